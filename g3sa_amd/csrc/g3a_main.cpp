@@ -1,0 +1,45 @@
+// g3a: command-line driver.
+//   g3a index [-s SA_INTV] <ref.fa> [prefix]
+//   g3a mem   [options] <prefix> <reads.fq>        (see g3a_mem.cpp)
+// `index` writes the BWA-format index plus the dense SA sample; `mem` keeps the
+// reference's `g3.exe mem` flags (bwa/src/fastmap.cpp:443-484).
+#include "g3a_index.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+int g3a_mem_main(int argc, char **argv);
+
+static int index_main(int argc, char **argv) {
+  int intv = 4;
+  int i = 1;
+  for (; i < argc && argv[i][0] == '-'; ++i) {
+    if (!std::strcmp(argv[i], "-s") && i + 1 < argc) intv = std::atoi(argv[++i]);
+    else { std::fprintf(stderr, "g3a index: unknown option %s\n", argv[i]); return 1; }
+  }
+  if (i >= argc) {
+    std::fprintf(stderr, "usage: g3a index [-s SA_INTV] <ref.fa> [prefix]\n");
+    return 1;
+  }
+  const char *fa = argv[i];
+  std::string prefix = i + 1 < argc ? argv[i + 1] : fa;
+  std::string err;
+  if (!g3a::build_index(fa, prefix.c_str(), intv, err)) {
+    std::fprintf(stderr, "g3a index: %s\n", err.c_str());
+    return 1;
+  }
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: g3a index|mem ...\n");
+    return 1;
+  }
+  if (!std::strcmp(argv[1], "index")) return index_main(argc - 1, argv + 1);
+  if (!std::strcmp(argv[1], "mem")) return g3a_mem_main(argc - 1, argv + 1);
+  std::fprintf(stderr, "g3a: unknown command %s\n", argv[1]);
+  return 1;
+}
