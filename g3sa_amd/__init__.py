@@ -1,0 +1,219 @@
+"""g3sa_amd: MI355X-native BWA-MEM short-read alignment hot path.
+
+The product is the native library ``_build/libg3a.so`` (HIP kernels behind the
+C ABI in ``include/g3a.h``).  This module binds it with ctypes; it never falls
+back to a CPU implementation: if the library is missing it raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+BUILD = PKG / "_build"
+LIB_PATH = BUILD / "libg3a.so"
+CLI_PATH = BUILD / "g3a"
+GEN_PATH = BUILD / "g3a-gen"
+
+G3A_OK = 0
+
+
+class Opt(C.Structure):
+    """g3a_opt (include/g3a.h)."""
+
+    _fields_ = [
+        ("a", C.c_int), ("b", C.c_int), ("o_del", C.c_int), ("e_del", C.c_int),
+        ("o_ins", C.c_int), ("e_ins", C.c_int), ("w", C.c_int), ("T", C.c_int),
+        ("pen_clip5", C.c_int), ("pen_clip3", C.c_int),
+        ("min_seed_len", C.c_int), ("split_width", C.c_int), ("max_occ", C.c_int),
+        ("max_chain_gap", C.c_int), ("flag", C.c_int),
+        ("max_mem_intv", C.c_int64),
+        ("split_factor", C.c_float), ("mask_level", C.c_float), ("drop_ratio", C.c_float),
+        ("mask_level_redun", C.c_float),
+        ("mat", C.c_int8 * 25),
+    ]
+
+
+class BatchOut(C.Structure):
+    """g3a_batch_out (include/g3a.h)."""
+
+    _fields_ = [
+        ("n_reads", C.c_int32),
+        ("aln_off", C.POINTER(C.c_int64)),
+        ("n_alns", C.c_int64),
+        ("rid", C.POINTER(C.c_int32)), ("pos", C.POINTER(C.c_int64)), ("flag", C.POINTER(C.c_int32)),
+        ("is_rev", C.POINTER(C.c_int32)), ("score", C.POINTER(C.c_int32)), ("nw_score", C.POINTER(C.c_int32)),
+        ("nm", C.POINTER(C.c_int32)), ("qb", C.POINTER(C.c_int32)), ("qe", C.POINTER(C.c_int32)),
+        ("rb", C.POINTER(C.c_int64)), ("re", C.POINTER(C.c_int64)),
+        ("n_cigar", C.POINTER(C.c_int32)), ("cigar_off", C.POINTER(C.c_int64)),
+        ("n_core", C.POINTER(C.c_int32)), ("core_off", C.POINTER(C.c_int64)),
+        ("cigar", C.POINTER(C.c_uint32)), ("n_cigar_words", C.c_int64),
+        ("ms_seed", C.c_float), ("ms_sa", C.c_float), ("ms_chain", C.c_float),
+        ("ms_extend", C.c_float), ("ms_traceback", C.c_float), ("ms_total", C.c_float),
+        ("n_seeds", C.c_int64), ("n_tasks", C.c_int64), ("ext_cells", C.c_int64),
+        ("fm_steps", C.c_int64), ("n_slow_reads", C.c_int64),
+    ]
+
+
+EXPORTS = [
+    "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
+    "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
+    "g3a_build_index", "g3a_last_stats",
+]
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Load libg3a.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(str(LIB_PATH))
+    lib.g3a_opt_init.argtypes = [C.POINTER(Opt)]
+    lib.g3a_device_alloc.argtypes = [C.c_int, C.c_char_p, C.POINTER(Opt), C.c_int, C.c_int64, C.POINTER(C.c_void_p)]
+    lib.g3a_memcpy_index.argtypes = [C.c_void_p]
+    lib.g3a_memcpy_input.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    lib.g3a_bwa_align.argtypes = [C.c_void_p]
+    lib.g3a_memcpy_output.argtypes = [C.c_void_p, C.POINTER(BatchOut)]
+    lib.g3a_align_batch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(BatchOut)]
+    lib.g3a_batch_out_free.argtypes = [C.POINTER(BatchOut)]
+    lib.g3a_destroy.argtypes = [C.c_void_p]
+    lib.g3a_last_error.argtypes = [C.c_void_p]
+    lib.g3a_last_error.restype = C.c_char_p
+    lib.g3a_build_index.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+    lib.g3a_last_stats.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+    _lib = lib
+    return lib
+
+
+def default_opt() -> Opt:
+    o = Opt()
+    load_library().g3a_opt_init(C.byref(o))
+    return o
+
+
+def build_index(fasta: str, prefix: str, sa_intv: int = 4) -> None:
+    lib = load_library()
+    err = C.create_string_buffer(512)
+    rc = lib.g3a_build_index(str(fasta).encode(), str(prefix).encode(), sa_intv, err, 512)
+    if rc != G3A_OK:
+        raise RuntimeError(f"g3a_build_index failed ({rc}): {err.value.decode()}")
+
+
+class Result:
+    """Alignments of one batch as numpy arrays (copied out of g3a_batch_out)."""
+
+    FIELDS = ["rid", "pos", "flag", "is_rev", "score", "nw_score", "nm", "qb", "qe", "rb", "re",
+              "n_cigar", "cigar_off", "n_core", "core_off"]
+
+    def __init__(self, out: BatchOut):
+        n, na = out.n_reads, out.n_alns
+        self.n_reads = n
+        self.n_alns = na
+        self.aln_off = np.ctypeslib.as_array(out.aln_off, shape=(n + 1,)).copy()
+        for f in self.FIELDS:
+            ptr = getattr(out, f)
+            setattr(self, f, np.ctypeslib.as_array(ptr, shape=(max(na, 1),))[:na].copy())
+        nw = out.n_cigar_words
+        self.cigar = np.ctypeslib.as_array(out.cigar, shape=(max(nw, 1),))[:nw].copy()
+        self.timing = {k: getattr(out, k) for k in
+                       ["ms_seed", "ms_sa", "ms_chain", "ms_extend", "ms_traceback", "ms_total"]}
+        self.work = {k: getattr(out, k) for k in ["n_seeds", "n_tasks", "ext_cells", "fm_steps", "n_slow_reads"]}
+
+
+class Aligner:
+    """A device context: index in HBM, batches aligned through the C ABI."""
+
+    def __init__(self, prefix: str, device: int = 0, max_reads: int = 1 << 20, max_bases: int | None = None,
+                 opt: Opt | None = None):
+        self.lib = load_library()
+        self.ctx = C.c_void_p()
+        if max_bases is None:
+            max_bases = max_reads * 320
+        rc = self.lib.g3a_device_alloc(device, str(prefix).encode(), C.byref(opt) if opt else None,
+                                       max_reads, max_bases, C.byref(self.ctx))
+        if rc != G3A_OK:
+            raise RuntimeError(f"g3a_device_alloc failed ({rc})")
+        self._check(self.lib.g3a_memcpy_index(self.ctx))
+
+    def _check(self, rc: int) -> None:
+        if rc != G3A_OK:
+            raise RuntimeError(f"g3a call failed ({rc}): {self.lib.g3a_last_error(self.ctx).decode()}")
+
+    def upload(self, seq: np.ndarray, offsets: np.ndarray) -> None:
+        seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        self._keep = (seq, offsets)
+        self._check(self.lib.g3a_memcpy_input(self.ctx, len(offsets) - 1, seq.ctypes.data, offsets.ctypes.data))
+
+    def run(self) -> None:
+        self._check(self.lib.g3a_bwa_align(self.ctx))
+
+    def download(self) -> Result:
+        out = BatchOut()
+        self._check(self.lib.g3a_memcpy_output(self.ctx, C.byref(out)))
+        try:
+            return Result(out)
+        finally:
+            self.lib.g3a_batch_out_free(C.byref(out))
+
+    def stats(self) -> dict:
+        ms = (C.c_float * 6)()
+        work = (C.c_int64 * 6)()
+        self._check(self.lib.g3a_last_stats(self.ctx, ms, work))
+        keys_ms = ["seed", "sa", "chain", "extend", "traceback", "total"]
+        keys_w = ["n_seeds", "n_tasks", "ext_cells", "fm_steps", "n_slow_reads", "n_alns"]
+        return {"ms": dict(zip(keys_ms, list(ms))), "work": dict(zip(keys_w, list(work)))}
+
+    def align(self, seq: np.ndarray, offsets: np.ndarray) -> Result:
+        self.upload(seq, offsets)
+        self.run()
+        return self.download()
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.g3a_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_fastq(path: str, limit: int = 0):
+    """Parse FASTQ with the reference's rules: name up to the first space,
+    sequence encoded A0 C1 G2 T3 (case-insensitive), anything else 4."""
+    names, seqs, quals = [], [], []
+    table = np.full(256, 4, dtype=np.uint8)
+    for i, ch in enumerate(b"ACGT"):
+        table[ch] = i
+        table[ch + 32] = i
+    with open(path, "rb") as f:
+        data = f.read()
+    lines = data.split(b"\n")
+    i = 0
+    while i + 3 < len(lines) + 1 and i < len(lines) and lines[i].startswith(b"@"):
+        hdr = lines[i][1:]
+        names.append(hdr.split(b" ", 1)[0])
+        seqs.append(table[np.frombuffer(lines[i + 1], dtype=np.uint8)])
+        quals.append(lines[i + 3] if i + 3 < len(lines) else b"")
+        i += 4
+        if limit and len(names) >= limit:
+            break
+    offsets = np.zeros(len(seqs) + 1, dtype=np.int32)
+    offsets[1:] = np.cumsum([len(s) for s in seqs]) if seqs else []
+    seq = np.concatenate(seqs) if seqs else np.zeros(0, dtype=np.uint8)
+    return names, seq, offsets, quals

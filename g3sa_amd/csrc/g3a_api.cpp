@@ -1,0 +1,573 @@
+// C-ABI implementation: context, HBM buffers and stage orchestration for the
+// BWA-MEM hot path (the reference's device_alloc / memcpy_index /
+// memcpy_input / bwa_align / memcpy_output, bwa/src/cuda_wrapper.cu and
+// bwa/src/bwamem.cu:92-425).
+//
+// All per-batch intermediate arrays are sized from exact counts: each stage
+// writes per-read counts, a device scan turns them into offsets and the
+// total is read back to size the next stage's buffers (grow-only, kept
+// across batches).
+#include "../../include/g3a.h"
+#include "g3a_index.h"
+#include "g3a_kernels.h"
+#include "g3a_layout.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace g3a;
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct g3a_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Opt opt{};
+  HostIndex h;
+  bool index_on_device = false;
+  DevIndex idx{};
+  int max_reads = 0;
+  int64_t max_bases = 0;
+  // index
+  DevBuf occ, pac, sa, ctg_off, ctg_len, ctg_alt;
+  // batch input
+  int n_reads = 0;
+  bool has_input = false, has_output = false;
+  DevBuf seq, off;
+  // seeding
+  DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
+  // seeds / chains
+  DevBuf seeds, seeds2, ch, nxt, order, tmpi, w16, kept, nodes, tasks_tmp, n_task, task_off;
+  // tasks / regions
+  DevBuf tasks, regs, flag, sec, n_aln, aln_off;
+  // alignments
+  DevBuf aln_reg, aln_read, aln_rank, z_size, cig_size, z_off, cig_off, z, cigar, alns;
+  DevBuf scan_tmp;
+  // host staging
+  int64_t *pinned = nullptr;
+  hipEvent_t ev[8] = {};
+  // last batch results
+  int64_t total_seeds = 0, total_tasks = 0, total_alns = 0, total_cig = 0, n_slow = 0;
+  float ms[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(g3a_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCK(call)                                                                              \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return fail(ctx, G3A_E_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(g3a_ctx *ctx, DevBuf &b, size_t bytes) {
+  if (bytes == 0) bytes = 64;
+  if (b.bytes >= bytes) return 0;
+  if (b.p) {
+    HIPCK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes + bytes / 4;  // headroom so repeated batches do not reallocate
+  HIPCK(hipMalloc(&b.p, want));
+  b.bytes = want;
+  return 0;
+}
+
+template <class T>
+T *P(DevBuf &b) {
+  return (T *)b.p;
+}
+
+void free_buf(DevBuf &b) {
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void g3a_opt_init(g3a_opt *o) {
+  std::memset(o, 0, sizeof *o);
+  o->a = 1; o->b = 4;
+  o->o_del = o->o_ins = 6;
+  o->e_del = o->e_ins = 1;
+  o->w = 100;
+  o->T = 30;
+  o->pen_clip5 = o->pen_clip3 = 5;
+  o->max_mem_intv = 20;
+  o->min_seed_len = 19;
+  o->split_width = 10;
+  o->max_occ = 500;
+  o->max_chain_gap = 10000;
+  o->mask_level = 0.50f;
+  o->drop_ratio = 0.50f;
+  o->split_factor = 1.5f;
+  o->mask_level_redun = 0.95f;
+  int k = 0;
+  for (int i = 0; i < 4; ++i) {
+    for (int j = 0; j < 4; ++j) o->mat[k++] = (int8_t)(i == j ? o->a : -o->b);
+    o->mat[k++] = -1;
+  }
+  for (int j = 0; j < 5; ++j) o->mat[k++] = -1;
+}
+
+const char *g3a_last_error(const g3a_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int g3a_build_index(const char *fasta, const char *prefix, int sa_intv, char *err, int err_len) {
+  std::string e;
+  if (!fasta || !prefix) e = "null argument";
+  else if (build_index(fasta, prefix, sa_intv, e)) return G3A_OK;
+  if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", e.c_str());
+  return e == "null argument" ? G3A_E_ARG : G3A_E_INDEX;
+}
+
+int g3a_device_alloc(int device, const char *prefix, const g3a_opt *o, int max_reads, int64_t max_bases,
+                     g3a_ctx **out) {
+  if (!out || !prefix || max_reads < 1 || max_bases < 1) return G3A_E_ARG;
+  *out = nullptr;
+  g3a_ctx *ctx = new g3a_ctx;
+  ctx->device = device;
+  ctx->max_reads = max_reads;
+  ctx->max_bases = max_bases;
+  g3a_opt d;
+  if (!o) { g3a_opt_init(&d); o = &d; }
+  Opt &p = ctx->opt;
+  p.a = o->a; p.b = o->b; p.o_del = o->o_del; p.e_del = o->e_del; p.o_ins = o->o_ins; p.e_ins = o->e_ins;
+  p.w = o->w; p.T = o->T; p.pen_clip5 = o->pen_clip5; p.pen_clip3 = o->pen_clip3;
+  p.min_seed_len = o->min_seed_len; p.split_width = o->split_width; p.max_occ = o->max_occ;
+  p.max_chain_gap = o->max_chain_gap; p.flag = o->flag; p.max_mem_intv = o->max_mem_intv;
+  p.split_factor = o->split_factor; p.mask_level = o->mask_level; p.drop_ratio = o->drop_ratio;
+  p.mask_level_redun = o->mask_level_redun;
+  std::memcpy(p.mat, o->mat, 25);
+  if (p.max_occ < 1 || p.min_seed_len < 1 || p.e_del < 1 || p.e_ins < 1) {
+    delete ctx;
+    return G3A_E_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    delete ctx;
+    return G3A_E_HIP;
+  }
+  std::string e;
+  if (!load_index(prefix, ctx->h, e)) {
+    delete ctx;
+    return G3A_E_INDEX;
+  }
+  for (auto &c : ctx->h.anns)
+    if (c.n_ambs < 0) { delete ctx; return G3A_E_INDEX; }
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return G3A_E_HIP;
+  }
+  for (auto &ev : ctx->ev) hipEventCreate(&ev);
+  if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+    g3a_destroy(ctx);
+    return G3A_E_HIP;
+  }
+  *out = ctx;
+  return G3A_OK;
+}
+
+int g3a_memcpy_index(g3a_ctx *ctx) {
+  if (!ctx) return G3A_E_ARG;
+  HIPCK(hipSetDevice(ctx->device));
+  const HostIndex &h = ctx->h;
+  std::vector<uint64_t> lines;
+  build_occ_lines(h, lines);
+  if (ensure(ctx, ctx->occ, lines.size() * 8)) return G3A_E_HIP;
+  HIPCK(hipMemcpy(ctx->occ.p, lines.data(), lines.size() * 8, hipMemcpyHostToDevice));
+  if (ensure(ctx, ctx->pac, h.pac.size())) return G3A_E_HIP;
+  HIPCK(hipMemcpy(ctx->pac.p, h.pac.data(), h.pac.size(), hipMemcpyHostToDevice));
+  if (ensure(ctx, ctx->sa, h.sa.size())) return G3A_E_HIP;
+  HIPCK(hipMemcpy(ctx->sa.p, h.sa.data(), h.sa.size(), hipMemcpyHostToDevice));
+  std::vector<int64_t> co, cl;
+  std::vector<int32_t> ca;
+  for (auto &c : h.anns) { co.push_back(c.offset); cl.push_back(c.len); ca.push_back(c.is_alt); }
+  if (ensure(ctx, ctx->ctg_off, co.size() * 8) || ensure(ctx, ctx->ctg_len, cl.size() * 8) ||
+      ensure(ctx, ctx->ctg_alt, ca.size() * 4))
+    return G3A_E_HIP;
+  HIPCK(hipMemcpy(ctx->ctg_off.p, co.data(), co.size() * 8, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(ctx->ctg_len.p, cl.data(), cl.size() * 8, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(ctx->ctg_alt.p, ca.data(), ca.size() * 4, hipMemcpyHostToDevice));
+  DevIndex &x = ctx->idx;
+  x.occ = P<uint64_t>(ctx->occ);
+  build_counts(h, x.count);
+  x.primary = (int64_t)h.primary;
+  x.n = (int64_t)h.seq_len;
+  x.l_pac = h.l_pac;
+  x.pac = P<uint8_t>(ctx->pac);
+  x.sa32 = h.sa_width == 4 ? P<uint32_t>(ctx->sa) : nullptr;
+  x.sa64 = h.sa_width == 8 ? P<uint64_t>(ctx->sa) : nullptr;
+  x.sa_shift = log2_exact(h.sa_intv);
+  x.n_contigs = (int)h.anns.size();
+  x.ctg_off = P<int64_t>(ctx->ctg_off);
+  x.ctg_len = P<int64_t>(ctx->ctg_len);
+  x.ctg_alt = P<int32_t>(ctx->ctg_alt);
+  ctx->index_on_device = true;
+  return G3A_OK;
+}
+
+int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t *offsets) {
+  if (!ctx || n_reads < 0 || (n_reads > 0 && (!seq || !offsets))) return G3A_E_ARG;
+  if (n_reads > ctx->max_reads) return fail(ctx, G3A_E_ARG, "batch larger than max_reads");
+  if (n_reads > 0) {
+    if (offsets[0] != 0) return fail(ctx, G3A_E_ARG, "offsets[0] must be 0");
+    for (int i = 0; i < n_reads; ++i)
+      if (offsets[i + 1] < offsets[i]) return fail(ctx, G3A_E_ARG, "offsets must be non-decreasing");
+    if (offsets[n_reads] > ctx->max_bases) return fail(ctx, G3A_E_ARG, "batch larger than max_bases");
+  }
+  HIPCK(hipSetDevice(ctx->device));
+  const int64_t nb = n_reads ? offsets[n_reads] : 0;
+  if (ensure(ctx, ctx->seq, (size_t)nb + 64) || ensure(ctx, ctx->off, (size_t)(n_reads + 1) * 4)) return G3A_E_HIP;
+  if (nb) HIPCK(hipMemcpyAsync(ctx->seq.p, seq, (size_t)nb, hipMemcpyHostToDevice, ctx->stream));
+  if (n_reads) HIPCK(hipMemcpyAsync(ctx->off.p, offsets, (size_t)(n_reads + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  ctx->n_reads = n_reads;
+  ctx->has_input = true;
+  ctx->has_output = false;
+  return G3A_OK;
+}
+
+namespace {
+
+// read back k int64 values that sit in device memory at the given addresses
+int readback(g3a_ctx *ctx, std::initializer_list<const int64_t *> src) {
+  int i = 0;
+  for (const int64_t *p : src) {
+    HIPCK(hipMemcpyAsync(ctx->pinned + i, p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    ++i;
+  }
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // namespace
+
+int g3a_bwa_align(g3a_ctx *ctx) {
+  if (!ctx) return G3A_E_ARG;
+  if (!ctx->index_on_device || !ctx->has_input) return fail(ctx, G3A_E_STATE, "index or input not on device");
+  HIPCK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int n = ctx->n_reads;
+  ctx->has_output = false;
+  ctx->total_seeds = ctx->total_tasks = ctx->total_alns = ctx->total_cig = ctx->n_slow = 0;
+  if (ensure(ctx, ctx->n_intv, (size_t)n * 4 + 4) || ensure(ctx, ctx->n_seed, (size_t)n * 8 + 8) ||
+      ensure(ctx, ctx->slow, (size_t)n * 4 + 4) || ensure(ctx, ctx->intv_base, (size_t)n * 8 + 8) ||
+      ensure(ctx, ctx->seed_off, (size_t)(n + 1) * 8) || ensure(ctx, ctx->counters, 64 * 8) ||
+      ensure(ctx, ctx->n_task, (size_t)n * 4 + 4) || ensure(ctx, ctx->task_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->n_aln, (size_t)n * 4 + 4) || ensure(ctx, ctx->aln_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->slow_list, (size_t)n * 4 + 4) || ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(ctx->max_reads + 1)))
+    return G3A_E_HIP;
+  // fast-path interval slots plus a pool that grows for slow reads
+  const size_t fast_entries = (size_t)n * kSlotCap;
+  if (ensure(ctx, ctx->intv, (fast_entries + 1024) * sizeof(Intv))) return G3A_E_HIP;
+  int64_t *cnt = P<int64_t>(ctx->counters);
+  HIPCK(hipMemsetAsync(cnt, 0, 64 * 8, s));
+  HIPCK(hipEventRecord(ctx->ev[0], s));
+  // ---- seeding
+  SeedArgs sa{};
+  sa.idx = ctx->idx;
+  sa.opt = ctx->opt;
+  sa.n_reads = n;
+  sa.seq = P<uint8_t>(ctx->seq);
+  sa.off = P<int32_t>(ctx->off);
+  sa.intv = P<Intv>(ctx->intv);
+  sa.slot_cap = kSlotCap;
+  sa.n_intv = P<int32_t>(ctx->n_intv);
+  sa.n_seed = P<int64_t>(ctx->n_seed);
+  sa.slow = P<int32_t>(ctx->slow);
+  sa.fm_steps = cnt + 0;
+  launch_seed(sa, s);
+  launch_collect_slow(n, P<int32_t>(ctx->slow), P<int32_t>(ctx->slow_list), P<int32_t>(ctx->counters) + 2 * 8,
+                      P<int64_t>(ctx->intv_base), kSlotCap, s);
+  if (readback(ctx, {cnt + 8})) return G3A_E_HIP;
+  const int n_slow = (int)(ctx->pinned[0] & 0xffffffff);
+  ctx->n_slow = n_slow;
+  if (n_slow > 0) {
+    // counts, then pool offsets, then the intervals themselves
+    const int chunk = 16384;
+    if (ensure(ctx, ctx->slow_scratch, (size_t)chunk * kSlowScratch * sizeof(Intv)) ||
+        ensure(ctx, ctx->slow_off, (size_t)(n_slow + 1) * 8) || ensure(ctx, ctx->slow_n, (size_t)n_slow * 8 + 8))
+      return G3A_E_HIP;
+    const int32_t *list = P<int32_t>(ctx->slow_list);
+    for (int c0 = 0; c0 < n_slow; c0 += chunk) {
+      int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
+      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), nullptr, nullptr, 0, s);
+    }
+    launch_gather_slow_counts(n_slow, list, P<int32_t>(ctx->n_intv), P<int64_t>(ctx->slow_n), s);
+    scan_i64(P<int64_t>(ctx->slow_n), P<int64_t>(ctx->slow_off), n_slow, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+    if (readback(ctx, {P<int64_t>(ctx->slow_off) + n_slow})) return G3A_E_HIP;
+    const int64_t pool = ctx->pinned[0];
+    // grow the interval buffer, keeping the fast slots
+    const size_t need = (fast_entries + (size_t)pool + 1024) * sizeof(Intv);
+    if (ctx->intv.bytes < need) {
+      DevBuf nb;
+      if (ensure(ctx, nb, need)) return G3A_E_HIP;
+      HIPCK(hipMemcpyAsync(nb.p, ctx->intv.p, fast_entries * sizeof(Intv), hipMemcpyDeviceToDevice, s));
+      HIPCK(hipStreamSynchronize(s));
+      free_buf(ctx->intv);
+      ctx->intv = nb;
+      sa.intv = P<Intv>(ctx->intv);
+    }
+    Intv *pool_base = P<Intv>(ctx->intv) + fast_entries;
+    for (int c0 = 0; c0 < n_slow; c0 += chunk) {
+      int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
+      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), P<int64_t>(ctx->slow_off) + c0, pool_base, 1, s);
+    }
+    launch_set_slow_base(n_slow, list, P<int64_t>(ctx->slow_off), (int64_t)fast_entries, P<int64_t>(ctx->intv_base), s);
+  }
+  HIPCK(hipEventRecord(ctx->ev[1], s));
+  // ---- SA lookup and seed sort
+  scan_i64(P<int64_t>(ctx->n_seed), P<int64_t>(ctx->seed_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->seed_off) + n})) return G3A_E_HIP;
+  const int64_t T_s = ctx->pinned[0];
+  ctx->total_seeds = T_s;
+  if (ensure(ctx, ctx->seeds, (size_t)T_s * sizeof(Seed)) || ensure(ctx, ctx->seeds2, (size_t)T_s * sizeof(Seed)))
+    return G3A_E_HIP;
+  SaArgs la{};
+  la.idx = ctx->idx;
+  la.opt = ctx->opt;
+  la.n_reads = n;
+  la.total = T_s;
+  la.seed_off = P<int64_t>(ctx->seed_off);
+  la.intv = P<Intv>(ctx->intv);
+  la.intv_base = P<int64_t>(ctx->intv_base);
+  la.n_intv = P<int32_t>(ctx->n_intv);
+  la.seeds = P<Seed>(ctx->seeds);
+  launch_sa_lookup(la, s);
+  SortArgs so{n, P<int64_t>(ctx->seed_off), P<Seed>(ctx->seeds), P<Seed>(ctx->seeds2)};
+  launch_sort(so, s);
+  HIPCK(hipEventRecord(ctx->ev[2], s));
+  // ---- chaining
+  const size_t n_nodes = (size_t)((T_s + 32 * (int64_t)(n + 1)) / 4 + 2);
+  if (ensure(ctx, ctx->ch, (size_t)T_s * sizeof(ChainTmp)) || ensure(ctx, ctx->nxt, (size_t)T_s * 4) ||
+      ensure(ctx, ctx->order, (size_t)T_s * 4) || ensure(ctx, ctx->tmpi, (size_t)T_s * 8 + 8) ||
+      ensure(ctx, ctx->w16, (size_t)T_s * 2) || ensure(ctx, ctx->kept, (size_t)T_s) ||
+      ensure(ctx, ctx->nodes, n_nodes * sizeof(KbNode)) || ensure(ctx, ctx->tasks_tmp, (size_t)T_s * sizeof(Task)))
+    return G3A_E_HIP;
+  ChainArgs ca{};
+  ca.idx = ctx->idx;
+  ca.opt = ctx->opt;
+  ca.n_reads = n;
+  ca.off = P<int32_t>(ctx->off);
+  ca.seed_off = P<int64_t>(ctx->seed_off);
+  ca.seeds = P<Seed>(ctx->seeds);
+  ca.ch = P<ChainTmp>(ctx->ch);
+  ca.nxt = P<int32_t>(ctx->nxt);
+  ca.order = P<int32_t>(ctx->order);
+  ca.tmpi = P<int32_t>(ctx->tmpi);
+  ca.w16 = P<uint16_t>(ctx->w16);
+  ca.kept = P<uint8_t>(ctx->kept);
+  ca.nodes = P<KbNode>(ctx->nodes);
+  ca.tasks_tmp = P<Task>(ctx->tasks_tmp);
+  ca.n_task = P<int32_t>(ctx->n_task);
+  ca.error = P<int32_t>(ctx->counters) + 2 * 16;
+  launch_chain(ca, s);
+  scan_i32(P<int32_t>(ctx->n_task), P<int64_t>(ctx->task_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->task_off) + n, cnt + 16})) return G3A_E_HIP;
+  const int64_t T_t = ctx->pinned[0];
+  if (ctx->pinned[1] & 0xffffffff) return fail(ctx, G3A_E_INTERNAL, "chain node arena overflow");
+  ctx->total_tasks = T_t;
+  if (ensure(ctx, ctx->tasks, (size_t)T_t * sizeof(Task)) || ensure(ctx, ctx->regs, (size_t)T_t * sizeof(Region)) ||
+      ensure(ctx, ctx->flag, (size_t)T_t) || ensure(ctx, ctx->sec, (size_t)T_t * 4))
+    return G3A_E_HIP;
+  CompactArgs cp{n, P<int64_t>(ctx->seed_off), P<int64_t>(ctx->task_off), P<Task>(ctx->tasks_tmp), P<Task>(ctx->tasks)};
+  launch_task_compact(cp, s);
+  HIPCK(hipEventRecord(ctx->ev[3], s));
+  // ---- extension
+  ExtArgs ea{};
+  ea.idx = ctx->idx;
+  ea.opt = ctx->opt;
+  ea.n_tasks = T_t;
+  ea.seq = P<uint8_t>(ctx->seq);
+  ea.off = P<int32_t>(ctx->off);
+  ea.tasks = P<Task>(ctx->tasks);
+  ea.regs = P<Region>(ctx->regs);
+  ea.cells = cnt + 24;
+  launch_extend(ea, s);
+  HIPCK(hipEventRecord(ctx->ev[4], s));
+  // ---- region filter, traceback, finalize
+  RegionArgs ra{ctx->opt, n, P<int64_t>(ctx->task_off), P<Region>(ctx->regs), P<uint8_t>(ctx->flag),
+                P<int32_t>(ctx->sec), P<int32_t>(ctx->n_aln)};
+  launch_region_filter(ra, s);
+  scan_i32(P<int32_t>(ctx->n_aln), P<int64_t>(ctx->aln_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->aln_off) + n})) return G3A_E_HIP;
+  const int64_t T_a = ctx->pinned[0];
+  ctx->total_alns = T_a;
+  if (ensure(ctx, ctx->aln_reg, (size_t)T_a * sizeof(Region)) || ensure(ctx, ctx->aln_read, (size_t)T_a * 4) ||
+      ensure(ctx, ctx->aln_rank, (size_t)T_a * 4) || ensure(ctx, ctx->z_size, (size_t)T_a * 8) ||
+      ensure(ctx, ctx->cig_size, (size_t)T_a * 8) || ensure(ctx, ctx->z_off, (size_t)(T_a + 1) * 8) ||
+      ensure(ctx, ctx->cig_off, (size_t)(T_a + 1) * 8) || ensure(ctx, ctx->alns, (size_t)T_a * sizeof(Aln)) ||
+      ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(T_a + 1)))
+    return G3A_E_HIP;
+  AlnPrepArgs ap{ctx->opt, n, P<int64_t>(ctx->task_off), P<int64_t>(ctx->aln_off), P<Region>(ctx->regs),
+                 P<Region>(ctx->aln_reg), P<int32_t>(ctx->aln_read), P<int32_t>(ctx->aln_rank),
+                 P<int64_t>(ctx->z_size), P<int64_t>(ctx->cig_size)};
+  launch_aln_prep(ap, s);
+  scan_i64(P<int64_t>(ctx->z_size), P<int64_t>(ctx->z_off), T_a, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  scan_i64(P<int64_t>(ctx->cig_size), P<int64_t>(ctx->cig_off), T_a, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->z_off) + T_a, P<int64_t>(ctx->cig_off) + T_a})) return G3A_E_HIP;
+  const int64_t T_z = ctx->pinned[0], T_c = ctx->pinned[1];
+  ctx->total_cig = T_c;
+  if (ensure(ctx, ctx->z, (size_t)T_z + 64) || ensure(ctx, ctx->cigar, (size_t)T_c * 4 + 64)) return G3A_E_HIP;
+  TbArgs tb{};
+  tb.idx = ctx->idx;
+  tb.opt = ctx->opt;
+  tb.n_alns = T_a;
+  tb.seq = P<uint8_t>(ctx->seq);
+  tb.off = P<int32_t>(ctx->off);
+  tb.aln_reg = P<Region>(ctx->aln_reg);
+  tb.aln_read = P<int32_t>(ctx->aln_read);
+  tb.aln_rank = P<int32_t>(ctx->aln_rank);
+  tb.z_off = P<int64_t>(ctx->z_off);
+  tb.cig_off = P<int64_t>(ctx->cig_off);
+  tb.z = P<uint8_t>(ctx->z);
+  tb.cigar = P<uint32_t>(ctx->cigar);
+  tb.alns = P<Aln>(ctx->alns);
+  launch_traceback(tb, s);
+  HIPCK(hipEventRecord(ctx->ev[5], s));
+  HIPCK(hipStreamSynchronize(s));
+  HIPCK(hipGetLastError());
+  float t[5];
+  for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], ctx->ev[i], ctx->ev[i + 1]);
+  for (int i = 0; i < 5; ++i) ctx->ms[i] = t[i];
+  hipEventElapsedTime(&ctx->ms[5], ctx->ev[0], ctx->ev[5]);
+  ctx->has_output = true;
+  return G3A_OK;
+}
+
+int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
+  if (!ctx || !out) return G3A_E_ARG;
+  if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "no alignment results");
+  HIPCK(hipSetDevice(ctx->device));
+  std::memset(out, 0, sizeof *out);
+  const int n = ctx->n_reads;
+  const int64_t na = ctx->total_alns;
+  out->n_reads = n;
+  out->n_alns = na;
+  out->aln_off = (int64_t *)std::malloc(sizeof(int64_t) * (size_t)(n + 1));
+  std::vector<Aln> alns((size_t)na);
+  std::vector<uint32_t> cig((size_t)ctx->total_cig);
+  if (n) HIPCK(hipMemcpy(out->aln_off, ctx->aln_off.p, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyDeviceToHost));
+  else out->aln_off[0] = 0;
+  if (na) HIPCK(hipMemcpy(alns.data(), ctx->alns.p, sizeof(Aln) * (size_t)na, hipMemcpyDeviceToHost));
+  if (ctx->total_cig) HIPCK(hipMemcpy(cig.data(), ctx->cigar.p, 4 * (size_t)ctx->total_cig, hipMemcpyDeviceToHost));
+  const size_t m = (size_t)(na ? na : 1);
+  out->rid = (int32_t *)std::malloc(4 * m);
+  out->pos = (int64_t *)std::malloc(8 * m);
+  out->flag = (int32_t *)std::malloc(4 * m);
+  out->is_rev = (int32_t *)std::malloc(4 * m);
+  out->score = (int32_t *)std::malloc(4 * m);
+  out->nw_score = (int32_t *)std::malloc(4 * m);
+  out->nm = (int32_t *)std::malloc(4 * m);
+  out->qb = (int32_t *)std::malloc(4 * m);
+  out->qe = (int32_t *)std::malloc(4 * m);
+  out->rb = (int64_t *)std::malloc(8 * m);
+  out->re = (int64_t *)std::malloc(8 * m);
+  out->n_cigar = (int32_t *)std::malloc(4 * m);
+  out->cigar_off = (int64_t *)std::malloc(8 * m);
+  out->n_core = (int32_t *)std::malloc(4 * m);
+  out->core_off = (int64_t *)std::malloc(8 * m);
+  int64_t words = 0;
+  for (int64_t i = 0; i < na; ++i) words += alns[(size_t)i].n_cigar + alns[(size_t)i].n_core;
+  out->cigar = (uint32_t *)std::malloc(4 * (size_t)(words ? words : 1));
+  int64_t w = 0;
+  for (int64_t i = 0; i < na; ++i) {
+    const Aln &a = alns[(size_t)i];
+    out->rid[i] = a.rid; out->pos[i] = a.pos; out->flag[i] = a.flag; out->is_rev[i] = a.is_rev;
+    out->score[i] = a.score; out->nw_score[i] = a.nw_score; out->nm[i] = a.nm;
+    out->qb[i] = a.qb; out->qe[i] = a.qe; out->rb[i] = a.rb; out->re[i] = a.re;
+    out->n_cigar[i] = a.n_cigar; out->cigar_off[i] = w;
+    std::memcpy(out->cigar + w, cig.data() + a.cigar_off, 4 * (size_t)a.n_cigar);
+    w += a.n_cigar;
+    out->n_core[i] = a.n_core; out->core_off[i] = w;
+    std::memcpy(out->cigar + w, cig.data() + a.core_off, 4 * (size_t)a.n_core);
+    w += a.n_core;
+  }
+  out->n_cigar_words = w;
+  out->ms_seed = ctx->ms[0]; out->ms_sa = ctx->ms[1]; out->ms_chain = ctx->ms[2];
+  out->ms_extend = ctx->ms[3]; out->ms_traceback = ctx->ms[4]; out->ms_total = ctx->ms[5];
+  int64_t c[4] = {0, 0, 0, 0};
+  HIPCK(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  int64_t cells = 0;
+  HIPCK(hipMemcpy(&cells, P<int64_t>(ctx->counters) + 24, 8, hipMemcpyDeviceToHost));
+  out->fm_steps = c[0];
+  out->ext_cells = cells;
+  out->n_seeds = ctx->total_seeds;
+  out->n_tasks = ctx->total_tasks;
+  out->n_slow_reads = ctx->n_slow;
+  return G3A_OK;
+}
+
+int g3a_align_batch(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t *offsets, g3a_batch_out *out) {
+  int rc = g3a_memcpy_input(ctx, n_reads, seq, offsets);
+  if (rc) return rc;
+  rc = g3a_bwa_align(ctx);
+  if (rc) return rc;
+  return g3a_memcpy_output(ctx, out);
+}
+
+int g3a_last_stats(const g3a_ctx *ctx, float ms[6], int64_t work[6]) {
+  if (!ctx || !ms || !work) return G3A_E_ARG;
+  if (!ctx->has_output) return G3A_E_STATE;
+  for (int i = 0; i < 6; ++i) ms[i] = ctx->ms[i];
+  int64_t c[32];
+  if (hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) return G3A_E_HIP;
+  work[0] = ctx->total_seeds;
+  work[1] = ctx->total_tasks;
+  work[2] = c[24];
+  work[3] = c[0];
+  work[4] = ctx->n_slow;
+  work[5] = ctx->total_alns;
+  return G3A_OK;
+}
+
+void g3a_batch_out_free(g3a_batch_out *o) {
+  if (!o) return;
+  std::free(o->aln_off); std::free(o->rid); std::free(o->pos); std::free(o->flag); std::free(o->is_rev);
+  std::free(o->score); std::free(o->nw_score); std::free(o->nm); std::free(o->qb); std::free(o->qe);
+  std::free(o->rb); std::free(o->re); std::free(o->n_cigar); std::free(o->cigar_off); std::free(o->n_core);
+  std::free(o->core_off); std::free(o->cigar);
+  std::memset(o, 0, sizeof *o);
+}
+
+void g3a_destroy(g3a_ctx *ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&ctx->occ, &ctx->pac, &ctx->sa, &ctx->ctg_off, &ctx->ctg_len, &ctx->ctg_alt, &ctx->seq, &ctx->off,
+                    &ctx->intv, &ctx->n_intv, &ctx->n_seed, &ctx->slow, &ctx->intv_base, &ctx->seed_off,
+                    &ctx->counters, &ctx->slow_list, &ctx->slow_scratch, &ctx->slow_n, &ctx->slow_off, &ctx->seeds,
+                    &ctx->seeds2, &ctx->ch, &ctx->nxt, &ctx->order, &ctx->tmpi, &ctx->w16, &ctx->kept, &ctx->nodes,
+                    &ctx->tasks_tmp, &ctx->n_task, &ctx->task_off, &ctx->tasks, &ctx->regs, &ctx->flag, &ctx->sec,
+                    &ctx->n_aln, &ctx->aln_off, &ctx->aln_reg, &ctx->aln_read, &ctx->aln_rank, &ctx->z_size,
+                    &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp};
+  for (DevBuf *b : bufs) free_buf(*b);
+  for (auto &ev : ctx->ev)
+    if (ev) hipEventDestroy(ev);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+}  // extern "C"

@@ -1,0 +1,1049 @@
+// Per-read device logic of the BWA-MEM hot path for MI355X (gfx950).
+//
+// Everything here is __host__ __device__ so that the same code can also be
+// compiled by g++ for CPU-side debugging; the product always runs it inside
+// the HIP kernels of g3a_kernels.hip.  Each function cites the reference
+// file:line whose behaviour it reproduces (paths relative to the reference).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define G3A_HD __host__ __device__ __forceinline__
+#if defined(__HIP_DEVICE_COMPILE__)
+#define G3A_POPC(x) __popcll(x)
+#else
+#define G3A_POPC(x) __builtin_popcountll(x)
+#endif
+#else
+#define G3A_HD inline
+#define G3A_POPC(x) __builtin_popcountll(x)
+#endif
+
+namespace g3a {
+
+constexpr int kMaxReadLen = 320;  // MAX_LEN_READ, bwa/src/macro.h:27
+
+// Options used on the path (subset of mem_opt_t, bwa/src/bwa.h:157-189).
+struct Opt {
+  int a, b, o_del, e_del, o_ins, e_ins, w, T, pen_clip5, pen_clip3;
+  int min_seed_len, split_width, max_occ, max_chain_gap, flag;
+  int64_t max_mem_intv;
+  float split_factor, mask_level, drop_ratio, mask_level_redun;
+  int8_t mat[25];
+};
+
+// Device view of the index.  `occ` holds one 64-byte line per 64 BWT rows:
+// four cumulative base counts (rows before the line) then four one-hot masks
+// (bit 63-y set when row 64*line+y holds that base; '$' sets no bit).  One
+// Occ lookup is a single line fetch.
+struct DevIndex {
+  const uint64_t *occ;
+  int64_t count[5];  // count[c] = 1 + #bases < c (the '$' row first), count[4] = n+1
+  int64_t primary;   // row whose suffix starts at text position 0 (BWT '$')
+  int64_t n;         // text length 2*l_pac; rows 0..n
+  int64_t l_pac;
+  const uint8_t *pac;       // forward strand, 2 bits per base, MSB first
+  const uint32_t *sa32;     // dense SA sample (one of sa32/sa64 is set)
+  const uint64_t *sa64;
+  int sa_shift;             // sample interval = 1 << sa_shift
+  int n_contigs;
+  const int64_t *ctg_off;   // n_contigs
+  const int64_t *ctg_len;
+  const int32_t *ctg_alt;
+};
+
+struct Intv {
+  int64_t k, l, s;
+  int32_t m, n;  // read[m..n]
+};
+
+struct Seed {
+  int64_t rbeg;
+  int32_t qbeg, len;
+  int32_t rid, pad;
+};
+
+// ---------------------------------------------------------------------------
+// FM-index primitives (bwa/src/fmindex.cu:29-45, 107-180; macro.h:50-71)
+// ---------------------------------------------------------------------------
+G3A_HD int64_t occ1(const DevIndex &x, int64_t p, int c) {
+  const uint64_t *b = x.occ + ((p >> 6) << 3);
+  uint64_t mask = ~0ULL << (63 - (p & 63));
+  return (int64_t)b[c] + G3A_POPC(b[4 + c] & mask);
+}
+
+// all four bases at one row: one line
+G3A_HD void occ4(const DevIndex &x, int64_t p, int64_t o[4]) {
+  const uint64_t *b = x.occ + ((p >> 6) << 3);
+  uint64_t mask = ~0ULL << (63 - (p & 63));
+#pragma unroll
+  for (int c = 0; c < 4; ++c) o[c] = (int64_t)b[c] + G3A_POPC(b[4 + c] & mask);
+}
+
+// BWT character of row k (4 for the '$' row)
+G3A_HD int bwt_char(const DevIndex &x, int64_t k) {
+  const uint64_t *b = x.occ + ((k >> 6) << 3);
+  uint64_t bit = 1ULL << (63 - (k & 63));
+  int c = 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (b[4 + i] & bit) c = i;
+  return c;
+}
+
+G3A_HD void intv_init(const DevIndex &x, Intv &p, int c) {
+  p.k = x.count[c];
+  p.l = x.count[3 - c];
+  p.s = x.count[c + 1] - x.count[c];
+}
+
+// backwardExt (fmindex.cu:107-160): extension by `base` with the l update
+G3A_HD void backward_ext(const DevIndex &x, const Intv &in, int base, Intv &out) {
+  int64_t osp[4], oep[4];
+  occ4(x, in.k - 1, osp);
+  occ4(x, in.k + in.s - 1, oep);
+  int64_t s[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) s[b] = oep[b] - osp[b];
+  int64_t so = (in.k <= x.primary && in.k + in.s > x.primary) ? 1 : 0;
+  int64_t l3 = in.l + so, l2 = l3 + s[3], l1 = l2 + s[2], l0 = l1 + s[1];
+  int64_t l = base == 0 ? l0 : base == 1 ? l1 : base == 2 ? l2 : l3;
+  out.k = x.count[base] + osp[base];
+  out.l = l;
+  out.s = s[base];
+}
+
+// backwardExtBackward (fmindex.cu:162-180): k and s only
+G3A_HD void backward_ext_b(const DevIndex &x, Intv &io, int base) {
+  int64_t a = occ1(x, io.k - 1, base), b = occ1(x, io.k + io.s - 1, base);
+  io.k = x.count[base] + a;
+  io.s = b - a;
+}
+
+// FORWARD_EXT (macro.h:64-71)
+G3A_HD void forward_ext(const DevIndex &x, Intv &io, int base) {
+  Intv sw = io, o;
+  sw.k = io.l;
+  sw.l = io.k;
+  backward_ext(x, sw, 3 - base, o);
+  io.k = o.l;
+  io.l = o.k;
+  io.s = o.s;
+}
+
+// SA value of row k through the dense sample (LF walk to a sampled row)
+G3A_HD int64_t sa_value(const DevIndex &x, int64_t k, int64_t *steps) {
+  const int64_t mask = ((int64_t)1 << x.sa_shift) - 1;
+  int64_t t = 0;
+  while (k & mask) {
+    if (k == x.primary) return t;
+    int c = bwt_char(x, k);
+    k = x.count[c] - 1 + occ1(x, k, c);
+    ++t;
+  }
+  if (steps) *steps += t;
+  int64_t i = k >> x.sa_shift;
+  return t + (x.sa32 ? (int64_t)x.sa32[i] : (int64_t)x.sa64[i]);
+}
+
+// bns_pos2rid_gpu / bns_intv2rid_gpu (bwa/src/bntseq.cu:7-37)
+G3A_HD int pos2rid(const DevIndex &x, int64_t pos_f) {
+  if (pos_f >= x.l_pac) return -1;
+  int left = 0, mid = 0, right = x.n_contigs;
+  while (left < right) {
+    mid = (left + right) >> 1;
+    if (pos_f >= x.ctg_off[mid]) {
+      if (mid == x.n_contigs - 1) break;
+      if (pos_f < x.ctg_off[mid + 1]) break;
+      left = mid + 1;
+    } else right = mid;
+  }
+  return mid;
+}
+G3A_HD int64_t depos(const DevIndex &x, int64_t pos, int &is_rev) {
+  is_rev = pos >= x.l_pac;
+  return is_rev ? (x.l_pac << 1) - 1 - pos : pos;
+}
+G3A_HD int intv2rid(const DevIndex &x, int64_t rb, int64_t re) {
+  int is_rev;
+  if (rb < x.l_pac && re > x.l_pac) return -2;
+  int rid_b = pos2rid(x, depos(x, rb, is_rev));
+  int rid_e = rb < re ? pos2rid(x, depos(x, re - 1, is_rev)) : rid_b;
+  return rid_b == rid_e ? rid_b : -1;
+}
+
+// base of the fwd+revcomp text at position p
+G3A_HD int text_base(const DevIndex &x, int64_t p) {
+  if (p < x.l_pac) return (x.pac[p >> 2] >> ((~p & 3) << 1)) & 3;
+  int64_t q = (x.l_pac << 1) - 1 - p;
+  return 3 - ((x.pac[q >> 2] >> ((~q & 3) << 1)) & 3);
+}
+
+// ---------------------------------------------------------------------------
+// Seeding
+// ---------------------------------------------------------------------------
+
+// Output sink for one read's intervals: the first `cap` go to `slot`, the
+// rest are only counted (the read is then redone with a bigger buffer).
+struct IntvSink {
+  Intv *slot;
+  int cap, n;
+  int64_t n_seed;  // sum over intervals of min(s, max_occ)
+  int max_occ;
+  G3A_HD void push(const Intv &p) {
+    if (n < cap) slot[n] = p;
+    ++n;
+    n_seed += p.s > max_occ ? max_occ : p.s;
+  }
+};
+
+// Scratch for the LEP lists (two alternating lists and the found list).
+struct LepBuf {
+  Intv *a, *b, *f;
+  int cap;
+};
+
+// Round 1 (preseedAndFilterV2, bwa/src/seed.cu:25-104 with
+// preseedOnePos2Backward, seed.cu:454-531).  The reference keeps, for each
+// end n, the longest match read[m(n)..n] and outputs it when it is right
+// maximal; these are exactly the super-maximal exact matches of length >=
+// min_seed_len.  They are found here with the bidirectional SMEM search
+// (forward from a pivot collecting the points where the interval shrinks,
+// then backward), which visits each read position a few times instead of
+// once per end position.  Output order is by end position, as the
+// reference's.  Returns false when a LEP list outgrows `buf`.
+G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
+                        LepBuf &buf, int64_t *steps) {
+  int x0 = 0;
+  while (x0 < len) {
+    if (q[x0] > 3) { ++x0; continue; }
+    Intv *prev = buf.a, *curr = buf.b;
+    int np = 0;
+    Intv ik;
+    intv_init(x, ik, q[x0]);
+    ik.m = x0;
+    ik.n = x0;
+    int i;
+    for (i = x0 + 1; i < len; ++i) {
+      if (q[i] < 4) {
+        Intv ok = ik;
+        forward_ext(x, ok, q[i]);
+        if (steps) ++*steps;
+        if (ok.s != ik.s) {
+          if (np == buf.cap) return false;
+          prev[np++] = ik;
+          if (ok.s < 1) break;
+        }
+        ik = ok;
+        ik.n = i;
+      } else {
+        if (np == buf.cap) return false;
+        prev[np++] = ik;
+        break;
+      }
+    }
+    if (i == len) {
+      if (np == buf.cap) return false;
+      prev[np++] = ik;
+    }
+    // longest first
+    for (int a = 0, b = np - 1; a < b; ++a, --b) { Intv t = prev[a]; prev[a] = prev[b]; prev[b] = t; }
+    const int ret = prev[0].n + 1;
+    // backward search; SMEMs are found in decreasing start order
+    int nf = 0, last_start = 0;
+    Intv *fl = buf.f;
+    for (i = x0 - 1; i >= -1; --i) {
+      int c = i < 0 ? -1 : (q[i] < 4 ? q[i] : -1);
+      int nc = 0;
+      for (int j = 0; j < np; ++j) {
+        Intv ok;
+        bool ext_ok = false;
+        if (c >= 0) {
+          backward_ext(x, prev[j], c, ok);
+          if (steps) ++*steps;
+          ext_ok = ok.s >= 1;
+        }
+        if (!ext_ok) {
+          if (nc == 0) {
+            if (nf == 0 || i + 1 < last_start) {
+              Intv p = prev[j];
+              p.m = i + 1;
+              if (nf == buf.cap) return false;
+              fl[nf++] = p;
+              last_start = i + 1;
+            }
+          }
+        } else if (nc == 0 || ok.s != curr[nc - 1].s) {
+          ok.m = prev[j].m;
+          ok.n = prev[j].n;
+          if (nc == buf.cap) return false;
+          curr[nc++] = ok;
+        }
+      }
+      if (nc == 0) break;
+      Intv *t = prev; prev = curr; curr = t;
+      np = nc;
+    }
+    // found in decreasing start (and end) order: emit the long-enough ones by end
+    for (int j = nf - 1; j >= 0; --j)
+      if (fl[j].n - fl[j].m + 1 >= o.min_seed_len) out.push(fl[j]);
+    x0 = ret;
+  }
+  return true;
+}
+
+// Round 1 by the reference's own per-end backward search; needs no scratch.
+// Used for reads whose LEP lists do not fit the fast path's buffers.
+G3A_HD void smem_round1_per_end(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
+                                int64_t *steps) {
+  int prev_m = -1, prev_valid = 0;
+  Intv prev;
+  prev.k = prev.l = prev.s = 0;
+  prev.m = prev.n = 0;
+  for (int n = 0; n < len; ++n) {
+    int valid = 0;
+    Intv cur;
+    cur.k = cur.l = cur.s = 0;
+    cur.m = cur.n = 0;
+    if (n >= o.min_seed_len - 1 && q[n] < 4) {
+      intv_init(x, cur, q[n]);
+      int m = n;
+      while (m > 0 && q[m - 1] < 4) {
+        Intv t = cur;
+        backward_ext_b(x, t, q[m - 1]);
+        if (steps) ++*steps;
+        if (t.s < 1) break;
+        cur.k = t.k;
+        cur.s = t.s;
+        --m;
+      }
+      cur.m = m;
+      cur.n = n;
+      valid = n - m + 1 >= o.min_seed_len;
+    }
+    // previous end is kept unless this end extends the same start
+    if (prev_valid && !(valid && cur.m == prev_m)) out.push(prev);
+    prev = cur;
+    prev_m = cur.m;
+    prev_valid = valid;
+  }
+  if (prev_valid) out.push(prev);
+}
+
+// Round 2, reseedV2 (bwa/src/seed.cu:639-913), for one SMEM.  `leps` and
+// `seeds` are scratch lists of capacity cap.  Returns false on overflow.
+G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int len, const Intv &smem,
+                       Intv *leps, Intv *seeds, int cap, IntvSink &out, int64_t *steps) {
+  const int split_len = (int)(o.min_seed_len * o.split_factor + .499);
+  const int slen = smem.n - smem.m + 1;
+  if (slen < split_len || smem.s > o.split_width) return true;
+  const int64_t min_seed_intv = smem.s + 1;
+  const int pivot = (smem.m + smem.n + 1) >> 1;
+  if (pivot > 77) return true;  // seed.cu:718
+  if (q[pivot] >= 4) return true;
+  int nl = 0, ns = 0;
+  Intv seed, old;
+  intv_init(x, seed, q[pivot]);
+  seed.m = pivot;
+  seed.n = pivot;
+  int n;
+  for (n = pivot; n < len - 1; ++n) {
+    old = seed;
+    if (q[n + 1] >= 4) break;
+    forward_ext(x, seed, q[n + 1]);
+    if (steps) ++*steps;
+    seed.n = n + 1;
+    if (seed.s < old.s) {
+      if (nl == cap) return false;
+      leps[nl++] = old;
+      if (seed.s < min_seed_intv) break;
+    }
+  }
+  if (seed.s >= min_seed_intv) {
+    seed.n = n;
+    if (nl == cap) return false;
+    leps[nl++] = seed;
+  }
+  for (int a = 0, b = nl - 1; a < b; ++a, --b) { Intv t = leps[a]; leps[a] = leps[b]; leps[b] = t; }
+  for (int m = pivot; m > 0; --m) {
+    int base = q[m - 1];
+    if (base >= 4) break;
+    int num_old = nl;
+    int64_t curr = min_seed_intv;
+    bool collected = false;
+    nl = 0;
+    for (int j = 0; j < num_old; ++j) {
+      Intv lep = leps[j], ext;
+      backward_ext(x, lep, base, ext);
+      if (steps) ++*steps;
+      if (ext.s >= curr) {
+        ext.m = m - 1;
+        ext.n = lep.n;
+        leps[nl++] = ext;
+        curr = ext.s;
+      } else if (!collected && lep.n - lep.m + 1 >= o.min_seed_len) {
+        if (ns == cap) return false;
+        seeds[ns++] = lep;
+        collected = true;
+      }
+    }
+    if (nl == 0) break;
+  }
+  if (nl > 0 && leps[0].n - leps[0].m + 1 >= o.min_seed_len) {
+    if (ns == cap) return false;
+    seeds[ns++] = leps[0];
+  }
+  for (int j = 0; j < ns; ++j) out.push(seeds[j]);
+  return true;
+}
+
+// Round 3, reseedThirdRound (bwa/src/seed.cu:1143-1190) via reseedLastRound
+// (seed.cu:1518-1550).
+G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
+                          int64_t *steps) {
+  const int min_len = o.min_seed_len + 1;
+  const int64_t max_intv = o.max_mem_intv;
+  int m = 0;
+  while (m < len) {
+    int next_m = m + 1;
+    int base = q[m];
+    if (base < 4) {
+      Intv seed;
+      intv_init(x, seed, base);
+      for (int n = m + 1; n < len; ++n) {
+        next_m = n + 1;
+        base = q[n];
+        if (base < 4) {
+          forward_ext(x, seed, base);
+          if (steps) ++*steps;
+          if (seed.s < max_intv && (n - m + 1) >= min_len) {
+            if (seed.s > 0) {
+              seed.m = m;
+              seed.n = n;
+              out.push(seed);
+            }
+            break;
+          }
+        } else break;
+      }
+    }
+    m = next_m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Seed sort (sortSeedsLowDim/HighDim, bwa/src/aux.cu:1436-1610): stable by
+// (qbeg, len) for <= 64 seeds; for more, the first 2048 positions are sorted
+// stably by (qbeg, qbeg+len) and the rest stay in place.
+// ---------------------------------------------------------------------------
+G3A_HD bool seed_less(const Seed &a, const Seed &b) {
+  return a.qbeg < b.qbeg || (a.qbeg == b.qbeg && a.len < b.len);
+}
+G3A_HD void sort_seeds(Seed *a, int n, Seed *tmp) {
+  int m = n <= 64 ? n : (n < 2048 ? n : 2048);
+  if (m <= 32) {
+    for (int i = 1; i < m; ++i) {
+      Seed v = a[i];
+      int j = i - 1;
+      while (j >= 0 && seed_less(v, a[j])) { a[j + 1] = a[j]; --j; }
+      a[j + 1] = v;
+    }
+    return;
+  }
+  for (int width = 1; width < m; width *= 2) {
+    for (int lo = 0; lo < m; lo += 2 * width) {
+      int mid = lo + width < m ? lo + width : m, hi = lo + 2 * width < m ? lo + 2 * width : m;
+      int i = lo, j = mid, k = lo;
+      while (i < mid && j < hi) tmp[k++] = seed_less(a[j], a[i]) ? a[j++] : a[i++];
+      while (i < mid) tmp[k++] = a[i++];
+      while (j < hi) tmp[k++] = a[j++];
+    }
+    for (int i = 0; i < m; ++i) a[i] = tmp[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Chaining: BTreeChaining (bwa/src/aux.cu:1755-1825) over klib's B-tree
+// (bwa/src/kbtree_CUDA.cu:3-144) with t = 6, the value kb_init_chn(512)
+// yields for a 32-byte chain record.  Keys are chain ids; nodes live in a
+// per-read arena.
+// ---------------------------------------------------------------------------
+constexpr int kKbT = 6, kKbN = 2 * kKbT - 1;
+struct KbNode {
+  int32_t n, internal;
+  int32_t key[kKbN];
+  int32_t child[kKbN + 1];
+};
+struct ChainTmp {  // chain under construction
+  int64_t pos;
+  int32_t rid, n, head, tail;  // seed list as links through `next`
+};
+
+struct KbTree {
+  KbNode *nodes;
+  int n_nodes, cap_nodes, root, n_keys;
+  const ChainTmp *ch;
+  G3A_HD int cmp(int64_t a, int64_t b) const { return (b < a) - (a < b); }
+  G3A_HD int getp(const KbNode &xn, int64_t kpos, int *r) const {
+    int tr, *rr = r ? r : &tr, begin = 0, end = xn.n;
+    if (xn.n == 0) return -1;
+    while (begin < end) {
+      int mid = (begin + end) >> 1;
+      if (cmp(ch[xn.key[mid]].pos, kpos) < 0) begin = mid + 1;
+      else end = mid;
+    }
+    if (begin == xn.n) { *rr = 1; return xn.n - 1; }
+    if ((*rr = cmp(kpos, ch[xn.key[begin]].pos)) < 0) --begin;
+    return begin;
+  }
+  G3A_HD int lower(int64_t kpos) const {
+    int low = -1, r = 0, xi = root;
+    while (xi >= 0) {
+      const KbNode &xn = nodes[xi];
+      int i = getp(xn, kpos, &r);
+      if (i >= 0 && r == 0) return xn.key[i];
+      if (i >= 0) low = xn.key[i];
+      if (!xn.internal) return low;
+      xi = xn.child[i + 1];
+    }
+    return low;
+  }
+  G3A_HD int new_node(int internal) {
+    if (n_nodes == cap_nodes) return -1;
+    KbNode &z = nodes[n_nodes];
+    z.n = 0;
+    z.internal = internal;
+    return n_nodes++;
+  }
+  G3A_HD bool split(int xi, int i, int yi) {
+    int zi = new_node(nodes[yi].internal);
+    if (zi < 0) return false;
+    KbNode &xn = nodes[xi], &yn = nodes[yi], &zn = nodes[zi];
+    zn.n = kKbT - 1;
+    for (int a = 0; a < kKbT - 1; ++a) zn.key[a] = yn.key[kKbT + a];
+    if (yn.internal)
+      for (int a = 0; a < kKbT; ++a) zn.child[a] = yn.child[kKbT + a];
+    yn.n = kKbT - 1;
+    for (int a = xn.n; a >= i + 1; --a) xn.child[a + 1] = xn.child[a];
+    xn.child[i + 1] = zi;
+    for (int a = xn.n - 1; a >= i; --a) xn.key[a + 1] = xn.key[a];
+    xn.key[i] = yn.key[kKbT - 1];
+    ++xn.n;
+    return true;
+  }
+  G3A_HD bool put(int cid) {
+    ++n_keys;
+    int64_t kpos = ch[cid].pos;
+    int ri = root;
+    if (nodes[ri].n == kKbN) {
+      int si = new_node(1);
+      if (si < 0) return false;
+      nodes[si].child[0] = ri;
+      root = si;
+      if (!split(si, 0, ri)) return false;
+      ri = si;
+    }
+    int xi = ri;
+    for (;;) {
+      KbNode &xn = nodes[xi];
+      if (!xn.internal) {
+        int i = getp(xn, kpos, nullptr);
+        for (int a = xn.n - 1; a >= i + 1; --a) xn.key[a + 1] = xn.key[a];
+        xn.key[i + 1] = cid;
+        ++xn.n;
+        return true;
+      }
+      int i = getp(xn, kpos, nullptr) + 1;
+      if (nodes[xn.child[i]].n == kKbN) {
+        if (!split(xi, i, xn.child[i])) return false;
+        if (cmp(kpos, ch[nodes[xi].key[i]].pos) > 0) ++i;
+      }
+      xi = nodes[xi].child[i];
+    }
+  }
+};
+
+
+// test_and_merge (bwa/src/aux.cu:54-84); seeds of a chain are linked through nxt
+G3A_HD int test_and_merge(const DevIndex &x, const Opt &o, ChainTmp &c, const Seed *s, int32_t *nxt, int pi) {
+  const Seed &p = s[pi], &last = s[c.tail], &first = s[c.head];
+  int64_t qend = last.qbeg + last.len, rend = last.rbeg + last.len;
+  if (p.rid != c.rid) return 0;
+  if (p.qbeg >= first.qbeg && p.qbeg + p.len <= qend && p.rbeg >= first.rbeg && p.rbeg + p.len <= rend) return 1;
+  if ((last.rbeg < x.l_pac || first.rbeg < x.l_pac) && p.rbeg >= x.l_pac) return 0;
+  int64_t xd = p.qbeg - last.qbeg, yd = p.rbeg - last.rbeg;
+  if (yd >= 0 && xd - yd <= o.w && yd - xd <= o.w && xd - last.len < o.max_chain_gap &&
+      yd - last.len < o.max_chain_gap) {
+    nxt[c.tail] = pi;
+    nxt[pi] = -1;
+    c.tail = pi;
+    ++c.n;
+    return 1;
+  }
+  return 0;
+}
+
+// Builds the chains of one read from its sorted seeds.  Chain ids are
+// written to `order` in B-tree order.  Returns the number of chains, or -1
+// when the node arena is too small.
+G3A_HD int chain_build(const DevIndex &x, const Opt &o, const Seed *s, int ns, ChainTmp *ch, int32_t *nxt,
+                       KbNode *nodes, int cap_nodes, int32_t *order) {
+  KbTree t;
+  t.nodes = nodes;
+  t.n_nodes = 0;
+  t.cap_nodes = cap_nodes;
+  t.n_keys = 0;
+  t.ch = ch;
+  t.root = t.new_node(0);
+  if (t.root < 0) return -1;
+  int nch = 0;
+  for (int i = 0; i < ns; ++i) {
+    const Seed &p = s[i];
+    bool start_new = true;
+    if (t.n_keys) {
+      int low = t.lower(p.rbeg);
+      if (low >= 0 && test_and_merge(x, o, ch[low], s, nxt, i)) start_new = false;
+    }
+    if (!start_new) continue;
+    if (p.rid < 0 || p.rid >= 455) continue;  // bwa/src/aux.cu:1799
+    ChainTmp &c = ch[nch];
+    c.pos = p.rbeg;
+    c.rid = p.rid;
+    c.n = 1;
+    c.head = c.tail = i;
+    nxt[i] = -1;
+    if (!t.put(nch)) return -1;
+    ++nch;
+  }
+  // in-order traversal (__kb_traverse, kbtree_CUDA.cu:121-144); each frame
+  // is (node, next key index, whether child `index` was already visited)
+  int k = 0;
+  if (nch) {
+    int sn[40], si[40], sd[40], sp = 0;
+    sn[0] = t.root; si[0] = 0; sd[0] = 0;
+    while (sp >= 0) {
+      const KbNode &xn = nodes[sn[sp]];
+      const int i = si[sp];
+      if (i > xn.n) { --sp; continue; }
+      if (xn.internal && !sd[sp]) {
+        sd[sp] = 1;
+        ++sp;
+        sn[sp] = xn.child[i]; si[sp] = 0; sd[sp] = 0;
+        continue;
+      }
+      if (i < xn.n) order[k++] = xn.key[i];
+      si[sp] = i + 1;
+      sd[sp] = 0;
+    }
+  }
+  return k;
+}
+
+// mem_chain_weight (bwa/src/aux.cu:91-110) over a linked chain
+G3A_HD int chain_weight(const Seed *s, const int32_t *nxt, const ChainTmp &c) {
+  int64_t end = 0;
+  int w = 0, tmp;
+  for (int j = c.head; j >= 0; j = nxt[j]) {
+    const Seed &p = s[j];
+    if (p.qbeg >= end) w += p.len;
+    else if (p.qbeg + p.len > end) w += (int)(p.qbeg + p.len - end);
+    end = end > p.qbeg + p.len ? end : p.qbeg + p.len;
+  }
+  tmp = w;
+  w = 0;
+  end = 0;
+  for (int j = c.head; j >= 0; j = nxt[j]) {
+    const Seed &p = s[j];
+    if (p.rbeg >= end) w += p.len;
+    else if (p.rbeg + p.len > end) w += (int)(p.rbeg + p.len - end);
+    end = end > p.rbeg + p.len ? end : p.rbeg + p.len;
+  }
+  w = w < tmp ? w : tmp;
+  return w < 1 << 30 ? w : (1 << 30) - 1;
+}
+
+// sortChainsDecreasingWeight + CHAINFILTERING_filter_kernel
+// (bwa/src/aux.cu:1828-1988).  `order` holds chain ids in B-tree order and is
+// rewritten with the kept chains in output order; `w16` and `tmp` are scratch
+// of n entries.  Stable descending sort of the first 2048 chains when
+// n <= 3072 (more chains stay unsorted); the filter is the sequential fixed
+// point of the reference's spin-wait (i dropped iff an earlier kept chain j
+// overlaps it by >= mask_level of the shorter and outweighs it); no filter
+// above 4096 chains.  Returns the number of kept chains.
+G3A_HD int chain_filter(const Opt &o, const Seed *s, const int32_t *nxt, const ChainTmp *ch, int32_t *order,
+                        int n, uint16_t *w16, int32_t *tmp, uint8_t *kept) {
+  if (n == 0) return 0;
+  for (int i = 0; i < n; ++i) w16[i] = (uint16_t)chain_weight(s, nxt, ch[order[i]]);
+  if (n <= 3072) {
+    int m = n < 2048 ? n : 2048;
+    // stable descending merge sort of (order, w16)[0..m)
+    uint16_t *wt = (uint16_t *)(tmp + m);
+    for (int width = 1; width < m; width *= 2) {
+      for (int lo = 0; lo < m; lo += 2 * width) {
+        int mid = lo + width < m ? lo + width : m, hi = lo + 2 * width < m ? lo + 2 * width : m;
+        int i = lo, j = mid, k = lo;
+        while (i < mid && j < hi) {
+          if (w16[j] > w16[i]) { tmp[k] = order[j]; wt[k++] = w16[j++]; }
+          else { tmp[k] = order[i]; wt[k++] = w16[i++]; }
+        }
+        while (i < mid) { tmp[k] = order[i]; wt[k++] = w16[i++]; }
+        while (j < hi) { tmp[k] = order[j]; wt[k++] = w16[j++]; }
+      }
+      for (int i = 0; i < m; ++i) { order[i] = tmp[i]; w16[i] = wt[i]; }
+    }
+  }
+  if (n > 4096) return n;
+  for (int i = 0; i < n; ++i) {
+    const ChainTmp &ci = ch[order[i]];
+    uint16_t bi = (uint16_t)s[ci.head].qbeg, ei = (uint16_t)(s[ci.tail].qbeg + s[ci.tail].len);
+    kept[i] = 1;
+    for (int j = 0; j < i; ++j) {
+      if (!kept[j]) continue;
+      const ChainTmp &cj = ch[order[j]];
+      uint16_t bj = (uint16_t)s[cj.head].qbeg, ej = (uint16_t)(s[cj.tail].qbeg + s[cj.tail].len);
+      int b_max = bj > bi ? bj : bi, e_min = ej < ei ? ej : ei;
+      if (e_min > b_max) {  // no ALT contigs are supported (is_alt == 0)
+        int li = ei - bi, lj = ej - bj, min_l = li < lj ? li : lj;
+        if (e_min - b_max >= min_l * o.mask_level && min_l < o.max_chain_gap) {
+          if (w16[i] < w16[j] * o.drop_ratio && w16[j] - w16[i] >= o.min_seed_len << 1) {
+            kept[i] = 0;
+            break;
+          }
+        }
+      }
+    }
+  }
+  int k = 0;
+  for (int i = 0; i < n; ++i)
+    if (kept[i]) order[k++] = order[i];
+  return k;
+}
+
+// cal_max_gap (bwa/src/aux.cu:168-175)
+G3A_HD int cal_max_gap(const Opt &o, int qlen) {
+  int l_del = (int)((double)(qlen * o.a - o.o_del) / o.e_del + 1.);
+  int l_ins = (int)((double)(qlen * o.a - o.o_ins) / o.e_ins + 1.);
+  int l = l_del > l_ins ? l_del : l_ins;
+  l = l > 1 ? l : 1;
+  return l < o.w << 1 ? l : o.w << 1;
+}
+
+// One SW extension task: a seed of a kept chain with the chain's reference
+// window [rmax0, rmax1) (ExtendingPairGenerate, bwa/src/aux.cu:2101-2192).
+struct Task {
+  int64_t rbeg, rmax0, rmax1;
+  int32_t qbeg, len, read, rid;
+};
+
+// Reference window of a chain, clipped to its strand and contig
+// (aux.cu:2111-2132 with bns_fetch_seq_gpu, bwa/src/bntseq.cu:62-87).
+G3A_HD void chain_window(const DevIndex &x, const Opt &o, const Seed *s, const int32_t *nxt, const ChainTmp &c,
+                         int l_seq, int64_t &rmax0, int64_t &rmax1, int &rid) {
+  const int64_t l_pac = x.l_pac;
+  rmax0 = l_pac << 1;
+  rmax1 = 0;
+  for (int j = c.head; j >= 0; j = nxt[j]) {
+    const Seed &p = s[j];
+    int64_t b = p.rbeg - (p.qbeg + cal_max_gap(o, p.qbeg));
+    int64_t e = p.rbeg + p.len + ((l_seq - p.qbeg - p.len) + cal_max_gap(o, l_seq - p.qbeg - p.len));
+    rmax0 = rmax0 < b ? rmax0 : b;
+    rmax1 = rmax1 > e ? rmax1 : e;
+  }
+  rmax0 = rmax0 > 0 ? rmax0 : 0;
+  rmax1 = rmax1 < l_pac << 1 ? rmax1 : l_pac << 1;
+  int64_t mid = s[c.head].rbeg;
+  if (rmax0 < l_pac && l_pac < rmax1) {
+    if (mid < l_pac) rmax1 = l_pac;
+    else rmax0 = l_pac;
+  }
+  int is_rev;
+  rid = pos2rid(x, depos(x, mid, is_rev));
+  int64_t far_beg = x.ctg_off[rid], far_end = far_beg + x.ctg_len[rid];
+  if (is_rev) {
+    int64_t t = far_beg;
+    far_beg = (l_pac << 1) - far_end;
+    far_end = (l_pac << 1) - t;
+  }
+  rmax0 = rmax0 > far_beg ? rmax0 : far_beg;
+  rmax1 = rmax1 < far_end ? rmax1 : far_end;
+}
+
+// Result of one extension task (mem_alnreg_t subset, bwa/src/bwa.h:229-248).
+struct Region {
+  int64_t rb, re;
+  int32_t qb, qe, score, rid;
+  int32_t seedlen0, secondary;
+};
+
+// filterRegions + sortRegions (bwa/src/aux.cu:2408-2613) for one read, in
+// place; returns the number of kept regions.  The reference's threads race on
+// the keep/secondary flags and write kept regions through an atomic counter;
+// here anchors are taken in ascending index order and kept regions are
+// written in ascending index order.  `flag`/`sec` are scratch of n entries.
+G3A_HD int region_filter(const Opt &o, Region *a, int n, uint8_t *flag, int32_t *sec) {
+  for (int i = 0; i < n; ++i) flag[i] = a[i].score >= o.T;
+  for (int i = 0; i < n; ++i) {
+    if (!flag[i]) continue;
+    for (int j = i + 1; j < n; ++j) {
+      if (!flag[j]) continue;
+      if (a[i].rid != a[j].rid) continue;
+      int64_t ai_rb = a[i].rb, ai_re = a[i].re, aj_rb = a[j].rb, aj_re = a[j].re;
+      if (ai_rb < aj_rb && ai_re < aj_rb) continue;
+      if (aj_rb < ai_rb && aj_re < ai_rb) continue;
+      int or_ = (int)(ai_rb < aj_rb ? ai_re - aj_rb : aj_re - ai_rb);
+      int oq = a[i].qb < a[j].qb ? a[i].qe - a[j].qb : a[j].qe - a[i].qb;
+      int rli = (int)(ai_re - ai_rb), rlj = (int)(aj_re - aj_rb);
+      int mr = rli < rlj ? rli : rlj;
+      int mq = (a[i].qe - a[i].qb < a[j].qe - a[j].qb) ? a[i].qe - a[i].qb : a[j].qe - a[j].qb;
+      if (or_ > o.mask_level_redun * mr && oq > o.mask_level_redun * mq) {
+        if (a[i].score < a[j].score) { flag[i] = 0; break; }
+        else flag[j] = 0;
+      }
+    }
+  }
+  int m = 0;
+  for (int i = 0; i < n; ++i)
+    if (flag[i]) a[m++] = a[i];
+  for (int i = 0; i < m; ++i) sec[i] = -1;
+  for (int i = 0; i < m; ++i) {
+    for (int j = i + 1; j < m; ++j) {
+      if (sec[i] >= 0) break;
+      if (sec[j] >= 0) continue;
+      int qb_max = a[i].qb > a[j].qb ? a[i].qb : a[j].qb;
+      int qe_min = a[i].qe < a[j].qe ? a[i].qe : a[j].qe;
+      if (qe_min > qb_max) {
+        int li = a[i].qe - a[i].qb, lj = a[j].qe - a[j].qb, min_l = li < lj ? li : lj;
+        if (qe_min - qb_max >= min_l * o.mask_level) {
+          if (a[i].score > a[j].score) sec[j] = i;
+          else sec[i] = j;
+        }
+      }
+    }
+  }
+  int k = 0;
+  for (int i = 0; i < m; ++i) {
+    if (sec[i] >= 0 && !(o.flag & 0x8)) continue;
+    if (sec[i] >= 0 && a[i].score < a[sec[i]].score * o.drop_ratio) continue;
+    Region r = a[i];
+    r.secondary = sec[i];
+    a[k++] = r;
+  }
+  // sortRegions: bubble sort by score descending (no ALT contigs)
+  if (k == 2) {
+    if (a[0].score < a[1].score) { Region t = a[0]; a[0] = a[1]; a[1] = t; }
+  } else if (k > 2) {
+    for (int i = 0; i < k - 1; ++i) {
+      bool swapped = false;
+      for (int j = 0; j < k - 1 - i; ++j)
+        if (a[j].score < a[j + 1].score) { Region t = a[j]; a[j] = a[j + 1]; a[j + 1] = t; swapped = true; }
+      if (!swapped) break;
+    }
+  }
+  return k;
+}
+
+// infer_bw (bwa/src/aux.cu:506-513)
+G3A_HD int infer_bw(int l1, int l2, int score, int a, int q, int r) {
+  int w;
+  if (l1 == l2 && l1 * a - score < (q + r - a) << 1) return 0;
+  w = (int)((double)((l1 < l2 ? l1 : l2) * a - score - q) / r + 2.);
+  int d = l1 > l2 ? l1 - l2 : l2 - l1;
+  if (w < d) w = d;
+  return w;
+}
+
+// Band width of the global alignment (FINALIZEALN_preprocessing2,
+// bwa/src/aux.cu:2676-2752), capped at GLOBALSW_BANDWITH_CUTOFF = 500.
+G3A_HD int global_bw(const Opt &o, int lq, int64_t rlen, int score) {
+  if (lq == rlen) return 0;
+  int w = infer_bw(lq, (int)rlen, score, o.a, o.o_del, o.e_del);
+  int tmp = infer_bw(lq, (int)rlen, score, o.a, o.o_ins, o.e_ins);
+  w = w > tmp ? w : tmp;
+  int max_ins = (int)((double)(((lq + 1) >> 1) * o.a - o.o_ins) / o.e_ins + 1.);
+  int max_del = (int)((double)(((lq + 1) >> 1) * o.a - o.o_del) / o.e_del + 1.);
+  int max_gap = max_ins > max_del ? max_ins : max_del;
+  max_gap = max_gap > 1 ? max_gap : 1;
+  int d = (int)rlen > lq ? (int)rlen - lq : lq - (int)rlen;
+  tmp = (max_gap + d + 1) >> 1;
+  w = w < tmp ? w : tmp;
+  tmp = d + 3;
+  w = w > tmp ? w : tmp;
+  return w < 500 ? w : 500;
+}
+
+G3A_HD int push_cigar(uint32_t *cigar, int n, int op, int len) {
+  if (n == 0 || op != (int)(cigar[n - 1] & 0xf)) {
+    cigar[n] = (uint32_t)(len << 4 | op);
+    return n + 1;
+  }
+  cigar[n - 1] += (uint32_t)(len << 4);
+  return n;
+}
+
+// ksw_global2 (bwa/src/ksw.cu:706-812): banded global alignment with
+// traceback; the reference's ksw_global3 keeps int8_t DP state and a shared
+// border buffer raced by 32 threads, so its defined intent (this routine) is
+// used.  `tq`/`tt` give query/target bases; eh_h/eh_e hold qlen+1 entries and
+// z holds tlen*min(qlen, 2w+1) direction bytes.  Returns the score.
+template <class Q, class T>
+G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, int32_t *eh_h, int32_t *eh_e,
+                        uint8_t *z, uint32_t *cigar, int *n_cigar_out) {
+  const int MINUS_INF = -0x40000000;
+  const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
+  const int n_col = qlen < 2 * w + 1 ? qlen : 2 * w + 1;
+  eh_h[0] = 0;
+  eh_e[0] = MINUS_INF;
+  int j;
+  for (j = 1; j <= qlen && j <= w; ++j) { eh_h[j] = -(o.o_ins + o.e_ins * j); eh_e[j] = MINUS_INF; }
+  for (; j <= qlen; ++j) eh_h[j] = eh_e[j] = MINUS_INF;
+  for (int i = 0; i < tlen; ++i) {
+    int32_t f = MINUS_INF, h1, beg, end, t;
+    const int tb = tt(i);
+    beg = i > w ? i - w : 0;
+    end = i + w + 1 < qlen ? i + w + 1 : qlen;
+    h1 = beg == 0 ? -(o.o_del + o.e_del * (i + 1)) : MINUS_INF;
+    uint8_t *zi = &z[(int64_t)i * n_col];
+    for (j = beg; j < end; ++j) {
+      int32_t h, m = eh_h[j], e = eh_e[j];
+      uint8_t d;
+      eh_h[j] = h1;
+      m += o.mat[tb * 5 + tq(j)];
+      d = m >= e ? 0 : 1;
+      h = m >= e ? m : e;
+      d = h >= f ? d : 2;
+      h = h >= f ? h : f;
+      h1 = h;
+      t = m - oe_del;
+      e -= o.e_del;
+      d |= e > t ? 1 << 2 : 0;
+      e = e > t ? e : t;
+      eh_e[j] = e;
+      t = m - oe_ins;
+      f -= o.e_ins;
+      d |= f > t ? 2 << 4 : 0;
+      f = f > t ? f : t;
+      zi[j - beg] = d;
+    }
+    eh_h[end] = h1;
+    eh_e[end] = MINUS_INF;
+  }
+  int score = eh_h[qlen];
+  int n = 0, which = 0, i = tlen - 1, k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
+  while (i >= 0 && k >= 0) {
+    which = z[(int64_t)i * n_col + (k - (i > w ? i - w : 0))] >> (which << 1) & 3;
+    if (which == 0) { n = push_cigar(cigar, n, 0, 1); --i; --k; }
+    else if (which == 1) { n = push_cigar(cigar, n, 2, 1); --i; }
+    else { n = push_cigar(cigar, n, 1, 1); --k; }
+  }
+  if (i >= 0) n = push_cigar(cigar, n, 2, i + 1);
+  if (k >= 0) n = push_cigar(cigar, n, 1, k + 1);
+  for (int a = 0; a < n >> 1; ++a) { uint32_t tmp = cigar[a]; cigar[a] = cigar[n - 1 - a]; cigar[n - 1 - a] = tmp; }
+  *n_cigar_out = n;
+  return score;
+}
+
+// Final alignment record (traceback + finalize, bwa/src/aux.cu:2874-3059).
+struct Aln {
+  int64_t pos, rb, re, cigar_off, core_off;
+  int32_t rid, is_rev, flag, score, nw_score, nm, qb, qe, n_cigar, n_core;
+};
+
+// Cigar words an alignment may need: the global-alignment CIGAR (at most
+// lq + rlen operations) followed by the clipped reference-format copy.
+G3A_HD int64_t aln_cigar_cap(int lq, int64_t rlen) { return 2 * ((int64_t)lq + rlen) + 4; }
+// Direction bytes of the global alignment.
+G3A_HD int64_t aln_z_size(int lq, int64_t rlen, int w) {
+  if (w == 0) return 0;
+  int n_col = lq < 2 * w + 1 ? lq : 2 * w + 1;
+  return (int64_t)n_col * rlen;
+}
+
+// traceback (aux.cu:2874-2949) and finalize (aux.cu:2952-3059) of the ai-th
+// kept region of a read of length l_seq.  `read` is the read, `cig` has
+// aln_cigar_cap() words (core at cig + lq + rlen + 4 - ... see below), z has
+// aln_z_size() bytes, eh_h/eh_e lq+1 entries.
+G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *read, int l_seq, const Region &g,
+                               int ai, int32_t *eh_h, int32_t *eh_e, uint8_t *z, uint32_t *cig, int64_t cig_base,
+                               Aln &a) {
+  const int64_t l_pac = x.l_pac;
+  const int lq = g.qe - g.qb;
+  const int64_t rlen = g.re - g.rb;
+  const uint8_t *query = read + g.qb;
+  const int64_t rb = g.rb;
+  const int w = global_bw(o, lq, rlen, g.score);
+  const int64_t core_at = (int64_t)lq + rlen + 4;  // core CIGAR offset inside this record
+  uint32_t *core = cig + core_at;
+  int n_core = 0, nw = 0;
+  if (w == 0) {
+    core[0] = (uint32_t)(lq << 4);
+    n_core = 1;
+    for (int i = 0; i < lq; ++i) nw += o.mat[text_base(x, rb + i) * 5 + query[i]];
+  } else {
+    nw = global_align(
+        o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return text_base(x, rb + i); }, w,
+        eh_h, eh_e, z, core, &n_core);
+  }
+  // NM as the reference's traceback counts it (aux.cu:2910-2941)
+  int n_mm = 0, n_gap = 0;
+  {
+    int xq = 0;
+    int64_t yr = 0;
+    for (int k = 0; k < n_core; ++k) {
+      int op = core[k] & 0xf, ln = (int)(core[k] >> 4);
+      if (op == 0) {
+        for (int i = 0; i < ln; ++i) n_mm += query[xq + i] != text_base(x, rb + yr + i);
+        xq += ln;
+        yr += ln;
+      } else if (op == 2) {
+        if (k > 0 && k < n_core - 1) n_gap += ln;
+        yr += ln;
+      } else if (op == 1) {
+        xq += ln;
+        n_gap += ln;
+      }
+    }
+  }
+  // finalize
+  const int rev_strand = g.rb >= l_pac;
+  int is_rev;
+  int64_t pos = depos(x, !rev_strand ? g.rb : g.re - 1, is_rev);
+  const uint32_t *cg = core;
+  int nc = n_core;
+  if (nc > 0) {
+    if ((cg[0] & 0xf) == 2) { pos += cg[0] >> 4; --nc; cg = cg + 1; }
+    else if ((cg[nc - 1] & 0xf) == 2) --nc;
+  }
+  int nf = 0;
+  if (g.qb != 0 || g.qe != l_seq) {
+    int clip5 = is_rev ? l_seq - g.qe : g.qb, clip3 = is_rev ? g.qb : l_seq - g.qe;
+    if (clip5) cig[nf++] = (uint32_t)(clip5 << 4 | 3);
+    for (int k = 0; k < nc; ++k) cig[nf++] = cg[k];
+    if (clip3) cig[nf++] = (uint32_t)(clip3 << 4 | 3);
+  } else {
+    // the squeezed CIGAR is kept only when clipping is added (aux.cu:2985-3022)
+    for (int k = 0; k < n_core; ++k) cig[nf++] = core[k];
+  }
+  a.rid = pos2rid(x, pos);
+  a.pos = pos - x.ctg_off[a.rid];
+  a.is_rev = is_rev;
+  int flag = 0;
+  if (g.secondary >= 0) flag |= 0x100;
+  else if (ai > 0) flag |= 0x800;
+  if (g.rid < 0) flag |= 0x4;
+  if (is_rev) flag |= 0x10;
+  a.flag = flag;
+  a.score = g.score;
+  a.nw_score = nw;
+  a.nm = n_mm + n_gap;
+  a.qb = g.qb;
+  a.qe = g.qe;
+  a.rb = g.rb;
+  a.re = g.re;
+  a.n_cigar = nf;
+  a.n_core = n_core;
+  a.cigar_off = cig_base;
+  a.core_off = cig_base + core_at;
+}
+
+}  // namespace g3a

@@ -1,0 +1,548 @@
+// HIP kernels of the BWA-MEM hot path for MI355X (gfx950, wave64).
+//
+// Stage kernels (one launch each, all on the context's stream):
+//   k_seed           thread per read: SMEM + reseed + third-round intervals
+//   k_seed_slow      thread per read, for reads whose lists outgrew the fast
+//                    path's private buffers (global-memory scratch)
+//   k_sa_lookup      thread per seed: SA interval row -> reference position
+//   k_sort_seeds     thread per read
+//   k_chain          thread per read: B-tree chaining, chain filter, tasks
+//   k_task_compact   thread per read: per-read task lists -> dense task array
+//   k_extend         one wavefront per task: banded local extension, left
+//                    then right, anti-diagonal sweep over 64-row tiles
+//   k_region_filter  thread per read
+//   k_aln_prep       thread per read: kept regions -> alignment list
+//   k_traceback      thread per alignment: global alignment + finalize
+#include "g3a_dev.h"
+#include "g3a_kernels.h"
+
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
+
+namespace g3a {
+
+// ---------------------------------------------------------------------------
+// seeding
+// ---------------------------------------------------------------------------
+constexpr int kLepCap = 32;  // fast-path LEP list capacity (private memory)
+
+__global__ void __launch_bounds__(128) k_seed(SeedArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const int off = a.off[r], len = a.off[r + 1] - off;
+  const uint8_t *q = a.seq + off;
+  IntvSink sink{a.intv + (int64_t)r * a.slot_cap, a.slot_cap, 0, 0, a.opt.max_occ};
+  if (len > kMaxReadLen) {  // unaligned (documented limit)
+    a.n_intv[r] = 0;
+    a.n_seed[r] = 0;
+    a.slow[r] = 0;
+    return;
+  }
+  Intv la[kLepCap], lb[kLepCap], lf[kLepCap];
+  LepBuf buf{la, lb, lf, kLepCap};
+  int64_t steps = 0;
+  bool ok = smem_round1(a.idx, a.opt, q, len, sink, buf, &steps);
+  if (ok) {
+    const int nsm = sink.n < sink.cap ? sink.n : sink.cap;
+    if (sink.n <= sink.cap) {
+      for (int k = 0; ok && k < nsm; ++k) {
+        Intv smem = sink.slot[k];
+        ok = reseed_one(a.idx, a.opt, q, len, smem, la, lb, kLepCap, sink, &steps);
+      }
+    } else ok = false;
+    if (ok) reseed_round3(a.idx, a.opt, q, len, sink, &steps);
+  }
+  ok = ok && sink.n <= sink.cap;
+  a.slow[r] = ok ? 0 : 1;
+  a.n_intv[r] = ok ? sink.n : 0;
+  a.n_seed[r] = ok ? sink.n_seed : 0;
+  if (a.fm_steps) atomicAdd((unsigned long long *)a.fm_steps, (unsigned long long)steps);
+}
+
+// Slow path for the reads listed in `list`: global-memory LEP lists of
+// kSlowLep entries, which no read of <= 320 bases can outgrow.  Pass 0 counts
+// (storing only the round-1 SMEMs, which round 2 needs), pass 1 stores all
+// intervals at the pool offsets computed from the counts.
+__global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *list, int n_list, Intv *scratch,
+                                                  const int64_t *pool_off, Intv *pool, int pass) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_list) return;
+  const int r = list[t];
+  const int off = a.off[r], len = a.off[r + 1] - off;
+  const uint8_t *q = a.seq + off;
+  Intv *sc = scratch + (int64_t)t * kSlowScratch;
+  Intv *slot = pass ? pool + pool_off[t] : sc + 3 * kSlowLep;
+  const int cap = pass ? (int)(pool_off[t + 1] - pool_off[t]) : 2 * kSlowLep;
+  IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};
+  LepBuf buf{sc, sc + kSlowLep, sc + 2 * kSlowLep, kSlowLep};
+  if (!smem_round1(a.idx, a.opt, q, len, sink, buf, nullptr)) {
+    sink.n = 0;
+    sink.n_seed = 0;
+    smem_round1_per_end(a.idx, a.opt, q, len, sink, nullptr);
+  }
+  const int nsm = sink.n;
+  for (int k = 0; k < nsm; ++k) {
+    Intv smem = slot[k];
+    reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.b, kSlowLep, sink, nullptr);
+  }
+  reseed_round3(a.idx, a.opt, q, len, sink, nullptr);
+  a.n_intv[r] = sink.n;
+  a.n_seed[r] = sink.n_seed;
+}
+
+// ---------------------------------------------------------------------------
+// SA lookup (saLookup, bwa/src/aux.cu:1365-1425): thread per seed
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.total) return;
+  // read owning seed s: last r with seed_off[r] <= s
+  int lo = 0, hi = a.n_reads;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a.seed_off[mid] <= s) lo = mid;
+    else hi = mid;
+  }
+  const int r = lo;
+  const int64_t local = s - a.seed_off[r];
+  const Intv *iv = a.intv + a.intv_base[r];
+  const int ni = a.n_intv[r];
+  // interval owning the local seed index (intervals carry cumulative counts)
+  int64_t acc = 0;
+  int k = 0;
+  const int64_t max_occ = a.opt.max_occ;
+  for (; k < ni; ++k) {
+    int64_t c = iv[k].s > max_occ ? max_occ : iv[k].s;
+    if (local < acc + c) break;
+    acc += c;
+  }
+  const Intv p = iv[k];
+  const int64_t j = local - acc;
+  const int64_t step = p.s > max_occ ? p.s / max_occ : 1;
+  Seed out;
+  out.rbeg = sa_value(a.idx, p.k + step * j, nullptr);
+  out.qbeg = p.m;
+  out.len = p.n - p.m + 1;
+  out.rid = intv2rid(a.idx, out.rbeg, out.rbeg + out.len);
+  out.pad = 0;
+  a.seeds[s] = out;
+}
+
+__global__ void __launch_bounds__(128) k_sort_seeds(SortArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const int64_t b = a.seed_off[r];
+  const int n = (int)(a.seed_off[r + 1] - b);
+  if (n > 1) sort_seeds(a.seeds + b, n, a.tmp + b);
+}
+
+// ---------------------------------------------------------------------------
+// chaining + chain filter + per-read task list
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_chain(ChainArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const int64_t b = a.seed_off[r];
+  const int ns = (int)(a.seed_off[r + 1] - b);
+  a.n_task[r] = 0;
+  if (ns == 0) return;
+  const int l_seq = a.off[r + 1] - a.off[r];
+  const Seed *s = a.seeds + b;
+  ChainTmp *ch = a.ch + b;
+  int32_t *nxt = a.nxt + b, *order = a.order + b;
+  const int64_t nb0 = (b + 32 * (int64_t)r) / 4, nb1 = (a.seed_off[r + 1] + 32 * (int64_t)(r + 1)) / 4;
+  int nch = chain_build(a.idx, a.opt, s, ns, ch, nxt, a.nodes + nb0, (int)(nb1 - nb0), order);
+  if (nch < 0) {  // cannot happen with the arena sizing above; reported as an error
+    atomicOr(a.error, 1);
+    return;
+  }
+  int nk = chain_filter(a.opt, s, nxt, ch, order, nch, a.w16 + b, a.tmpi + 2 * b, a.kept + b);
+  int t = 0;
+  Task *out = a.tasks_tmp + b;
+  for (int c = 0; c < nk; ++c) {
+    const ChainTmp &cc = ch[order[c]];
+    int64_t r0, r1;
+    int rid;
+    chain_window(a.idx, a.opt, s, nxt, cc, l_seq, r0, r1, rid);
+    for (int j = cc.head; j >= 0; j = nxt[j]) {
+      Task tk;
+      tk.rbeg = s[j].rbeg;
+      tk.rmax0 = r0;
+      tk.rmax1 = r1;
+      tk.qbeg = s[j].qbeg;
+      tk.len = s[j].len;
+      tk.read = r;
+      tk.rid = rid;
+      out[t++] = tk;
+    }
+  }
+  a.n_task[r] = t;
+}
+
+__global__ void __launch_bounds__(128) k_task_compact(CompactArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const int64_t src = a.seed_off[r], dst = a.task_off[r];
+  const int n = (int)(a.task_off[r + 1] - dst);
+  for (int t = 0; t < n; ++t) a.tasks[dst + t] = a.tasks_tmp[src + t];
+}
+
+// ---------------------------------------------------------------------------
+// banded local extension (ksw_extend_warp2, bwa/src/ksw.cu:814-1034)
+//
+// One wavefront per task.  Lane l owns row 64*T+l of tile T and sweeps the
+// tile's anti-diagonals: at step d it fills column j = d - l.  The diagonal
+// input comes from lane l-1 two steps back and the E input from lane l-1 one
+// step back, both moved by DPP wave_shr:1; lane 0 takes them from an LDS row
+// buffer that lane 63 filled during the previous tile.  The cell set and the
+// position rule are those of the reference's 32-lane program (see g3a_dev.h
+// and the oracle), which do not depend on the lane schedule.
+// ---------------------------------------------------------------------------
+constexpr int kQMax = kMaxReadLen + 2;
+
+__device__ __forceinline__ int shr1(int src, int old) {
+  return __builtin_amdgcn_update_dpp(old, src, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    int t = __shfl_xor(v, o);
+    v = v > t ? v : t;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    uint64_t t = __shfl_xor(v, o);
+    v = v > t ? v : t;
+  }
+  return v;
+}
+
+struct ExtOut {
+  int score, qle, tle;
+};
+
+// qbuf: query (qlen bytes) in LDS; tb_of(i): target base of row i.
+template <class TB>
+__device__ ExtOut extend_wave(const Opt &o, int qlen, const uint8_t *qbuf, int tlen, TB tb_of, int h0, int end_bonus,
+                              int *Hbuf, int *Ebuf, int64_t *cells) {
+  const int lane = threadIdx.x & 63;
+  const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
+  int max_ = 0;
+  for (int i = 0; i < 25; ++i) max_ = max_ > o.mat[i] ? max_ : o.mat[i];
+  int w = o.w;
+  {
+    int max_ins = (int)((double)(qlen * max_ + end_bonus - o.o_ins) / o.e_ins + 1.);
+    max_ins = max_ins > 1 ? max_ins : 1;
+    w = w < max_ins ? w : max_ins;
+    int max_del = (int)((double)(qlen * max_ + end_bonus - o.o_del) / o.e_del + 1.);
+    max_del = max_del > 1 ? max_del : 1;
+    w = w < max_del ? w : max_del;
+  }
+  const int Q = qlen;
+  const int stride = kQMax + 2;
+  // row buffer 0: the first row's inputs (H(-1,j-1), E = 0)
+  for (int j = lane; j <= Q; j += 64) {
+    int ht = h0 - oe_ins - (j - 1) * o.e_ins;
+    Hbuf[j] = j == 0 ? h0 : (ht > 0 ? ht : 0);
+    Ebuf[j] = 0;
+  }
+  __syncthreads();
+  int best_h = h0, best_i = -1, best_j = -1;
+  int64_t ncell = 0;
+  const int ntile = (tlen + 63) >> 6;
+  for (int T = 0; T < ntile; ++T) {
+    const int cur = T & 1, nxt = cur ^ 1;
+    const int *Hc = Hbuf + cur * stride, *Ec = Ebuf + cur * stride;
+    int *Hn = Hbuf + nxt * stride, *En = Ebuf + nxt * stride;
+    const int i = 64 * T + lane;
+    const bool row_ok = i < tlen;
+    const int tbase = row_ok ? tb_of(i) : 0;
+    uint64_t srow = 0;  // packed scores of this row's target base against query bases 0..4
+#pragma unroll
+    for (int qc = 0; qc < 5; ++qc) srow |= (uint64_t)(uint8_t)o.mat[tbase * 5 + qc] << (8 * qc);
+    const int lim = 32 * (i / 32 + 1) + w + 1;
+    const int hb = h0 - (o.o_del + o.e_del * (i + 1));
+    const int boundary = hb > 0 ? hb : 0;
+    const bool quirk_row = (i & 31) == 0 && i >= 32;
+    const int dlo = 64 * T - w > 0 ? 64 * T - w : 0;
+    // values of the two steps before dlo: column -1 carries the row's left
+    // boundary (lane 0 reaches it only at step -1), earlier columns are empty
+    int H1 = dlo - 1 - lane == -1 ? boundary : 0, H2 = dlo - 2 - lane == -1 ? boundary : 0, E1 = 0, F = 0;
+    const int dhi = Q + 62;
+    for (int d = dlo; d <= dhi; ++d) {
+      const int j = d - lane;
+      const int di = d < Q ? d : Q;
+      const int hd0 = shr1(H2, Hc[di]);
+      const int ein0 = shr1(E1, Ec[di]);
+      const bool in_q = j >= 0 && j < Q;
+      const int qb = in_q ? qbuf[j] : 0;
+      const bool valid = in_q && row_ok && j >= i - w && j <= lim && (i + j > 30 || j <= i + w + 1);
+      const bool quirk = quirk_row && j > i + w + 1;
+      int ht = h0 - oe_ins - (j - 1) * o.e_ins;
+      const int hd = quirk ? (ht > 0 ? ht : 0) : hd0;
+      const int ein = quirk ? 0 : ein0;
+      const int sc = (int)(int8_t)(srow >> (8 * qb));
+      const int M = hd ? hd + sc : 0;
+      int h = M > ein ? M : ein;
+      h = h > F ? h : F;
+      int t1 = M - oe_del;
+      t1 = t1 > 0 ? t1 : 0;
+      int e = ein - o.e_del;
+      e = e > t1 ? e : t1;
+      int t2 = M - oe_ins;
+      t2 = t2 > 0 ? t2 : 0;
+      int f = F - o.e_ins;
+      f = f > t2 ? f : t2;
+      const int hcur = valid ? h : (j == -1 ? boundary : 0);
+      const int ecur = valid ? e : 0;
+      F = valid ? f : 0;
+      if (valid) {
+        ++ncell;
+        if (h > best_h) { best_h = h; best_i = i; best_j = j; }
+        else if (h == best_h && (i == best_i || best_i < 0)) { best_i = i; best_j = j; }
+      }
+      if (lane == 63) {
+        if (in_q) { Hn[j + 1] = hcur; En[j] = ecur; }
+        else if (j == -1) Hn[0] = hcur;
+      }
+      H2 = H1;
+      H1 = hcur;
+      E1 = ecur;
+    }
+    __syncthreads();
+  }
+  // combine lanes (ksw.cu:1008-1033 with the 32-lane residue rule)
+  const int best = wave_max_i32(best_h);
+  const uint64_t key = best_i >= 0 && best_h == best
+                           ? ((uint64_t)(uint32_t)best << 42) | ((uint64_t)(0x1FFFFF - best_i) << 21) |
+                                 (uint64_t)(uint32_t)best_j
+                           : 0;
+  const uint64_t kmax = wave_max_u64(key);
+  const uint64_t covered = __ballot(best_i >= 0);
+  int bi = -1, bj = -1;
+  if (best > h0 || (((covered | (covered >> 32)) & 0xffffffffULL) == 0xffffffffULL)) {
+    if (kmax) {
+      bi = 0x1FFFFF - (int)((kmax >> 21) & 0x1FFFFF);
+      bj = (int)(kmax & 0x1FFFFF);
+    }
+  }
+  if (cells) *cells += ncell;
+  ExtOut r;
+  r.score = best;
+  r.qle = bj + 1;
+  r.tle = bi + 1;
+  return r;
+}
+
+__global__ void __launch_bounds__(64) k_extend(ExtArgs a) {
+  __shared__ uint8_t qbuf[kQMax];
+  __shared__ int Hbuf[2 * (kQMax + 2)], Ebuf[2 * (kQMax + 2)];
+  const int64_t t = blockIdx.x;
+  if (t >= a.n_tasks) return;
+  const Task tk = a.tasks[t];
+  const int off = a.off[tk.read], l_seq = a.off[tk.read + 1] - off;
+  const uint8_t *read = a.seq + off;
+  const int lane = threadIdx.x;
+  int64_t cells = 0;
+  Region g;
+  g.rid = tk.rid;
+  g.seedlen0 = tk.len;
+  g.secondary = 0;
+  int score;
+  // left extension: reversed read prefix against reversed reference
+  {
+    const int qlen = tk.qbeg, tlen = (int)(tk.rbeg - tk.rmax0);
+    const int h0 = tk.len * a.opt.a;
+    if (qlen > 0) {
+      for (int j = lane; j < qlen; j += 64) qbuf[j] = read[qlen - 1 - j];
+      __syncthreads();
+      const int64_t rb = tk.rbeg;
+      ExtOut e = extend_wave(
+          a.opt, qlen, qbuf, tlen, [&](int i) { return text_base(a.idx, rb - 1 - i); }, h0, a.opt.pen_clip5, Hbuf,
+          Ebuf, &cells);
+      score = e.score;
+      g.qb = tk.qbeg - e.qle;
+      g.rb = tk.rbeg - e.tle;
+      __syncthreads();
+    } else {
+      score = h0;
+      g.qb = 0;
+      g.rb = tk.rbeg;
+    }
+  }
+  // right extension
+  {
+    const int qlen = l_seq - tk.qbeg - tk.len, tlen = (int)(tk.rmax1 - tk.rbeg - tk.len);
+    const int h0 = score;
+    if (qlen > 0) {
+      for (int j = lane; j < qlen; j += 64) qbuf[j] = read[tk.qbeg + tk.len + j];
+      __syncthreads();
+      const int64_t re0 = tk.rbeg + tk.len;
+      ExtOut e = extend_wave(
+          a.opt, qlen, qbuf, tlen, [&](int i) { return text_base(a.idx, re0 + i); }, h0, a.opt.pen_clip3, Hbuf,
+          Ebuf, &cells);
+      score = e.score;
+      g.qe = tk.qbeg + tk.len + e.qle;
+      g.re = tk.rbeg + tk.len + e.tle;
+    } else {
+      g.qe = tk.qbeg + tk.len;
+      g.re = tk.rbeg + tk.len;
+    }
+  }
+  g.score = score;
+  if (lane == 0) a.regs[t] = g;
+  if (a.cells) {
+    // per-lane cell counts summed across the wave, one atomic per task
+    int64_t c = cells;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) atomicAdd((unsigned long long *)a.cells, (unsigned long long)c);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// region filter, alignment list, traceback + finalize
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128) k_region_filter(RegionArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const int64_t b = a.task_off[r];
+  const int n = (int)(a.task_off[r + 1] - b);
+  a.n_aln[r] = n ? region_filter(a.opt, a.regs + b, n, a.flag + b, a.sec + b) : 0;
+}
+
+__global__ void __launch_bounds__(128) k_aln_prep(AlnPrepArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const int64_t src = a.task_off[r], dst = a.aln_off[r];
+  const int n = (int)(a.aln_off[r + 1] - dst);
+  for (int k = 0; k < n; ++k) {
+    const Region g = a.regs[src + k];
+    a.aln_reg[dst + k] = g;
+    a.aln_read[dst + k] = r;
+    a.aln_rank[dst + k] = k;
+    const int lq = g.qe - g.qb;
+    const int64_t rlen = g.re - g.rb;
+    const int w = global_bw(a.opt, lq, rlen, g.score);
+    a.z_size[dst + k] = aln_z_size(lq, rlen, w);
+    a.cig_size[dst + k] = aln_cigar_cap(lq, rlen);
+  }
+}
+
+__global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_alns) return;
+  const int r = a.aln_read[t];
+  const int off = a.off[r], l_seq = a.off[r + 1] - off;
+  const Region g = a.aln_reg[t];
+  int32_t eh_h[kQMax], eh_e[kQMax];
+  Aln out;
+  trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], eh_h, eh_e, a.z + a.z_off[t],
+                     a.cigar + a.cig_off[t], a.cig_off[t], out);
+  a.alns[t] = out;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_seed(const SeedArgs &a, hipStream_t s) {
+  if (a.n_reads) hipLaunchKernelGGL(k_seed, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+}
+void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
+                      Intv *pool, int pass, hipStream_t s) {
+  if (n_list)
+    hipLaunchKernelGGL(k_seed_slow, dim3(blocks(n_list, 64)), dim3(64), 0, s, a, list, n_list, scratch, pool_off, pool,
+                       pass);
+}
+void launch_sa_lookup(const SaArgs &a, hipStream_t s) {
+  if (a.total) hipLaunchKernelGGL(k_sa_lookup, dim3(blocks(a.total, 256)), dim3(256), 0, s, a);
+}
+void launch_sort(const SortArgs &a, hipStream_t s) {
+  if (a.n_reads) hipLaunchKernelGGL(k_sort_seeds, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+}
+void launch_chain(const ChainArgs &a, hipStream_t s) {
+  if (a.n_reads) hipLaunchKernelGGL(k_chain, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+}
+void launch_task_compact(const CompactArgs &a, hipStream_t s) {
+  if (a.n_reads) hipLaunchKernelGGL(k_task_compact, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+}
+void launch_extend(const ExtArgs &a, hipStream_t s) {
+  if (a.n_tasks) hipLaunchKernelGGL(k_extend, dim3((unsigned)a.n_tasks), dim3(64), 0, s, a);
+}
+void launch_region_filter(const RegionArgs &a, hipStream_t s) {
+  if (a.n_reads) hipLaunchKernelGGL(k_region_filter, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+}
+void launch_aln_prep(const AlnPrepArgs &a, hipStream_t s) {
+  if (a.n_reads) hipLaunchKernelGGL(k_aln_prep, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+}
+void launch_traceback(const TbArgs &a, hipStream_t s) {
+  if (a.n_alns) hipLaunchKernelGGL(k_traceback, dim3(blocks(a.n_alns, 64)), dim3(64), 0, s, a);
+}
+
+
+// ---------------------------------------------------------------------------
+// small helpers: slow-read list, interval bases, scans
+// ---------------------------------------------------------------------------
+__global__ void k_collect_slow(int n, const int32_t *slow, int32_t *list, int32_t *count, int64_t *intv_base,
+                               int slot_cap) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  intv_base[r] = (int64_t)r * slot_cap;
+  if (slow[r]) list[atomicAdd(count, 1)] = r;
+}
+__global__ void k_gather_slow_counts(int n, const int32_t *list, const int32_t *n_intv, int64_t *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = n_intv[list[t]];
+}
+__global__ void k_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_t base, int64_t *intv_base) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) intv_base[list[t]] = base + off[t];
+}
+__global__ void k_zero_first(int64_t *p) { *p = 0; }
+
+void launch_collect_slow(int n, const int32_t *slow, int32_t *list, int32_t *count, int64_t *intv_base, int slot_cap,
+                         hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_collect_slow, dim3(blocks(n, 256)), dim3(256), 0, s, n, slow, list, count, intv_base,
+                            slot_cap);
+}
+void launch_gather_slow_counts(int n, const int32_t *list, const int32_t *n_intv, int64_t *out, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_gather_slow_counts, dim3(blocks(n, 256)), dim3(256), 0, s, n, list, n_intv, out);
+}
+void launch_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_t base, int64_t *intv_base,
+                          hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_set_slow_base, dim3(blocks(n, 256)), dim3(256), 0, s, n, list, off, base, intv_base);
+}
+
+struct WidenI32 {
+  __host__ __device__ int64_t operator()(int32_t v) const { return (int64_t)v; }
+};
+
+size_t scan_tmp_bytes(int64_t n) {
+  size_t a = 0, b = 0;
+  int64_t *p = nullptr;
+  int32_t *q = nullptr;
+  rocprim::inclusive_scan(nullptr, a, p, p, (size_t)(n > 0 ? n : 1), rocprim::plus<int64_t>());
+  rocprim::inclusive_scan(nullptr, b, rocprim::make_transform_iterator(q, WidenI32()), p, (size_t)(n > 0 ? n : 1),
+                          rocprim::plus<int64_t>());
+  return (a > b ? a : b) + 256;
+}
+void scan_i64(const int64_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, s, out);
+  if (n > 0) rocprim::inclusive_scan(tmp, tmp_bytes, in, out + 1, (size_t)n, rocprim::plus<int64_t>(), s);
+}
+void scan_i32(const int32_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, s, out);
+  if (n > 0)
+    rocprim::inclusive_scan(tmp, tmp_bytes, rocprim::make_transform_iterator(in, WidenI32()), out + 1, (size_t)n,
+                            rocprim::plus<int64_t>(), s);
+}
+
+}  // namespace g3a

@@ -1,0 +1,402 @@
+// `g3a mem`: host driver keeping the reference's `g3.exe mem` command line
+// (bwa/src/fastmap.cpp:416-566).  Reads FASTQ with the reference's field
+// rules (bwa/src/pipeline.cpp:111-227), aligns batches through the C ABI and
+// writes either the reference's records (pipeline.cpp:287-351):
+//     NAME RNAME POS CIGAR SEQ QUAL        (POS 0-based)
+//     NAME * 0 * SEQ QUAL                  (no alignment)
+// or, with -S, standard SAM.  Output is in input order.
+//
+// Options: -o FILE output, -Z N reads per batch, -S standard SAM, -g N GPUs
+// (1 only), -t N formatting threads; -b -v -i -z -l -p -F -L -m are accepted
+// for compatibility with the reference's command line and ignored.
+#include "../../include/g3a.h"
+#include "g3a_index.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct Batch {
+  std::string names, quals;
+  std::vector<int64_t> name_off{0}, qual_off{0};
+  std::vector<uint8_t> seq;
+  std::vector<int32_t> seq_off{0};
+  int n() const { return (int)seq_off.size() - 1; }
+  void clear() {
+    names.clear(); quals.clear(); seq.clear();
+    name_off.assign(1, 0); qual_off.assign(1, 0); seq_off.assign(1, 0);
+  }
+};
+
+uint8_t nt4(int c) {
+  switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 4;
+  }
+}
+
+// Sequential FASTQ reader over a memory buffer: NAME runs to the first space
+// (or end of line), then the rest of the header line, SEQ, the separator line
+// and QUAL.
+struct FastqReader {
+  std::vector<char> buf;
+  size_t pos = 0;
+  bool open(const char *fn) {
+    FILE *fp = std::fopen(fn, "rb");
+    if (!fp) return false;
+    std::fseek(fp, 0, SEEK_END);
+    long sz = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    buf.resize((size_t)(sz > 0 ? sz : 0));
+    bool ok = buf.empty() || std::fread(buf.data(), 1, buf.size(), fp) == buf.size();
+    std::fclose(fp);
+    while (pos < buf.size() && buf[pos] != '@') ++pos;
+    return ok;
+  }
+  bool next(Batch &b) {
+    const size_t end = buf.size();
+    if (pos >= end) return false;
+    const char *p = buf.data();
+    size_t q = ++pos;
+    while (pos < end && p[pos] != ' ' && p[pos] != '\n') ++pos;
+    b.names.append(p + q, pos - q);
+    b.name_off.push_back((int64_t)b.names.size());
+    while (pos < end && p[pos] != '\n') ++pos;
+    q = ++pos;
+    while (pos < end && p[pos] != '\n') ++pos;
+    for (size_t i = q; i < pos && i < end; ++i) b.seq.push_back(nt4((unsigned char)p[i]));
+    b.seq_off.push_back((int32_t)b.seq.size());
+    ++pos;
+    while (pos < end && p[pos] != '\n') ++pos;
+    q = ++pos;
+    while (pos < end && p[pos] != '\n') ++pos;
+    if (q < end) b.quals.append(p + q, (pos < end ? pos : end) - q);
+    b.qual_off.push_back((int64_t)b.quals.size());
+    ++pos;
+    while (pos < end && p[pos] != '@') ++pos;
+    return true;
+  }
+};
+
+void put_int(std::string &s, long long v) {
+  char t[32];
+  int k = std::snprintf(t, sizeof t, "%lld", v);
+  s.append(t, (size_t)k);
+}
+
+// mem_approx_mapq_se (bwa/src/aux.cu:676-700) with sub = csub = sub_n = 0 and
+// frac_rep = 0, evaluated on the host.
+int mapq_se(const g3a_opt &o, int score, int qb, int qe, int64_t rb, int64_t re) {
+  int mapq, l, sub = o.min_seed_len * o.a;
+  if (sub >= score) return 0;
+  l = qe - qb > re - rb ? qe - qb : (int)(re - rb);
+  double identity = 1. - (double)(l * o.a - score) / (o.a + o.b) / l;
+  if (score == 0) mapq = 0;
+  else {
+    const int coef_len = 50, coef_fac = (int)std::log(50.0);
+    double tmp = l < coef_len ? 1. : coef_fac / logf((float)l);
+    tmp *= identity * identity;
+    mapq = (int)(6.02 * (score - sub) / o.a * tmp * tmp + .499);
+  }
+  if (mapq > 60) mapq = 60;
+  if (mapq < 0) mapq = 0;
+  return mapq;
+}
+
+struct Ref {
+  std::vector<std::string> names;
+  std::vector<int64_t> offs, lens;
+  std::vector<uint8_t> pac;
+  int64_t l_pac = 0;
+  int base(int64_t l) const { return (pac[(size_t)(l >> 2)] >> ((~l & 3) << 1)) & 3; }
+};
+
+void format_range(const Batch &b, const g3a_batch_out &o, const Ref &ref, const g3a_opt &opt, bool sam, int r0, int r1,
+                  std::string &s) {
+  static const char kOp[] = "MIDSH";
+  std::vector<uint32_t> core;
+  std::string md;
+  for (int q = r0; q < r1; ++q) {
+    const char *name = b.names.data() + b.name_off[(size_t)q];
+    const size_t nl = (size_t)(b.name_off[(size_t)q + 1] - b.name_off[(size_t)q]);
+    const char *qual = b.quals.data() + b.qual_off[(size_t)q];
+    const size_t ql = (size_t)(b.qual_off[(size_t)q + 1] - b.qual_off[(size_t)q]);
+    const uint8_t *sq = b.seq.data() + b.seq_off[(size_t)q];
+    const int len = b.seq_off[(size_t)q + 1] - b.seq_off[(size_t)q];
+    const int64_t a0 = o.aln_off[q], a1 = o.aln_off[q + 1];
+    if (!sam) {
+      if (a1 <= a0) {
+        s.append(name, nl);
+        s.append(" * 0 * ");
+        for (int k = 0; k < len; ++k) s.push_back("ACGTN"[sq[k]]);
+        s.push_back(' ');
+        s.append(qual, ql);
+        s.push_back('\n');
+        continue;
+      }
+      for (int64_t a = a0; a < a1; ++a) {
+        s.append(name, nl);
+        s.push_back(' ');
+        s.append(ref.names[(size_t)o.rid[a]]);
+        s.push_back(' ');
+        put_int(s, o.pos[a]);
+        s.push_back(' ');
+        const uint32_t *cg = o.cigar + o.cigar_off[a];
+        for (int k = 0; k < o.n_cigar[a]; ++k) {
+          put_int(s, cg[k] >> 4);
+          s.push_back(kOp[cg[k] & 0xf]);
+        }
+        s.push_back(' ');
+        for (int k = 0; k < len; ++k) s.push_back("ACGTN"[sq[k]]);
+        s.push_back(' ');
+        s.append(qual, ql);
+        s.push_back('\n');
+      }
+      continue;
+    }
+    if (a1 <= a0) {
+      s.append(name, nl);
+      s.append("\t4\t*\t0\t0\t*\t*\t0\t0\t");
+      for (int k = 0; k < len; ++k) s.push_back("ACGTN"[sq[k]]);
+      s.push_back('\t');
+      s.append(qual, ql);
+      s.push_back('\n');
+      continue;
+    }
+    for (int64_t a = a0; a < a1; ++a) {
+      const bool rev = o.rb[a] >= ref.l_pac;
+      const int nc = o.n_core[a];
+      const uint32_t *c0 = o.cigar + o.core_off[a];
+      core.assign(c0, c0 + nc);
+      if (rev)
+        for (int i = 0, j = nc - 1; i < j; ++i, --j) std::swap(core[(size_t)i], core[(size_t)j]);
+      int64_t fpos = rev ? (ref.l_pac << 1) - o.re[a] : o.rb[a];
+      int beg = 0, endc = nc;
+      if (endc > 0 && (core[0] & 0xf) == 2) { fpos += core[0] >> 4; beg = 1; }
+      if (endc > beg && (core[(size_t)endc - 1] & 0xf) == 2) --endc;
+      const int qb = o.qb[a], qe = o.qe[a];
+      const int clip_l = rev ? len - qe : qb, clip_r = rev ? qb : len - qe;
+      md.clear();
+      int nm = 0, run = 0, xq = 0;
+      int64_t yr = fpos;
+      for (int k = beg; k < endc; ++k) {
+        const int op = core[(size_t)k] & 0xf, ln = (int)(core[(size_t)k] >> 4);
+        if (op == 0) {
+          for (int t = 0; t < ln; ++t) {
+            const int qi = xq + t;
+            uint8_t c = rev ? sq[qe - 1 - qi] : sq[qb + qi];
+            if (rev && c < 4) c = (uint8_t)(3 - c);
+            const int rbase = ref.base(yr + t);
+            if (c != rbase) {
+              put_int(md, run);
+              md.push_back("ACGT"[rbase]);
+              run = 0;
+              ++nm;
+            } else ++run;
+          }
+          xq += ln;
+          yr += ln;
+        } else if (op == 1) {
+          xq += ln;
+          nm += ln;
+        } else if (op == 2) {
+          put_int(md, run);
+          md.push_back('^');
+          for (int t = 0; t < ln; ++t) md.push_back("ACGT"[ref.base(yr + t)]);
+          run = 0;
+          yr += ln;
+          nm += ln;
+        }
+      }
+      put_int(md, run);
+      const int flag = (rev ? 0x10 : 0) | (a > a0 ? 0x800 : 0);
+      s.append(name, nl);
+      s.push_back('\t');
+      put_int(s, flag);
+      s.push_back('\t');
+      s.append(ref.names[(size_t)o.rid[a]]);
+      s.push_back('\t');
+      put_int(s, fpos - ref.offs[(size_t)o.rid[a]] + 1);
+      s.push_back('\t');
+      put_int(s, mapq_se(opt, o.score[a], qb, qe, o.rb[a], o.re[a]));
+      s.push_back('\t');
+      if (clip_l) { put_int(s, clip_l); s.push_back('S'); }
+      for (int k = beg; k < endc; ++k) {
+        put_int(s, core[(size_t)k] >> 4);
+        s.push_back(kOp[core[(size_t)k] & 0xf]);
+      }
+      if (clip_r) { put_int(s, clip_r); s.push_back('S'); }
+      s.append("\t*\t0\t0\t");
+      for (int k = 0; k < len; ++k) {
+        const uint8_t c = rev ? sq[len - 1 - k] : sq[k];
+        s.push_back(rev ? "TGCAN"[c] : "ACGTN"[c]);
+      }
+      s.push_back('\t');
+      for (size_t k = 0; k < ql; ++k) s.push_back(rev ? qual[ql - 1 - k] : qual[k]);
+      s.append("\tNM:i:");
+      put_int(s, nm);
+      s.append("\tMD:Z:");
+      s.append(md);
+      s.append("\tAS:i:");
+      put_int(s, o.score[a]);
+      s.push_back('\n');
+    }
+  }
+}
+
+bool load_ref(const char *prefix, Ref &ref) {
+  g3a::HostIndex h;
+  std::string fn = std::string(prefix) + ".ann";
+  FILE *fp = std::fopen(fn.c_str(), "r");
+  if (!fp) return false;
+  long long l_pac = 0;
+  int n = 0;
+  unsigned seed = 0;
+  if (std::fscanf(fp, "%lld %d %u", &l_pac, &n, &seed) != 3) { std::fclose(fp); return false; }
+  ref.l_pac = l_pac;
+  char line[65536];
+  if (!std::fgets(line, sizeof line, fp)) { std::fclose(fp); return false; }
+  for (int i = 0; i < n; ++i) {
+    if (!std::fgets(line, sizeof line, fp)) { std::fclose(fp); return false; }
+    char *p = std::strchr(line, ' ');
+    if (!p) { std::fclose(fp); return false; }
+    ++p;
+    std::string nm;
+    while (*p && *p != ' ' && *p != '\n' && *p != '\r') nm.push_back(*p++);
+    long long off, ln;
+    int namb;
+    if (!std::fgets(line, sizeof line, fp) || std::sscanf(line, "%lld %lld %d", &off, &ln, &namb) != 3) {
+      std::fclose(fp);
+      return false;
+    }
+    ref.names.push_back(nm);
+    ref.offs.push_back(off);
+    ref.lens.push_back(ln);
+  }
+  std::fclose(fp);
+  fn = std::string(prefix) + ".pac";
+  fp = std::fopen(fn.c_str(), "rb");
+  if (!fp) return false;
+  std::fseek(fp, 0, SEEK_END);
+  long sz = std::ftell(fp);
+  std::fseek(fp, 0, SEEK_SET);
+  ref.pac.resize((size_t)sz);
+  bool ok = std::fread(ref.pac.data(), 1, ref.pac.size(), fp) == ref.pac.size();
+  std::fclose(fp);
+  return ok;
+}
+
+}  // namespace
+
+int g3a_mem_main(int argc, char **argv) {
+  const char *out_fn = nullptr;
+  int batch = 500000, gpus = 1, threads = 8;
+  bool sam = false;
+  int i = 1;
+  for (; i < argc && argv[i][0] == '-' && argv[i][1]; ++i) {
+    const char *a = argv[i];
+    auto val = [&](void) -> const char * { return i + 1 < argc ? argv[++i] : nullptr; };
+    if (!std::strcmp(a, "-o")) out_fn = val();
+    else if (!std::strcmp(a, "-Z")) { const char *v = val(); batch = v ? std::atoi(v) : batch; }
+    else if (!std::strcmp(a, "-g")) { const char *v = val(); gpus = v ? std::atoi(v) : gpus; }
+    else if (!std::strcmp(a, "-t")) { const char *v = val(); threads = v ? std::atoi(v) : threads; }
+    else if (!std::strcmp(a, "-S")) sam = true;
+    else if (!std::strcmp(a, "-b")) {}
+    else if (!std::strcmp(a, "-v") || !std::strcmp(a, "-i") || !std::strcmp(a, "-z") || !std::strcmp(a, "-l") ||
+             !std::strcmp(a, "-p") || !std::strcmp(a, "-F") || !std::strcmp(a, "-L") || !std::strcmp(a, "-m"))
+      val();
+    else {
+      std::fprintf(stderr, "g3a mem: unknown option %s\n", a);
+      return 1;
+    }
+  }
+  if (i + 2 > argc) {
+    std::fprintf(stderr, "usage: g3a mem [-o out] [-Z reads_per_batch] [-S] [-t threads] <prefix> <reads.fq>\n");
+    return 1;
+  }
+  if (gpus != 1) {
+    std::fprintf(stderr, "g3a mem: this build drives one GPU (-g 1)\n");
+    return 1;
+  }
+  if (batch < 1) batch = 1;
+  if (threads < 1) threads = 1;
+  const char *prefix = argv[i], *fq = argv[i + 1];
+  Ref ref;
+  if (!load_ref(prefix, ref)) {
+    std::fprintf(stderr, "g3a mem: cannot read %s.ann/.pac\n", prefix);
+    return 1;
+  }
+  FastqReader rd;
+  if (!rd.open(fq)) {
+    std::fprintf(stderr, "g3a mem: cannot read %s\n", fq);
+    return 1;
+  }
+  g3a_opt opt;
+  g3a_opt_init(&opt);
+  g3a_ctx *ctx = nullptr;
+  int rc = g3a_device_alloc(0, prefix, &opt, batch, (int64_t)batch * 320, &ctx);
+  if (rc) {
+    std::fprintf(stderr, "g3a mem: cannot create context (%d)\n", rc);
+    return 1;
+  }
+  if ((rc = g3a_memcpy_index(ctx))) {
+    std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
+    g3a_destroy(ctx);
+    return 1;
+  }
+  FILE *out = out_fn ? std::fopen(out_fn, "wb") : stdout;
+  if (!out) {
+    std::fprintf(stderr, "g3a mem: cannot write %s\n", out_fn);
+    g3a_destroy(ctx);
+    return 1;
+  }
+  if (sam) {
+    std::string hdr = "@HD\tVN:1.6\tSO:unsorted\n";
+    for (size_t c = 0; c < ref.names.size(); ++c) {
+      hdr += "@SQ\tSN:" + ref.names[c] + "\tLN:";
+      put_int(hdr, ref.lens[c]);
+      hdr += "\n";
+    }
+    hdr += "@PG\tID:g3a\tPN:g3a\n";
+    std::fwrite(hdr.data(), 1, hdr.size(), out);
+  }
+  Batch b;
+  long long total = 0;
+  int status = 0;
+  for (;;) {
+    b.clear();
+    while (b.n() < batch && rd.next(b)) {}
+    if (b.n() == 0) break;
+    g3a_batch_out o;
+    rc = g3a_align_batch(ctx, b.n(), b.seq.data(), b.seq_off.data(), &o);
+    if (rc) {
+      std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
+      status = 1;
+      break;
+    }
+    const int nt = b.n() < 4096 ? 1 : threads;
+    std::vector<std::string> parts((size_t)nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+      int r0 = (int)((int64_t)b.n() * t / nt), r1 = (int)((int64_t)b.n() * (t + 1) / nt);
+      th.emplace_back([&, t, r0, r1] { format_range(b, o, ref, opt, sam, r0, r1, parts[(size_t)t]); });
+    }
+    for (auto &x : th) x.join();
+    for (auto &p : parts) std::fwrite(p.data(), 1, p.size(), out);
+    total += b.n();
+    g3a_batch_out_free(&o);
+  }
+  if (out_fn) std::fclose(out);
+  g3a_destroy(ctx);
+  std::fprintf(stderr, "g3a mem: %lld reads\n", total);
+  return status;
+}

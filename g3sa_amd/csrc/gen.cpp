@@ -279,6 +279,8 @@ int cmd_reads(const Args &a) {
   double hard_frac = a.num("--hard-frac", 0.0), hard_div = a.num("--hard-div", 0.08);
   int tail_min = (int)a.num("--tail-min", 20), tail_max = (int)a.num("--tail-max", 40);
   double err_min = a.num("--err-min", -1), err_max = a.num("--err-max", -1);
+  const char *truth_fn = a.get("--truth");
+  FILE *tfp = truth_fn ? std::fopen(truth_fn, "w") : nullptr;
   std::vector<Contig> ctg = read_fasta(ref);
   if (ctg.empty()) { std::fprintf(stderr, "g3a-gen reads: empty reference\n"); return 1; }
   int64_t total = 0;
@@ -308,9 +310,11 @@ int cmd_reads(const Args &a) {
       int64_t start = (int64_t)r.below((uint64_t)((int64_t)s.size() - body - 15));
       if (!sample_read(r, s, start, body, hard ? hard_div : s0, hard ? hard_div : s1, indel, rd)) continue;
       for (int i = 0; i < tail; ++i) rd.push_back((uint8_t)r.below(4));
-      if (r.below(2)) revcomp(rd);
+      bool rc = r.below(2);
+      if (rc) revcomp(rd);
       std::snprintf(name, sizeof name, "r%lld", (long long)made);
       write_fq(fp, name, rd);
+      if (tfp) std::fprintf(tfp, "%s\t%s\t%lld\t%c\n", name, ctg[ci].name.c_str(), (long long)start, rc ? '-' : '+');
       ++made;
     } else {
       int len = len_fixed;
@@ -334,6 +338,7 @@ int cmd_reads(const Args &a) {
     }
   }
   std::fclose(fp);
+  if (tfp) std::fclose(tfp);
   return 0;
 }
 

@@ -1,0 +1,119 @@
+/*
+ * g3a: MI355X-native BWA-MEM short-read alignment hot path (C ABI).
+ *
+ * Drop-in boundary for the reference's device interface
+ * (sunwookim028/G3SA, bwa/src/cuda_wrapper.h and bwa/src/bwamem.h).  The
+ * reference splits one GPU batch into five calls on a `process_data_t`; the
+ * same five phases are exposed here on an opaque context with plain pointers
+ * and sizes:
+ *
+ *   reference (file:line)                          here
+ *   device_alloc       cuda_wrapper.h:22-24        g3a_device_alloc
+ *   memcpy_index       cuda_wrapper.h:26-30        g3a_memcpy_index
+ *   memcpy_input       cuda_wrapper.h:85-86        g3a_memcpy_input
+ *   bwa_align          bwamem.h:10                 g3a_bwa_align
+ *   memcpy_output      cuda_wrapper.h:88-90        g3a_memcpy_output
+ *   destruct_proc      cuda_wrapper.cu:408-411     g3a_destroy
+ *
+ * g3a_align_batch runs input copy, alignment and output copy in one call.
+ * Every function returns 0 on success or a negative G3A_E* code; the message
+ * of the last failure on a context is available from g3a_last_error.  No
+ * function exits or aborts the process.
+ *
+ * Threading: one context per device; calls on one context must not overlap.
+ * Contexts on different devices may be used concurrently.
+ */
+#ifndef G3A_H
+#define G3A_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define G3A_OK 0
+#define G3A_E_ARG (-1)      /* invalid argument */
+#define G3A_E_INDEX (-2)    /* index files missing or malformed */
+#define G3A_E_HIP (-3)      /* HIP runtime error (includes device OOM) */
+#define G3A_E_STATE (-4)    /* call out of order (e.g. align before input) */
+#define G3A_E_INTERNAL (-5)
+
+typedef struct g3a_ctx g3a_ctx;
+
+/* Alignment options: the subset of the reference's mem_opt_t
+ * (bwa/src/bwa.h:157-189) read on this path.  g3a_opt_init fills BWA's
+ * defaults as mem_opt_init does (bwa/src/fastmap.cpp:77-113). */
+typedef struct {
+  int a, b, o_del, e_del, o_ins, e_ins, w, T, pen_clip5, pen_clip3;
+  int min_seed_len, split_width, max_occ, max_chain_gap, flag;
+  int64_t max_mem_intv;
+  float split_factor, mask_level, drop_ratio, mask_level_redun;
+  int8_t mat[25];
+} g3a_opt;
+
+/* Output of one batch (the reference's memcpy_output arrays,
+ * cuda_wrapper.cu:427-461, without its 16-operation CIGAR cap).  Alignments
+ * of read i are [aln_off[i], aln_off[i+1]); a read without alignments is
+ * unmapped.  All arrays are allocated by the library and released with
+ * g3a_batch_out_free. */
+typedef struct {
+  int32_t n_reads;
+  int64_t *aln_off;      /* n_reads + 1 */
+  int64_t n_alns;
+  int32_t *rid;          /* contig index */
+  int64_t *pos;          /* 0-based position in the contig (reference record) */
+  int32_t *flag;         /* 0x10 reverse strand, 0x800 supplementary, 0x100 secondary */
+  int32_t *is_rev;
+  int32_t *score;        /* extension score (AS) */
+  int32_t *nw_score;     /* global-alignment score */
+  int32_t *nm;           /* edit distance as the reference's traceback counts it */
+  int32_t *qb, *qe;      /* aligned read interval [qb, qe) */
+  int64_t *rb, *re;      /* aligned interval of the fwd+revcomp text [rb, re) */
+  int32_t *n_cigar;      /* reference-format CIGAR (with soft clips), BAM encoding */
+  int64_t *cigar_off;    /* into cigar[] */
+  int32_t *n_core;       /* global-alignment CIGAR in read orientation */
+  int64_t *core_off;     /* into cigar[] */
+  uint32_t *cigar;
+  int64_t n_cigar_words;
+  /* device time of the pipeline stages of this batch, milliseconds */
+  float ms_seed, ms_sa, ms_chain, ms_extend, ms_traceback, ms_total;
+  /* work counters of this batch */
+  int64_t n_seeds, n_tasks, ext_cells, fm_steps, n_slow_reads;
+} g3a_batch_out;
+
+void g3a_opt_init(g3a_opt *opt);
+
+/* device_alloc: create a context on HIP device `device` for batches of up to
+ * max_reads reads and max_bases bases, and load the index at `prefix`
+ * (.ann .amb .pac .bwt and .g3sa or .sa) into host memory. */
+int g3a_device_alloc(int device, const char *prefix, const g3a_opt *opt, int max_reads, int64_t max_bases,
+                     g3a_ctx **out);
+/* memcpy_index: upload the index to HBM. */
+int g3a_memcpy_index(g3a_ctx *ctx);
+/* memcpy_input: copy a batch to the device; seq holds 2-bit codes (A0 C1 G2
+ * T3, anything else 4) of n_reads reads concatenated; offsets[n_reads+1]. */
+int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t *offsets);
+/* bwa_align: run the pipeline on the batch resident on the device. */
+int g3a_bwa_align(g3a_ctx *ctx);
+/* memcpy_output: copy the results of the last g3a_bwa_align to the host. */
+int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out);
+/* memcpy_input + bwa_align + memcpy_output */
+int g3a_align_batch(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t *offsets, g3a_batch_out *out);
+void g3a_batch_out_free(g3a_batch_out *out);
+/* Stage times (ms: seed, sa, chain, extend, traceback, total) and work
+ * counters (seeds, tasks, extension cells, FM steps, slow-path reads,
+ * alignments) of the last g3a_bwa_align, without copying its output.  No
+ * reference counterpart: the reference's per-stage timers are the
+ * CUDA-event tprof table (bwa/src/bwamem.cu:31-49). */
+int g3a_last_stats(const g3a_ctx *ctx, float ms[6], int64_t work[6]);
+void g3a_destroy(g3a_ctx *ctx);
+const char *g3a_last_error(const g3a_ctx *ctx);
+
+/* Index construction (the `g3a index` command): BWA-format files plus the
+ * dense suffix-array sample with interval sa_intv (power of two, 1..32). */
+int g3a_build_index(const char *fasta, const char *prefix, int sa_intv, char *err, int err_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

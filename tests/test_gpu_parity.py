@@ -1,0 +1,118 @@
+"""GPU parity: the HIP pipeline, called through the C ABI, against the oracle.
+
+Every alignment field and both CIGARs must be identical (integer work:
+bit-exact).  Datasets: config 1 exactly (1 Mbp, 10,000 x 150 bp), config-3
+style variable-length reads with divergent tails, a repeat-rich 5-contig
+reference with paired reads aligned as single-end, and edge-case batches.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import g3sa_amd
+from conftest import oracle_threads
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = [("rid", "rid"), ("pos", "pos"), ("flag", "flag"), ("is_rev", "is_rev"), ("score", "score"),
+          ("nw_score", "nw_score"), ("nm", "NM"), ("qb", "qb"), ("qe", "qe"), ("rb", "rb"), ("re", "re"),
+          ("n_cigar", "n_cigar"), ("n_core", "n_core")]
+
+
+def compare(gpu: g3sa_amd.Result, ref: dict, label: str) -> None:
+    assert gpu.n_reads == len(ref["aln_off"]) - 1, label
+    np.testing.assert_array_equal(gpu.aln_off, ref["aln_off"], err_msg=f"{label}: alignments per read differ")
+    for g, r in FIELDS:
+        a, b = getattr(gpu, g), ref[r]
+        if not np.array_equal(a, b):
+            bad = np.nonzero(a != b)[0][:5]
+            raise AssertionError(f"{label}: field {g} differs at alignments {bad.tolist()}: gpu {a[bad]} oracle {b[bad]}")
+    for i in range(gpu.n_alns):
+        gc = gpu.cigar[gpu.cigar_off[i]: gpu.cigar_off[i] + gpu.n_cigar[i]]
+        rc = ref["cigar"][ref["cigar_off"][i]: ref["cigar_off"][i] + ref["n_cigar"][i]]
+        assert np.array_equal(gc, rc), f"{label}: CIGAR of alignment {i}"
+        gk = gpu.cigar[gpu.core_off[i]: gpu.core_off[i] + gpu.n_core[i]]
+        rk = ref["cigar"][ref["cigar_off"][i] + ref["n_cigar"][i]:
+                          ref["cigar_off"][i] + ref["n_cigar"][i] + ref["n_core"][i]]
+        assert np.array_equal(gk, rk), f"{label}: global CIGAR of alignment {i}"
+
+
+def run_both(ds, oracle, seq, off, label, max_reads=None):
+    n = len(off) - 1
+    print(f"[parity] {label}: {n} reads", flush=True)
+    with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(max_reads or n, 1),
+                          max_bases=max(int(off[-1]), 1)) as al:
+        got = al.align(seq, off)
+    idx = oracle.load(ds.prefix)
+    try:
+        ref = oracle.align(idx, seq, off, threads=oracle_threads())
+    finally:
+        oracle.lib.orc_index_free(idx)
+    compare(got, ref, label)
+    # seeds resolved by the GPU equal the oracle's SA lookups
+    assert got.work["n_seeds"] == ref["work"]["sa_reads"], label
+    return got, ref
+
+
+def test_config1_parity(ds_c1, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["main"]))
+    got, ref = run_both(ds_c1, oracle, seq, off, "config1")
+    assert got.n_alns >= 9900
+
+
+def test_config3_style_parity(ds_c1, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["c3"]))
+    run_both(ds_c1, oracle, seq, off, "config3-style")
+
+
+def test_repeats_parity(ds_rep, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_rep.reads["main"]))
+    got, _ = run_both(ds_rep, oracle, seq, off, "repeats")
+
+
+def _batch(reads):
+    off = np.zeros(len(reads) + 1, dtype=np.int32)
+    off[1:] = np.cumsum([len(r) for r in reads])
+    seq = np.concatenate([np.asarray(r, dtype=np.uint8) for r in reads]) if off[-1] else np.zeros(1, np.uint8)
+    return seq, off
+
+
+def test_edge_cases_parity(ds_c1, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["main"]), limit=50)
+    reads = [seq[off[i]:off[i + 1]] for i in range(50)]
+    rng = np.random.default_rng(7)
+    cases = [
+        np.zeros(0, np.uint8),                              # empty read
+        np.full(150, 4, np.uint8),                          # all N
+        reads[0][:18],                                      # shorter than a seed
+        reads[1][:19],                                      # exactly one seed long
+        np.concatenate([reads[2][:70], [4], reads[2][71:]]),  # N inside
+        np.concatenate([reads[3], reads[4], reads[5]]),     # 450 bp: over the 320 bp limit
+        np.concatenate([reads[6], reads[7][:170]]),         # 320 bp: at the limit
+        rng.integers(0, 4, 150).astype(np.uint8),           # random (unmapped)
+        reads[8][::-1].copy(),                              # reversed (not complemented)
+        np.tile(np.array([0, 1], np.uint8), 75),            # low complexity
+    ] + reads[9:20]
+    seq2, off2 = _batch(cases)
+    run_both(ds_c1, oracle, seq2, off2, "edge-cases")
+
+
+def test_single_read_and_repeated_batches(ds_c1, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["main"]), limit=300)
+    idx = oracle.load(ds_c1.prefix)
+    try:
+        with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=300, max_bases=int(off[-1])) as al:
+            for lo, hi in [(0, 1), (0, 300), (17, 18), (100, 250)]:
+                s, o = seq[off[lo]:off[hi]], (off[lo:hi + 1] - off[lo]).astype(np.int32)
+                got = al.align(s, o)
+                ref = oracle.align(idx, s, o, threads=2)
+                compare(got, ref, f"batch {lo}:{hi}")
+    finally:
+        oracle.lib.orc_index_free(idx)
+
+
+def test_empty_batch(ds_c1):
+    with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=4, max_bases=1000) as al:
+        got = al.align(np.zeros(1, np.uint8), np.zeros(1, np.int32))
+        assert got.n_reads == 0 and got.n_alns == 0
