@@ -18,6 +18,9 @@ CLI_PATH = BUILD / "g3a"
 GEN_PATH = BUILD / "g3a-gen"
 
 G3A_OK = 0
+FORM_PLAIN = 0
+FORM_BATCHED = 1
+FORMS = {"plain": FORM_PLAIN, "batched": FORM_BATCHED}
 
 
 class Opt(C.Structure):
@@ -60,7 +63,7 @@ class BatchOut(C.Structure):
 EXPORTS = [
     "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
-    "g3a_build_index", "g3a_last_stats",
+    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms",
 ]
 
 _lib = None
@@ -87,6 +90,9 @@ def load_library() -> C.CDLL:
     lib.g3a_last_error.restype = C.c_char_p
     lib.g3a_build_index.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
     lib.g3a_last_stats.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+    lib.g3a_set_form.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_kernel_ms.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    lib.g3a_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     _lib = lib
     return lib
 
@@ -130,7 +136,7 @@ class Aligner:
     """A device context: index in HBM, batches aligned through the C ABI."""
 
     def __init__(self, prefix: str, device: int = 0, max_reads: int = 1 << 20, max_bases: int | None = None,
-                 opt: Opt | None = None):
+                 opt: Opt | None = None, form: str = "batched", tuning: tuple[int, int, int] | None = None):
         self.lib = load_library()
         self.ctx = C.c_void_p()
         if max_bases is None:
@@ -140,6 +146,16 @@ class Aligner:
         if rc != G3A_OK:
             raise RuntimeError(f"g3a_device_alloc failed ({rc})")
         self._check(self.lib.g3a_memcpy_index(self.ctx))
+        self.set_form(form)
+        if tuning is not None:
+            self.set_tuning(*tuning)
+
+    def set_form(self, form: str) -> None:
+        """'plain' (one wavefront per extension task) or 'batched'."""
+        self._check(self.lib.g3a_set_form(self.ctx, FORMS[form]))
+
+    def set_tuning(self, strip_rows: int, lane_blocks_per_cu: int, wave_blocks_per_cu: int) -> None:
+        self._check(self.lib.g3a_set_tuning(self.ctx, strip_rows, lane_blocks_per_cu, wave_blocks_per_cu))
 
     def _check(self, rc: int) -> None:
         if rc != G3A_OK:
@@ -168,7 +184,11 @@ class Aligner:
         self._check(self.lib.g3a_last_stats(self.ctx, ms, work))
         keys_ms = ["seed", "sa", "chain", "extend", "traceback", "total"]
         keys_w = ["n_seeds", "n_tasks", "ext_cells", "fm_steps", "n_slow_reads", "n_alns"]
-        return {"ms": dict(zip(keys_ms, list(ms))), "work": dict(zip(keys_w, list(work)))}
+        kms = (C.c_float * 4)()
+        self._check(self.lib.g3a_kernel_ms(self.ctx, kms))
+        keys_k = ["k_seed", "ext_lane", "ext_wave", "k_traceback"]
+        return {"ms": dict(zip(keys_ms, list(ms))), "work": dict(zip(keys_w, list(work))),
+                "kernel_ms": dict(zip(keys_k, list(kms)))}
 
     def align(self, seq: np.ndarray, offsets: np.ndarray) -> Result:
         self.upload(seq, offsets)

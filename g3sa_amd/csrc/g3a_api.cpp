@@ -56,9 +56,19 @@ struct g3a_ctx {
   // alignments
   DevBuf aln_reg, aln_read, aln_rank, z_size, cig_size, z_off, cig_off, z, cigar, alns;
   DevBuf scan_tmp;
+  // batched-form extension
+  int form = G3A_FORM_BATCHED;
+  int n_cu = 0;
+  int strip_rows = 16, lane_blocks_per_cu = 8, wave_blocks_per_cu = 8;
+  DevBuf ext_res[2], ext_key, ext_key2, ext_idx, ext_sorted, ext_slow, ext_ctr, ext_sort_tmp, ext_rowbuf;
   // host staging
   int64_t *pinned = nullptr;
   hipEvent_t ev[8] = {};
+  // single-kernel timing: k_seed, lane engine (2 sides), one-wavefront
+  // extension (2 sides or the plain form's k_extend), k_traceback
+  hipEvent_t kev[12] = {};
+  int kev_used[6] = {0, 0, 0, 0, 0, 0};
+  float kms[4] = {0, 0, 0, 0};
   // last batch results
   int64_t total_seeds = 0, total_tasks = 0, total_alns = 0, total_cig = 0, n_slow = 0;
   float ms[6] = {0, 0, 0, 0, 0, 0};
@@ -180,6 +190,10 @@ int g3a_device_alloc(int device, const char *prefix, const g3a_opt *o, int max_r
     return G3A_E_HIP;
   }
   for (auto &ev : ctx->ev) hipEventCreate(&ev);
+  for (auto &ev : ctx->kev) hipEventCreate(&ev);
+  if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      ctx->n_cu < 1)
+    ctx->n_cu = 256;
   if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
     g3a_destroy(ctx);
     return G3A_E_HIP;
@@ -240,6 +254,7 @@ int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_
   const int64_t nb = n_reads ? offsets[n_reads] : 0;
   if (ensure(ctx, ctx->seq, (size_t)nb + 64) || ensure(ctx, ctx->off, (size_t)(n_reads + 1) * 4)) return G3A_E_HIP;
   if (nb) HIPCK(hipMemcpyAsync(ctx->seq.p, seq, (size_t)nb, hipMemcpyHostToDevice, ctx->stream));
+  launch_clamp_codes(P<uint8_t>(ctx->seq), nb, ctx->stream);
   if (n_reads) HIPCK(hipMemcpyAsync(ctx->off.p, offsets, (size_t)(n_reads + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPCK(hipStreamSynchronize(ctx->stream));
   ctx->n_reads = n_reads;
@@ -297,7 +312,11 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.n_seed = P<int64_t>(ctx->n_seed);
   sa.slow = P<int32_t>(ctx->slow);
   sa.fm_steps = cnt + 0;
+  for (int &u : ctx->kev_used) u = 0;
+  HIPCK(hipEventRecord(ctx->kev[0], s));
   launch_seed(sa, s);
+  HIPCK(hipEventRecord(ctx->kev[1], s));
+  ctx->kev_used[0] = 1;
   launch_collect_slow(n, P<int32_t>(ctx->slow), P<int32_t>(ctx->slow_list), P<int32_t>(ctx->counters) + 2 * 8,
                       P<int64_t>(ctx->intv_base), kSlotCap, s);
   if (readback(ctx, {cnt + 8})) return G3A_E_HIP;
@@ -395,16 +414,65 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   launch_task_compact(cp, s);
   HIPCK(hipEventRecord(ctx->ev[3], s));
   // ---- extension
-  ExtArgs ea{};
-  ea.idx = ctx->idx;
-  ea.opt = ctx->opt;
-  ea.n_tasks = T_t;
-  ea.seq = P<uint8_t>(ctx->seq);
-  ea.off = P<int32_t>(ctx->off);
-  ea.tasks = P<Task>(ctx->tasks);
-  ea.regs = P<Region>(ctx->regs);
-  ea.cells = cnt + 24;
-  launch_extend(ea, s);
+  if (ctx->form == G3A_FORM_PLAIN) {
+    ExtArgs ea{};
+    ea.idx = ctx->idx;
+    ea.opt = ctx->opt;
+    ea.n_tasks = T_t;
+    ea.seq = P<uint8_t>(ctx->seq);
+    ea.off = P<int32_t>(ctx->off);
+    ea.tasks = P<Task>(ctx->tasks);
+    ea.regs = P<Region>(ctx->regs);
+    ea.cells = cnt + 24;
+    HIPCK(hipEventRecord(ctx->kev[3], s));
+    launch_extend(ea, s);
+    HIPCK(hipEventRecord(ctx->kev[7], s));
+    ctx->kev_used[3] = 1;
+  } else if (T_t > 0) {
+    const int lane_blocks = ctx->n_cu * ctx->lane_blocks_per_cu, wave_blocks = ctx->n_cu * ctx->wave_blocks_per_cu;
+    if (ensure(ctx, ctx->ext_res[0], (size_t)T_t * sizeof(ExtRes)) ||
+        ensure(ctx, ctx->ext_res[1], (size_t)T_t * sizeof(ExtRes)) || ensure(ctx, ctx->ext_key, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->ext_key2, (size_t)T_t * 4) || ensure(ctx, ctx->ext_idx, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->ext_sorted, (size_t)T_t * 4) || ensure(ctx, ctx->ext_slow, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->ext_ctr, 2 * 4 * sizeof(int32_t)) ||
+        ensure(ctx, ctx->ext_sort_tmp, ext_sort_tmp_bytes(T_t)) ||
+        ensure(ctx, ctx->ext_rowbuf, (size_t)lane_blocks * kExtQCap * 64 * 4))
+      return G3A_E_HIP;
+    launch_fill_i32(P<int32_t>(ctx->ext_ctr), 8, 0, s);
+    for (int side = 0; side < 2; ++side) {
+      ExtSideArgs ea{};
+      ea.idx = ctx->idx;
+      ea.opt = ctx->opt;
+      ea.side = side;
+      ea.lane_opt_ok = ext_lane_opt_ok(ctx->opt);
+      ea.n_tasks = T_t;
+      ea.seq = P<uint8_t>(ctx->seq);
+      ea.off = P<int32_t>(ctx->off);
+      ea.tasks = P<Task>(ctx->tasks);
+      ea.left = P<ExtRes>(ctx->ext_res[0]);
+      ea.res = P<ExtRes>(ctx->ext_res[side]);
+      ea.key = P<uint32_t>(ctx->ext_key);
+      ea.tid = P<int32_t>(ctx->ext_idx);
+      ea.sorted = P<int32_t>(ctx->ext_sorted);
+      ea.slow_list = P<int32_t>(ctx->ext_slow);
+      ea.ctr = P<int32_t>(ctx->ext_ctr) + 4 * side;
+      ea.rowbuf = P<uint32_t>(ctx->ext_rowbuf);
+      ea.cells = cnt + 24;
+      launch_ext_classify(ea, s);
+      if (ext_sort(P<uint32_t>(ctx->ext_key), P<uint32_t>(ctx->ext_key2), P<int32_t>(ctx->ext_idx),
+                   P<int32_t>(ctx->ext_sorted), T_t, ctx->ext_sort_tmp.p, ctx->ext_sort_tmp.bytes, s))
+        return fail(ctx, G3A_E_HIP, "extension task sort failed");
+      HIPCK(hipEventRecord(ctx->kev[2 + 2 * side], s));
+      launch_ext_lane(ea, ctx->strip_rows, lane_blocks, s);
+      HIPCK(hipEventRecord(ctx->kev[3 + 2 * side], s));
+      launch_ext_wave(ea, wave_blocks, s);
+      HIPCK(hipEventRecord(ctx->kev[7 + 2 * side], s));
+      ctx->kev_used[1 + side] = 1;
+    }
+    ExtFinishArgs fa{T_t, P<Task>(ctx->tasks), P<ExtRes>(ctx->ext_res[0]), P<ExtRes>(ctx->ext_res[1]),
+                     P<Region>(ctx->regs)};
+    launch_ext_finish(fa, s);
+  }
   HIPCK(hipEventRecord(ctx->ev[4], s));
   // ---- region filter, traceback, finalize
   RegionArgs ra{ctx->opt, n, P<int64_t>(ctx->task_off), P<Region>(ctx->regs), P<uint8_t>(ctx->flag),
@@ -444,7 +512,10 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   tb.z = P<uint8_t>(ctx->z);
   tb.cigar = P<uint32_t>(ctx->cigar);
   tb.alns = P<Aln>(ctx->alns);
+  HIPCK(hipEventRecord(ctx->kev[10], s));
   launch_traceback(tb, s);
+  HIPCK(hipEventRecord(ctx->kev[11], s));
+  ctx->kev_used[5] = 1;
   HIPCK(hipEventRecord(ctx->ev[5], s));
   HIPCK(hipStreamSynchronize(s));
   HIPCK(hipGetLastError());
@@ -452,6 +523,16 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], ctx->ev[i], ctx->ev[i + 1]);
   for (int i = 0; i < 5; ++i) ctx->ms[i] = t[i];
   hipEventElapsedTime(&ctx->ms[5], ctx->ev[0], ctx->ev[5]);
+  float e = 0;
+  for (float &k : ctx->kms) k = 0;
+  if (ctx->kev_used[0] && hipEventElapsedTime(&e, ctx->kev[0], ctx->kev[1]) == hipSuccess) ctx->kms[0] = e;
+  for (int side = 0; side < 2; ++side)
+    if (ctx->kev_used[1 + side]) {
+      if (hipEventElapsedTime(&e, ctx->kev[2 + 2 * side], ctx->kev[3 + 2 * side]) == hipSuccess) ctx->kms[1] += e;
+      if (hipEventElapsedTime(&e, ctx->kev[3 + 2 * side], ctx->kev[7 + 2 * side]) == hipSuccess) ctx->kms[2] += e;
+    }
+  if (ctx->kev_used[3] && hipEventElapsedTime(&e, ctx->kev[3], ctx->kev[7]) == hipSuccess) ctx->kms[2] = e;
+  if (ctx->kev_used[5] && hipEventElapsedTime(&e, ctx->kev[10], ctx->kev[11]) == hipSuccess) ctx->kms[3] = e;
   ctx->has_output = true;
   return G3A_OK;
 }
@@ -527,6 +608,29 @@ int g3a_align_batch(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t
   return g3a_memcpy_output(ctx, out);
 }
 
+int g3a_kernel_ms(const g3a_ctx *ctx, float ms[4]) {
+  if (!ctx || !ms) return G3A_E_ARG;
+  if (!ctx->has_output) return G3A_E_STATE;
+  for (int i = 0; i < 4; ++i) ms[i] = ctx->kms[i];
+  return G3A_OK;
+}
+
+int g3a_set_form(g3a_ctx *ctx, int form) {
+  if (!ctx || (form != G3A_FORM_PLAIN && form != G3A_FORM_BATCHED)) return G3A_E_ARG;
+  ctx->form = form;
+  return G3A_OK;
+}
+
+int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wave_blocks_per_cu) {
+  if (!ctx || (strip_rows != 16 && strip_rows != 32) || lane_blocks_per_cu < 1 || lane_blocks_per_cu > 64 ||
+      wave_blocks_per_cu < 1 || wave_blocks_per_cu > 64)
+    return G3A_E_ARG;
+  ctx->strip_rows = strip_rows;
+  ctx->lane_blocks_per_cu = lane_blocks_per_cu;
+  ctx->wave_blocks_per_cu = wave_blocks_per_cu;
+  return G3A_OK;
+}
+
 int g3a_last_stats(const g3a_ctx *ctx, float ms[6], int64_t work[6]) {
   if (!ctx || !ms || !work) return G3A_E_ARG;
   if (!ctx->has_output) return G3A_E_STATE;
@@ -561,9 +665,13 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->seeds2, &ctx->ch, &ctx->nxt, &ctx->order, &ctx->tmpi, &ctx->w16, &ctx->kept, &ctx->nodes,
                     &ctx->tasks_tmp, &ctx->n_task, &ctx->task_off, &ctx->tasks, &ctx->regs, &ctx->flag, &ctx->sec,
                     &ctx->n_aln, &ctx->aln_off, &ctx->aln_reg, &ctx->aln_read, &ctx->aln_rank, &ctx->z_size,
-                    &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp};
+                    &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp,
+                    &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
+                    &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
+    if (ev) hipEventDestroy(ev);
+  for (auto &ev : ctx->kev)
     if (ev) hipEventDestroy(ev);
   if (ctx->pinned) hipHostFree(ctx->pinned);
   if (ctx->stream) hipStreamDestroy(ctx->stream);

@@ -174,11 +174,12 @@ G3A_HD int intv2rid(const DevIndex &x, int64_t rb, int64_t re) {
 }
 
 // base of the fwd+revcomp text at position p
-G3A_HD int text_base(const DevIndex &x, int64_t p) {
-  if (p < x.l_pac) return (x.pac[p >> 2] >> ((~p & 3) << 1)) & 3;
-  int64_t q = (x.l_pac << 1) - 1 - p;
-  return 3 - ((x.pac[q >> 2] >> ((~q & 3) << 1)) & 3);
+G3A_HD int text_base_p(const uint8_t *pac, int64_t l_pac, int64_t p) {
+  if (p < l_pac) return (pac[p >> 2] >> ((~p & 3) << 1)) & 3;
+  int64_t q = (l_pac << 1) - 1 - p;
+  return 3 - ((pac[q >> 2] >> ((~q & 3) << 1)) & 3);
 }
+G3A_HD int text_base(const DevIndex &x, int64_t p) { return text_base_p(x.pac, x.l_pac, p); }
 
 // ---------------------------------------------------------------------------
 // Seeding

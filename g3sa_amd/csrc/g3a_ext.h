@@ -1,0 +1,225 @@
+// Banded local extension, lane-per-task engine (ksw_extend_warp2,
+// bwa/src/ksw.cu:814-1034, as its cell set and result are restated in the
+// oracle's orc_ksw_extend).
+//
+// Each lane of a wavefront runs one extension task.  The DP is swept in
+// strips of R target rows: the strip's current column of H (and the F value
+// of each row) lives in registers, so a cell costs a dozen VALU operations
+// and no memory access; the row above the strip (H, E and the query code of
+// each column, packed in one 32-bit word) is read and rewritten once per
+// column from a per-lane row buffer laid out column-major across the
+// wavefront, so a column is one coalesced 256-byte access.
+//
+// The engine is exact for the tasks ext_lane_ok() admits: those for which the
+// reference's column limit 32*(i/32+1)+w+1, its corner rule and its quirk
+// rows never reach a column below qlen, and whose scores fit 12 bits.  Other
+// tasks, and tasks whose best score equals h0 after at least 32 rows (where
+// the reference's 32-lane residue rule decides the position), go to the
+// one-wavefront-per-task kernel.
+//
+// Exact pruning: after each strip the lane stops once no later cell can
+// reach the current best.  A later cell's value comes from a positive cell of
+// the strip's last row (through the diagonal or E, both bounded by that
+// cell's H) gaining at most max(mat) per remaining query column, or from the
+// column -1 boundary max(h0 - o_del - e_del*i, 0) of a later row; when all of
+// these bounds are below the best, no later cell can equal or exceed it, so
+// neither the score nor its position can change.
+#pragma once
+#include "g3a_dev.h"
+
+namespace g3a {
+
+constexpr int kExtQCap = kMaxReadLen;  // row-buffer columns per lane
+constexpr int kExtPad = 5;             // query/target code of padding cells
+
+// Band width after clipping by max_ins/max_del (ksw.cu:840-851).
+G3A_HD int ext_band_w(const Opt &o, int qlen, int end_bonus) {
+  int max_ = 0;
+  for (int i = 0; i < 25; ++i) max_ = max_ > o.mat[i] ? max_ : o.mat[i];
+  int w = o.w;
+  int max_ins = (int)((double)(qlen * max_ + end_bonus - o.o_ins) / o.e_ins + 1.);
+  max_ins = max_ins > 1 ? max_ins : 1;
+  w = w < max_ins ? w : max_ins;
+  int max_del = (int)((double)(qlen * max_ + end_bonus - o.o_del) / o.e_del + 1.);
+  max_del = max_del > 1 ? max_del : 1;
+  w = w < max_del ? w : max_del;
+  return w;
+}
+
+G3A_HD int mat_max(const Opt &o) {
+  int m = 0;
+  for (int i = 0; i < 25; ++i) m = m > o.mat[i] ? m : o.mat[i];
+  return m;
+}
+
+// Scoring admitted by the engine: every score in [-8, 7] (4-bit fields).
+G3A_HD bool ext_lane_opt_ok(const Opt &o) {
+  for (int i = 0; i < 25; ++i)
+    if (o.mat[i] < -8 || o.mat[i] > 7) return false;
+  return o.e_del >= 0 && o.e_ins >= 0 && o.o_del >= 0 && o.o_ins >= 0;
+}
+
+// Can the engine compute this task exactly?
+G3A_HD bool ext_lane_ok(const Opt &o, int qlen, int w, int h0) {
+  if (qlen <= 0 || qlen > kExtQCap || h0 < 0) return false;
+  if (qlen > w + 33) return false;            // column limit and quirk rows stay right of qlen
+  if (w < 29 && qlen > w + 2) return false;  // corner rule stays right of qlen
+  return h0 + mat_max(o) * qlen <= 4095;     // 12-bit H and E
+}
+
+struct ExtLaneOut {
+  int score, qle, tle;
+  int redo;  // best == h0 after >= 32 rows: the position needs the full rule
+  int64_t cells;
+};
+
+// One task on one lane.
+//   qb(j): query code of column j < qlen; tb(i): target code of row i < tlen
+//   qmax >= qlen, w_min <= w <= w_max: bounds shared by the lanes of a
+//     wavefront (they keep the column loop and the memory accesses uniform;
+//     any looser values give the same result)
+//   rowbuf: column j of this lane at rowbuf[j * rs]
+//   wt: score words of query code c at wt[c * cs + k], k < R/8
+template <int R, class QB, class TB>
+G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w, int h0, int qmax, int w_min,
+                           int w_max, uint32_t *rowbuf, int rs, uint32_t *wt, int cs) {
+  static_assert(R == 16 || R == 32, "strip height");
+  constexpr int NW = R / 8;
+  const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
+  const int ms = mat_max(o);
+  // score + 8 of target code t against query codes 0..4, 4 bits each
+  uint32_t rwt[5];
+  for (int t = 0; t < 5; ++t) {
+    uint32_t v = 0;
+    for (int c = 0; c < 5; ++c) v |= (uint32_t)(o.mat[t * 5 + c] + 8) << (4 * c);
+    rwt[t] = v;
+  }
+  const uint32_t rw0 = rwt[0], rw1 = rwt[1], rw2 = rwt[2], rw3 = rwt[3], rw4 = rwt[4];
+  // row -1: H(-1, j) is the diagonal input of cell (0, j+1), E(-1, j) = 0
+  for (int j = 0; j < qmax; ++j) {
+    int hv = h0 - oe_ins - j * o.e_ins;
+    hv = hv > 0 ? hv : 0;
+    const uint32_t qc = j < qlen ? (uint32_t)qb(j) : (uint32_t)kExtPad;
+    rowbuf[(int64_t)j * rs] = (uint32_t)hv | (qc << 24);
+  }
+  int best = h0, bi = -1, bj = -1;
+  int64_t cells = 0;
+  int i0 = 0;
+  for (;;) {
+    // 4-bit fields (score + 8) of the strip's rows against each query code;
+    // padding rows and columns score -8, below any real score
+    {
+      uint32_t wl[6][NW];
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int k = 0; k < NW; ++k) wl[c][k] = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = i0 + r;
+        const int tc = i < tlen ? tb(i) : kExtPad;
+        uint32_t rw = tc == 0 ? rw0 : tc == 1 ? rw1 : tc == 2 ? rw2 : tc == 3 ? rw3 : tc == 4 ? rw4 : 0u;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) wl[c][r >> 3] |= ((rw >> (4 * c)) & 15u) << ((r & 7) * 4);
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int k = 0; k < NW; ++k) wt[c * cs + k] = wl[c][k];
+    }
+    const int jstart = i0 - w_max > 0 ? i0 - w_max : 0;
+    const int jfull = i0 + R - 1 - w_min;  // columns below this need the band mask
+    int hc[R], F[R];
+    int htop;  // diagonal input of row i0 at the next column
+    if (jstart == 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int b = h0 - o.o_del - o.e_del * (i0 + r + 1);
+        hc[r] = b > 0 ? b : 0;
+      }
+      const int b0 = h0 - o.o_del - o.e_del * i0;
+      htop = i0 == 0 ? h0 : (b0 > 0 ? b0 : 0);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) hc[r] = 0;
+      htop = (int)(rowbuf[(int64_t)(jstart - 1) * rs] & 0xfff);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) F[r] = 0;
+    int u = 0;
+    const int lim0 = w - i0;  // cell (i0+r, j) is in the band iff r <= j + lim0
+    for (int j = jstart; j < qmax; ++j) {
+      const uint32_t word = rowbuf[(int64_t)j * rs];
+      const int Ht = (int)(word & 0xfff), Et = (int)((word >> 12) & 0xfff);
+      const uint32_t qc = word >> 24;
+      uint32_t W[NW];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) W[k] = wt[qc * cs + k];
+      // M of every row from the previous column's H (bottom-up, in place)
+#pragma unroll
+      for (int r = R - 1; r >= 1; --r) {
+        const int hd = hc[r - 1];
+        const int m = hd + (int)((W[r >> 3] >> ((r & 7) * 4)) & 15) - 8;
+        hc[r] = hd ? m : 0;
+      }
+      {
+        const int m = htop + (int)(W[0] & 15) - 8;
+        hc[0] = htop ? m : 0;
+        htop = Ht;
+      }
+      int e = Et, ck = 0;
+      const bool masked = j < jfull;
+      const int lim = j + lim0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int m = hc[r], f = F[r];
+        int h = m > e ? m : e;
+        h = h > f ? h : f;
+        int e2 = e - o.e_del, t = m - oe_del;
+        e2 = e2 > t ? e2 : t;
+        e2 = e2 > 0 ? e2 : 0;
+        int f2 = f - o.e_ins;
+        t = m - oe_ins;
+        f2 = f2 > t ? f2 : t;
+        f2 = f2 > 0 ? f2 : 0;
+        if (masked && r > lim) {
+          h = 0;
+          f2 = 0;
+        }
+        hc[r] = h;
+        F[r] = f2;
+        e = e2;
+        const int key = (h << 5) | (31 - r);
+        ck = ck > key ? ck : key;
+      }
+      rowbuf[(int64_t)j * rs] = (uint32_t)hc[R - 1] | ((uint32_t)e << 12) | (qc << 24);
+      const int ch = ck >> 5, ci = i0 + 31 - (ck & 31);
+      if (ch > best || (ch == best && ci <= bi)) {
+        best = ch;
+        bi = ci;
+        bj = j;
+      }
+      const int hl = hc[R - 1], ub = hl + ms * (qlen - 1 - j);
+      if (hl > 0 && ub > u) u = ub;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = i0 + r, lo = i - w > 0 ? i - w : 0;
+      if (i < tlen && qlen > lo) cells += qlen - lo;
+    }
+    i0 += R;
+    if (i0 >= tlen) break;
+    const int b = h0 - o.o_del - o.e_del * i0;
+    const int ub = b > 0 ? b + ms * qlen : 0;
+    if ((u > ub ? u : ub) < best) break;
+  }
+  ExtLaneOut out;
+  out.score = best;
+  out.qle = bj + 1;
+  out.tle = bi + 1;
+  out.redo = best == h0 && (i0 < tlen ? i0 : tlen) >= 32;
+  out.cells = cells;
+  return out;
+}
+
+}  // namespace g3a

@@ -1,6 +1,7 @@
 // Kernel argument blocks and launch entry points (g3a_kernels.hip).
 #pragma once
 #include "g3a_dev.h"
+#include "g3a_ext.h"
 
 #include <hip/hip_runtime.h>
 
@@ -77,6 +78,42 @@ struct ExtArgs {
   int64_t *cells;
 };
 
+// Result of one side of one extension task.
+struct ExtRes {
+  int32_t score, qle, tle, pad;
+};
+
+// One side (0 = left, 1 = right) of the extension stage, batched form:
+// classify -> sort lane tasks by descending query length -> lane engine
+// (persistent, chunks of 64 from a device cursor) -> one wavefront per
+// remaining task (persistent) -> finish.
+struct ExtSideArgs {
+  DevIndex idx;
+  Opt opt;
+  int side;
+  int lane_opt_ok;        // scoring admitted by the lane engine
+  int64_t n_tasks;
+  const uint8_t *seq;
+  const int32_t *off;
+  const Task *tasks;
+  const ExtRes *left;     // left results: h0 of the right side
+  ExtRes *res;            // this side's results
+  uint32_t *key;          // sort key: query length of lane tasks, 0 otherwise
+  int32_t *tid;           // sort values: task index
+  const int32_t *sorted;  // task indices by descending key
+  int32_t *slow_list;     // tasks for the one-wavefront kernel
+  int32_t *ctr;           // [0] lane tasks [1] slow tasks [2] lane chunk cursor
+  uint32_t *rowbuf;       // kExtQCap * 64 words per lane-kernel block
+  int64_t *cells;
+};
+
+struct ExtFinishArgs {
+  int64_t n_tasks;
+  const Task *tasks;
+  const ExtRes *left, *right;
+  Region *regs;
+};
+
 struct RegionArgs {
   Opt opt;
   int n_reads;
@@ -119,6 +156,16 @@ void launch_sort(const SortArgs &a, hipStream_t s);
 void launch_chain(const ChainArgs &a, hipStream_t s);
 void launch_task_compact(const CompactArgs &a, hipStream_t s);
 void launch_extend(const ExtArgs &a, hipStream_t s);
+void launch_ext_classify(const ExtSideArgs &a, hipStream_t s);
+void launch_ext_lane(const ExtSideArgs &a, int strip_rows, int n_blocks, hipStream_t s);
+void launch_ext_wave(const ExtSideArgs &a, int n_blocks, hipStream_t s);
+void launch_ext_finish(const ExtFinishArgs &a, hipStream_t s);
+void launch_fill_i32(int32_t *p, int n, int32_t v, hipStream_t s);
+void launch_clamp_codes(uint8_t *seq, int64_t n, hipStream_t s);
+// descending radix sort of (key < 512, value) pairs
+size_t ext_sort_tmp_bytes(int64_t n);
+int ext_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,
+             size_t tmp_bytes, hipStream_t s);
 void launch_region_filter(const RegionArgs &a, hipStream_t s);
 void launch_aln_prep(const AlnPrepArgs &a, hipStream_t s);
 void launch_traceback(const TbArgs &a, hipStream_t s);

@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 
@@ -407,6 +408,217 @@ __global__ void __launch_bounds__(64) k_extend(ExtArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// banded local extension, batched form: one side at a time, tasks binned by
+// query length, lane engine (g3a_ext.h) for the tasks it admits and one
+// wavefront per task (extend_wave) for the rest.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    int t = __shfl_xor(v, o);
+    v = v < t ? v : t;
+  }
+  return v;
+}
+
+// Slot of this lane in a wave-aggregated append (one atomic per wavefront).
+// Every lane of the wavefront must call it.
+__device__ __forceinline__ int wave_reserve(bool pred, int32_t *ctr) {
+  const uint64_t m = __ballot(pred);
+  if (!m) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(ctr, __popcll(m));
+  base = __shfl(base, leader);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1)) : -1;
+}
+
+// Inputs of one side of task t (ExtendingPairGenerate, bwa/src/aux.cu:2101-2192;
+// localExtending, aux.cu:2289-2386): left = reversed read prefix against the
+// reversed reference before the seed, right = read suffix against the
+// reference after the seed; right h0 = left score.
+struct SideTask {
+  int qlen, tlen, h0, end_bonus;
+  const uint8_t *read;
+  int qbeg, qend;
+  int64_t rbeg, rend;
+};
+
+__device__ __forceinline__ SideTask side_task(const ExtSideArgs &a, int64_t t) {
+  const Task tk = a.tasks[t];
+  const int off = a.off[tk.read], l_seq = a.off[tk.read + 1] - off;
+  SideTask st;
+  st.read = a.seq + off;
+  st.qbeg = tk.qbeg;
+  st.qend = tk.qbeg + tk.len;
+  st.rbeg = tk.rbeg;
+  st.rend = tk.rbeg + tk.len;
+  if (a.side == 0) {
+    st.qlen = tk.qbeg;
+    st.tlen = (int)(tk.rbeg - tk.rmax0);
+    st.h0 = tk.len * a.opt.a;
+    st.end_bonus = a.opt.pen_clip5;
+  } else {
+    st.qlen = l_seq - tk.qbeg - tk.len;
+    st.tlen = (int)(tk.rmax1 - tk.rbeg - tk.len);
+    st.h0 = a.left[t].score;
+    st.end_bonus = a.opt.pen_clip3;
+  }
+  return st;
+}
+
+__global__ void __launch_bounds__(256) k_ext_classify(ExtSideArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = t < a.n_tasks;
+  bool lane = false, slow = false;
+  if (in) {
+    const SideTask st = side_task(a, t);
+    if (st.qlen <= 0) {
+      ExtRes r{st.h0, 0, 0, 0};
+      a.res[t] = r;
+    } else {
+      const int w = ext_band_w(a.opt, st.qlen, st.end_bonus);
+      lane = a.lane_opt_ok && ext_lane_ok(a.opt, st.qlen, w, st.h0);
+      slow = !lane;
+    }
+    a.key[t] = lane ? (uint32_t)st.qlen : 0u;
+    a.tid[t] = (int32_t)t;
+  }
+  wave_reserve(lane, a.ctr);
+  const int slot = wave_reserve(slow, a.ctr + 1);
+  if (slow) a.slow_list[slot] = (int32_t)t;
+}
+
+template <int R>
+__global__ void __launch_bounds__(64) k_ext_lane(ExtSideArgs a) {
+  constexpr int NW = R / 8;
+  __shared__ uint32_t wt[6 * 64 * NW];
+  const int lane = threadIdx.x;
+  uint32_t *rb = a.rowbuf + (size_t)blockIdx.x * (kExtQCap * 64) + lane;
+  const int n_lane = a.ctr[0];
+  int64_t cells = 0;
+  for (;;) {
+    int chunk = 0;
+    if (lane == 0) chunk = atomicAdd(a.ctr + 2, 1);
+    chunk = __shfl(chunk, 0);
+    if ((int64_t)chunk * 64 >= n_lane) break;
+    const int64_t k = (int64_t)chunk * 64 + lane;
+    const bool valid = k < n_lane;
+    const int64_t t = valid ? a.sorted[k] : 0;
+    SideTask st;
+    int w = 0;
+    if (valid) {
+      st = side_task(a, t);
+      w = ext_band_w(a.opt, st.qlen, st.end_bonus);
+    }
+    const int qmax = wave_max_i32(valid ? st.qlen : 0);
+    const int wmin = wave_min_i32(valid ? w : 1 << 20);
+    const int wmax = wave_max_i32(valid ? w : 0);
+    if (valid) {
+      // everything the accessors need, by value (a capture by reference
+      // would move the kernel arguments to private memory)
+      const uint8_t *rd = st.read;
+      const uint8_t *pac = a.idx.pac;
+      const int64_t l_pac = a.idx.l_pac;
+      const int qbeg = st.qbeg, qend = st.qend;
+      const int64_t rbeg = st.rbeg, rend = st.rend;
+      ExtLaneOut r;
+      if (a.side == 0)
+        r = ext_lane<R>(
+            a.opt, st.qlen, [=](int j) { return (int)rd[qbeg - 1 - j]; }, st.tlen,
+            [=](int i) { return text_base_p(pac, l_pac, rbeg - 1 - i); }, w, st.h0, qmax, wmin, wmax, rb, 64,
+            wt + lane * NW, 64 * NW);
+      else
+        r = ext_lane<R>(
+            a.opt, st.qlen, [=](int j) { return (int)rd[qend + j]; }, st.tlen,
+            [=](int i) { return text_base_p(pac, l_pac, rend + i); }, w, st.h0, qmax, wmin, wmax, rb, 64,
+            wt + lane * NW, 64 * NW);
+      ExtRes o{r.score, r.qle, r.tle, 0};
+      a.res[t] = o;
+      cells += r.cells;
+      if (r.redo) a.slow_list[atomicAdd(a.ctr + 1, 1)] = (int32_t)t;
+    }
+  }
+  if (a.cells) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) cells += __shfl_xor(cells, o);
+    if (lane == 0 && cells) atomicAdd((unsigned long long *)a.cells, (unsigned long long)cells);
+  }
+}
+
+__global__ void __launch_bounds__(64) k_ext_wave(ExtSideArgs a) {
+  __shared__ uint8_t qbuf[kQMax];
+  __shared__ int Hbuf[2 * (kQMax + 2)], Ebuf[2 * (kQMax + 2)];
+  const int lane = threadIdx.x;
+  const int n_slow = a.ctr[1];
+  int64_t cells = 0;
+  for (int e = blockIdx.x; e < n_slow; e += gridDim.x) {
+    const int64_t t = a.slow_list[e];
+    const SideTask st = side_task(a, t);
+    const uint8_t *rd = st.read;
+    if (a.side == 0)
+      for (int j = lane; j < st.qlen; j += 64) qbuf[j] = rd[st.qbeg - 1 - j];
+    else
+      for (int j = lane; j < st.qlen; j += 64) qbuf[j] = rd[st.qend + j];
+    __syncthreads();
+    const DevIndex &x = a.idx;
+    const int64_t rbeg = st.rbeg, rend = st.rend;
+    ExtOut r;
+    if (a.side == 0)
+      r = extend_wave(
+          a.opt, st.qlen, qbuf, st.tlen, [&](int i) { return text_base(x, rbeg - 1 - i); }, st.h0, st.end_bonus, Hbuf,
+          Ebuf, &cells);
+    else
+      r = extend_wave(
+          a.opt, st.qlen, qbuf, st.tlen, [&](int i) { return text_base(x, rend + i); }, st.h0, st.end_bonus, Hbuf,
+          Ebuf, &cells);
+    if (lane == 0) {
+      ExtRes o{r.score, r.qle, r.tle, 0};
+      a.res[t] = o;
+    }
+    __syncthreads();
+  }
+  if (a.cells) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) cells += __shfl_xor(cells, o);
+    if (lane == 0 && cells) atomicAdd((unsigned long long *)a.cells, (unsigned long long)cells);
+  }
+}
+
+// Region of each task from its two sides (localExtending, aux.cu:2326-2370).
+__global__ void __launch_bounds__(256) k_ext_finish(ExtFinishArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_tasks) return;
+  const Task tk = a.tasks[t];
+  const ExtRes l = a.left[t], r = a.right[t];
+  Region g;
+  g.rid = tk.rid;
+  g.seedlen0 = tk.len;
+  g.secondary = 0;
+  g.qb = tk.qbeg - l.qle;
+  g.rb = tk.rbeg - l.tle;
+  g.qe = tk.qbeg + tk.len + r.qle;
+  g.re = tk.rbeg + tk.len + r.tle;
+  g.score = r.score;
+  a.regs[t] = g;
+}
+
+// Input codes above 4 (anything but A, C, G, T) become 4.
+__global__ void k_clamp_codes(uint8_t *seq, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && seq[i] > 4) seq[i] = 4;
+}
+void launch_clamp_codes(uint8_t *seq, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_clamp_codes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, seq, n);
+}
+
+__global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
 // region filter, alignment list, traceback + finalize
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) k_region_filter(RegionArgs a) {
@@ -477,6 +689,23 @@ void launch_task_compact(const CompactArgs &a, hipStream_t s) {
 void launch_extend(const ExtArgs &a, hipStream_t s) {
   if (a.n_tasks) hipLaunchKernelGGL(k_extend, dim3((unsigned)a.n_tasks), dim3(64), 0, s, a);
 }
+void launch_ext_classify(const ExtSideArgs &a, hipStream_t s) {
+  if (a.n_tasks) hipLaunchKernelGGL(k_ext_classify, dim3(blocks(a.n_tasks, 256)), dim3(256), 0, s, a);
+}
+void launch_ext_lane(const ExtSideArgs &a, int strip_rows, int n_blocks, hipStream_t s) {
+  if (!a.n_tasks) return;
+  if (strip_rows == 32) hipLaunchKernelGGL(k_ext_lane<32>, dim3(n_blocks), dim3(64), 0, s, a);
+  else hipLaunchKernelGGL(k_ext_lane<16>, dim3(n_blocks), dim3(64), 0, s, a);
+}
+void launch_ext_wave(const ExtSideArgs &a, int n_blocks, hipStream_t s) {
+  if (a.n_tasks) hipLaunchKernelGGL(k_ext_wave, dim3(n_blocks), dim3(64), 0, s, a);
+}
+void launch_ext_finish(const ExtFinishArgs &a, hipStream_t s) {
+  if (a.n_tasks) hipLaunchKernelGGL(k_ext_finish, dim3(blocks(a.n_tasks, 256)), dim3(256), 0, s, a);
+}
+void launch_fill_i32(int32_t *p, int n, int32_t v, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, s, p, n, v);
+}
 void launch_region_filter(const RegionArgs &a, hipStream_t s) {
   if (a.n_reads) hipLaunchKernelGGL(k_region_filter, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
 }
@@ -543,6 +772,21 @@ void scan_i32(const int32_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_
   if (n > 0)
     rocprim::inclusive_scan(tmp, tmp_bytes, rocprim::make_transform_iterator(in, WidenI32()), out + 1, (size_t)n,
                             rocprim::plus<int64_t>(), s);
+}
+
+size_t ext_sort_tmp_bytes(int64_t n) {
+  size_t b = 0;
+  rocprim::radix_sort_pairs_desc(nullptr, b, (const uint32_t *)nullptr, (uint32_t *)nullptr, (const int32_t *)nullptr,
+                                 (int32_t *)nullptr, (size_t)(n > 0 ? n : 1), 0, 9);
+  return b + 256;
+}
+int ext_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,
+             size_t tmp_bytes, hipStream_t s) {
+  if (n <= 0) return 0;
+  return rocprim::radix_sort_pairs_desc(tmp, tmp_bytes, key_in, key_out, val_in, val_out, (size_t)n, 0, 9, s) ==
+                 hipSuccess
+             ? 0
+             : -1;
 }
 
 }  // namespace g3a

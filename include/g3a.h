@@ -106,7 +106,29 @@ void g3a_batch_out_free(g3a_batch_out *out);
  * reference counterpart: the reference's per-stage timers are the
  * CUDA-event tprof table (bwa/src/bwamem.cu:31-49). */
 int g3a_last_stats(const g3a_ctx *ctx, float ms[6], int64_t work[6]);
+/* Device time (ms) of single kernels of the last g3a_bwa_align, from events
+ * recorded on the context's stream around each launch: [0] k_seed, [1] the
+ * lane extension engine (both sides; 0 in the plain form), [2] one-wavefront
+ * extension (both sides, or the plain form's k_extend), [3] k_traceback.
+ * No reference counterpart. */
+int g3a_kernel_ms(const g3a_ctx *ctx, float ms[4]);
 void g3a_destroy(g3a_ctx *ctx);
+
+/* Execution form of the extension stage.  Both forms give identical output.
+ *   G3A_FORM_PLAIN    one wavefront per extension task (anti-diagonal DP)
+ *   G3A_FORM_BATCHED  tasks binned by query length; one task per lane in
+ *                     persistent kernels pulling 64-task chunks from a
+ *                     device cursor; tasks outside the lane engine's exact
+ *                     domain fall back to one wavefront per task
+ * No reference counterpart (the reference has one form, ksw_extend_warp2
+ * under localExtending, bwa/src/aux.cu:2289-2386).  Default: BATCHED. */
+#define G3A_FORM_PLAIN 0
+#define G3A_FORM_BATCHED 1
+int g3a_set_form(g3a_ctx *ctx, int form);
+/* Tuning of the batched form: rows per register strip (16 or 32) and
+ * persistent blocks per compute unit for the lane and wavefront kernels.
+ * No reference counterpart. */
+int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wave_blocks_per_cu);
 const char *g3a_last_error(const g3a_ctx *ctx);
 
 /* Index construction (the `g3a index` command): BWA-format files plus the

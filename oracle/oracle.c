@@ -1597,7 +1597,11 @@ static void *worker(void *arg) {
       /* DR-20: reads longer than MAX_LEN_READ (bwa/src/macro.h:27) overflow
        * the reference's shared arrays; here they are left unaligned. */
       if (len > 320) continue;
-      align_read(j->x, j->opt, j->seq + j->off[i], len, &j->outs[i]);
+      /* codes above 4 (no base) are N, as the reference's encoder maps every
+       * non-ACGT character (pipeline.cpp:90-107) */
+      uint8_t rd[320];
+      for (int k = 0; k < len; ++k) rd[k] = j->seq[j->off[i] + k] > 4 ? 4 : j->seq[j->off[i] + k];
+      align_read(j->x, j->opt, rd, len, &j->outs[i]);
     }
   }
   return NULL;

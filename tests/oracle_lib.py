@@ -126,11 +126,11 @@ class Oracle:
                                      C.byref(nc), cig.ctypes.data, cap)
         return s, cig[: nc.value].copy()
 
-    def align(self, idx, seq: np.ndarray, offsets: np.ndarray, threads: int = 4):
+    def align(self, idx, seq: np.ndarray, offsets: np.ndarray, threads: int = 4, opt: OrcOpt | None = None):
         seq = np.ascontiguousarray(seq, dtype=np.uint8)
         offsets = np.ascontiguousarray(offsets, dtype=np.int32)
         b = OrcBatch()
-        rc = self.lib.orc_align_batch(idx, C.byref(self.opt), len(offsets) - 1, seq.ctypes.data, offsets.ctypes.data,
+        rc = self.lib.orc_align_batch(idx, C.byref(opt if opt is not None else self.opt), len(offsets) - 1, seq.ctypes.data, offsets.ctypes.data,
                                       threads, C.byref(b))
         if rc != 0:
             raise RuntimeError("oracle alignment failed")

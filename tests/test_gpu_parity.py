@@ -7,6 +7,8 @@ reference with paired reads aligned as single-end, and edge-case batches.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -38,20 +40,28 @@ def compare(gpu: g3sa_amd.Result, ref: dict, label: str) -> None:
         assert np.array_equal(gk, rk), f"{label}: global CIGAR of alignment {i}"
 
 
+# (form, tuning): the plain form and the batched form at both strip heights
+VARIANTS = [("plain", None), ("batched", (16, 8, 8)), ("batched", (32, 4, 8))]
+
+
 def run_both(ds, oracle, seq, off, label, max_reads=None):
     n = len(off) - 1
-    print(f"[parity] {label}: {n} reads", flush=True)
-    with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(max_reads or n, 1),
-                          max_bases=max(int(off[-1]), 1)) as al:
-        got = al.align(seq, off)
     idx = oracle.load(ds.prefix)
     try:
         ref = oracle.align(idx, seq, off, threads=oracle_threads())
     finally:
         oracle.lib.orc_index_free(idx)
-    compare(got, ref, label)
-    # seeds resolved by the GPU equal the oracle's SA lookups
-    assert got.work["n_seeds"] == ref["work"]["sa_reads"], label
+    with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(max_reads or n, 1),
+                          max_bases=max(int(off[-1]), 1)) as al:
+        for form, tuning in VARIANTS:
+            print(f"[parity] {label}: {n} reads, form {form} {tuning}", flush=True)
+            al.set_form(form)
+            if tuning:
+                al.set_tuning(*tuning)
+            got = al.align(seq, off)
+            compare(got, ref, f"{label} [{form} {tuning}]")
+            # seeds resolved by the GPU equal the oracle's SA lookups
+            assert got.work["n_seeds"] == ref["work"]["sa_reads"], label
     return got, ref
 
 
@@ -110,6 +120,43 @@ def test_single_read_and_repeated_batches(ds_c1, oracle):
                 compare(got, ref, f"batch {lo}:{hi}")
     finally:
         oracle.lib.orc_index_free(idx)
+
+
+def _with_scoring(oracle, w=None, b=None):
+    """Matching option blocks for the oracle and the library."""
+    oo = type(oracle.opt)()
+    C.pointer(oo)[0] = oracle.opt
+    go = g3sa_amd.default_opt()
+    for o in (oo, go):
+        if w is not None:
+            o.w = w
+        if b is not None:
+            o.b = b
+            for i in range(4):
+                for j in range(4):
+                    o.mat[i * 5 + j] = o.a if i == j else -b
+    return oo, go
+
+
+@pytest.mark.parametrize("w,b", [(20, None), (None, 9)])
+def test_scoring_variants_parity(ds_c1, oracle, w, b):
+    """A narrow band (the lane engine's column-limit and corner conditions
+    bind for many tasks) and a mismatch score outside the lane engine's
+    4-bit range (every task takes the one-wavefront kernel)."""
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["c3"]), limit=1500)
+    oo, go = _with_scoring(oracle, w, b)
+    idx = oracle.load(ds_c1.prefix)
+    try:
+        ref = oracle.align(idx, seq, off, threads=oracle_threads(), opt=oo)
+    finally:
+        oracle.lib.orc_index_free(idx)
+    with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=len(off) - 1, max_bases=int(off[-1]), opt=go) as al:
+        for form, tuning in VARIANTS:
+            print(f"[parity] scoring w={w} b={b}: form {form} {tuning}", flush=True)
+            al.set_form(form)
+            if tuning:
+                al.set_tuning(*tuning)
+            compare(al.align(seq, off), ref, f"scoring w={w} b={b} [{form} {tuning}]")
 
 
 def test_empty_batch(ds_c1):
