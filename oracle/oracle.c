@@ -621,6 +621,7 @@ typedef struct {
   int64_t pos;
   int n, m, rid, is_alt;
   int w;
+  int id; /* B-tree-order index (stage dumps) */
   seed_t *seeds;
 } chain_t;
 
@@ -754,13 +755,16 @@ static int test_and_merge(const orc_opt *opt, int64_t l_pac, chain_t *c, const s
   if (yd >= 0 && xd - yd <= opt->w && yd - xd <= opt->w && xd - last->len < opt->max_chain_gap &&
       yd - last->len < opt->max_chain_gap) {
     chain_push(c, p);
-    return 1;
+    return 2;
   }
   return 0;
 }
 
-/* returns chains in B-tree order */
-static chain_t *chain_seeds(const orc_index *x, const orc_opt *opt, const seed_t *a, int n, int *n_chains) {
+/* returns chains in B-tree order; if chain_of_seed is given, it receives for
+ * each seed the B-tree-order index of the chain that took it, -1 when the
+ * seed was contained in a chain (and dropped) or started none (rid rule) */
+static chain_t *chain_seeds(const orc_index *x, const orc_opt *opt, const seed_t *a, int n, int *n_chains,
+                            int32_t *chain_of_seed) {
   kbtree_t b;
   memset(&b, 0, sizeof b);
   int m_ch = 16, n_ch = 0;
@@ -769,12 +773,16 @@ static chain_t *chain_seeds(const orc_index *x, const orc_opt *opt, const seed_t
   for (int i = 0; i < n; ++i) {
     seed_t s = a[i];
     int start_new = 0;
+    if (chain_of_seed) chain_of_seed[i] = -1;
     if (b.n_keys) {
       int lower = kb_lower(&b, s.rbeg);
-      if (lower < 0 || !test_and_merge(opt, x->l_pac, &b.chains[lower], &s)) start_new = 1;
+      int merged = lower < 0 ? 0 : test_and_merge(opt, x->l_pac, &b.chains[lower], &s);
+      if (!merged) start_new = 1;
+      else if (merged == 2 && chain_of_seed) chain_of_seed[i] = lower;
     } else start_new = 1;
     if (start_new) {
       if (s.rid < 0 || s.rid >= 455) continue; /* bwa/src/aux.cu:1799 */
+      if (chain_of_seed) chain_of_seed[i] = n_ch;
       if (n_ch == m_ch) {
         m_ch *= 2;
         b.chains = (chain_t *)realloc(b.chains, (size_t)m_ch * sizeof(chain_t));
@@ -795,6 +803,13 @@ static chain_t *chain_seeds(const orc_index *x, const orc_opt *opt, const seed_t
   if (n_ch) kb_traverse(&b, b.root, order, &k);
   chain_t *out = (chain_t *)xmalloc((size_t)(n_ch ? n_ch : 1) * sizeof(chain_t));
   for (int i = 0; i < k; ++i) out[i] = b.chains[order[i]];
+  if (chain_of_seed) {
+    int *rank = (int *)xmalloc((size_t)(n_ch ? n_ch : 1) * sizeof(int));
+    for (int i = 0; i < k; ++i) rank[order[i]] = i;
+    for (int i = 0; i < n; ++i)
+      if (chain_of_seed[i] >= 0) chain_of_seed[i] = rank[chain_of_seed[i]];
+    free(rank);
+  }
   free(order);
   free(b.chains);
   free(b.nodes);
@@ -881,6 +896,33 @@ static int filter_chains(const orc_opt *opt, chain_t *a, int n) {
   }
   free(kept);
   return k;
+}
+
+/* Stage dump for tests: chaining and chain filtering of one read's seeds
+ * (already in sort_seeds order).  chain_of_seed[i]: chain (B-tree order) that
+ * took seed i, or -1.  weight[c]: chain_weight of chain c (B-tree order).
+ * kept[k]: B-tree index of the k-th chain after sorting and filtering.
+ * Returns the number of chains; *n_kept receives the number kept. */
+int orc_chain_dump(const orc_index *x, const orc_opt *opt, const orc_seed *seeds, int n, int32_t *chain_of_seed,
+                   int32_t *weight, int32_t *kept, int *n_kept) {
+  seed_t *a = (seed_t *)xmalloc((size_t)(n ? n : 1) * sizeof(seed_t));
+  for (int i = 0; i < n; ++i) {
+    a[i].rbeg = seeds[i].rbeg; a[i].qbeg = seeds[i].qbeg; a[i].len = seeds[i].len; a[i].rid = seeds[i].rid;
+    a[i].score = seeds[i].len;
+  }
+  int n_ch = 0;
+  chain_t *ch = n ? chain_seeds(x, opt, a, n, &n_ch, chain_of_seed) : NULL;
+  free(a);
+  for (int c = 0; c < n_ch; ++c) {
+    ch[c].id = c;
+    weight[c] = chain_weight(&ch[c]);
+  }
+  int k = filter_chains(opt, ch, n_ch);
+  for (int i = 0; i < k; ++i) kept[i] = ch[i].id;
+  for (int i = 0; i < k; ++i) free(ch[i].seeds);
+  free(ch);
+  *n_kept = k;
+  return n_ch;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1307,7 +1349,7 @@ static void align_read(const orc_index *x, const orc_opt *opt, const uint8_t *re
   free(mem.a);
   sort_seeds(seeds, n_seeds);
   int n_ch = 0;
-  chain_t *ch = n_seeds ? chain_seeds(x, opt, seeds, n_seeds, &n_ch) : NULL;
+  chain_t *ch = n_seeds ? chain_seeds(x, opt, seeds, n_seeds, &n_ch, NULL) : NULL;
   free(seeds);
   n_ch = filter_chains(opt, ch, n_ch);
   /* SWSeed + ExtendingPairGenerate + localExtending, aux.cu:2041-2386 */

@@ -94,6 +94,10 @@ int orc_seed_intervals(const orc_index *idx, const orc_opt *opt, const uint8_t *
 int orc_read_seeds(const orc_index *idx, const orc_opt *opt, const uint8_t *read, int len,
                    orc_seed *out, int cap);
 
+/* Chaining + chain filter of one read's sorted seeds (stage dump). */
+int orc_chain_dump(const orc_index *idx, const orc_opt *opt, const orc_seed *seeds, int n, int32_t *chain_of_seed,
+                   int32_t *weight, int32_t *kept, int *n_kept);
+
 /* Align a batch: reads are 2-bit codes (A0 C1 G2 T3, other 4) concatenated
  * with offsets[n+1].  Returns 0 on success. */
 int orc_align_batch(const orc_index *idx, const orc_opt *opt, int n, const uint8_t *seq,
