@@ -1271,6 +1271,7 @@ int orc_mapq(const orc_opt *opt, int score, int qb, int qe, int64_t rb, int64_t 
 typedef struct {
   int64_t rb, re;
   int qb, qe, rid, score, seedlen0, secondary, is_alt;
+  int idx; /* position in the read's task order (stage dumps) */
 } reg_t;
 
 static void cigar_append(uint32_t **buf, int64_t *n, int64_t *m, const uint32_t *src, int k) {
@@ -1338,6 +1339,114 @@ static void out_push(read_out_t *o, const orc_aln *a) {
     if (!o->a) abort();
   }
   o->a[o->n++] = *a;
+}
+
+/* filterRegions (aux.cu:2408-2548) + sortRegions (aux.cu:2552-2613) over one
+ * read's regions in task order; frees reg and returns the kept regions in
+ * output order (their `idx` is the position in the input). */
+static reg_t *filter_regions(const orc_index *x, const orc_opt *opt, reg_t *reg, int n_reg, int *n_out) {
+  for (int i = 0; i < n_reg; ++i) reg[i].idx = i;
+  /* filterRegions.
+   * DR: the reference's threads race on the keep/secondary flags; the oracle
+   * uses the sequential order of anchors (ascending index), and writes kept
+   * regions in ascending index order (DR-8). */
+  uint8_t *kept = (uint8_t *)xmalloc((size_t)(n_reg ? n_reg : 1));
+  for (int i = 0; i < n_reg; ++i) kept[i] = reg[i].score >= opt->T;
+  for (int i = 0; i < n_reg; ++i) {
+    if (!kept[i]) continue;
+    for (int j = i + 1; j < n_reg; ++j) {
+      if (!kept[j]) continue;
+      if (reg[i].rid != reg[j].rid) continue;
+      int64_t ai_rb = reg[i].rb, ai_re = reg[i].re, aj_rb = reg[j].rb, aj_re = reg[j].re;
+      if (ai_rb < aj_rb && ai_re < aj_rb) continue;
+      if (aj_rb < ai_rb && aj_re < ai_rb) continue;
+      int or_ = (int)(ai_rb < aj_rb ? ai_re - aj_rb : aj_re - ai_rb);
+      int oq = reg[i].qb < reg[j].qb ? reg[i].qe - reg[j].qb : reg[j].qe - reg[i].qb;
+      int rli = (int)(ai_re - ai_rb), rlj = (int)(aj_re - aj_rb);
+      int mr = rli < rlj ? rli : rlj;
+      int mq = (reg[i].qe - reg[i].qb < reg[j].qe - reg[j].qb) ? reg[i].qe - reg[i].qb : reg[j].qe - reg[j].qb;
+      if (or_ > opt->mask_level_redun * mr && oq > opt->mask_level_redun * mq) {
+        if (reg[i].score < reg[j].score) { kept[i] = 0; break; }
+        else kept[j] = 0;
+      }
+    }
+  }
+  int m = 0;
+  for (int i = 0; i < n_reg; ++i)
+    if (kept[i]) reg[m++] = reg[i];
+  free(kept);
+  for (int i = 0; i < m; ++i) reg[i].is_alt = (reg[i].rid >= 0 && x->contigs[reg[i].rid].is_alt) ? 1 : 0;
+  int *sec = (int *)xmalloc((size_t)(m ? m : 1) * sizeof(int));
+  for (int i = 0; i < m; ++i) sec[i] = -1;
+  for (int i = 0; i < m; ++i) {
+    for (int j = i + 1; j < m; ++j) {
+      if (sec[i] >= 0) continue;
+      if (sec[j] >= 0) continue;
+      int qb_max = reg[i].qb > reg[j].qb ? reg[i].qb : reg[j].qb;
+      int qe_min = reg[i].qe < reg[j].qe ? reg[i].qe : reg[j].qe;
+      if (qe_min > qb_max) {
+        int li = reg[i].qe - reg[i].qb, lj = reg[j].qe - reg[j].qb, min_l = li < lj ? li : lj;
+        if (qe_min - qb_max >= min_l * opt->mask_level) {
+          if (reg[i].score > reg[j].score) {
+            if (reg[i].is_alt && reg[j].is_alt) sec[j] = i;
+            else if (!reg[i].is_alt) sec[j] = i;
+          } else {
+            if (reg[j].is_alt && reg[i].is_alt) sec[i] = j;
+            else if (!reg[j].is_alt) sec[i] = j;
+          }
+        }
+      }
+    }
+  }
+  int nk = 0;
+  reg_t *kr = (reg_t *)xmalloc((size_t)(m ? m : 1) * sizeof(reg_t));
+  for (int i = 0; i < m; ++i) {
+    if (sec[i] >= 0 && (reg[i].is_alt || !(opt->flag & 0x8))) continue;
+    if (sec[i] >= 0 && reg[i].score < reg[sec[i]].score * opt->drop_ratio) continue;
+    kr[nk] = reg[i];
+    kr[nk].secondary = sec[i];
+    ++nk;
+  }
+  free(sec);
+  free(reg);
+  /* sortRegions, aux.cu:2552-2613 */
+  if (nk == 2) {
+    if (kr[0].is_alt > kr[1].is_alt || (kr[0].is_alt <= kr[1].is_alt && kr[0].score < kr[1].score)) {
+      reg_t t = kr[0]; kr[0] = kr[1]; kr[1] = t;
+    }
+  } else if (nk > 2) {
+    for (int i = 0; i < nk - 1; ++i) {
+      int swapped = 0;
+      for (int j = 0; j < nk - 1 - i; ++j) {
+        if (kr[j].is_alt > kr[j + 1].is_alt || kr[j].score < kr[j + 1].score) {
+          reg_t t = kr[j]; kr[j] = kr[j + 1]; kr[j + 1] = t;
+          swapped = 1;
+        }
+      }
+      if (!swapped) break;
+    }
+  }
+  *n_out = nk;
+  return kr;
+}
+
+/* Stage dump for tests: the region filter and sort of one read's regions
+ * (rb, re, qb, qe, score, rid in task order).  out_idx[k] = input position of
+ * the k-th output region, out_sec[k] = its secondary anchor (an index into
+ * the filtered list before sorting, -1 if primary).  Returns the count. */
+int orc_region_dump(const orc_index *x, const orc_opt *opt, int n, const int64_t *rb, const int64_t *re,
+                    const int32_t *qb, const int32_t *qe, const int32_t *score, const int32_t *rid, int32_t *out_idx,
+                    int32_t *out_sec) {
+  reg_t *reg = (reg_t *)xcalloc((size_t)(n ? n : 1), sizeof(reg_t));
+  for (int i = 0; i < n; ++i) {
+    reg[i].rb = rb[i]; reg[i].re = re[i]; reg[i].qb = qb[i]; reg[i].qe = qe[i];
+    reg[i].score = score[i]; reg[i].rid = rid[i];
+  }
+  int nk = 0;
+  reg_t *kr = filter_regions(x, opt, reg, n, &nk);
+  for (int i = 0; i < nk; ++i) { out_idx[i] = kr[i].idx; out_sec[i] = kr[i].secondary; }
+  free(kr);
+  return nk;
 }
 
 static void align_read(const orc_index *x, const orc_opt *opt, const uint8_t *read, int len, read_out_t *o) {
@@ -1422,86 +1531,8 @@ static void align_read(const orc_index *x, const orc_opt *opt, const uint8_t *re
   }
   for (int i = 0; i < n_ch; ++i) free(ch[i].seeds);
   free(ch);
-  /* filterRegions, aux.cu:2408-2548.
-   * DR: the reference's threads race on the keep/secondary flags; the oracle
-   * uses the sequential order of anchors (ascending index), and writes kept
-   * regions in ascending index order (DR-8). */
-  uint8_t *kept = (uint8_t *)xmalloc((size_t)(n_reg ? n_reg : 1));
-  for (int i = 0; i < n_reg; ++i) kept[i] = reg[i].score >= opt->T;
-  for (int i = 0; i < n_reg; ++i) {
-    if (!kept[i]) continue;
-    for (int j = i + 1; j < n_reg; ++j) {
-      if (!kept[j]) continue;
-      if (reg[i].rid != reg[j].rid) continue;
-      int64_t ai_rb = reg[i].rb, ai_re = reg[i].re, aj_rb = reg[j].rb, aj_re = reg[j].re;
-      if (ai_rb < aj_rb && ai_re < aj_rb) continue;
-      if (aj_rb < ai_rb && aj_re < ai_rb) continue;
-      int or_ = (int)(ai_rb < aj_rb ? ai_re - aj_rb : aj_re - ai_rb);
-      int oq = reg[i].qb < reg[j].qb ? reg[i].qe - reg[j].qb : reg[j].qe - reg[i].qb;
-      int rli = (int)(ai_re - ai_rb), rlj = (int)(aj_re - aj_rb);
-      int mr = rli < rlj ? rli : rlj;
-      int mq = (reg[i].qe - reg[i].qb < reg[j].qe - reg[j].qb) ? reg[i].qe - reg[i].qb : reg[j].qe - reg[j].qb;
-      if (or_ > opt->mask_level_redun * mr && oq > opt->mask_level_redun * mq) {
-        if (reg[i].score < reg[j].score) { kept[i] = 0; break; }
-        else kept[j] = 0;
-      }
-    }
-  }
-  int m = 0;
-  for (int i = 0; i < n_reg; ++i)
-    if (kept[i]) reg[m++] = reg[i];
-  free(kept);
-  for (int i = 0; i < m; ++i) reg[i].is_alt = (reg[i].rid >= 0 && x->contigs[reg[i].rid].is_alt) ? 1 : 0;
-  int *sec = (int *)xmalloc((size_t)(m ? m : 1) * sizeof(int));
-  for (int i = 0; i < m; ++i) sec[i] = -1;
-  for (int i = 0; i < m; ++i) {
-    for (int j = i + 1; j < m; ++j) {
-      if (sec[i] >= 0) continue;
-      if (sec[j] >= 0) continue;
-      int qb_max = reg[i].qb > reg[j].qb ? reg[i].qb : reg[j].qb;
-      int qe_min = reg[i].qe < reg[j].qe ? reg[i].qe : reg[j].qe;
-      if (qe_min > qb_max) {
-        int li = reg[i].qe - reg[i].qb, lj = reg[j].qe - reg[j].qb, min_l = li < lj ? li : lj;
-        if (qe_min - qb_max >= min_l * opt->mask_level) {
-          if (reg[i].score > reg[j].score) {
-            if (reg[i].is_alt && reg[j].is_alt) sec[j] = i;
-            else if (!reg[i].is_alt) sec[j] = i;
-          } else {
-            if (reg[j].is_alt && reg[i].is_alt) sec[i] = j;
-            else if (!reg[j].is_alt) sec[i] = j;
-          }
-        }
-      }
-    }
-  }
   int nk = 0;
-  reg_t *kr = (reg_t *)xmalloc((size_t)(m ? m : 1) * sizeof(reg_t));
-  for (int i = 0; i < m; ++i) {
-    if (sec[i] >= 0 && (reg[i].is_alt || !(opt->flag & 0x8))) continue;
-    if (sec[i] >= 0 && reg[i].score < reg[sec[i]].score * opt->drop_ratio) continue;
-    kr[nk] = reg[i];
-    kr[nk].secondary = sec[i];
-    ++nk;
-  }
-  free(sec);
-  free(reg);
-  /* sortRegions, aux.cu:2552-2613 */
-  if (nk == 2) {
-    if (kr[0].is_alt > kr[1].is_alt || (kr[0].is_alt <= kr[1].is_alt && kr[0].score < kr[1].score)) {
-      reg_t t = kr[0]; kr[0] = kr[1]; kr[1] = t;
-    }
-  } else if (nk > 2) {
-    for (int i = 0; i < nk - 1; ++i) {
-      int swapped = 0;
-      for (int j = 0; j < nk - 1 - i; ++j) {
-        if (kr[j].is_alt > kr[j + 1].is_alt || kr[j].score < kr[j + 1].score) {
-          reg_t t = kr[j]; kr[j] = kr[j + 1]; kr[j + 1] = t;
-          swapped = 1;
-        }
-      }
-      if (!swapped) break;
-    }
-  }
+  reg_t *kr = filter_regions(x, opt, reg, n_reg, &nk);
   /* FINALIZEALN_preprocessing2 + traceback + finalize, aux.cu:2676-3059 */
   for (int ai = 0; ai < nk; ++ai) {
     const reg_t *g = &kr[ai];

@@ -98,6 +98,11 @@ int orc_read_seeds(const orc_index *idx, const orc_opt *opt, const uint8_t *read
 int orc_chain_dump(const orc_index *idx, const orc_opt *opt, const orc_seed *seeds, int n, int32_t *chain_of_seed,
                    int32_t *weight, int32_t *kept, int *n_kept);
 
+/* Region filter + sort of one read's regions (stage dump). */
+int orc_region_dump(const orc_index *idx, const orc_opt *opt, int n, const int64_t *rb, const int64_t *re,
+                    const int32_t *qb, const int32_t *qe, const int32_t *score, const int32_t *rid, int32_t *out_idx,
+                    int32_t *out_sec);
+
 /* Align a batch: reads are 2-bit codes (A0 C1 G2 T3, other 4) concatenated
  * with offsets[n+1].  Returns 0 on success. */
 int orc_align_batch(const orc_index *idx, const orc_opt *opt, int n, const uint8_t *seq,

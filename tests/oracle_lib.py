@@ -66,6 +66,7 @@ class Oracle:
         lib.orc_seed_intervals.argtypes = [C.c_void_p, C.POINTER(OrcOpt), C.c_void_p, C.c_int, C.POINTER(OrcIntv),
                                            C.c_int, C.POINTER(C.c_int64)]
         lib.orc_read_seeds.argtypes = [C.c_void_p, C.POINTER(OrcOpt), C.c_void_p, C.c_int, C.POINTER(OrcSeed), C.c_int]
+        lib.orc_region_dump.argtypes = [C.c_void_p, C.POINTER(OrcOpt), C.c_int] + [C.c_void_p] * 8
         lib.orc_chain_dump.argtypes = [C.c_void_p, C.POINTER(OrcOpt), C.POINTER(OrcSeed), C.c_int, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         lib.orc_align_batch.argtypes = [C.c_void_p, C.POINTER(OrcOpt), C.c_int, C.c_void_p, C.c_void_p, C.c_int,
