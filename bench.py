@@ -119,6 +119,12 @@ def run_form(al, n: int, steps: int, warmup: int, form: str) -> dict:
     for i in range(warmup):
         al.run()
         log(f"[{form}] warmup step {i + 1}/{warmup} done")
+    tuned = None
+    if form == "batched":
+        if warmup == 0:
+            al.run()
+        tuned = al.autotune()
+        log(f"[{form}] autotuned: {tuned}")
     stage_ms, kern_ms = [], []
     st = None
     t0 = time.perf_counter()
@@ -135,6 +141,7 @@ def run_form(al, n: int, steps: int, warmup: int, form: str) -> dict:
         "stages_ms": {k: round(float(np.mean([s[k] for s in stage_ms])), 3) for k in stage_ms[0]},
         "kernel_ms": {k: round(float(np.mean([s[k] for s in kern_ms])), 3) for k in kern_ms[0]},
         "work": st["work"],
+        "tuning": tuned,
     }
 
 
@@ -192,7 +199,11 @@ def main() -> int:
             "algorithmic_bytes": seed_bytes, "launch_ms": round(kms["k_seed"], 3)}
     if dominant != "k_seed":
         roof["dominant"] = dominant
-    gcups = work["ext_cells"] / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    # cells of the reference's extension cell set; the batched form counts
+    # only the cells it computes (pruning skips the rest), so take the plain
+    # form's count when it ran
+    ref_cells = per_form["plain"]["work"]["ext_cells"] if "plain" in per_form else work["ext_cells"]
+    gcups = ref_cells / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
     result = {
         "metric": "reads/s (byte-identical SAM)",
         "value": m["reads_per_s"],
@@ -211,6 +222,8 @@ def main() -> int:
         "kernel_ms": kms,
         "work": work,
         "extension_gcups": round(gcups, 2),
+        "extension_cells": {"reference_cell_set": ref_cells, "computed": work["ext_cells"]},
+        "tuning": m.get("tuning"),
         "forms": {f: {"reads_per_s": v["reads_per_s"], "ms_per_step": v["ms_per_step"], "kernel_ms": v["kernel_ms"],
                       "stages_ms": v["stages_ms"]} for f, v in per_form.items()},
         "roofline": roof,

@@ -63,7 +63,7 @@ class BatchOut(C.Structure):
 EXPORTS = [
     "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
-    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms",
+    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
 ]
 
 _lib = None
@@ -92,6 +92,7 @@ def load_library() -> C.CDLL:
     lib.g3a_last_stats.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int64)]
     lib.g3a_set_form.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_kernel_ms.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    lib.g3a_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     _lib = lib
     return lib
@@ -153,6 +154,12 @@ class Aligner:
     def set_form(self, form: str) -> None:
         """'plain' (one wavefront per extension task) or 'batched'."""
         self._check(self.lib.g3a_set_form(self.ctx, FORMS[form]))
+
+    def autotune(self) -> dict:
+        """Pick the batched form's fastest tile shape on the last aligned batch."""
+        sr, lb, ms = C.c_int(), C.c_int(), C.c_float()
+        self._check(self.lib.g3a_autotune(self.ctx, C.byref(sr), C.byref(lb), C.byref(ms)))
+        return {"strip_rows": sr.value, "lane_blocks_per_cu": lb.value, "extension_ms": ms.value}
 
     def set_tuning(self, strip_rows: int, lane_blocks_per_cu: int, wave_blocks_per_cu: int) -> None:
         self._check(self.lib.g3a_set_tuning(self.ctx, strip_rows, lane_blocks_per_cu, wave_blocks_per_cu))

@@ -47,6 +47,8 @@ struct g3a_ctx {
   int n_reads = 0;
   bool has_input = false, has_output = false;
   DevBuf seq, off;
+  DevBuf lep_pool;
+  int seed_threads = 0;
   // seeding
   DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
   // seeds / chains
@@ -278,6 +280,71 @@ int readback(g3a_ctx *ctx, std::initializer_list<const int64_t *> src) {
 
 }  // namespace
 
+static // The extension stage over the T_t tasks in ctx->tasks, into ctx->regs.
+int run_extension(g3a_ctx *ctx, int64_t T_t, int64_t *cells) {
+  hipStream_t s = ctx->stream;
+  if (ctx->form == G3A_FORM_PLAIN) {
+    ExtArgs ea{};
+    ea.idx = ctx->idx;
+    ea.opt = ctx->opt;
+    ea.n_tasks = T_t;
+    ea.seq = P<uint8_t>(ctx->seq);
+    ea.off = P<int32_t>(ctx->off);
+    ea.tasks = P<Task>(ctx->tasks);
+    ea.regs = P<Region>(ctx->regs);
+    ea.cells = cells;
+    HIPCK(hipEventRecord(ctx->kev[3], s));
+    launch_extend(ea, s);
+    HIPCK(hipEventRecord(ctx->kev[7], s));
+    ctx->kev_used[3] = 1;
+  } else if (T_t > 0) {
+    const int lane_blocks = ctx->n_cu * ctx->lane_blocks_per_cu, wave_blocks = ctx->n_cu * ctx->wave_blocks_per_cu;
+    if (ensure(ctx, ctx->ext_res[0], (size_t)T_t * sizeof(ExtRes)) ||
+        ensure(ctx, ctx->ext_res[1], (size_t)T_t * sizeof(ExtRes)) || ensure(ctx, ctx->ext_key, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->ext_key2, (size_t)T_t * 4) || ensure(ctx, ctx->ext_idx, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->ext_sorted, (size_t)T_t * 4) || ensure(ctx, ctx->ext_slow, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->ext_ctr, 2 * 4 * sizeof(int32_t)) ||
+        ensure(ctx, ctx->ext_sort_tmp, ext_sort_tmp_bytes(T_t)) ||
+        ensure(ctx, ctx->ext_rowbuf, (size_t)lane_blocks * kExtQCap * 64 * 4))
+      return G3A_E_HIP;
+    launch_fill_i32(P<int32_t>(ctx->ext_ctr), 8, 0, s);
+    for (int side = 0; side < 2; ++side) {
+      ExtSideArgs ea{};
+      ea.idx = ctx->idx;
+      ea.opt = ctx->opt;
+      ea.side = side;
+      ea.lane_opt_ok = ext_lane_opt_ok(ctx->opt);
+      ea.n_tasks = T_t;
+      ea.seq = P<uint8_t>(ctx->seq);
+      ea.off = P<int32_t>(ctx->off);
+      ea.tasks = P<Task>(ctx->tasks);
+      ea.left = P<ExtRes>(ctx->ext_res[0]);
+      ea.res = P<ExtRes>(ctx->ext_res[side]);
+      ea.key = P<uint32_t>(ctx->ext_key);
+      ea.tid = P<int32_t>(ctx->ext_idx);
+      ea.sorted = P<int32_t>(ctx->ext_sorted);
+      ea.slow_list = P<int32_t>(ctx->ext_slow);
+      ea.ctr = P<int32_t>(ctx->ext_ctr) + 4 * side;
+      ea.rowbuf = P<uint32_t>(ctx->ext_rowbuf);
+      ea.cells = cells;
+      launch_ext_classify(ea, s);
+      if (ext_sort(P<uint32_t>(ctx->ext_key), P<uint32_t>(ctx->ext_key2), P<int32_t>(ctx->ext_idx),
+                   P<int32_t>(ctx->ext_sorted), T_t, ctx->ext_sort_tmp.p, ctx->ext_sort_tmp.bytes, s))
+        return fail(ctx, G3A_E_HIP, "extension task sort failed");
+      HIPCK(hipEventRecord(ctx->kev[2 + 2 * side], s));
+      launch_ext_lane(ea, ctx->strip_rows, lane_blocks, s);
+      HIPCK(hipEventRecord(ctx->kev[3 + 2 * side], s));
+      launch_ext_wave(ea, wave_blocks, s);
+      HIPCK(hipEventRecord(ctx->kev[7 + 2 * side], s));
+      ctx->kev_used[1 + side] = 1;
+    }
+    ExtFinishArgs fa{T_t, P<Task>(ctx->tasks), P<ExtRes>(ctx->ext_res[0]), P<ExtRes>(ctx->ext_res[1]),
+                     P<Region>(ctx->regs)};
+    launch_ext_finish(fa, s);
+  }
+  return 0;
+}
+
 int g3a_bwa_align(g3a_ctx *ctx) {
   if (!ctx) return G3A_E_ARG;
   if (!ctx->index_on_device || !ctx->has_input) return fail(ctx, G3A_E_STATE, "index or input not on device");
@@ -312,9 +379,12 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.n_seed = P<int64_t>(ctx->n_seed);
   sa.slow = P<int32_t>(ctx->slow);
   sa.fm_steps = cnt + 0;
+  if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu);
+  if (ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * 2 * kLepCap * sizeof(Intv))) return G3A_E_HIP;
+  sa.lep_pool = P<Intv>(ctx->lep_pool);
   for (int &u : ctx->kev_used) u = 0;
   HIPCK(hipEventRecord(ctx->kev[0], s));
-  launch_seed(sa, s);
+  launch_seed(sa, ctx->seed_threads, s);
   HIPCK(hipEventRecord(ctx->kev[1], s));
   ctx->kev_used[0] = 1;
   launch_collect_slow(n, P<int32_t>(ctx->slow), P<int32_t>(ctx->slow_list), P<int32_t>(ctx->counters) + 2 * 8,
@@ -414,65 +484,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   launch_task_compact(cp, s);
   HIPCK(hipEventRecord(ctx->ev[3], s));
   // ---- extension
-  if (ctx->form == G3A_FORM_PLAIN) {
-    ExtArgs ea{};
-    ea.idx = ctx->idx;
-    ea.opt = ctx->opt;
-    ea.n_tasks = T_t;
-    ea.seq = P<uint8_t>(ctx->seq);
-    ea.off = P<int32_t>(ctx->off);
-    ea.tasks = P<Task>(ctx->tasks);
-    ea.regs = P<Region>(ctx->regs);
-    ea.cells = cnt + 24;
-    HIPCK(hipEventRecord(ctx->kev[3], s));
-    launch_extend(ea, s);
-    HIPCK(hipEventRecord(ctx->kev[7], s));
-    ctx->kev_used[3] = 1;
-  } else if (T_t > 0) {
-    const int lane_blocks = ctx->n_cu * ctx->lane_blocks_per_cu, wave_blocks = ctx->n_cu * ctx->wave_blocks_per_cu;
-    if (ensure(ctx, ctx->ext_res[0], (size_t)T_t * sizeof(ExtRes)) ||
-        ensure(ctx, ctx->ext_res[1], (size_t)T_t * sizeof(ExtRes)) || ensure(ctx, ctx->ext_key, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->ext_key2, (size_t)T_t * 4) || ensure(ctx, ctx->ext_idx, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->ext_sorted, (size_t)T_t * 4) || ensure(ctx, ctx->ext_slow, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->ext_ctr, 2 * 4 * sizeof(int32_t)) ||
-        ensure(ctx, ctx->ext_sort_tmp, ext_sort_tmp_bytes(T_t)) ||
-        ensure(ctx, ctx->ext_rowbuf, (size_t)lane_blocks * kExtQCap * 64 * 4))
-      return G3A_E_HIP;
-    launch_fill_i32(P<int32_t>(ctx->ext_ctr), 8, 0, s);
-    for (int side = 0; side < 2; ++side) {
-      ExtSideArgs ea{};
-      ea.idx = ctx->idx;
-      ea.opt = ctx->opt;
-      ea.side = side;
-      ea.lane_opt_ok = ext_lane_opt_ok(ctx->opt);
-      ea.n_tasks = T_t;
-      ea.seq = P<uint8_t>(ctx->seq);
-      ea.off = P<int32_t>(ctx->off);
-      ea.tasks = P<Task>(ctx->tasks);
-      ea.left = P<ExtRes>(ctx->ext_res[0]);
-      ea.res = P<ExtRes>(ctx->ext_res[side]);
-      ea.key = P<uint32_t>(ctx->ext_key);
-      ea.tid = P<int32_t>(ctx->ext_idx);
-      ea.sorted = P<int32_t>(ctx->ext_sorted);
-      ea.slow_list = P<int32_t>(ctx->ext_slow);
-      ea.ctr = P<int32_t>(ctx->ext_ctr) + 4 * side;
-      ea.rowbuf = P<uint32_t>(ctx->ext_rowbuf);
-      ea.cells = cnt + 24;
-      launch_ext_classify(ea, s);
-      if (ext_sort(P<uint32_t>(ctx->ext_key), P<uint32_t>(ctx->ext_key2), P<int32_t>(ctx->ext_idx),
-                   P<int32_t>(ctx->ext_sorted), T_t, ctx->ext_sort_tmp.p, ctx->ext_sort_tmp.bytes, s))
-        return fail(ctx, G3A_E_HIP, "extension task sort failed");
-      HIPCK(hipEventRecord(ctx->kev[2 + 2 * side], s));
-      launch_ext_lane(ea, ctx->strip_rows, lane_blocks, s);
-      HIPCK(hipEventRecord(ctx->kev[3 + 2 * side], s));
-      launch_ext_wave(ea, wave_blocks, s);
-      HIPCK(hipEventRecord(ctx->kev[7 + 2 * side], s));
-      ctx->kev_used[1 + side] = 1;
-    }
-    ExtFinishArgs fa{T_t, P<Task>(ctx->tasks), P<ExtRes>(ctx->ext_res[0]), P<ExtRes>(ctx->ext_res[1]),
-                     P<Region>(ctx->regs)};
-    launch_ext_finish(fa, s);
-  }
+  if (int rc = run_extension(ctx, T_t, cnt + 24)) return rc;
   HIPCK(hipEventRecord(ctx->ev[4], s));
   // ---- region filter, traceback, finalize
   RegionArgs ra{ctx->opt, n, P<int64_t>(ctx->task_off), P<Region>(ctx->regs), P<uint8_t>(ctx->flag),
@@ -615,6 +627,50 @@ int g3a_kernel_ms(const g3a_ctx *ctx, float ms[4]) {
   return G3A_OK;
 }
 
+int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *best_ms) {
+  if (!ctx) return G3A_E_ARG;
+  if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
+  HIPCK(hipSetDevice(ctx->device));
+  const int form = ctx->form, sr0 = ctx->strip_rows, lb0 = ctx->lane_blocks_per_cu;
+  ctx->form = G3A_FORM_BATCHED;
+  int64_t *cells = P<int64_t>(ctx->counters) + 24;
+  int64_t saved = 0;
+  HIPCK(hipMemcpy(&saved, cells, 8, hipMemcpyDeviceToHost));
+  const int srs[2] = {16, 32}, lbs[4] = {2, 4, 8, 16};
+  float best = -1;
+  int bsr = sr0, blb = lb0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 4; ++b) {
+      ctx->strip_rows = srs[a];
+      ctx->lane_blocks_per_cu = lbs[b];
+      float t = 0;
+      // first run warms the buffers of this shape, the second is timed
+      for (int rep = 0; rep < 2; ++rep) {
+        HIPCK(hipEventRecord(ctx->ev[6], ctx->stream));
+        if (int rc = run_extension(ctx, ctx->total_tasks, cells)) return rc;
+        HIPCK(hipEventRecord(ctx->ev[7], ctx->stream));
+        HIPCK(hipEventSynchronize(ctx->ev[7]));
+        HIPCK(hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
+      }
+      if (best < 0 || t < best) {
+        best = t;
+        bsr = srs[a];
+        blb = lbs[b];
+      }
+    }
+  ctx->strip_rows = bsr;
+  ctx->lane_blocks_per_cu = blb;
+  // leave the regions and counters as the aligned batch had them
+  if (int rc = run_extension(ctx, ctx->total_tasks, cells)) return rc;
+  HIPCK(hipMemcpyAsync(cells, &saved, 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  ctx->form = form;
+  if (strip_rows) *strip_rows = bsr;
+  if (lane_blocks_per_cu) *lane_blocks_per_cu = blb;
+  if (best_ms) *best_ms = best;
+  return G3A_OK;
+}
+
 int g3a_set_form(g3a_ctx *ctx, int form) {
   if (!ctx || (form != G3A_FORM_PLAIN && form != G3A_FORM_BATCHED)) return G3A_E_ARG;
   ctx->form = form;
@@ -667,7 +723,8 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->n_aln, &ctx->aln_off, &ctx->aln_reg, &ctx->aln_read, &ctx->aln_rank, &ctx->z_size,
                     &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp,
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
-                    &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf};
+                    &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
+                    &ctx->lep_pool};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

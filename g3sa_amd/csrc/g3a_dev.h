@@ -199,9 +199,9 @@ struct IntvSink {
   }
 };
 
-// Scratch for the LEP lists (two alternating lists and the found list).
+// Scratch for the LEP list and the found list.
 struct LepBuf {
-  Intv *a, *b, *f;
+  Intv *a, *f;
   int cap;
 };
 
@@ -219,7 +219,9 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
   int x0 = 0;
   while (x0 < len) {
     if (q[x0] > 3) { ++x0; continue; }
-    Intv *prev = buf.a, *curr = buf.b;
+    // the backward pass rebuilds the list in place: entry nc is written only
+    // after entry j >= nc has been read
+    Intv *prev = buf.a, *curr = buf.a;
     int np = 0;
     Intv ik;
     intv_init(x, ik, q[x0]);
@@ -283,7 +285,6 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
         }
       }
       if (nc == 0) break;
-      Intv *t = prev; prev = curr; curr = t;
       np = nc;
     }
     // found in decreasing start (and end) order: emit the long-enough ones by end

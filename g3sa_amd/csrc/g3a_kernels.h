@@ -10,6 +10,7 @@ namespace g3a {
 constexpr int kSlowLep = 400;                  // slow-path LEP list capacity
 constexpr int kSlowScratch = 5 * kSlowLep;     // Intv entries of scratch per slow read
 constexpr int kSlotCap = 64;                   // fast-path interval slots per read
+constexpr int kLepCap = 32;                    // fast-path LEP list capacity
 
 struct SeedArgs {
   DevIndex idx;
@@ -23,6 +24,7 @@ struct SeedArgs {
   int64_t *n_seed;
   int32_t *slow;
   int64_t *fm_steps;
+  Intv *lep_pool;    // 2 * kLepCap entries per k_seed thread
 };
 
 struct SaArgs {
@@ -148,7 +150,9 @@ struct TbArgs {
   Aln *alns;
 };
 
-void launch_seed(const SeedArgs &a, hipStream_t s);
+// threads of k_seed's persistent grid (occupancy-limited, n_cu compute units)
+int seed_grid_threads(int n_cu);
+void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s);
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
                       Intv *pool, int pass, hipStream_t s);
 void launch_sa_lookup(const SaArgs &a, hipStream_t s);

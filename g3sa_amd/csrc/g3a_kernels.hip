@@ -27,39 +27,47 @@ namespace g3a {
 // ---------------------------------------------------------------------------
 // seeding
 // ---------------------------------------------------------------------------
-constexpr int kLepCap = 32;  // fast-path LEP list capacity (private memory)
-
+// Persistent grid: each thread takes reads r = tid, tid + T, ...  Its LEP
+// lists live in a global pool slice of its own, contiguous per thread (a
+// divergent list access then touches one cache line; private memory would
+// scatter every entry over eight lines of the per-lane swizzle).
 __global__ void __launch_bounds__(128) k_seed(SeedArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.n_reads) return;
-  const int off = a.off[r], len = a.off[r + 1] - off;
-  const uint8_t *q = a.seq + off;
-  IntvSink sink{a.intv + (int64_t)r * a.slot_cap, a.slot_cap, 0, 0, a.opt.max_occ};
-  if (len > kMaxReadLen) {  // unaligned (documented limit)
-    a.n_intv[r] = 0;
-    a.n_seed[r] = 0;
-    a.slow[r] = 0;
-    return;
-  }
-  Intv la[kLepCap], lb[kLepCap], lf[kLepCap];
-  LepBuf buf{la, lb, lf, kLepCap};
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  Intv *la = a.lep_pool + (size_t)gtid * 2 * kLepCap, *lf = la + kLepCap;
+  LepBuf buf{la, lf, kLepCap};
   int64_t steps = 0;
-  bool ok = smem_round1(a.idx, a.opt, q, len, sink, buf, &steps);
-  if (ok) {
-    const int nsm = sink.n < sink.cap ? sink.n : sink.cap;
-    if (sink.n <= sink.cap) {
-      for (int k = 0; ok && k < nsm; ++k) {
-        Intv smem = sink.slot[k];
-        ok = reseed_one(a.idx, a.opt, q, len, smem, la, lb, kLepCap, sink, &steps);
-      }
-    } else ok = false;
-    if (ok) reseed_round3(a.idx, a.opt, q, len, sink, &steps);
+  for (int r = gtid; r < a.n_reads; r += nthreads) {
+    const int off = a.off[r], len = a.off[r + 1] - off;
+    const uint8_t *q = a.seq + off;
+    IntvSink sink{a.intv + (int64_t)r * a.slot_cap, a.slot_cap, 0, 0, a.opt.max_occ};
+    if (len > kMaxReadLen) {  // unaligned (documented limit)
+      a.n_intv[r] = 0;
+      a.n_seed[r] = 0;
+      a.slow[r] = 0;
+      continue;
+    }
+    bool ok = smem_round1(a.idx, a.opt, q, len, sink, buf, &steps);
+    if (ok) {
+      const int nsm = sink.n < sink.cap ? sink.n : sink.cap;
+      if (sink.n <= sink.cap) {
+        for (int k = 0; ok && k < nsm; ++k) {
+          Intv smem = sink.slot[k];
+          ok = reseed_one(a.idx, a.opt, q, len, smem, la, lf, kLepCap, sink, &steps);
+        }
+      } else ok = false;
+      if (ok) reseed_round3(a.idx, a.opt, q, len, sink, &steps);
+    }
+    ok = ok && sink.n <= sink.cap;
+    a.slow[r] = ok ? 0 : 1;
+    a.n_intv[r] = ok ? sink.n : 0;
+    a.n_seed[r] = ok ? sink.n_seed : 0;
   }
-  ok = ok && sink.n <= sink.cap;
-  a.slow[r] = ok ? 0 : 1;
-  a.n_intv[r] = ok ? sink.n : 0;
-  a.n_seed[r] = ok ? sink.n_seed : 0;
-  if (a.fm_steps) atomicAdd((unsigned long long *)a.fm_steps, (unsigned long long)steps);
+  if (a.fm_steps) {  // one atomic per wavefront
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) steps += __shfl_xor(steps, o);
+    if ((threadIdx.x & 63) == 0 && steps) atomicAdd((unsigned long long *)a.fm_steps, (unsigned long long)steps);
+  }
 }
 
 // Slow path for the reads listed in `list`: global-memory LEP lists of
@@ -77,7 +85,7 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
   Intv *slot = pass ? pool + pool_off[t] : sc + 3 * kSlowLep;
   const int cap = pass ? (int)(pool_off[t + 1] - pool_off[t]) : 2 * kSlowLep;
   IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};
-  LepBuf buf{sc, sc + kSlowLep, sc + 2 * kSlowLep, kSlowLep};
+  LepBuf buf{sc, sc + kSlowLep, kSlowLep};
   if (!smem_round1(a.idx, a.opt, q, len, sink, buf, nullptr)) {
     sink.n = 0;
     sink.n_seed = 0;
@@ -86,7 +94,7 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
   const int nsm = sink.n;
   for (int k = 0; k < nsm; ++k) {
     Intv smem = slot[k];
-    reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.b, kSlowLep, sink, nullptr);
+    reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.f, kSlowLep, sink, nullptr);
   }
   reseed_round3(a.idx, a.opt, q, len, sink, nullptr);
   a.n_intv[r] = sink.n;
@@ -665,8 +673,15 @@ __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
 // ---------------------------------------------------------------------------
 static inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-void launch_seed(const SeedArgs &a, hipStream_t s) {
-  if (a.n_reads) hipLaunchKernelGGL(k_seed, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+int seed_grid_threads(int n_cu) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed, 128, 0) != hipSuccess || per_cu < 1) per_cu = 8;
+  return n_cu * per_cu * 128;
+}
+void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s) {
+  if (!a.n_reads) return;
+  const int need = blocks(a.n_reads, 128), have = n_threads / 128;
+  hipLaunchKernelGGL(k_seed, dim3(need < have ? need : have), dim3(128), 0, s, a);
 }
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
                       Intv *pool, int pass, hipStream_t s) {

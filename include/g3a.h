@@ -129,6 +129,13 @@ int g3a_set_form(g3a_ctx *ctx, int form);
  * persistent blocks per compute unit for the lane and wavefront kernels.
  * No reference counterpart. */
 int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wave_blocks_per_cu);
+/* Autotuner of the batched form: on the batch last aligned (its tasks are
+ * still on the device), time the extension stage for each strip height
+ * (16, 32) and persistent lane blocks per compute unit (2, 4, 8, 16), keep
+ * the fastest setting on the context and report it.  Results are identical
+ * for every setting; the batch's output is left as it was.  No reference
+ * counterpart. */
+int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *best_ms);
 const char *g3a_last_error(const g3a_ctx *ctx);
 
 /* Index construction (the `g3a index` command): BWA-format files plus the

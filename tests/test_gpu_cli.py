@@ -1,0 +1,61 @@
+"""The `g3a mem` command line (the reference's `bwa mem` entry) against the
+oracle's command line: byte-identical output in the reference's record format
+and in standard SAM, across batch boundaries and both extension forms."""
+from __future__ import annotations
+
+import subprocess
+
+import pytest
+
+import g3sa_amd
+from conftest import ROOT, oracle_threads
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_CLI = ROOT / "oracle" / "_build" / "g3a_oracle"
+
+
+def oracle_out(ds, fq, mode, tmp_path):
+    out = tmp_path / f"oracle_{mode}.txt"
+    subprocess.run([str(ORACLE_CLI), "-t", str(oracle_threads()), "-m", mode, "-o", str(out), str(ds.prefix),
+                    str(fq)], check=True, capture_output=True)
+    return out.read_bytes()
+
+
+def gpu_out(ds, fq, extra, tmp_path, name):
+    out = tmp_path / f"gpu_{name}.txt"
+    r = subprocess.run([str(g3sa_amd.CLI_PATH), "mem", "-o", str(out), *extra, str(ds.prefix), str(fq)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return out.read_bytes()
+
+
+def first_diff(a: bytes, b: bytes) -> str:
+    la, lb = a.split(b"\n"), b.split(b"\n")
+    for i, (x, y) in enumerate(zip(la, lb)):
+        if x != y:
+            return f"line {i}: gpu {x[:200]!r} oracle {y[:200]!r}"
+    return f"line counts {len(la)} vs {len(lb)}"
+
+
+@pytest.mark.parametrize("mode", ["compat", "sam"])
+def test_cli_byte_identical(ds_c1, tmp_path, mode):
+    fq = ds_c1.reads["c3"]
+    ref = oracle_out(ds_c1, fq, mode, tmp_path)
+    flags = ["-S"] if mode == "sam" else []
+    print(f"[cli] {mode}: one batch", flush=True)
+    got = gpu_out(ds_c1, fq, flags, tmp_path, f"{mode}_one")
+    assert got == ref, first_diff(got, ref)
+    print(f"[cli] {mode}: batches of 700 reads", flush=True)
+    got = gpu_out(ds_c1, fq, flags + ["-Z", "700"], tmp_path, f"{mode}_many")
+    assert got == ref, first_diff(got, ref)
+
+
+def test_cli_repeats_sam(ds_rep, tmp_path):
+    fq = ds_rep.reads["main"]
+    ref = oracle_out(ds_rep, fq, "sam", tmp_path)
+    print("[cli] repeats: sam", flush=True)
+    got = gpu_out(ds_rep, fq, ["-S", "-Z", "5000"], tmp_path, "rep_sam")
+    assert got == ref, first_diff(got, ref)
+    lines = [ln for ln in got.split(b"\n") if ln and not ln.startswith(b"@")]
+    assert len(lines) >= 12000

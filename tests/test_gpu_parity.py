@@ -163,3 +163,22 @@ def test_empty_batch(ds_c1):
     with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=4, max_bases=1000) as al:
         got = al.align(np.zeros(1, np.uint8), np.zeros(1, np.int32))
         assert got.n_reads == 0 and got.n_alns == 0
+
+
+def test_autotune_keeps_output(ds_c1, oracle):
+    """The autotuner reruns the extension stage under every tile shape; the
+    batch's output and every later batch stay identical to the oracle."""
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["c3"]), limit=2000)
+    idx = oracle.load(ds_c1.prefix)
+    try:
+        ref = oracle.align(idx, seq, off, threads=oracle_threads())
+    finally:
+        oracle.lib.orc_index_free(idx)
+    with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=len(off) - 1, max_bases=int(off[-1])) as al:
+        al.upload(seq, off)
+        al.run()
+        print("[parity] autotune", flush=True)
+        t = al.autotune()
+        assert t["strip_rows"] in (16, 32) and t["lane_blocks_per_cu"] in (2, 4, 8, 16)
+        compare(al.download(), ref, "after autotune")
+        compare(al.align(seq, off), ref, "next batch after autotune")
