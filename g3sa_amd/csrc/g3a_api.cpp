@@ -208,10 +208,11 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   if (!ctx) return G3A_E_ARG;
   HIPCK(hipSetDevice(ctx->device));
   const HostIndex &h = ctx->h;
-  std::vector<uint64_t> lines;
-  build_occ_lines(h, lines);
-  if (ensure(ctx, ctx->occ, lines.size() * 8)) return G3A_E_HIP;
-  HIPCK(hipMemcpy(ctx->occ.p, lines.data(), lines.size() * 8, hipMemcpyHostToDevice));
+  std::vector<uint32_t> lines;
+  DevIndex &x = ctx->idx;
+  if (!build_occ_lines(h, lines, x.super)) return fail(ctx, G3A_E_INDEX, "index larger than 2^34 rows");
+  if (ensure(ctx, ctx->occ, lines.size() * 4)) return G3A_E_HIP;
+  HIPCK(hipMemcpy(ctx->occ.p, lines.data(), lines.size() * 4, hipMemcpyHostToDevice));
   if (ensure(ctx, ctx->pac, h.pac.size())) return G3A_E_HIP;
   HIPCK(hipMemcpy(ctx->pac.p, h.pac.data(), h.pac.size(), hipMemcpyHostToDevice));
   if (ensure(ctx, ctx->sa, h.sa.size())) return G3A_E_HIP;
@@ -225,8 +226,7 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   HIPCK(hipMemcpy(ctx->ctg_off.p, co.data(), co.size() * 8, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(ctx->ctg_len.p, cl.data(), cl.size() * 8, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(ctx->ctg_alt.p, ca.data(), ca.size() * 4, hipMemcpyHostToDevice));
-  DevIndex &x = ctx->idx;
-  x.occ = P<uint64_t>(ctx->occ);
+  x.occ = P<uint32_t>(ctx->occ);
   build_counts(h, x.count);
   x.primary = (int64_t)h.primary;
   x.n = (int64_t)h.seq_len;

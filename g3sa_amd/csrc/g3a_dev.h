@@ -33,12 +33,14 @@ struct Opt {
   int8_t mat[25];
 };
 
-// Device view of the index.  `occ` holds one 64-byte line per 64 BWT rows:
-// four cumulative base counts (rows before the line) then four one-hot masks
-// (bit 63-y set when row 64*line+y holds that base; '$' sets no bit).  One
-// Occ lookup is a single line fetch.
+// Device view of the index.  `occ` holds one 32-byte line per 64 BWT rows:
+// four uint32 base counts of the rows before the line, relative to its
+// 2^32-row superblock (`super`), then the 64 rows' 2-bit BWT symbols in two
+// uint64 words, first row in the highest bits; the '$' row is stored as 0
+// and discounted.  One Occ lookup is two 16-byte loads of one line.
 struct DevIndex {
-  const uint64_t *occ;
+  const uint32_t *occ;
+  int64_t super[4][4];
   int64_t count[5];  // count[c] = 1 + #bases < c (the '$' row first), count[4] = n+1
   int64_t primary;   // row whose suffix starts at text position 0 (BWT '$')
   int64_t n;         // text length 2*l_pac; rows 0..n
@@ -67,29 +69,55 @@ struct Seed {
 // ---------------------------------------------------------------------------
 // FM-index primitives (bwa/src/fmindex.cu:29-45, 107-180; macro.h:50-71)
 // ---------------------------------------------------------------------------
-G3A_HD int64_t occ1(const DevIndex &x, int64_t p, int c) {
-  const uint64_t *b = x.occ + ((p >> 6) << 3);
-  uint64_t mask = ~0ULL << (63 - (p & 63));
-  return (int64_t)b[c] + G3A_POPC(b[4 + c] & mask);
+// Occurrences of each base in rows [line start, line start + y] from the two
+// symbol words: bit planes hi/lo of the even bit of each pair.
+G3A_HD void line_counts(uint64_t w0, uint64_t w1, int y, uint32_t o[4]) {
+  const uint64_t E = 0x5555555555555555ULL;
+  uint64_t m0, m1;
+  if (y < 32) {
+    m0 = (~0ULL << (62 - 2 * y)) & E;
+    m1 = 0;
+  } else {
+    m0 = E;
+    m1 = (~0ULL << (62 - 2 * (y - 32))) & E;
+  }
+  const uint64_t h0 = (w0 >> 1) & m0, l0 = w0 & m0, h1 = (w1 >> 1) & m1, l1 = w1 & m1;
+  const uint32_t n3 = (uint32_t)(G3A_POPC(h0 & l0) + G3A_POPC(h1 & l1));
+  const uint32_t n2 = (uint32_t)(G3A_POPC(h0 & ~l0) + G3A_POPC(h1 & ~l1));
+  const uint32_t n1 = (uint32_t)(G3A_POPC(l0 & ~h0) + G3A_POPC(l1 & ~h1));
+  o[3] = n3;
+  o[2] = n2;
+  o[1] = n1;
+  o[0] = (uint32_t)(y + 1) - n1 - n2 - n3;
 }
 
-// all four bases at one row: one line
+// Occ(p, c): occurrences of c in rows [0, p], '$' excluded
 G3A_HD void occ4(const DevIndex &x, int64_t p, int64_t o[4]) {
-  const uint64_t *b = x.occ + ((p >> 6) << 3);
-  uint64_t mask = ~0ULL << (63 - (p & 63));
+  const uint32_t *b = x.occ + ((p >> 6) << 3);
+  const uint64_t w0 = (uint64_t)b[4] | (uint64_t)b[5] << 32, w1 = (uint64_t)b[6] | (uint64_t)b[7] << 32;
+  uint32_t in[4];
+  line_counts(w0, w1, (int)(p & 63), in);
+  const int64_t *sb = x.super[p >> 32];
+  const int dollar = (p >> 6) == (x.primary >> 6) && p >= x.primary;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) o[c] = (int64_t)b[c] + G3A_POPC(b[4 + c] & mask);
+  for (int c = 0; c < 4; ++c) o[c] = sb[c] + (int64_t)b[c] + in[c];
+  o[0] -= dollar;
+}
+
+G3A_HD int64_t occ1(const DevIndex &x, int64_t p, int c) {
+  int64_t o[4];
+  occ4(x, p, o);
+  return o[c];
 }
 
 // BWT character of row k (4 for the '$' row)
 G3A_HD int bwt_char(const DevIndex &x, int64_t k) {
-  const uint64_t *b = x.occ + ((k >> 6) << 3);
-  uint64_t bit = 1ULL << (63 - (k & 63));
-  int c = 4;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    if (b[4 + i] & bit) c = i;
-  return c;
+  if (k == x.primary) return 4;
+  const uint32_t *b = x.occ + ((k >> 6) << 3);
+  const int y = (int)(k & 63);
+  // rows 0-15 are the high half of the first uint64 word, i.e. uint32 b[5]
+  const uint32_t w = b[4 + ((y >> 4) ^ 1)];
+  return (int)((w >> (30 - 2 * (y & 15))) & 3);
 }
 
 G3A_HD void intv_init(const DevIndex &x, Intv &p, int c) {

@@ -31,7 +31,7 @@ namespace g3a {
 // lists live in a global pool slice of its own, contiguous per thread (a
 // divergent list access then touches one cache line; private memory would
 // scatter every entry over eight lines of the per-lane swizzle).
-__global__ void __launch_bounds__(128) k_seed(SeedArgs a) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
   Intv *la = a.lep_pool + (size_t)gtid * 2 * kLepCap, *lf = la + kLepCap;
