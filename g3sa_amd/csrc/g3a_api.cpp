@@ -47,8 +47,8 @@ struct g3a_ctx {
   int n_reads = 0;
   bool has_input = false, has_output = false;
   DevBuf seq, off;
-  DevBuf lep_pool;
-  int seed_threads = 0;
+  DevBuf lep_pool, eh_pool;
+  int seed_threads = 0, tb_threads = 0;
   // seeding
   DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
   // seeds / chains
@@ -213,7 +213,9 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   if (!build_occ_lines(h, lines, x.super)) return fail(ctx, G3A_E_INDEX, "index larger than 2^34 rows");
   if (ensure(ctx, ctx->occ, lines.size() * 4)) return G3A_E_HIP;
   HIPCK(hipMemcpy(ctx->occ.p, lines.data(), lines.size() * 4, hipMemcpyHostToDevice));
-  if (ensure(ctx, ctx->pac, h.pac.size())) return G3A_E_HIP;
+  // 16 zero bytes of padding: word-wise readers may read past the last base
+  if (ensure(ctx, ctx->pac, h.pac.size() + 16)) return G3A_E_HIP;
+  HIPCK(hipMemset(ctx->pac.p, 0, h.pac.size() + 16));
   HIPCK(hipMemcpy(ctx->pac.p, h.pac.data(), h.pac.size(), hipMemcpyHostToDevice));
   if (ensure(ctx, ctx->sa, h.sa.size())) return G3A_E_HIP;
   HIPCK(hipMemcpy(ctx->sa.p, h.sa.data(), h.sa.size(), hipMemcpyHostToDevice));
@@ -524,8 +526,11 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   tb.z = P<uint8_t>(ctx->z);
   tb.cigar = P<uint32_t>(ctx->cigar);
   tb.alns = P<Aln>(ctx->alns);
+  if (!ctx->tb_threads) ctx->tb_threads = traceback_grid_threads(ctx->n_cu);
+  if (ensure(ctx, ctx->eh_pool, (size_t)traceback_pool_words(ctx->tb_threads) * 4)) return G3A_E_HIP;
+  tb.eh_pool = P<int32_t>(ctx->eh_pool);
   HIPCK(hipEventRecord(ctx->kev[10], s));
-  launch_traceback(tb, s);
+  launch_traceback(tb, ctx->tb_threads, s);
   HIPCK(hipEventRecord(ctx->kev[11], s));
   ctx->kev_used[5] = 1;
   HIPCK(hipEventRecord(ctx->ev[5], s));
@@ -724,7 +729,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp,
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
                     &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
-                    &ctx->lep_pool};
+                    &ctx->lep_pool, &ctx->eh_pool};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

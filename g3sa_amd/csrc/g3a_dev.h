@@ -209,6 +209,34 @@ G3A_HD int text_base_p(const uint8_t *pac, int64_t l_pac, int64_t p) {
 }
 G3A_HD int text_base(const DevIndex &x, int64_t p) { return text_base_p(x.pac, x.l_pac, p); }
 
+// Sequential reader of the fwd+revcomp text: keeps one 32-bit word (16
+// bases) of the packed forward strand, so a run of neighbouring positions
+// costs one load per 16 bases.  The packed strand must be readable up to the
+// next 4-byte boundary.
+struct TextReader {
+  const uint8_t *pac;
+  int64_t l_pac;
+  int64_t widx;
+  uint32_t word;
+  G3A_HD TextReader(const uint8_t *p, int64_t l) : pac(p), l_pac(l), widx(-1), word(0) {}
+  G3A_HD int operator()(int64_t p) {
+    const bool fwd = p < l_pac;
+    const int64_t q = fwd ? p : (l_pac << 1) - 1 - p;
+    const int64_t wi = q >> 4;
+    if (wi != widx) {
+      widx = wi;
+      const uint8_t *b = pac + (wi << 2);
+#if defined(__HIP_DEVICE_COMPILE__)
+      word = *(const uint32_t *)b;
+#else
+      word = (uint32_t)b[0] | (uint32_t)b[1] << 8 | (uint32_t)b[2] << 16 | (uint32_t)b[3] << 24;
+#endif
+    }
+    const int v = (int)((word >> (8 * ((q >> 2) & 3) + 2 * (3 - (q & 3)))) & 3);
+    return fwd ? v : 3 - v;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Seeding
 // ---------------------------------------------------------------------------
@@ -1005,24 +1033,26 @@ G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *r
   const int64_t core_at = (int64_t)lq + rlen + 4;  // core CIGAR offset inside this record
   uint32_t *core = cig + core_at;
   int n_core = 0, nw = 0;
+  TextReader tr(x.pac, l_pac);
   if (w == 0) {
     core[0] = (uint32_t)(lq << 4);
     n_core = 1;
-    for (int i = 0; i < lq; ++i) nw += o.mat[text_base(x, rb + i) * 5 + query[i]];
+    for (int i = 0; i < lq; ++i) nw += o.mat[tr(rb + i) * 5 + query[i]];
   } else {
     nw = global_align(
-        o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return text_base(x, rb + i); }, w,
-        eh_h, eh_e, z, core, &n_core);
+        o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return tr(rb + i); }, w, eh_h, eh_e,
+        z, core, &n_core);
   }
   // NM as the reference's traceback counts it (aux.cu:2910-2941)
   int n_mm = 0, n_gap = 0;
   {
+    TextReader trn(x.pac, l_pac);
     int xq = 0;
     int64_t yr = 0;
     for (int k = 0; k < n_core; ++k) {
       int op = core[k] & 0xf, ln = (int)(core[k] >> 4);
       if (op == 0) {
-        for (int i = 0; i < ln; ++i) n_mm += query[xq + i] != text_base(x, rb + yr + i);
+        for (int i = 0; i < ln; ++i) n_mm += query[xq + i] != trn(rb + yr + i);
         xq += ln;
         yr += ln;
       } else if (op == 2) {

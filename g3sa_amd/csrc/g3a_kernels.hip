@@ -655,17 +655,21 @@ __global__ void __launch_bounds__(128) k_aln_prep(AlnPrepArgs a) {
   }
 }
 
+// Persistent grid; the global-alignment row buffers of each thread live in a
+// global pool slice of its own (2 * kQMax words), contiguous per thread.
 __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.n_alns) return;
-  const int r = a.aln_read[t];
-  const int off = a.off[r], l_seq = a.off[r + 1] - off;
-  const Region g = a.aln_reg[t];
-  int32_t eh_h[kQMax], eh_e[kQMax];
-  Aln out;
-  trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], eh_h, eh_e, a.z + a.z_off[t],
-                     a.cigar + a.cig_off[t], a.cig_off[t], out);
-  a.alns[t] = out;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+  int32_t *eh_h = a.eh_pool + gtid * 2 * kQMax, *eh_e = eh_h + kQMax;
+  for (int64_t t = gtid; t < a.n_alns; t += nthreads) {
+    const int r = a.aln_read[t];
+    const int off = a.off[r], l_seq = a.off[r + 1] - off;
+    const Region g = a.aln_reg[t];
+    Aln out;
+    trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], eh_h, eh_e, a.z + a.z_off[t],
+                       a.cigar + a.cig_off[t], a.cig_off[t], out);
+    a.alns[t] = out;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -727,8 +731,17 @@ void launch_region_filter(const RegionArgs &a, hipStream_t s) {
 void launch_aln_prep(const AlnPrepArgs &a, hipStream_t s) {
   if (a.n_reads) hipLaunchKernelGGL(k_aln_prep, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
 }
-void launch_traceback(const TbArgs &a, hipStream_t s) {
-  if (a.n_alns) hipLaunchKernelGGL(k_traceback, dim3(blocks(a.n_alns, 64)), dim3(64), 0, s, a);
+int traceback_grid_threads(int n_cu) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traceback, 64, 0) != hipSuccess || per_cu < 1)
+    per_cu = 16;
+  return n_cu * per_cu * 64;
+}
+int64_t traceback_pool_words(int n_threads) { return (int64_t)n_threads * 2 * kQMax; }
+void launch_traceback(const TbArgs &a, int n_threads, hipStream_t s) {
+  if (!a.n_alns) return;
+  const int64_t need = (a.n_alns + 63) / 64, have = n_threads / 64;
+  hipLaunchKernelGGL(k_traceback, dim3((unsigned)(need < have ? need : have)), dim3(64), 0, s, a);
 }
 
 
