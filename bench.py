@@ -14,7 +14,7 @@ alignment + finalize) over the 2M-read batch already resident in HBM; outputs
 stay in HBM.  The CPU baseline is the oracle (C, all host cores) on a bounded
 sample of the same reads.
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c2|c1] [--form both|batched|plain]
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c2|c3|c1] [--form both|batched|plain]
 
 Both extension forms give identical output; `value` is the batched form's
 rate and the plain form's is reported beside it.
@@ -49,6 +49,13 @@ CONFIGS = {
         reads=["--n", "2000000", "--paired", "--len", "150", "--insert-mean", "400", "--insert-sd", "50",
                "--sub-start", "0.005", "--sub-end", "0.015", "--indel", "0.0005", "--seed", "12"],
         ref_len=100_000_000, n_reads=2_000_000),
+    "c3": dict(
+        workload="bwa-mem SE, 1 Mbp uniform synthetic reference, 2M reads of 70-250 bp (5% at 8% divergence "
+                 "with a 20-40 bp random tail)",
+        ref=["--len", "1000000", "--contigs", "1", "--seed", "1"],
+        reads=["--n", "2000000", "--len-min", "70", "--len-max", "250", "--sub", "0.01", "--indel", "0.0005",
+               "--hard-frac", "0.05", "--hard-div", "0.08", "--tail-min", "20", "--tail-max", "40", "--seed", "3"],
+        ref_len=1_000_000, n_reads=2_000_000),
     "c1": dict(
         workload="bwa-mem SE, 1 Mbp uniform synthetic reference, 10k x 150 bp reads",
         ref=["--len", "1000000", "--contigs", "1", "--seed", "1"],

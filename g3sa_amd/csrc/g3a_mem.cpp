@@ -16,6 +16,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -44,27 +49,64 @@ uint8_t nt4(int c) {
   }
 }
 
-// Sequential FASTQ reader over a memory buffer: NAME runs to the first space
-// (or end of line), then the rest of the header line, SEQ, the separator line
-// and QUAL.
+// Streaming FASTQ reader: the file is read in chunks of kChunk bytes and
+// parsed record by record (NAME runs to the first space or end of line, then
+// the rest of the header line, SEQ, the separator line and QUAL); a record
+// cut by a chunk boundary is completed from the next chunk.
 struct FastqReader {
+  size_t kChunk = 64u << 20;  // G3A_FASTQ_CHUNK overrides (bytes)
+  FILE *fp = nullptr;
   std::vector<char> buf;
-  size_t pos = 0;
+  size_t pos = 0, len = 0;
+  bool eof = false;
+  ~FastqReader() {
+    if (fp) std::fclose(fp);
+  }
   bool open(const char *fn) {
-    FILE *fp = std::fopen(fn, "rb");
+    if (const char *e = std::getenv("G3A_FASTQ_CHUNK")) {
+      const long long v = std::atoll(e);
+      if (v >= 16) kChunk = (size_t)v;
+    }
+    fp = std::fopen(fn, "rb");
     if (!fp) return false;
-    std::fseek(fp, 0, SEEK_END);
-    long sz = std::ftell(fp);
-    std::fseek(fp, 0, SEEK_SET);
-    buf.resize((size_t)(sz > 0 ? sz : 0));
-    bool ok = buf.empty() || std::fread(buf.data(), 1, buf.size(), fp) == buf.size();
-    std::fclose(fp);
-    while (pos < buf.size() && buf[pos] != '@') ++pos;
-    return ok;
+    fill();
+    skip_to_record();
+    return true;
+  }
+  // append more of the file after the unparsed tail
+  void fill() {
+    if (eof) return;
+    if (pos > 0) {
+      std::memmove(buf.data(), buf.data() + pos, len - pos);
+      len -= pos;
+      pos = 0;
+    }
+    if (buf.size() < len + kChunk) buf.resize(len + kChunk);
+    const size_t got = std::fread(buf.data() + len, 1, kChunk, fp);
+    len += got;
+    if (got < kChunk) eof = true;
+  }
+  void skip_to_record() {
+    for (;;) {
+      while (pos < len && buf[pos] != '@') ++pos;
+      if (pos < len || eof) return;
+      fill();
+    }
+  }
+  // a whole record (four lines) is buffered, or the file ends inside it
+  bool have_record() {
+    for (;;) {
+      int nl = 0;
+      for (size_t i = pos; i < len; ++i)
+        if (buf[i] == '\n' && ++nl == 4) return true;
+      if (eof) return pos < len;
+      fill();
+    }
   }
   bool next(Batch &b) {
-    const size_t end = buf.size();
-    if (pos >= end) return false;
+    if (pos >= len && eof) return false;
+    if (!have_record()) return false;
+    const size_t end = len;
     const char *p = buf.data();
     size_t q = ++pos;
     while (pos < end && p[pos] != ' ' && p[pos] != '\n') ++pos;
@@ -82,9 +124,37 @@ struct FastqReader {
     if (q < end) b.quals.append(p + q, (pos < end ? pos : end) - q);
     b.qual_off.push_back((int64_t)b.quals.size());
     ++pos;
-    while (pos < end && p[pos] != '@') ++pos;
+    if (pos > len) pos = len;
+    skip_to_record();
     return true;
   }
+};
+
+// Bounded FIFO between pipeline stages.
+template <class T>
+class Queue {
+ public:
+  explicit Queue(size_t cap) : cap_(cap) {}
+  void push(T v) {
+    std::unique_lock<std::mutex> l(m_);
+    not_full_.wait(l, [&] { return q_.size() < cap_; });
+    q_.push_back(std::move(v));
+    not_empty_.notify_one();
+  }
+  T pop() {
+    std::unique_lock<std::mutex> l(m_);
+    not_empty_.wait(l, [&] { return !q_.empty(); });
+    T v = std::move(q_.front());
+    q_.pop_front();
+    not_full_.notify_one();
+    return v;
+  }
+
+ private:
+  size_t cap_;
+  std::deque<T> q_;
+  std::mutex m_;
+  std::condition_variable not_full_, not_empty_;
 };
 
 void put_int(std::string &s, long long v) {
@@ -369,34 +439,67 @@ int g3a_mem_main(int argc, char **argv) {
     hdr += "@PG\tID:g3a\tPN:g3a\n";
     std::fwrite(hdr.data(), 1, hdr.size(), out);
   }
-  Batch b;
+  // three stages overlap: the loader parses batch k+1 while the GPU aligns
+  // batch k and the formatter writes batch k-1, in input order
+  struct Job {
+    std::unique_ptr<Batch> b;
+    g3a_batch_out o;
+  };
+  Queue<std::unique_ptr<Batch>> q_in(2);
+  Queue<std::unique_ptr<Job>> q_out(2);
+  const auto t0 = std::chrono::steady_clock::now();
+  std::thread loader([&] {
+    for (;;) {
+      std::unique_ptr<Batch> b(new Batch);
+      while (b->n() < batch && rd.next(*b)) {}
+      if (b->n() == 0) {
+        q_in.push(nullptr);
+        return;
+      }
+      q_in.push(std::move(b));
+    }
+  });
   long long total = 0;
+  std::thread writer([&] {
+    for (;;) {
+      std::unique_ptr<Job> j = q_out.pop();
+      if (!j) return;
+      const Batch &b = *j->b;
+      const int nt = b.n() < 4096 ? 1 : threads;
+      std::vector<std::string> parts((size_t)nt);
+      std::vector<std::thread> th;
+      for (int t = 0; t < nt; ++t) {
+        int r0 = (int)((int64_t)b.n() * t / nt), r1 = (int)((int64_t)b.n() * (t + 1) / nt);
+        th.emplace_back([&, t, r0, r1] { format_range(b, j->o, ref, opt, sam, r0, r1, parts[(size_t)t]); });
+      }
+      for (auto &x : th) x.join();
+      for (auto &p : parts) std::fwrite(p.data(), 1, p.size(), out);
+      total += b.n();
+      g3a_batch_out_free(&j->o);
+    }
+  });
   int status = 0;
   for (;;) {
-    b.clear();
-    while (b.n() < batch && rd.next(b)) {}
-    if (b.n() == 0) break;
-    g3a_batch_out o;
-    rc = g3a_align_batch(ctx, b.n(), b.seq.data(), b.seq_off.data(), &o);
+    std::unique_ptr<Batch> b = q_in.pop();
+    if (!b) break;
+    if (status) continue;  // drain the loader after a failure
+    std::unique_ptr<Job> j(new Job);
+    rc = g3a_align_batch(ctx, b->n(), b->seq.data(), b->seq_off.data(), &j->o);
     if (rc) {
       std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
       status = 1;
-      break;
+      continue;
     }
-    const int nt = b.n() < 4096 ? 1 : threads;
-    std::vector<std::string> parts((size_t)nt);
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) {
-      int r0 = (int)((int64_t)b.n() * t / nt), r1 = (int)((int64_t)b.n() * (t + 1) / nt);
-      th.emplace_back([&, t, r0, r1] { format_range(b, o, ref, opt, sam, r0, r1, parts[(size_t)t]); });
-    }
-    for (auto &x : th) x.join();
-    for (auto &p : parts) std::fwrite(p.data(), 1, p.size(), out);
-    total += b.n();
-    g3a_batch_out_free(&o);
+    j->b = std::move(b);
+    q_out.push(std::move(j));
   }
+  q_out.push(nullptr);
+  loader.join();
+  writer.join();
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::fprintf(stderr, "g3a mem: %lld reads in %.2f s (%.0f reads/s end to end)\n", total, sec,
+               sec > 0 ? total / sec : 0.0);
   if (out_fn) std::fclose(out);
   g3a_destroy(ctx);
-  std::fprintf(stderr, "g3a mem: %lld reads\n", total);
   return status;
 }

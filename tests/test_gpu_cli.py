@@ -3,6 +3,7 @@ oracle's command line: byte-identical output in the reference's record format
 and in standard SAM, across batch boundaries and both extension forms."""
 from __future__ import annotations
 
+import os
 import subprocess
 
 import pytest
@@ -22,10 +23,13 @@ def oracle_out(ds, fq, mode, tmp_path):
     return out.read_bytes()
 
 
-def gpu_out(ds, fq, extra, tmp_path, name):
+def gpu_out(ds, fq, extra, tmp_path, name, chunk=None):
     out = tmp_path / f"gpu_{name}.txt"
+    env = dict(os.environ)
+    if chunk:
+        env["G3A_FASTQ_CHUNK"] = str(chunk)
     r = subprocess.run([str(g3sa_amd.CLI_PATH), "mem", "-o", str(out), *extra, str(ds.prefix), str(fq)],
-                       capture_output=True, text=True, timeout=300)
+                       capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, r.stderr
     return out.read_bytes()
 
@@ -48,6 +52,10 @@ def test_cli_byte_identical(ds_c1, tmp_path, mode):
     assert got == ref, first_diff(got, ref)
     print(f"[cli] {mode}: batches of 700 reads", flush=True)
     got = gpu_out(ds_c1, fq, flags + ["-Z", "700"], tmp_path, f"{mode}_many")
+    assert got == ref, first_diff(got, ref)
+    # FASTQ read in 997-byte chunks: records cut at every possible offset
+    print(f"[cli] {mode}: 997-byte input chunks", flush=True)
+    got = gpu_out(ds_c1, fq, flags + ["-Z", "1500"], tmp_path, f"{mode}_chunks", chunk=997)
     assert got == ref, first_diff(got, ref)
 
 
