@@ -152,6 +152,19 @@ def run_form(al, n: int, steps: int, warmup: int, form: str) -> dict:
     }
 
 
+def end_to_end(prefix: Path, fq: Path) -> dict | None:
+    """`g3a mem` over the whole FASTQ (parse, align, format, write to /dev/null),
+    timed by the driver from the first read parsed to the last line written."""
+    r = subprocess.run([str(g3sa_amd.CLI_PATH), "mem", "-S", "-t", "16", "-o", "/dev/null", str(prefix), str(fq)],
+                       capture_output=True, text=True)
+    m = re.search(r"(\d+) reads in ([0-9.]+) s \(([0-9.]+) reads/s end to end\)", r.stderr)
+    if r.returncode != 0 or not m:
+        log(f"g3a mem failed: {r.stderr[-300:]}")
+        return None
+    return {"reads_per_s": float(m.group(3)), "reads": int(m.group(1)), "seconds": float(m.group(2)),
+            "what": "g3a mem -S, standard SAM to /dev/null, batches of 500k reads, index load excluded"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +174,7 @@ def main() -> int:
     ap.add_argument("--data", default=os.environ.get("G3A_DATA", "/tmp/g3a_bench"))
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line run")
     ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
                     help="extension form(s) to time; with both, value is the batched form's")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -236,6 +250,9 @@ def main() -> int:
         "roofline": roof,
     }
     al.close()
+    if not args.no_e2e:
+        log("end-to-end: g3a mem")
+        result["end_to_end"] = end_to_end(prefix, fq)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(prefix, fq, min(args.cpu_sample, n))
     print(json.dumps(result), flush=True)

@@ -78,6 +78,9 @@ class Oracle:
         lib.orc_ksw_extend_warp_emulate.argtypes = ext_args
         lib.orc_ksw_global2.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_int]
+        lib.orc_index_contig_name.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_index_contig_name.restype = C.c_char_p
+        lib.orc_index_n_contigs.argtypes = [C.c_void_p]
         lib.orc_mapq.argtypes = [C.POINTER(OrcOpt), C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64]
         self.lib = lib
         self.opt = OrcOpt()
