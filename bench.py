@@ -122,12 +122,12 @@ def cpu_baseline(prefix: Path, fq: Path, sample: int) -> dict | None:
             "sample": f"first {m.group(3)} reads of the same FASTQ, alignment only ({float(m.group(4)):.1f} s)"}
 
 
-def run_form(al, n: int, steps: int, warmup: int, form: str) -> dict:
+def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = True) -> dict:
     for i in range(warmup):
         al.run()
         log(f"[{form}] warmup step {i + 1}/{warmup} done")
     tuned = None
-    if form == "batched":
+    if form == "batched" and autotune:
         if warmup == 0:
             al.run()
         tuned = al.autotune()
@@ -175,6 +175,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line run")
+    ap.add_argument("--tuning", default=None, metavar="ROWS,LANE_BLOCKS",
+                    help="fixed batched-form tuning instead of the autotuner (e.g. 16,16)")
     ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
                     help="extension form(s) to time; with both, value is the batched form's")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -206,7 +208,10 @@ def main() -> int:
     per_form = {}
     for form in forms:
         al.set_form(form)
-        per_form[form] = run_form(al, n, args.steps, args.warmup, form)
+        if form == "batched" and args.tuning:
+            rows, lb = (int(v) for v in args.tuning.split(","))
+            al.set_tuning(rows, lb, 8)
+        per_form[form] = run_form(al, n, args.steps, args.warmup, form, autotune=not args.tuning)
     main_form = "batched" if "batched" in per_form else forms[0]
     m = per_form[main_form]
     mean, kms, work = m["stages_ms"], m["kernel_ms"], m["work"]

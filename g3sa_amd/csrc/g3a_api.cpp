@@ -381,7 +381,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.n_seed = P<int64_t>(ctx->n_seed);
   sa.slow = P<int32_t>(ctx->slow);
   sa.fm_steps = cnt + 0;
-  if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu);
+  if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx);
   if (ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * 2 * kLepCap * sizeof(Intv))) return G3A_E_HIP;
   sa.lep_pool = P<Intv>(ctx->lep_pool);
   for (int &u : ctx->kev_used) u = 0;

@@ -255,9 +255,42 @@ struct IntvSink {
   }
 };
 
+// Storage of the LEP and found lists.  LepWide keeps the whole interval;
+// LepNarrow packs it into 16 bytes (one vector load or store per entry) and
+// holds for texts of fewer than 2^32 rows.
+struct LepWide {
+  Intv v;
+  static G3A_HD Intv get(const LepWide *p, int i) { return p[i].v; }
+  static G3A_HD void put(LepWide *p, int i, const Intv &x) { p[i].v = x; }
+};
+struct alignas(16) LepNarrow {
+  uint32_t k, l, s;
+  int16_t m, n;
+  static G3A_HD Intv get(const LepNarrow *p, int i) {
+    const LepNarrow e = p[i];
+    Intv r;
+    r.k = e.k;
+    r.l = e.l;
+    r.s = e.s;
+    r.m = e.m;
+    r.n = e.n;
+    return r;
+  }
+  static G3A_HD void put(LepNarrow *p, int i, const Intv &x) {
+    LepNarrow e;
+    e.k = (uint32_t)x.k;
+    e.l = (uint32_t)x.l;
+    e.s = (uint32_t)x.s;
+    e.m = (int16_t)x.m;
+    e.n = (int16_t)x.n;
+    p[i] = e;
+  }
+};
+
 // Scratch for the LEP list and the found list.
+template <class E>
 struct LepBuf {
-  Intv *a, *f;
+  E *a, *f;
   int cap;
 };
 
@@ -270,14 +303,15 @@ struct LepBuf {
 // then backward), which visits each read position a few times instead of
 // once per end position.  Output order is by end position, as the
 // reference's.  Returns false when a LEP list outgrows `buf`.
+template <class E>
 G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
-                        LepBuf &buf, int64_t *steps) {
+                        LepBuf<E> &buf, int64_t *steps) {
   int x0 = 0;
   while (x0 < len) {
     if (q[x0] > 3) { ++x0; continue; }
     // the backward pass rebuilds the list in place: entry nc is written only
     // after entry j >= nc has been read
-    Intv *prev = buf.a, *curr = buf.a;
+    E *prev = buf.a, *curr = buf.a;
     int np = 0;
     Intv ik;
     intv_init(x, ik, q[x0]);
@@ -291,61 +325,70 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
         if (steps) ++*steps;
         if (ok.s != ik.s) {
           if (np == buf.cap) return false;
-          prev[np++] = ik;
+          E::put(prev, np++, ik);
           if (ok.s < 1) break;
         }
         ik = ok;
         ik.n = i;
       } else {
         if (np == buf.cap) return false;
-        prev[np++] = ik;
+        E::put(prev, np++, ik);
         break;
       }
     }
     if (i == len) {
       if (np == buf.cap) return false;
-      prev[np++] = ik;
+      E::put(prev, np++, ik);
     }
     // longest first
-    for (int a = 0, b = np - 1; a < b; ++a, --b) { Intv t = prev[a]; prev[a] = prev[b]; prev[b] = t; }
-    const int ret = prev[0].n + 1;
+    for (int a = 0, b = np - 1; a < b; ++a, --b) {
+      const Intv ta = E::get(prev, a), tb = E::get(prev, b);
+      E::put(prev, a, tb);
+      E::put(prev, b, ta);
+    }
+    const int ret = E::get(prev, 0).n + 1;
     // backward search; SMEMs are found in decreasing start order
     int nf = 0, last_start = 0;
-    Intv *fl = buf.f;
+    E *fl = buf.f;
+    int64_t last_s = -1;  // size of the last entry written to curr
     for (i = x0 - 1; i >= -1; --i) {
       int c = i < 0 ? -1 : (q[i] < 4 ? q[i] : -1);
       int nc = 0;
       for (int j = 0; j < np; ++j) {
+        const Intv pj = E::get(prev, j);
         Intv ok;
         bool ext_ok = false;
         if (c >= 0) {
-          backward_ext(x, prev[j], c, ok);
+          backward_ext(x, pj, c, ok);
           if (steps) ++*steps;
           ext_ok = ok.s >= 1;
         }
         if (!ext_ok) {
           if (nc == 0) {
             if (nf == 0 || i + 1 < last_start) {
-              Intv p = prev[j];
+              Intv p = pj;
               p.m = i + 1;
               if (nf == buf.cap) return false;
-              fl[nf++] = p;
+              E::put(fl, nf++, p);
               last_start = i + 1;
             }
           }
-        } else if (nc == 0 || ok.s != curr[nc - 1].s) {
-          ok.m = prev[j].m;
-          ok.n = prev[j].n;
+        } else if (nc == 0 || ok.s != last_s) {
+          ok.m = pj.m;
+          ok.n = pj.n;
           if (nc == buf.cap) return false;
-          curr[nc++] = ok;
+          E::put(curr, nc++, ok);
+          last_s = ok.s;
         }
       }
       if (nc == 0) break;
       np = nc;
     }
     // found in decreasing start (and end) order: emit the long-enough ones by end
-    for (int j = nf - 1; j >= 0; --j)
-      if (fl[j].n - fl[j].m + 1 >= o.min_seed_len) out.push(fl[j]);
+    for (int j = nf - 1; j >= 0; --j) {
+      const Intv fj = E::get(fl, j);
+      if (fj.n - fj.m + 1 >= o.min_seed_len) out.push(fj);
+    }
     x0 = ret;
   }
   return true;
@@ -391,8 +434,9 @@ G3A_HD void smem_round1_per_end(const DevIndex &x, const Opt &o, const uint8_t *
 
 // Round 2, reseedV2 (bwa/src/seed.cu:639-913), for one SMEM.  `leps` and
 // `seeds` are scratch lists of capacity cap.  Returns false on overflow.
+template <class E>
 G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int len, const Intv &smem,
-                       Intv *leps, Intv *seeds, int cap, IntvSink &out, int64_t *steps) {
+                       E *leps, E *seeds, int cap, IntvSink &out, int64_t *steps) {
   const int split_len = (int)(o.min_seed_len * o.split_factor + .499);
   const int slen = smem.n - smem.m + 1;
   if (slen < split_len || smem.s > o.split_width) return true;
@@ -414,16 +458,20 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     seed.n = n + 1;
     if (seed.s < old.s) {
       if (nl == cap) return false;
-      leps[nl++] = old;
+      E::put(leps, nl++, old);
       if (seed.s < min_seed_intv) break;
     }
   }
   if (seed.s >= min_seed_intv) {
     seed.n = n;
     if (nl == cap) return false;
-    leps[nl++] = seed;
+    E::put(leps, nl++, seed);
   }
-  for (int a = 0, b = nl - 1; a < b; ++a, --b) { Intv t = leps[a]; leps[a] = leps[b]; leps[b] = t; }
+  for (int a = 0, b = nl - 1; a < b; ++a, --b) {
+    const Intv ta = E::get(leps, a), tb = E::get(leps, b);
+    E::put(leps, a, tb);
+    E::put(leps, b, ta);
+  }
   for (int m = pivot; m > 0; --m) {
     int base = q[m - 1];
     if (base >= 4) break;
@@ -432,27 +480,30 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     bool collected = false;
     nl = 0;
     for (int j = 0; j < num_old; ++j) {
-      Intv lep = leps[j], ext;
+      Intv lep = E::get(leps, j), ext;
       backward_ext(x, lep, base, ext);
       if (steps) ++*steps;
       if (ext.s >= curr) {
         ext.m = m - 1;
         ext.n = lep.n;
-        leps[nl++] = ext;
+        E::put(leps, nl++, ext);
         curr = ext.s;
       } else if (!collected && lep.n - lep.m + 1 >= o.min_seed_len) {
         if (ns == cap) return false;
-        seeds[ns++] = lep;
+        E::put(seeds, ns++, lep);
         collected = true;
       }
     }
     if (nl == 0) break;
   }
-  if (nl > 0 && leps[0].n - leps[0].m + 1 >= o.min_seed_len) {
-    if (ns == cap) return false;
-    seeds[ns++] = leps[0];
+  if (nl > 0) {
+    const Intv l0 = E::get(leps, 0);
+    if (l0.n - l0.m + 1 >= o.min_seed_len) {
+      if (ns == cap) return false;
+      E::put(seeds, ns++, l0);
+    }
   }
-  for (int j = 0; j < ns; ++j) out.push(seeds[j]);
+  for (int j = 0; j < ns; ++j) out.push(E::get(seeds, j));
   return true;
 }
 

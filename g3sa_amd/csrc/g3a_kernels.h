@@ -152,7 +152,7 @@ struct TbArgs {
 };
 
 // threads of k_seed's persistent grid (occupancy-limited, n_cu compute units)
-int seed_grid_threads(int n_cu);
+int seed_grid_threads(int n_cu, const DevIndex &x);
 void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s);
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
                       Intv *pool, int pass, hipStream_t s);

@@ -31,11 +31,12 @@ namespace g3a {
 // lists live in a global pool slice of its own, contiguous per thread (a
 // divergent list access then touches one cache line; private memory would
 // scatter every entry over eight lines of the per-lane swizzle).
+template <class E>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
-  Intv *la = a.lep_pool + (size_t)gtid * 2 * kLepCap, *lf = la + kLepCap;
-  LepBuf buf{la, lf, kLepCap};
+  E *la = reinterpret_cast<E *>(a.lep_pool) + (size_t)gtid * 2 * kLepCap, *lf = la + kLepCap;
+  LepBuf<E> buf{la, lf, kLepCap};
   int64_t steps = 0;
   for (int r = gtid; r < a.n_reads; r += nthreads) {
     const int off = a.off[r], len = a.off[r + 1] - off;
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
   Intv *slot = pass ? pool + pool_off[t] : sc + 3 * kSlowLep;
   const int cap = pass ? (int)(pool_off[t + 1] - pool_off[t]) : 2 * kSlowLep;
   IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};
-  LepBuf buf{sc, sc + kSlowLep, kSlowLep};
+  LepBuf<LepWide> buf{reinterpret_cast<LepWide *>(sc), reinterpret_cast<LepWide *>(sc) + kSlowLep, kSlowLep};
   if (!smem_round1(a.idx, a.opt, q, len, sink, buf, nullptr)) {
     sink.n = 0;
     sink.n_seed = 0;
@@ -677,15 +678,21 @@ __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
 // ---------------------------------------------------------------------------
 static inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-int seed_grid_threads(int n_cu) {
+// 16-byte list entries when every SA row fits 32 bits
+static bool seed_narrow(const DevIndex &x) { return x.n + 1 < (1LL << 32); }
+int seed_grid_threads(int n_cu, const DevIndex &x) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed, 128, 0) != hipSuccess || per_cu < 1) per_cu = 8;
+  hipError_t e = seed_narrow(x) ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepNarrow>, 128, 0)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepWide>, 128, 0);
+  if (e != hipSuccess || per_cu < 1) per_cu = 8;
   return n_cu * per_cu * 128;
 }
 void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s) {
   if (!a.n_reads) return;
   const int need = blocks(a.n_reads, 128), have = n_threads / 128;
-  hipLaunchKernelGGL(k_seed, dim3(need < have ? need : have), dim3(128), 0, s, a);
+  const dim3 grid(need < have ? need : have);
+  if (seed_narrow(a.idx)) hipLaunchKernelGGL(k_seed<LepNarrow>, grid, dim3(128), 0, s, a);
+  else hipLaunchKernelGGL(k_seed<LepWide>, grid, dim3(128), 0, s, a);
 }
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
                       Intv *pool, int pass, hipStream_t s) {
