@@ -8,14 +8,21 @@
  *   index / Occ / SA ........ E. coli fixtures (.sa, .counts), brute-force SA
  *                             and Occ on small texts
  *   round-1 seeding ......... brute-force longest-match-ending-at-n search
+ *   round-2 reseeding ....... the reference's control flow over brute-force
+ *                             interval sizes (planted repeats)
+ *   round-3 seeding ......... brute-force forward scan
  *   SA resolution ........... full suffix array of small texts
+ *   chaining ................ sorted-list restatement (distinct positions);
+ *                             the B-tree's duplicate-key order is pinned by
+ *                             its own emulation only
+ *   chain weight ............ closed form (interval-union sizes)
+ *   chain / region filters .. known answers at every threshold + restatement
  *   extension ............... lane-by-lane emulation of the warp program and
  *                             a textbook banded affine DP where the band does
  *                             not bind
  *   global alignment ........ textbook Gotoh score and CIGAR consistency
- *   reseeding, chaining, chain/region filters: parity unpinned beyond
- *                             hand-built cases (the reference has no
- *                             executable or golden output for them)
+ *   finalize / MAPQ ......... reads sampled at known places; MAPQ known
+ *                             answers
  */
 #define _GNU_SOURCE
 #include "oracle.h"
