@@ -25,11 +25,26 @@
 // these bounds are below the best, no later cell can equal or exceed it, so
 // neither the score nor its position can change.
 #pragma once
+#include <type_traits>
+
 #include "g3a_dev.h"
 
 namespace g3a {
 
 constexpr int kExtQCap = kMaxReadLen;  // row-buffer columns per lane
+
+// max of three signed ints as one VALU instruction on the device (the
+// compiler otherwise splits a signed/unsigned pair of maxima)
+G3A_HD int max3i(int a, int b, int c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  int d;
+  asm("v_max3_i32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+#else
+  const int t = a > b ? a : b;
+  return t > c ? t : c;
+#endif
+}
 constexpr int kExtPad = 5;             // query/target code of padding cells
 
 // Band width after clipping by max_ins/max_del (ksw.cu:840-851).
@@ -80,10 +95,12 @@ struct ExtLaneOut {
 //     any looser values give the same result)
 //   rowbuf: column j of this lane at rowbuf[j * rs]
 //   wt: score words of query code c at wt[c * cs + k], k < R/8
-template <int R, class QB, class TB>
+//   SYM: o_del + e_del == o_ins + e_ins (one M - o - e per cell)
+template <int R, bool SYM, class QB, class TB>
 G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w, int h0, int qmax, int w_min,
                            int w_max, uint32_t *rowbuf, int rs, uint32_t *wt, int cs) {
   static_assert(R == 16 || R == 32, "strip height");
+  // SYM: the caller guarantees o_del + e_del == o_ins + e_ins
   constexpr int NW = R / 8;
   const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
   const int ms = mat_max(o);
@@ -148,7 +165,9 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
     for (int r = 0; r < R; ++r) F[r] = 0;
     int u = 0;
     const int lim0 = w - i0;  // cell (i0+r, j) is in the band iff r <= j + lim0
-    for (int j = jstart; j < qmax; ++j) {
+    // one column of the strip; kMasked: some rows may lie left of the band
+    auto column = [&](int j, auto masked_tag) {
+      constexpr bool kMasked = decltype(masked_tag)::value;
       const uint32_t word = rowbuf[(int64_t)j * rs];
       const int Ht = (int)(word & 0xfff), Et = (int)((word >> 12) & 0xfff);
       const uint32_t qc = word >> 24;
@@ -167,41 +186,50 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
         hc[0] = htop ? m : 0;
         htop = Ht;
       }
-      int e = Et, ck = 0;
-      const bool masked = j < jfull;
+      int e = Et, cmax = 0;
       const int lim = j + lim0;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int m = hc[r], f = F[r];
-        int h = m > e ? m : e;
-        h = h > f ? h : f;
-        int e2 = e - o.e_del, t = m - oe_del;
-        e2 = e2 > t ? e2 : t;
+        int h = max3i(m, e, f);
+        const int td = m - oe_del, ti = SYM ? td : m - oe_ins;
+        int e2 = e - o.e_del;
+        e2 = e2 > td ? e2 : td;
         e2 = e2 > 0 ? e2 : 0;
         int f2 = f - o.e_ins;
-        t = m - oe_ins;
-        f2 = f2 > t ? f2 : t;
+        f2 = f2 > ti ? f2 : ti;
         f2 = f2 > 0 ? f2 : 0;
-        if (masked && r > lim) {
+        if (kMasked && r > lim) {
           h = 0;
           f2 = 0;
         }
         hc[r] = h;
         F[r] = f2;
         e = e2;
-        const int key = (h << 5) | (31 - r);
-        ck = ck > key ? ck : key;
+        if (r & 1) cmax = max3i(cmax, hc[r - 1], h);
       }
       rowbuf[(int64_t)j * rs] = (uint32_t)hc[R - 1] | ((uint32_t)e << 12) | (qc << 24);
-      const int ch = ck >> 5, ci = i0 + 31 - (ck & 31);
-      if (ch > best || (ch == best && ci <= bi)) {
-        best = ch;
-        bi = ci;
-        bj = j;
+      // the column's best cell: only when it can change the result, find the
+      // smallest row holding the column maximum
+      if (cmax > best || (cmax == best && bi >= i0)) {
+        int rmin = R - 1;
+#pragma unroll
+        for (int r = R - 2; r >= 0; --r)
+          if (hc[r] == cmax) rmin = r;
+        const int ci = i0 + rmin;
+        if (cmax > best || ci <= bi) {
+          best = cmax;
+          bi = ci;
+          bj = j;
+        }
       }
       const int hl = hc[R - 1], ub = hl + ms * (qlen - 1 - j);
       if (hl > 0 && ub > u) u = ub;
-    }
+    };
+    const int jm = jfull < qmax ? jfull : qmax;
+    int j = jstart;
+    for (; j < jm; ++j) column(j, std::true_type());
+    for (; j < qmax; ++j) column(j, std::false_type());
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int i = i0 + r, lo = i - w > 0 ? i - w : 0;
