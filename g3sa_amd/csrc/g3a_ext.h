@@ -10,20 +10,22 @@
 // column from a per-lane row buffer laid out column-major across the
 // wavefront, so a column is one coalesced 256-byte access.
 //
-// The engine is exact for the tasks ext_lane_ok() admits: those for which the
-// reference's column limit 32*(i/32+1)+w+1, its corner rule and its quirk
-// rows never reach a column below qlen, and whose scores fit 12 bits.  Other
-// tasks, and tasks whose best score equals h0 after at least 32 rows (where
-// the reference's 32-lane residue rule decides the position), go to the
-// one-wavefront-per-task kernel.
+// The engine applies every rule of the reference's cell set: the band's left
+// edge, the column limit 32*(i/32+1)+w+1, the corner rule and the quirk rows
+// (only in the few columns of a strip where one of them can apply to some
+// lane; the other columns run without any test).  It admits every task
+// whose scores fit its 12-bit cells; tasks whose best score equals h0 after
+// at least 32 rows (where the reference's 32-lane residue rule decides the
+// position) go to the one-wavefront-per-task kernel.
 //
 // Exact pruning: after each strip the lane stops once no later cell can
 // reach the current best.  A later cell's value comes from a positive cell of
 // the strip's last row (through the diagonal or E, both bounded by that
-// cell's H) gaining at most max(mat) per remaining query column, or from the
-// column -1 boundary max(h0 - o_del - e_del*i, 0) of a later row; when all of
-// these bounds are below the best, no later cell can equal or exceed it, so
-// neither the score nor its position can change.
+// cell's H) gaining at most max(mat) per remaining query column, from the
+// column -1 boundary max(h0 - o_del - e_del*i, 0) of a later row, or from a
+// later quirk row's injected diagonal; when all of these bounds are below the
+// best, no later cell can equal or exceed it, so neither the score nor its
+// position can change.
 #pragma once
 #include <type_traits>
 
@@ -74,12 +76,12 @@ G3A_HD bool ext_lane_opt_ok(const Opt &o) {
   return o.e_del >= 0 && o.e_ins >= 0 && o.o_del >= 0 && o.o_ins >= 0;
 }
 
-// Can the engine compute this task exactly?
+// Can the engine compute this task exactly?  (Scores must fit its 12-bit
+// cells; every rule of the cell set is applied in the general columns.)
 G3A_HD bool ext_lane_ok(const Opt &o, int qlen, int w, int h0) {
+  (void)w;
   if (qlen <= 0 || qlen > kExtQCap || h0 < 0) return false;
-  if (qlen > w + 33) return false;            // column limit and quirk rows stay right of qlen
-  if (w < 29 && qlen > w + 2) return false;  // corner rule stays right of qlen
-  return h0 + mat_max(o) * qlen <= 4095;     // 12-bit H and E
+  return h0 + mat_max(o) * qlen <= 4095;
 }
 
 struct ExtLaneOut {
@@ -165,15 +167,29 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
     for (int r = 0; r < R; ++r) F[r] = 0;
     int u = 0;
     const int lim0 = w - i0;  // cell (i0+r, j) is in the band iff r <= j + lim0
-    // one column of the strip; kMasked: some rows may lie left of the band
-    auto column = [&](int j, auto masked_tag) {
-      constexpr bool kMasked = decltype(masked_tag)::value;
+    // the reference's other rules for this strip: the column limit of its
+    // 32-row block, the quirk row at the block start (i0 = 32k, k >= 1) whose
+    // cells right of i0+w+1 take an injected diagonal and no E, and the
+    // corner rule (i+j <= 30 needs j <= i+w+1)
+    const int jlim = 32 * (i0 / 32 + 1) + w + 1;
+    const bool quirk = i0 >= 32 && (i0 & 31) == 0;
+    // one column of the strip; kGeneral: some rule above or the band's left
+    // edge may apply to a row of this column
+    auto column = [&](int j, auto general_tag) {
+      constexpr bool kGeneral = decltype(general_tag)::value;
       const uint32_t word = rowbuf[(int64_t)j * rs];
-      const int Ht = (int)(word & 0xfff), Et = (int)((word >> 12) & 0xfff);
+      const int Ht = (int)(word & 0xfff);
+      int Et = (int)((word >> 12) & 0xfff);
       const uint32_t qc = word >> 24;
       uint32_t W[NW];
 #pragma unroll
       for (int k = 0; k < NW; ++k) W[k] = wt[qc * cs + k];
+      int hd0 = htop;
+      if (kGeneral && quirk && j > i0 + w + 1) {
+        const int v = h0 - oe_ins - (j - 1) * o.e_ins;
+        hd0 = v > 0 ? v : 0;
+        Et = 0;
+      }
       // M of every row from the previous column's H (bottom-up, in place)
 #pragma unroll
       for (int r = R - 1; r >= 1; --r) {
@@ -182,12 +198,14 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
         hc[r] = hd ? m : 0;
       }
       {
-        const int m = htop + (int)(W[0] & 15) - 8;
-        hc[0] = htop ? m : 0;
+        const int m = hd0 + (int)(W[0] & 15) - 8;
+        hc[0] = hd0 ? m : 0;
         htop = Ht;
       }
       int e = Et, cmax = 0;
-      const int lim = j + lim0;
+      const int lim = j + lim0;                  // band: r <= lim
+      const bool in_lim = j <= jlim;             // column limit
+      const int cb = 30 - i0 - j, cc = j - i0 - w - 1;  // corner: r > cb or r >= cc
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int m = hc[r], f = F[r];
@@ -199,9 +217,10 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
         int f2 = f - o.e_ins;
         f2 = f2 > ti ? f2 : ti;
         f2 = f2 > 0 ? f2 : 0;
-        if (kMasked && r > lim) {
+        if (kGeneral && !(r <= lim && in_lim && (r > cb || r >= cc))) {
           h = 0;
           f2 = 0;
+          e2 = 0;
         }
         hc[r] = h;
         F[r] = f2;
@@ -226,19 +245,42 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
       const int hl = hc[R - 1], ub = hl + ms * (qlen - 1 - j);
       if (hl > 0 && ub > u) u = ub;
     };
-    const int jm = jfull < qmax ? jfull : qmax;
+    // columns [ju0, ju1) need no rule for any lane of the wavefront
+    int ju0 = jfull > jstart ? jfull : jstart;
+    if (i0 + w_min + 1 < 30 - i0 && ju0 < 31 - i0) ju0 = 31 - i0;
+    int ju1 = 32 * (i0 / 32 + 1) + w_min + 2;
+    if (quirk && i0 + w_min + 2 < ju1) ju1 = i0 + w_min + 2;
+    if (ju1 > qmax) ju1 = qmax;
+    if (ju0 > qmax) ju0 = qmax;
+    if (ju1 < ju0) ju1 = ju0;
     int j = jstart;
-    for (; j < jm; ++j) column(j, std::true_type());
-    for (; j < qmax; ++j) column(j, std::false_type());
+    for (; j < ju0; ++j) column(j, std::true_type());
+    for (; j < ju1; ++j) column(j, std::false_type());
+    for (; j < qmax; ++j) column(j, std::true_type());
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int i = i0 + r, lo = i - w > 0 ? i - w : 0;
-      if (i < tlen && qlen > lo) cells += qlen - lo;
+      const int i = i0 + r;
+      if (i >= tlen) continue;
+      const int lo = i - w > 0 ? i - w : 0;
+      int hi = qlen - 1 < jlim ? qlen - 1 : jlim;
+      if (hi < lo) continue;
+      int n = hi - lo + 1;
+      const int c0 = lo > i + w + 2 ? lo : i + w + 2, c1 = hi < 30 - i ? hi : 30 - i;  // corner cells
+      if (c1 >= c0) n -= c1 - c0 + 1;
+      cells += n;
     }
     i0 += R;
     if (i0 >= tlen) break;
+    // bound on every later cell: paths through the strip's last row, the
+    // column -1 boundary of later rows, and the next quirk row's injections
     const int b = h0 - o.o_del - o.e_del * i0;
-    const int ub = b > 0 ? b + ms * qlen : 0;
+    int ub = b > 0 ? b + ms * qlen : 0;
+    const int iq = (i0 + 31) & ~31;
+    if (iq + w + 2 <= qlen - 1) {
+      const int dq = h0 - oe_ins - (iq + w + 1) * o.e_ins;
+      const int bq = dq > 0 ? dq + ms * (qlen - iq - w - 2) : 0;
+      ub = ub > bq ? ub : bq;
+    }
     if ((u > ub ? u : ub) < best) break;
   }
   ExtLaneOut out;

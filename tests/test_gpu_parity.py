@@ -140,9 +140,10 @@ def _with_scoring(oracle, w=None, b=None):
 
 @pytest.mark.parametrize("w,b", [(20, None), (None, 9)])
 def test_scoring_variants_parity(ds_c1, oracle, w, b):
-    """A narrow band (the lane engine's column-limit and corner conditions
-    bind for many tasks) and a mismatch score outside the lane engine's
-    4-bit range (every task takes the one-wavefront kernel)."""
+    """A narrow band (the column limit, corner rule and quirk rows of the
+    reference's cell set apply to many cells of the lane engine's general
+    columns) and a mismatch score outside the lane engine's 4-bit range
+    (every task takes the one-wavefront kernel)."""
     names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["c3"]), limit=1500)
     oo, go = _with_scoring(oracle, w, b)
     idx = oracle.load(ds_c1.prefix)
