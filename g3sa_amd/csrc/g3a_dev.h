@@ -91,17 +91,41 @@ G3A_HD void line_counts(uint64_t w0, uint64_t w1, int y, uint32_t o[4]) {
   o[0] = (uint32_t)(y + 1) - n1 - n2 - n3;
 }
 
-// Occ(p, c): occurrences of c in rows [0, p], '$' excluded
-G3A_HD void occ4(const DevIndex &x, int64_t p, int64_t o[4]) {
-  const uint32_t *b = x.occ + ((p >> 6) << 3);
-  const uint64_t w0 = (uint64_t)b[4] | (uint64_t)b[5] << 32, w1 = (uint64_t)b[6] | (uint64_t)b[7] << 32;
+// One Occ line: the four counts and the two symbol words.
+struct OccLine {
+  uint32_t c[4];
+  uint64_t w0, w1;
+};
+G3A_HD OccLine occ_line(const DevIndex &x, int64_t line) {
+  const uint32_t *b = x.occ + (line << 3);
+  OccLine L;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) L.c[k] = b[k];
+  L.w0 = (uint64_t)b[4] | (uint64_t)b[5] << 32;
+  L.w1 = (uint64_t)b[6] | (uint64_t)b[7] << 32;
+  return L;
+}
+// Occ(p, c) for all c from p's line
+G3A_HD void occ_at(const DevIndex &x, const OccLine &L, int64_t p, int64_t o[4]) {
   uint32_t in[4];
-  line_counts(w0, w1, (int)(p & 63), in);
+  line_counts(L.w0, L.w1, (int)(p & 63), in);
   const int64_t *sb = x.super[p >> 32];
   const int dollar = (p >> 6) == (x.primary >> 6) && p >= x.primary;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) o[c] = sb[c] + (int64_t)b[c] + in[c];
+  for (int c = 0; c < 4; ++c) o[c] = sb[c] + (int64_t)L.c[c] + in[c];
   o[0] -= dollar;
+}
+
+// Occ(p, c): occurrences of c in rows [0, p], '$' excluded
+G3A_HD void occ4(const DevIndex &x, int64_t p, int64_t o[4]) { occ_at(x, occ_line(x, p >> 6), p, o); }
+
+// Occ at two rows; one line fetch when both lie in the same line (the
+// common case once an interval has narrowed to a few rows)
+G3A_HD void occ4_pair(const DevIndex &x, int64_t p1, int64_t p2, int64_t o1[4], int64_t o2[4]) {
+  OccLine L = occ_line(x, p1 >> 6);
+  occ_at(x, L, p1, o1);
+  if ((p2 >> 6) != (p1 >> 6)) L = occ_line(x, p2 >> 6);
+  occ_at(x, L, p2, o2);
 }
 
 G3A_HD int64_t occ1(const DevIndex &x, int64_t p, int c) {
@@ -129,8 +153,7 @@ G3A_HD void intv_init(const DevIndex &x, Intv &p, int c) {
 // backwardExt (fmindex.cu:107-160): extension by `base` with the l update
 G3A_HD void backward_ext(const DevIndex &x, const Intv &in, int base, Intv &out) {
   int64_t osp[4], oep[4];
-  occ4(x, in.k - 1, osp);
-  occ4(x, in.k + in.s - 1, oep);
+  occ4_pair(x, in.k - 1, in.k + in.s - 1, osp, oep);
   int64_t s[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) s[b] = oep[b] - osp[b];
@@ -144,9 +167,10 @@ G3A_HD void backward_ext(const DevIndex &x, const Intv &in, int base, Intv &out)
 
 // backwardExtBackward (fmindex.cu:162-180): k and s only
 G3A_HD void backward_ext_b(const DevIndex &x, Intv &io, int base) {
-  int64_t a = occ1(x, io.k - 1, base), b = occ1(x, io.k + io.s - 1, base);
-  io.k = x.count[base] + a;
-  io.s = b - a;
+  int64_t oa[4], ob[4];
+  occ4_pair(x, io.k - 1, io.k + io.s - 1, oa, ob);
+  io.k = x.count[base] + oa[base];
+  io.s = ob[base] - oa[base];
 }
 
 // FORWARD_EXT (macro.h:64-71)

@@ -183,3 +183,34 @@ def test_autotune_keeps_output(ds_c1, oracle):
         assert t["strip_rows"] in (16, 32) and t["lane_blocks_per_cu"] in (2, 4, 8, 16)
         compare(al.download(), ref, "after autotune")
         compare(al.align(seq, off), ref, "next batch after autotune")
+
+
+def test_api_error_paths(ds_c1):
+    """Documented failures come back as codes with a message; the context
+    stays usable after each."""
+    lib = g3sa_amd.load_library()
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["main"]), limit=50)
+    ctx = C.c_void_p()
+    assert lib.g3a_device_alloc(0, str(ds_c1.prefix).encode(), None, 50, int(off[-1]), C.byref(ctx)) == 0
+    try:
+        assert lib.g3a_bwa_align(ctx) == -4                     # index not on the device yet
+        assert lib.g3a_memcpy_index(ctx) == 0
+        assert lib.g3a_bwa_align(ctx) == -4                     # no input yet
+        out = g3sa_amd.BatchOut()
+        assert lib.g3a_memcpy_output(ctx, C.byref(out)) == -4   # nothing aligned
+        bad = off.copy()
+        bad[3], bad[4] = bad[4], bad[3]                          # offsets not monotonic
+        assert lib.g3a_memcpy_input(ctx, 50, seq.ctypes.data, bad.ctypes.data) == -1
+        assert b"non-decreasing" in lib.g3a_last_error(ctx)
+        big = np.concatenate([off, off[-1:] + 150]).astype(np.int32)
+        assert lib.g3a_memcpy_input(ctx, 51, seq.ctypes.data, big.ctypes.data) == -1   # > max_reads
+        assert lib.g3a_set_form(ctx, 7) == -1
+        assert lib.g3a_set_tuning(ctx, 24, 8, 8) == -1
+        # still usable
+        assert lib.g3a_memcpy_input(ctx, 50, seq.ctypes.data, off.ctypes.data) == 0
+        assert lib.g3a_bwa_align(ctx) == 0
+        assert lib.g3a_memcpy_output(ctx, C.byref(out)) == 0
+        assert out.n_reads == 50
+        lib.g3a_batch_out_free(C.byref(out))
+    finally:
+        lib.g3a_destroy(ctx)
