@@ -42,7 +42,7 @@ struct g3a_ctx {
   int max_reads = 0;
   int64_t max_bases = 0;
   // index
-  DevBuf occ, pac, sa, ctg_off, ctg_len, ctg_alt;
+  DevBuf occ, pac, sa, ctg_off, ctg_len, ctg_alt, kmer;
   // batch input
   int n_reads = 0;
   bool has_input = false, has_output = false;
@@ -241,6 +241,14 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   x.ctg_off = P<int64_t>(ctx->ctg_off);
   x.ctg_len = P<int64_t>(ctx->ctg_len);
   x.ctg_alt = P<int32_t>(ctx->ctg_alt);
+  x.kmer = nullptr;
+  if (x.n + 1 < (1LL << 32)) {  // 32-bit interval fields
+    if (ensure(ctx, ctx->kmer, ((size_t)3 << (2 * kKmer)) * 4)) return G3A_E_HIP;
+    launch_kmer_table(x, P<uint32_t>(ctx->kmer), ctx->stream);
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    HIPCK(hipGetLastError());
+    x.kmer = P<uint32_t>(ctx->kmer);
+  }
   ctx->index_on_device = true;
   return G3A_OK;
 }
@@ -720,7 +728,7 @@ void g3a_destroy(g3a_ctx *ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  DevBuf *bufs[] = {&ctx->occ, &ctx->pac, &ctx->sa, &ctx->ctg_off, &ctx->ctg_len, &ctx->ctg_alt, &ctx->seq, &ctx->off,
+  DevBuf *bufs[] = {&ctx->occ, &ctx->kmer, &ctx->pac, &ctx->sa, &ctx->ctg_off, &ctx->ctg_len, &ctx->ctg_alt, &ctx->seq, &ctx->off,
                     &ctx->intv, &ctx->n_intv, &ctx->n_seed, &ctx->slow, &ctx->intv_base, &ctx->seed_off,
                     &ctx->counters, &ctx->slow_list, &ctx->slow_scratch, &ctx->slow_n, &ctx->slow_off, &ctx->seeds,
                     &ctx->seeds2, &ctx->ch, &ctx->nxt, &ctx->order, &ctx->tmpi, &ctx->w16, &ctx->kept, &ctx->nodes,

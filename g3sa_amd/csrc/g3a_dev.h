@@ -53,7 +53,12 @@ struct DevIndex {
   const int64_t *ctg_off;   // n_contigs
   const int64_t *ctg_len;
   const int32_t *ctg_alt;
+  // bidirectional interval (k, l, s) of every kKmer-mer, 2-bit code of the
+  // first base highest; null when the text has 2^32 rows or more
+  const uint32_t *kmer;
 };
+
+constexpr int kKmer = 12;
 
 struct Intv {
   int64_t k, l, s;
@@ -532,19 +537,38 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
 }
 
 // Round 3, reseedThirdRound (bwa/src/seed.cu:1143-1190) via reseedLastRound
-// (seed.cu:1518-1550).
+// (seed.cu:1518-1550).  Every start m extends forward at least to length
+// min_len before it can emit, so when the first kKmer bases hold no N their
+// interval is read from the k-mer table instead of kKmer-1 forward steps.
 G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
                           int64_t *steps) {
   const int min_len = o.min_seed_len + 1;
   const int64_t max_intv = o.max_mem_intv;
+  const bool table = x.kmer && min_len > kKmer;
   int m = 0;
   while (m < len) {
     int next_m = m + 1;
     int base = q[m];
     if (base < 4) {
       Intv seed;
-      intv_init(x, seed, base);
-      for (int n = m + 1; n < len; ++n) {
+      int n0 = m + 1;
+      uint32_t code = 0;
+      bool jumped = false;
+      if (table && m + kKmer <= len) {
+        int k = 0;
+        for (; k < kKmer && q[m + k] < 4; ++k) code = code << 2 | q[m + k];
+        if (k == kKmer) {
+          const uint32_t *e = x.kmer + 3 * (uint64_t)code;
+          seed.k = e[0];
+          seed.l = e[1];
+          seed.s = e[2];
+          n0 = m + kKmer;
+          next_m = n0;
+          jumped = true;
+        }
+      }
+      if (!jumped) intv_init(x, seed, base);
+      for (int n = n0; n < len; ++n) {
         next_m = n + 1;
         base = q[n];
         if (base < 4) {
@@ -563,6 +587,17 @@ G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int
     }
     m = next_m;
   }
+}
+
+// Interval of k-mer `code` (kKmer bases, first base in the highest bits) by
+// kKmer-1 forward steps from the first base: the k-mer table's entries.
+G3A_HD void kmer_interval(const DevIndex &x, uint32_t code, uint32_t out[3]) {
+  Intv seed;
+  intv_init(x, seed, (int)(code >> (2 * (kKmer - 1))) & 3);
+  for (int i = kKmer - 2; i >= 0; --i) forward_ext(x, seed, (int)(code >> (2 * i)) & 3);
+  out[0] = (uint32_t)seed.k;
+  out[1] = (uint32_t)seed.l;
+  out[2] = (uint32_t)seed.s;
 }
 
 // ---------------------------------------------------------------------------

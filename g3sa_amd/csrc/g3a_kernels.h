@@ -167,6 +167,8 @@ void launch_ext_wave(const ExtSideArgs &a, int n_blocks, hipStream_t s);
 void launch_ext_finish(const ExtFinishArgs &a, hipStream_t s);
 void launch_fill_i32(int32_t *p, int n, int32_t v, hipStream_t s);
 void launch_clamp_codes(uint8_t *seq, int64_t n, hipStream_t s);
+// fills the 3 * 4^kKmer words of the k-mer interval table
+void launch_kmer_table(const DevIndex &x, uint32_t *table, hipStream_t s);
 // descending radix sort of (key < 512, value) pairs
 size_t ext_sort_tmp_bytes(int64_t n);
 int ext_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,

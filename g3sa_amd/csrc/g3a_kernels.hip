@@ -613,6 +613,20 @@ __global__ void __launch_bounds__(256) k_ext_finish(ExtFinishArgs a) {
   a.regs[t] = g;
 }
 
+// k-mer table: thread per k-mer code
+__global__ void __launch_bounds__(256) k_kmer_table(DevIndex x, uint32_t *table) {
+  const uint32_t code = blockIdx.x * blockDim.x + threadIdx.x;
+  if (code >= (1u << (2 * kKmer))) return;
+  uint32_t e[3];
+  kmer_interval(x, code, e);
+  table[3 * (size_t)code] = e[0];
+  table[3 * (size_t)code + 1] = e[1];
+  table[3 * (size_t)code + 2] = e[2];
+}
+void launch_kmer_table(const DevIndex &x, uint32_t *table, hipStream_t s) {
+  hipLaunchKernelGGL(k_kmer_table, dim3((1u << (2 * kKmer)) / 256), dim3(256), 0, s, x, table);
+}
+
 // Input codes above 4 (anything but A, C, G, T) become 4.
 __global__ void k_clamp_codes(uint8_t *seq, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
