@@ -217,7 +217,9 @@ def main() -> int:
     mean, kms, work = m["stages_ms"], m["kernel_ms"], m["work"]
     # dominant kernel and its roofline (k_seed: HBM; extension: integer VALU)
     ext_ms = kms["ext_lane"] + kms["ext_wave"]
-    seed_bytes = 128 * work["fm_steps"] + int(off[-1])
+    # algorithmic bytes of k_seed: each FM step reads two 32-byte Occ lines
+    # (rows k-1 and k+s-1), plus the read's bases once
+    seed_bytes = 64 * work["fm_steps"] + int(off[-1])
     seed_gbs = seed_bytes / (kms["k_seed"] * 1e-3) / 1e9 if kms["k_seed"] > 0 else 0.0
     dominant = "k_seed" if kms["k_seed"] >= ext_ms else "extension"
     roof = {"bound": "hbm", "kernel": "k_seed", "achieved": round(seed_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -47,7 +47,7 @@ struct g3a_ctx {
   int n_reads = 0;
   bool has_input = false, has_output = false;
   DevBuf seq, off;
-  DevBuf lep_pool, eh_pool;
+  DevBuf lep_pool, eh_pool, tb_order, tb_ctr;
   int seed_threads = 0, tb_threads = 0;
   // seeding
   DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
@@ -537,6 +537,9 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   if (!ctx->tb_threads) ctx->tb_threads = traceback_grid_threads(ctx->n_cu);
   if (ensure(ctx, ctx->eh_pool, (size_t)traceback_pool_words(ctx->tb_threads) * 4)) return G3A_E_HIP;
   tb.eh_pool = P<int32_t>(ctx->eh_pool);
+  if (ensure(ctx, ctx->tb_order, (size_t)T_a * 4 + 4) || ensure(ctx, ctx->tb_ctr, 64)) return G3A_E_HIP;
+  tb.tb_order = P<int32_t>(ctx->tb_order);
+  tb.tb_ctr = P<int32_t>(ctx->tb_ctr);
   HIPCK(hipEventRecord(ctx->kev[10], s));
   launch_traceback(tb, ctx->tb_threads, s);
   HIPCK(hipEventRecord(ctx->kev[11], s));
@@ -737,7 +740,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp,
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
                     &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
-                    &ctx->lep_pool, &ctx->eh_pool};
+                    &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

@@ -149,6 +149,8 @@ struct TbArgs {
   uint32_t *cigar;
   Aln *alns;
   int32_t *eh_pool;  // traceback_pool_words(threads) words
+  int32_t *tb_order;  // n_alns: processing order
+  int32_t *tb_ctr;    // 2 counters
 };
 
 // threads of k_seed's persistent grid (occupancy-limited, n_cu compute units)

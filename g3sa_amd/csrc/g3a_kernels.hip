@@ -670,13 +670,30 @@ __global__ void __launch_bounds__(128) k_aln_prep(AlnPrepArgs a) {
   }
 }
 
+// Alignment order for k_traceback: those that need the banded global
+// alignment (w > 0) first, the ungapped ones (w = 0) last, so that the DP
+// loops of a wavefront run with all lanes busy.
+__global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = t < a.n_alns;
+  bool dp = false;
+  if (in) {
+    const Region g = a.aln_reg[t];
+    dp = global_bw(a.opt, g.qe - g.qb, g.re - g.rb, g.score) != 0;
+  }
+  const int f = wave_reserve(in && dp, a.tb_ctr);
+  const int b = wave_reserve(in && !dp, a.tb_ctr + 1);
+  if (in) a.tb_order[dp ? f : a.n_alns - 1 - b] = (int32_t)t;
+}
+
 // Persistent grid; the global-alignment row buffers of each thread live in a
 // global pool slice of its own (2 * kQMax words), contiguous per thread.
 __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
   int32_t *eh_h = a.eh_pool + gtid * 2 * kQMax, *eh_e = eh_h + kQMax;
-  for (int64_t t = gtid; t < a.n_alns; t += nthreads) {
+  for (int64_t e = gtid; e < a.n_alns; e += nthreads) {
+    const int64_t t = a.tb_order[e];
     const int r = a.aln_read[t];
     const int off = a.off[r], l_seq = a.off[r + 1] - off;
     const Region g = a.aln_reg[t];
@@ -767,6 +784,8 @@ int traceback_grid_threads(int n_cu) {
 int64_t traceback_pool_words(int n_threads) { return (int64_t)n_threads * 2 * kQMax; }
 void launch_traceback(const TbArgs &a, int n_threads, hipStream_t s) {
   if (!a.n_alns) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, a.tb_ctr, 2, 0);
+  hipLaunchKernelGGL(k_tb_order, dim3((unsigned)((a.n_alns + 255) / 256)), dim3(256), 0, s, a);
   const int64_t need = (a.n_alns + 63) / 64, have = n_threads / 64;
   hipLaunchKernelGGL(k_traceback, dim3((unsigned)(need < have ? need : have)), dim3(64), 0, s, a);
 }
