@@ -12,6 +12,11 @@
 #include "../../include/g3a.h"
 #include "g3a_index.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +28,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -36,6 +42,18 @@ struct Batch {
   void clear() {
     names.clear(); quals.clear(); seq.clear();
     name_off.assign(1, 0); qual_off.assign(1, 0); seq_off.assign(1, 0);
+  }
+};
+
+// A0 C1 G2 T3 (either case), anything else 4
+struct Nt4Table {
+  uint8_t v[256];
+  Nt4Table() {
+    std::memset(v, 4, sizeof v);
+    v['A'] = v['a'] = 0;
+    v['C'] = v['c'] = 1;
+    v['G'] = v['g'] = 2;
+    v['T'] = v['t'] = 3;
   }
 };
 
@@ -54,18 +72,42 @@ uint8_t nt4(int c) {
 // the rest of the header line, SEQ, the separator line and QUAL); a record
 // cut by a chunk boundary is completed from the next chunk.
 struct FastqReader {
-  size_t kChunk = 64u << 20;  // G3A_FASTQ_CHUNK overrides (bytes)
+  size_t kChunk = 64u << 20;  // G3A_FASTQ_CHUNK overrides (bytes) and disables the mapping
   FILE *fp = nullptr;
   std::vector<char> buf;
+  const char *map = nullptr;  // whole regular file mapped read-only
+  size_t map_len = 0;
   size_t pos = 0, len = 0;
   bool eof = false;
   ~FastqReader() {
     if (fp) std::fclose(fp);
+    if (map) munmap((void *)map, map_len);
   }
+  const char *data() const { return map ? map : buf.data(); }
   bool open(const char *fn) {
+    bool chunked = false;
     if (const char *e = std::getenv("G3A_FASTQ_CHUNK")) {
       const long long v = std::atoll(e);
       if (v >= 16) kChunk = (size_t)v;
+      chunked = true;
+    }
+    if (!chunked) {
+      const int fd = ::open(fn, O_RDONLY);
+      struct stat st;
+      if (fd >= 0 && fstat(fd, &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0) {
+        void *m = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m != MAP_FAILED) {
+          madvise(m, (size_t)st.st_size, MADV_SEQUENTIAL);
+          map = (const char *)m;
+          map_len = len = (size_t)st.st_size;
+          eof = true;
+        }
+      }
+      if (fd >= 0) ::close(fd);
+      if (map) {
+        skip_to_record();
+        return true;
+      }
     }
     fp = std::fopen(fn, "rb");
     if (!fp) return false;
@@ -88,7 +130,7 @@ struct FastqReader {
   }
   void skip_to_record() {
     for (;;) {
-      while (pos < len && buf[pos] != '@') ++pos;
+      while (pos < len && data()[pos] != '@') ++pos;
       if (pos < len || eof) return;
       fill();
     }
@@ -96,37 +138,150 @@ struct FastqReader {
   // a whole record (four lines) is buffered, or the file ends inside it
   bool have_record() {
     for (;;) {
+      const char *p = data() + pos, *e = data() + len;
       int nl = 0;
-      for (size_t i = pos; i < len; ++i)
-        if (buf[i] == '\n' && ++nl == 4) return true;
+      while (p < e) {
+        const char *q = (const char *)std::memchr(p, '\n', (size_t)(e - p));
+        if (!q) break;
+        if (++nl == 4) return true;
+        p = q + 1;
+      }
       if (eof) return pos < len;
       fill();
     }
   }
+  // end of the line starting at i (the newline, or len)
+  size_t eol(size_t i) const {
+    const char *q = (const char *)std::memchr(data() + i, '\n', len > i ? len - i : 0);
+    return q ? (size_t)(q - data()) : len;
+  }
   bool next(Batch &b) {
+    static const Nt4Table tab;
     if (pos >= len && eof) return false;
     if (!have_record()) return false;
-    const size_t end = len;
-    const char *p = buf.data();
+    const char *p = data();
+    // header: NAME up to the first space or the end of the line
     size_t q = ++pos;
-    while (pos < end && p[pos] != ' ' && p[pos] != '\n') ++pos;
-    b.names.append(p + q, pos - q);
+    const size_t h_end = eol(q);
+    const char *sp = (const char *)std::memchr(p + q, ' ', h_end - q);
+    const size_t n_end = sp ? (size_t)(sp - p) : h_end;
+    b.names.append(p + q, n_end - q);
     b.name_off.push_back((int64_t)b.names.size());
-    while (pos < end && p[pos] != '\n') ++pos;
-    q = ++pos;
-    while (pos < end && p[pos] != '\n') ++pos;
-    for (size_t i = q; i < pos && i < end; ++i) b.seq.push_back(nt4((unsigned char)p[i]));
+    // sequence
+    q = h_end + 1 < len ? h_end + 1 : len;
+    const size_t s_end = eol(q);
+    const size_t old = b.seq.size();
+    b.seq.resize(old + (s_end - q));
+    uint8_t *d = b.seq.data() + old;
+    for (size_t i = q; i < s_end; ++i) *d++ = tab.v[(unsigned char)p[i]];
     b.seq_off.push_back((int32_t)b.seq.size());
-    ++pos;
-    while (pos < end && p[pos] != '\n') ++pos;
-    q = ++pos;
-    while (pos < end && p[pos] != '\n') ++pos;
-    if (q < end) b.quals.append(p + q, (pos < end ? pos : end) - q);
+    // separator line, then qualities
+    q = s_end + 1 < len ? s_end + 1 : len;
+    const size_t p_end = eol(q);
+    q = p_end + 1 < len ? p_end + 1 : len;
+    const size_t q_end = eol(q);
+    if (q < len) b.quals.append(p + q, q_end - q);
     b.qual_off.push_back((int64_t)b.quals.size());
-    ++pos;
-    if (pos > len) pos = len;
+    pos = q_end + 1 < len ? q_end + 1 : len;
     skip_to_record();
     return true;
+  }
+
+  // Up to max_reads records into b, parsed by `threads` threads: a pass over
+  // the buffer finds the records' line ends, then the records found are split
+  // into equal parts whose lengths are summed and whose fields are copied in
+  // parallel; the buffer is refilled until the batch is full or the file ends.
+  void next_batch(Batch &b, int max_reads, int threads) {
+    struct Rec {
+      size_t h, s, p, q, e;  // header, sequence, separator, quality starts; end
+      size_t name_end;
+    };
+    std::vector<Rec> recs;
+    for (;;) {
+      recs.clear();
+      const int want = max_reads - b.n();
+      while ((int)recs.size() < want && pos < len) {
+        size_t ends[4], i = pos;
+        int k = 0;
+        for (; k < 4; ++k) {
+          const char *q = (const char *)std::memchr(data() + i, '\n', len - i);
+          if (!q) break;
+          ends[k] = (size_t)(q - data());
+          i = ends[k] + 1;
+        }
+        if (k < 4) break;  // cut by the end of the buffer
+        Rec r;
+        r.h = pos + 1;
+        r.s = ends[0] + 1;
+        r.p = ends[1] + 1;
+        r.q = ends[2] + 1;
+        r.e = ends[3];
+        recs.push_back(r);
+        pos = ends[3] + 1;
+        while (pos < len && data()[pos] != '@') ++pos;
+      }
+      copy_records(b, recs, threads);
+      if (b.n() >= max_reads) return;
+      if (eof) break;
+      fill();
+      skip_to_record();
+    }
+    // the file's last record without a final newline
+    while (b.n() < max_reads && next(b)) {}
+  }
+
+ private:
+  template <class Rec>
+  void copy_records(Batch &b, std::vector<Rec> &recs, int threads) {
+    const size_t n0 = (size_t)b.n(), n = recs.size();
+    if (!n) return;
+    std::vector<int64_t> nl(n + 1, 0), sl(n + 1, 0), ql(n + 1, 0);
+    const char *p = data();
+    const int nt = n < 20000 ? 1 : threads;
+    auto run = [&](auto fn) {
+      std::vector<std::thread> th;
+      for (int t = 1; t < nt; ++t) th.emplace_back(fn, t);
+      fn(0);
+      for (auto &x : th) x.join();
+    };
+    run([&](int t) {
+      const size_t r0 = n * (size_t)t / (size_t)nt, r1 = n * (size_t)(t + 1) / (size_t)nt;
+      for (size_t i = r0; i < r1; ++i) {
+        Rec &x = recs[i];
+        const size_t h_end = x.s - 1;
+        const char *sp = (const char *)std::memchr(p + x.h, ' ', h_end - x.h);
+        x.name_end = sp ? (size_t)(sp - p) : h_end;
+        nl[i + 1] = (int64_t)(x.name_end - x.h);
+        sl[i + 1] = (int64_t)(x.p - 1 - x.s);
+        ql[i + 1] = (int64_t)(x.e - x.q);
+      }
+    });
+    for (size_t i = 0; i < n; ++i) {
+      nl[i + 1] += nl[i];
+      sl[i + 1] += sl[i];
+      ql[i + 1] += ql[i];
+    }
+    const size_t nb0 = b.names.size(), sb0 = b.seq.size(), qb0 = b.quals.size();
+    b.names.resize(nb0 + (size_t)nl[n]);
+    b.seq.resize(sb0 + (size_t)sl[n]);
+    b.quals.resize(qb0 + (size_t)ql[n]);
+    b.name_off.resize(n0 + n + 1);
+    b.seq_off.resize(n0 + n + 1);
+    b.qual_off.resize(n0 + n + 1);
+    static const Nt4Table tab;
+    run([&](int t) {
+      const size_t r0 = n * (size_t)t / (size_t)nt, r1 = n * (size_t)(t + 1) / (size_t)nt;
+      for (size_t i = r0; i < r1; ++i) {
+        const Rec &x = recs[i];
+        std::memcpy(&b.names[nb0 + (size_t)nl[i]], p + x.h, x.name_end - x.h);
+        uint8_t *d = b.seq.data() + sb0 + sl[i];
+        for (size_t k = x.s; k + 1 < x.p; ++k) *d++ = tab.v[(unsigned char)p[k]];
+        std::memcpy(&b.quals[qb0 + (size_t)ql[i]], p + x.q, x.e - x.q);
+        b.name_off[n0 + i + 1] = (int64_t)(nb0 + nl[i + 1]);
+        b.seq_off[n0 + i + 1] = (int32_t)(sb0 + sl[i + 1]);
+        b.qual_off[n0 + i + 1] = (int64_t)(qb0 + ql[i + 1]);
+      }
+    });
   }
 };
 
@@ -447,11 +602,17 @@ int g3a_mem_main(int argc, char **argv) {
   };
   Queue<std::unique_ptr<Batch>> q_in(2);
   Queue<std::unique_ptr<Job>> q_out(2);
+  // batches circulate: loader -> aligner -> writer -> loader, so their
+  // buffers are allocated (and faulted in) once
+  const int kBatches = 5;
+  Queue<std::unique_ptr<Batch>> q_free((size_t)kBatches);
+  for (int k = 0; k < kBatches; ++k) q_free.push(std::unique_ptr<Batch>(new Batch));
   const auto t0 = std::chrono::steady_clock::now();
   std::thread loader([&] {
     for (;;) {
-      std::unique_ptr<Batch> b(new Batch);
-      while (b->n() < batch && rd.next(*b)) {}
+      std::unique_ptr<Batch> b = q_free.pop();
+      b->clear();
+      rd.next_batch(*b, batch, threads);
       if (b->n() == 0) {
         q_in.push(nullptr);
         return;
@@ -476,18 +637,23 @@ int g3a_mem_main(int argc, char **argv) {
       for (auto &p : parts) std::fwrite(p.data(), 1, p.size(), out);
       total += b.n();
       g3a_batch_out_free(&j->o);
+      q_free.push(std::move(j->b));
     }
   });
   int status = 0;
   for (;;) {
     std::unique_ptr<Batch> b = q_in.pop();
     if (!b) break;
-    if (status) continue;  // drain the loader after a failure
+    if (status) {  // drain the loader after a failure
+      q_free.push(std::move(b));
+      continue;
+    }
     std::unique_ptr<Job> j(new Job);
     rc = g3a_align_batch(ctx, b->n(), b->seq.data(), b->seq_off.data(), &j->o);
     if (rc) {
       std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
       status = 1;
+      q_free.push(std::move(b));
       continue;
     }
     j->b = std::move(b);
