@@ -541,7 +541,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   tb.tb_order = P<int32_t>(ctx->tb_order);
   tb.tb_ctr = P<int32_t>(ctx->tb_ctr);
   HIPCK(hipEventRecord(ctx->kev[10], s));
-  launch_traceback(tb, ctx->tb_threads, s);
+  launch_traceback(tb, ctx->tb_threads, ctx->n_cu, s);
   HIPCK(hipEventRecord(ctx->kev[11], s));
   ctx->kev_used[5] = 1;
   HIPCK(hipEventRecord(ctx->ev[5], s));

@@ -1054,28 +1054,54 @@ G3A_HD int push_cigar(uint32_t *cigar, int n, int op, int len) {
 // border buffer raced by 32 threads, so its defined intent (this routine) is
 // used.  `tq`/`tt` give query/target bases; eh_h/eh_e hold qlen+1 entries and
 // z holds tlen*min(qlen, 2w+1) direction bytes.  Returns the score.
-template <class Q, class T>
-G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, int32_t *eh_h, int32_t *eh_e,
-                        uint8_t *z, uint32_t *cigar, int *n_cigar_out) {
+// Row buffers of the banded global alignment: H and E of the previous row
+// by query column.  EhArray keeps whole rows; EhRing keeps a ring of RB
+// columns (enough when 2w+2 <= RB) at a stride, e.g. one lane's slice of a
+// wavefront's LDS block.
+struct EhArray {
+  int32_t *h, *e;
+  G3A_HD int32_t &H(int j) { return h[j]; }
+  G3A_HD int32_t &E(int j) { return e[j]; }
+};
+template <int RB>
+struct EhRing {
+  int32_t *h, *e;
+  int stride;
+  G3A_HD int32_t &H(int j) { return h[(j & (RB - 1)) * stride]; }
+  G3A_HD int32_t &E(int j) { return e[(j & (RB - 1)) * stride]; }
+};
+
+// z-matrix row stride: the band's column count rounded up to 4 bytes
+G3A_HD int z_stride(int qlen, int w) {
+  const int n_col = qlen < 2 * w + 1 ? qlen : 2 * w + 1;
+  return (n_col + 3) & ~3;
+}
+
+// ksw_global2 (bwa/src/ksw.cu:706-812, DR-9): banded global alignment with
+// traceback.  The direction bytes of a row are written four at a time.
+template <class Q, class T, class EH>
+G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH eh, uint8_t *z, uint32_t *cigar,
+                        int *n_cigar_out) {
   const int MINUS_INF = -0x40000000;
   const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
-  const int n_col = qlen < 2 * w + 1 ? qlen : 2 * w + 1;
-  eh_h[0] = 0;
-  eh_e[0] = MINUS_INF;
+  const int zs = z_stride(qlen, w);
+  eh.H(0) = 0;
+  eh.E(0) = MINUS_INF;
   int j;
-  for (j = 1; j <= qlen && j <= w; ++j) { eh_h[j] = -(o.o_ins + o.e_ins * j); eh_e[j] = MINUS_INF; }
-  for (; j <= qlen; ++j) eh_h[j] = eh_e[j] = MINUS_INF;
+  for (j = 1; j <= qlen && j <= w; ++j) { eh.H(j) = -(o.o_ins + o.e_ins * j); eh.E(j) = MINUS_INF; }
+  for (; j <= qlen && j <= w + 1; ++j) eh.H(j) = eh.E(j) = MINUS_INF;
   for (int i = 0; i < tlen; ++i) {
     int32_t f = MINUS_INF, h1, beg, end, t;
     const int tb = tt(i);
     beg = i > w ? i - w : 0;
     end = i + w + 1 < qlen ? i + w + 1 : qlen;
     h1 = beg == 0 ? -(o.o_del + o.e_del * (i + 1)) : MINUS_INF;
-    uint8_t *zi = &z[(int64_t)i * n_col];
+    uint32_t *zi = (uint32_t *)&z[(int64_t)i * zs];
+    uint32_t zw = 0;
     for (j = beg; j < end; ++j) {
-      int32_t h, m = eh_h[j], e = eh_e[j];
-      uint8_t d;
-      eh_h[j] = h1;
+      int32_t h, m = eh.H(j), e = eh.E(j);
+      uint32_t d;
+      eh.H(j) = h1;
       m += o.mat[tb * 5 + tq(j)];
       d = m >= e ? 0 : 1;
       h = m >= e ? m : e;
@@ -1086,20 +1112,26 @@ G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, int
       e -= o.e_del;
       d |= e > t ? 1 << 2 : 0;
       e = e > t ? e : t;
-      eh_e[j] = e;
+      eh.E(j) = e;
       t = m - oe_ins;
       f -= o.e_ins;
       d |= f > t ? 2 << 4 : 0;
       f = f > t ? f : t;
-      zi[j - beg] = d;
+      const int c = j - beg;
+      zw |= d << (8 * (c & 3));
+      if ((c & 3) == 3) {
+        zi[c >> 2] = zw;
+        zw = 0;
+      }
     }
-    eh_h[end] = h1;
-    eh_e[end] = MINUS_INF;
+    if ((end - beg) & 3) zi[(end - beg) >> 2] = zw;
+    eh.H(end) = h1;
+    eh.E(end) = MINUS_INF;
   }
-  int score = eh_h[qlen];
+  int score = eh.H(qlen);
   int n = 0, which = 0, i = tlen - 1, k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
   while (i >= 0 && k >= 0) {
-    which = z[(int64_t)i * n_col + (k - (i > w ? i - w : 0))] >> (which << 1) & 3;
+    which = z[(int64_t)i * zs + (k - (i > w ? i - w : 0))] >> (which << 1) & 3;
     if (which == 0) { n = push_cigar(cigar, n, 0, 1); --i; --k; }
     else if (which == 1) { n = push_cigar(cigar, n, 2, 1); --i; }
     else { n = push_cigar(cigar, n, 1, 1); --k; }
@@ -1123,17 +1155,16 @@ G3A_HD int64_t aln_cigar_cap(int lq, int64_t rlen) { return 2 * ((int64_t)lq + r
 // Direction bytes of the global alignment.
 G3A_HD int64_t aln_z_size(int lq, int64_t rlen, int w) {
   if (w == 0) return 0;
-  int n_col = lq < 2 * w + 1 ? lq : 2 * w + 1;
-  return (int64_t)n_col * rlen;
+  return (int64_t)z_stride(lq, w) * rlen;
 }
 
 // traceback (aux.cu:2874-2949) and finalize (aux.cu:2952-3059) of the ai-th
 // kept region of a read of length l_seq.  `read` is the read, `cig` has
 // aln_cigar_cap() words (core at cig + lq + rlen + 4 - ... see below), z has
 // aln_z_size() bytes, eh_h/eh_e lq+1 entries.
+template <class EH>
 G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *read, int l_seq, const Region &g,
-                               int ai, int32_t *eh_h, int32_t *eh_e, uint8_t *z, uint32_t *cig, int64_t cig_base,
-                               Aln &a) {
+                               int ai, EH eh, uint8_t *z, uint32_t *cig, int64_t cig_base, Aln &a) {
   const int64_t l_pac = x.l_pac;
   const int lq = g.qe - g.qb;
   const int64_t rlen = g.re - g.rb;
@@ -1150,8 +1181,8 @@ G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *r
     for (int i = 0; i < lq; ++i) nw += o.mat[tr(rb + i) * 5 + query[i]];
   } else {
     nw = global_align(
-        o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return tr(rb + i); }, w, eh_h, eh_e,
-        z, core, &n_core);
+        o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return tr(rb + i); }, w, eh, z, core,
+        &n_core);
   }
   // NM as the reference's traceback counts it (aux.cu:2910-2941)
   int n_mm = 0, n_gap = 0;

@@ -179,7 +179,7 @@ void launch_region_filter(const RegionArgs &a, hipStream_t s);
 void launch_aln_prep(const AlnPrepArgs &a, hipStream_t s);
 int traceback_grid_threads(int n_cu);
 int64_t traceback_pool_words(int n_threads);
-void launch_traceback(const TbArgs &a, int n_threads, hipStream_t s);
+void launch_traceback(const TbArgs &a, int n_threads, int n_cu, hipStream_t s);
 void launch_collect_slow(int n, const int32_t *slow, int32_t *list, int32_t *count, int64_t *intv_base, int slot_cap,
                          hipStream_t s);
 void launch_gather_slow_counts(int n, const int32_t *list, const int32_t *n_intv, int64_t *out, hipStream_t s);

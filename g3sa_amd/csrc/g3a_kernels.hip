@@ -686,19 +686,48 @@ __global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
   if (in) a.tb_order[dp ? f : a.n_alns - 1 - b] = (int32_t)t;
 }
 
-// Persistent grid; the global-alignment row buffers of each thread live in a
-// global pool slice of its own (2 * kQMax words), contiguous per thread.
+// Persistent grids over the ordered list: k_traceback_dp takes the first
+// tb_ctr[0] entries (w > 0) with its row buffers in a 64-column ring per lane
+// in LDS when 2w+2 <= 64, else in the thread's slice of the global pool
+// (2 * kQMax words, contiguous per thread); k_traceback takes the rest.
+constexpr int kTbRing = 64;
+
+__global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a) {
+  __shared__ int32_t ring_h[kTbRing * 64], ring_e[kTbRing * 64];
+  const int lane = threadIdx.x;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + lane;
+  const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+  int32_t *eh_h = a.eh_pool + gtid * 2 * kQMax, *eh_e = eh_h + kQMax;
+  const int64_t n_dp = a.tb_ctr[0];
+  for (int64_t e = gtid; e < n_dp; e += nthreads) {
+    const int64_t t = a.tb_order[e];
+    const int r = a.aln_read[t];
+    const int off = a.off[r], l_seq = a.off[r + 1] - off;
+    const Region g = a.aln_reg[t];
+    const int w = global_bw(a.opt, g.qe - g.qb, g.re - g.rb, g.score);
+    Aln out;
+    if (2 * w + 2 <= kTbRing)
+      trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t],
+                         EhRing<kTbRing>{ring_h + lane, ring_e + lane, 64}, a.z + a.z_off[t], a.cigar + a.cig_off[t],
+                         a.cig_off[t], out);
+    else
+      trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], EhArray{eh_h, eh_e},
+                         a.z + a.z_off[t], a.cigar + a.cig_off[t], a.cig_off[t], out);
+    a.alns[t] = out;
+  }
+}
+
 __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
   int32_t *eh_h = a.eh_pool + gtid * 2 * kQMax, *eh_e = eh_h + kQMax;
-  for (int64_t e = gtid; e < a.n_alns; e += nthreads) {
+  for (int64_t e = a.tb_ctr[0] + gtid; e < a.n_alns; e += nthreads) {
     const int64_t t = a.tb_order[e];
     const int r = a.aln_read[t];
     const int off = a.off[r], l_seq = a.off[r + 1] - off;
     const Region g = a.aln_reg[t];
     Aln out;
-    trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], eh_h, eh_e, a.z + a.z_off[t],
+    trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], EhArray{eh_h, eh_e}, a.z + a.z_off[t],
                        a.cigar + a.cig_off[t], a.cig_off[t], out);
     a.alns[t] = out;
   }
@@ -782,10 +811,19 @@ int traceback_grid_threads(int n_cu) {
   return n_cu * per_cu * 64;
 }
 int64_t traceback_pool_words(int n_threads) { return (int64_t)n_threads * 2 * kQMax; }
-void launch_traceback(const TbArgs &a, int n_threads, hipStream_t s) {
+void launch_traceback(const TbArgs &a, int n_threads, int n_cu, hipStream_t s) {
   if (!a.n_alns) return;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, a.tb_ctr, 2, 0);
   hipLaunchKernelGGL(k_tb_order, dim3((unsigned)((a.n_alns + 255) / 256)), dim3(256), 0, s, a);
+  {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traceback_dp, 64, 0) != hipSuccess || per_cu < 1)
+      per_cu = 4;
+    int blocks_dp = n_cu * per_cu;  // the pool has a slice for each of these threads
+    if (blocks_dp > n_threads / 64) blocks_dp = n_threads / 64;
+    if (blocks_dp < 1) blocks_dp = 1;
+    hipLaunchKernelGGL(k_traceback_dp, dim3((unsigned)blocks_dp), dim3(64), 0, s, a);
+  }
   const int64_t need = (a.n_alns + 63) / 64, have = n_threads / 64;
   hipLaunchKernelGGL(k_traceback, dim3((unsigned)(need < have ? need : have)), dim3(64), 0, s, a);
 }
