@@ -60,10 +60,34 @@ struct DevIndex {
 
 constexpr int kKmer = 12;
 
-struct Intv {
-  int64_t k, l, s;
-  int32_t m, n;  // read[m..n]
+// Bidirectional interval: rows [k, k+s) of the text, rows [l, l+s) for the
+// reverse complement; read[m..n].  P is int64_t for any text and uint32_t
+// for texts of fewer than 2^32 rows (the seeding kernel's narrow form).
+template <class P>
+struct IntvT {
+  P k, l, s;
+  int32_t m, n;
 };
+using Intv = IntvT<int64_t>;
+using IntvN = IntvT<uint32_t>;
+
+template <class Q, class P>
+G3A_HD IntvT<Q> intv_cast(const IntvT<P> &a) {
+  IntvT<Q> r;
+  r.k = (Q)a.k;
+  r.l = (Q)a.l;
+  r.s = (Q)a.s;
+  r.m = a.m;
+  r.n = a.n;
+  return r;
+}
+
+// v[c] for c in 0..3 as a chain of selects: with v in scalar registers (a
+// kernel argument) a per-lane c would otherwise turn into a memory load
+template <class T>
+G3A_HD T pick4(const T *v, int c) {
+  return c == 0 ? v[0] : c == 1 ? v[1] : c == 2 ? v[2] : v[3];
+}
 
 struct Seed {
   int64_t rbeg;
@@ -121,15 +145,28 @@ G3A_HD void occ_at(const DevIndex &x, const OccLine &L, int64_t p, int64_t o[4])
   o[0] -= dollar;
 }
 
+// The same for a text of fewer than 2^32 rows, in 32-bit arithmetic: the
+// only superblock is the first, whose counts are 0.
+G3A_HD void occ_at(const DevIndex &x, const OccLine &L, uint32_t p, uint32_t o[4]) {
+  uint32_t in[4];
+  line_counts(L.w0, L.w1, (int)(p & 63), in);
+  const uint32_t pr = (uint32_t)x.primary;
+  const uint32_t dollar = (p >> 6) == (pr >> 6) && p >= pr;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) o[c] = L.c[c] + in[c];
+  o[0] -= dollar;
+}
+
 // Occ(p, c): occurrences of c in rows [0, p], '$' excluded
 G3A_HD void occ4(const DevIndex &x, int64_t p, int64_t o[4]) { occ_at(x, occ_line(x, p >> 6), p, o); }
 
 // Occ at two rows; one line fetch when both lie in the same line (the
 // common case once an interval has narrowed to a few rows)
-G3A_HD void occ4_pair(const DevIndex &x, int64_t p1, int64_t p2, int64_t o1[4], int64_t o2[4]) {
-  OccLine L = occ_line(x, p1 >> 6);
+template <class P>
+G3A_HD void occ4_pair(const DevIndex &x, P p1, P p2, P o1[4], P o2[4]) {
+  OccLine L = occ_line(x, (int64_t)(p1 >> 6));
   occ_at(x, L, p1, o1);
-  if ((p2 >> 6) != (p1 >> 6)) L = occ_line(x, p2 >> 6);
+  if ((p2 >> 6) != (p1 >> 6)) L = occ_line(x, (int64_t)(p2 >> 6));
   occ_at(x, L, p2, o2);
 }
 
@@ -149,38 +186,43 @@ G3A_HD int bwt_char(const DevIndex &x, int64_t k) {
   return (int)((w >> (30 - 2 * (y & 15))) & 3);
 }
 
-G3A_HD void intv_init(const DevIndex &x, Intv &p, int c) {
-  p.k = x.count[c];
-  p.l = x.count[3 - c];
-  p.s = x.count[c + 1] - x.count[c];
+template <class P>
+G3A_HD void intv_init(const DevIndex &x, IntvT<P> &p, int c) {
+  const P k = (P)pick4(x.count, c), l = (P)pick4(x.count, 3 - c), e = (P)pick4(x.count + 1, c);
+  p.k = k;
+  p.l = l;
+  p.s = e - k;
 }
 
 // backwardExt (fmindex.cu:107-160): extension by `base` with the l update
-G3A_HD void backward_ext(const DevIndex &x, const Intv &in, int base, Intv &out) {
-  int64_t osp[4], oep[4];
-  occ4_pair(x, in.k - 1, in.k + in.s - 1, osp, oep);
-  int64_t s[4];
+template <class P>
+G3A_HD void backward_ext(const DevIndex &x, const IntvT<P> &in, int base, IntvT<P> &out) {
+  P osp[4], oep[4];
+  occ4_pair(x, (P)(in.k - 1), (P)(in.k + in.s - 1), osp, oep);
+  P s[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) s[b] = oep[b] - osp[b];
-  int64_t so = (in.k <= x.primary && in.k + in.s > x.primary) ? 1 : 0;
-  int64_t l3 = in.l + so, l2 = l3 + s[3], l1 = l2 + s[2], l0 = l1 + s[1];
-  int64_t l = base == 0 ? l0 : base == 1 ? l1 : base == 2 ? l2 : l3;
-  out.k = x.count[base] + osp[base];
+  const P pr = (P)x.primary;
+  const P so = (in.k <= pr && in.k + in.s > pr) ? 1 : 0;
+  const P l3 = in.l + so, l2 = l3 + s[3], l1 = l2 + s[2], l0 = l1 + s[1];
+  const P l = base == 0 ? l0 : base == 1 ? l1 : base == 2 ? l2 : l3;
+  out.k = (P)pick4(x.count, base) + pick4(osp, base);
   out.l = l;
-  out.s = s[base];
+  out.s = pick4(s, base);
 }
 
 // backwardExtBackward (fmindex.cu:162-180): k and s only
 G3A_HD void backward_ext_b(const DevIndex &x, Intv &io, int base) {
   int64_t oa[4], ob[4];
   occ4_pair(x, io.k - 1, io.k + io.s - 1, oa, ob);
-  io.k = x.count[base] + oa[base];
-  io.s = ob[base] - oa[base];
+  io.k = pick4(x.count, base) + pick4(oa, base);
+  io.s = pick4(ob, base) - pick4(oa, base);
 }
 
 // FORWARD_EXT (macro.h:64-71)
-G3A_HD void forward_ext(const DevIndex &x, Intv &io, int base) {
-  Intv sw = io, o;
+template <class P>
+G3A_HD void forward_ext(const DevIndex &x, IntvT<P> &io, int base) {
+  IntvT<P> sw = io, o;
   sw.k = io.l;
   sw.l = io.k;
   backward_ext(x, sw, 3 - base, o);
@@ -277,27 +319,31 @@ struct IntvSink {
   int cap, n;
   int64_t n_seed;  // sum over intervals of min(s, max_occ)
   int max_occ;
-  G3A_HD void push(const Intv &p) {
-    if (n < cap) slot[n] = p;
+  template <class P>
+  G3A_HD void push(const IntvT<P> &p) {
+    if (n < cap) slot[n] = intv_cast<int64_t>(p);
     ++n;
-    n_seed += p.s > max_occ ? max_occ : p.s;
+    n_seed += (int64_t)p.s > max_occ ? max_occ : (int64_t)p.s;
   }
 };
 
 // Storage of the LEP and found lists.  LepWide keeps the whole interval;
 // LepNarrow packs it into 16 bytes (one vector load or store per entry) and
 // holds for texts of fewer than 2^32 rows.
+// Each names the interval type its searches compute in (I).
 struct LepWide {
+  using I = Intv;
   Intv v;
   static G3A_HD Intv get(const LepWide *p, int i) { return p[i].v; }
   static G3A_HD void put(LepWide *p, int i, const Intv &x) { p[i].v = x; }
 };
 struct alignas(16) LepNarrow {
+  using I = IntvN;
   uint32_t k, l, s;
   int16_t m, n;
-  static G3A_HD Intv get(const LepNarrow *p, int i) {
+  static G3A_HD IntvN get(const LepNarrow *p, int i) {
     const LepNarrow e = p[i];
-    Intv r;
+    IntvN r;
     r.k = e.k;
     r.l = e.l;
     r.s = e.s;
@@ -305,11 +351,11 @@ struct alignas(16) LepNarrow {
     r.n = e.n;
     return r;
   }
-  static G3A_HD void put(LepNarrow *p, int i, const Intv &x) {
+  static G3A_HD void put(LepNarrow *p, int i, const IntvN &x) {
     LepNarrow e;
-    e.k = (uint32_t)x.k;
-    e.l = (uint32_t)x.l;
-    e.s = (uint32_t)x.s;
+    e.k = x.k;
+    e.l = x.l;
+    e.s = x.s;
     e.m = (int16_t)x.m;
     e.n = (int16_t)x.n;
     p[i] = e;
@@ -335,6 +381,8 @@ struct LepBuf {
 template <class E>
 G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
                         LepBuf<E> &buf, int64_t *steps) {
+  using I = typename E::I;
+  using P = decltype(I::s);
   int x0 = 0;
   while (x0 < len) {
     if (q[x0] > 3) { ++x0; continue; }
@@ -342,14 +390,14 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     // after entry j >= nc has been read
     E *prev = buf.a, *curr = buf.a;
     int np = 0;
-    Intv ik;
+    I ik;
     intv_init(x, ik, q[x0]);
     ik.m = x0;
     ik.n = x0;
     int i;
     for (i = x0 + 1; i < len; ++i) {
       if (q[i] < 4) {
-        Intv ok = ik;
+        I ok = ik;
         forward_ext(x, ok, q[i]);
         if (steps) ++*steps;
         if (ok.s != ik.s) {
@@ -371,7 +419,7 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     }
     // longest first
     for (int a = 0, b = np - 1; a < b; ++a, --b) {
-      const Intv ta = E::get(prev, a), tb = E::get(prev, b);
+      const I ta = E::get(prev, a), tb = E::get(prev, b);
       E::put(prev, a, tb);
       E::put(prev, b, ta);
     }
@@ -379,13 +427,13 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     // backward search; SMEMs are found in decreasing start order
     int nf = 0, last_start = 0;
     E *fl = buf.f;
-    int64_t last_s = -1;  // size of the last entry written to curr
+    P last_s = 0;  // size of the last entry written to curr (read only when nc > 0)
     for (i = x0 - 1; i >= -1; --i) {
       int c = i < 0 ? -1 : (q[i] < 4 ? q[i] : -1);
       int nc = 0;
       for (int j = 0; j < np; ++j) {
-        const Intv pj = E::get(prev, j);
-        Intv ok;
+        const I pj = E::get(prev, j);
+        I ok;
         bool ext_ok = false;
         if (c >= 0) {
           backward_ext(x, pj, c, ok);
@@ -395,7 +443,7 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
         if (!ext_ok) {
           if (nc == 0) {
             if (nf == 0 || i + 1 < last_start) {
-              Intv p = pj;
+              I p = pj;
               p.m = i + 1;
               if (nf == buf.cap) return false;
               E::put(fl, nf++, p);
@@ -415,7 +463,7 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     }
     // found in decreasing start (and end) order: emit the long-enough ones by end
     for (int j = nf - 1; j >= 0; --j) {
-      const Intv fj = E::get(fl, j);
+      const I fj = E::get(fl, j);
       if (fj.n - fj.m + 1 >= o.min_seed_len) out.push(fj);
     }
     x0 = ret;
@@ -466,15 +514,17 @@ G3A_HD void smem_round1_per_end(const DevIndex &x, const Opt &o, const uint8_t *
 template <class E>
 G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int len, const Intv &smem,
                        E *leps, E *seeds, int cap, IntvSink &out, int64_t *steps) {
+  using I = typename E::I;
+  using P = decltype(I::s);
   const int split_len = (int)(o.min_seed_len * o.split_factor + .499);
   const int slen = smem.n - smem.m + 1;
   if (slen < split_len || smem.s > o.split_width) return true;
-  const int64_t min_seed_intv = smem.s + 1;
+  const P min_seed_intv = (P)(smem.s + 1);
   const int pivot = (smem.m + smem.n + 1) >> 1;
   if (pivot > 77) return true;  // seed.cu:718
   if (q[pivot] >= 4) return true;
   int nl = 0, ns = 0;
-  Intv seed, old;
+  I seed, old;
   intv_init(x, seed, q[pivot]);
   seed.m = pivot;
   seed.n = pivot;
@@ -497,7 +547,7 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     E::put(leps, nl++, seed);
   }
   for (int a = 0, b = nl - 1; a < b; ++a, --b) {
-    const Intv ta = E::get(leps, a), tb = E::get(leps, b);
+    const I ta = E::get(leps, a), tb = E::get(leps, b);
     E::put(leps, a, tb);
     E::put(leps, b, ta);
   }
@@ -505,11 +555,11 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     int base = q[m - 1];
     if (base >= 4) break;
     int num_old = nl;
-    int64_t curr = min_seed_intv;
+    P curr = min_seed_intv;
     bool collected = false;
     nl = 0;
     for (int j = 0; j < num_old; ++j) {
-      Intv lep = E::get(leps, j), ext;
+      I lep = E::get(leps, j), ext;
       backward_ext(x, lep, base, ext);
       if (steps) ++*steps;
       if (ext.s >= curr) {
@@ -526,7 +576,7 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     if (nl == 0) break;
   }
   if (nl > 0) {
-    const Intv l0 = E::get(leps, 0);
+    const I l0 = E::get(leps, 0);
     if (l0.n - l0.m + 1 >= o.min_seed_len) {
       if (ns == cap) return false;
       E::put(seeds, ns++, l0);
@@ -540,6 +590,7 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
 // (seed.cu:1518-1550).  Every start m extends forward at least to length
 // min_len before it can emit, so when the first kKmer bases hold no N their
 // interval is read from the k-mer table instead of kKmer-1 forward steps.
+template <class P>
 G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
                           int64_t *steps) {
   const int min_len = o.min_seed_len + 1;
@@ -550,7 +601,7 @@ G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int
     int next_m = m + 1;
     int base = q[m];
     if (base < 4) {
-      Intv seed;
+      IntvT<P> seed;
       int n0 = m + 1;
       uint32_t code = 0;
       bool jumped = false;
@@ -559,9 +610,9 @@ G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int
         for (; k < kKmer && q[m + k] < 4; ++k) code = code << 2 | q[m + k];
         if (k == kKmer) {
           const uint32_t *e = x.kmer + 3 * (uint64_t)code;
-          seed.k = e[0];
-          seed.l = e[1];
-          seed.s = e[2];
+          seed.k = (P)e[0];
+          seed.l = (P)e[1];
+          seed.s = (P)e[2];
           n0 = m + kKmer;
           next_m = n0;
           jumped = true;
@@ -574,7 +625,7 @@ G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int
         if (base < 4) {
           forward_ext(x, seed, base);
           if (steps) ++*steps;
-          if (seed.s < max_intv && (n - m + 1) >= min_len) {
+          if ((int64_t)seed.s < max_intv && (n - m + 1) >= min_len) {
             if (seed.s > 0) {
               seed.m = m;
               seed.n = n;

@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
           ok = reseed_one(a.idx, a.opt, q, len, smem, la, lf, kLepCap, sink, &steps);
         }
       } else ok = false;
-      if (ok) reseed_round3(a.idx, a.opt, q, len, sink, &steps);
+      if (ok) reseed_round3<decltype(E::I::s)>(a.idx, a.opt, q, len, sink, &steps);
     }
     ok = ok && sink.n <= sink.cap;
     a.slow[r] = ok ? 0 : 1;
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
     Intv smem = slot[k];
     reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.f, kSlowLep, sink, nullptr);
   }
-  reseed_round3(a.idx, a.opt, q, len, sink, nullptr);
+  reseed_round3<int64_t>(a.idx, a.opt, q, len, sink, nullptr);
   a.n_intv[r] = sink.n;
   a.n_seed[r] = sink.n_seed;
 }

@@ -313,9 +313,33 @@ class Queue {
 };
 
 void put_int(std::string &s, long long v) {
-  char t[32];
-  int k = std::snprintf(t, sizeof t, "%lld", v);
-  s.append(t, (size_t)k);
+  char t[24];
+  char *e = t + sizeof t, *p = e;
+  unsigned long long u = v < 0 ? 0ull - (unsigned long long)v : (unsigned long long)v;
+  do {
+    *--p = (char)('0' + u % 10);
+    u /= 10;
+  } while (u);
+  if (v < 0) *--p = '-';
+  s.append(p, (size_t)(e - p));
+}
+
+// len bases of sq as letters: forward, or reverse-complemented
+void put_seq(std::string &s, const uint8_t *sq, int len, bool rc) {
+  const size_t at = s.size();
+  s.resize(at + (size_t)len);
+  char *d = &s[at];
+  if (!rc)
+    for (int k = 0; k < len; ++k) d[k] = "ACGTN"[sq[k]];
+  else
+    for (int k = 0; k < len; ++k) d[k] = "TGCAN"[sq[len - 1 - k]];
+}
+
+void put_rev(std::string &s, const char *q, size_t n) {
+  const size_t at = s.size();
+  s.resize(at + n);
+  char *d = &s[at];
+  for (size_t k = 0; k < n; ++k) d[k] = q[n - 1 - k];
 }
 
 // mem_approx_mapq_se (bwa/src/aux.cu:676-700) with sub = csub = sub_n = 0 and
@@ -350,6 +374,12 @@ void format_range(const Batch &b, const g3a_batch_out &o, const Ref &ref, const 
   static const char kOp[] = "MIDSH";
   std::vector<uint32_t> core;
   std::string md;
+  {
+    const int64_t na = o.aln_off[r1] - o.aln_off[r0], nr = r1 - r0;
+    const int64_t bytes = (b.name_off[(size_t)r1] - b.name_off[(size_t)r0]) + 2 * (b.qual_off[(size_t)r1] -
+                                                                                    b.qual_off[(size_t)r0]);
+    s.reserve(s.size() + (size_t)((bytes / (nr ? nr : 1) + 96) * (na + nr)));
+  }
   for (int q = r0; q < r1; ++q) {
     const char *name = b.names.data() + b.name_off[(size_t)q];
     const size_t nl = (size_t)(b.name_off[(size_t)q + 1] - b.name_off[(size_t)q]);
@@ -362,7 +392,7 @@ void format_range(const Batch &b, const g3a_batch_out &o, const Ref &ref, const 
       if (a1 <= a0) {
         s.append(name, nl);
         s.append(" * 0 * ");
-        for (int k = 0; k < len; ++k) s.push_back("ACGTN"[sq[k]]);
+        put_seq(s, sq, len, false);
         s.push_back(' ');
         s.append(qual, ql);
         s.push_back('\n');
@@ -381,7 +411,7 @@ void format_range(const Batch &b, const g3a_batch_out &o, const Ref &ref, const 
           s.push_back(kOp[cg[k] & 0xf]);
         }
         s.push_back(' ');
-        for (int k = 0; k < len; ++k) s.push_back("ACGTN"[sq[k]]);
+        put_seq(s, sq, len, false);
         s.push_back(' ');
         s.append(qual, ql);
         s.push_back('\n');
@@ -391,7 +421,7 @@ void format_range(const Batch &b, const g3a_batch_out &o, const Ref &ref, const 
     if (a1 <= a0) {
       s.append(name, nl);
       s.append("\t4\t*\t0\t0\t*\t*\t0\t0\t");
-      for (int k = 0; k < len; ++k) s.push_back("ACGTN"[sq[k]]);
+      put_seq(s, sq, len, false);
       s.push_back('\t');
       s.append(qual, ql);
       s.push_back('\n');
@@ -461,12 +491,10 @@ void format_range(const Batch &b, const g3a_batch_out &o, const Ref &ref, const 
       }
       if (clip_r) { put_int(s, clip_r); s.push_back('S'); }
       s.append("\t*\t0\t0\t");
-      for (int k = 0; k < len; ++k) {
-        const uint8_t c = rev ? sq[len - 1 - k] : sq[k];
-        s.push_back(rev ? "TGCAN"[c] : "ACGTN"[c]);
-      }
+      put_seq(s, sq, len, rev);
       s.push_back('\t');
-      for (size_t k = 0; k < ql; ++k) s.push_back(rev ? qual[ql - 1 - k] : qual[k]);
+      if (rev) put_rev(s, qual, ql);
+      else s.append(qual, ql);
       s.append("\tNM:i:");
       put_int(s, nm);
       s.append("\tMD:Z:");
@@ -608,11 +636,16 @@ int g3a_mem_main(int argc, char **argv) {
   Queue<std::unique_ptr<Batch>> q_free((size_t)kBatches);
   for (int k = 0; k < kBatches; ++k) q_free.push(std::unique_ptr<Batch>(new Batch));
   const auto t0 = std::chrono::steady_clock::now();
+  using clk = std::chrono::steady_clock;
+  auto since = [](clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); };
+  double t_load = 0, t_align = 0, t_write = 0;  // busy time of each stage
   std::thread loader([&] {
     for (;;) {
       std::unique_ptr<Batch> b = q_free.pop();
+      const auto tl = clk::now();
       b->clear();
       rd.next_batch(*b, batch, threads);
+      t_load += since(tl);
       if (b->n() == 0) {
         q_in.push(nullptr);
         return;
@@ -625,6 +658,7 @@ int g3a_mem_main(int argc, char **argv) {
     for (;;) {
       std::unique_ptr<Job> j = q_out.pop();
       if (!j) return;
+      const auto tw = clk::now();
       const Batch &b = *j->b;
       const int nt = b.n() < 4096 ? 1 : threads;
       std::vector<std::string> parts((size_t)nt);
@@ -637,6 +671,7 @@ int g3a_mem_main(int argc, char **argv) {
       for (auto &p : parts) std::fwrite(p.data(), 1, p.size(), out);
       total += b.n();
       g3a_batch_out_free(&j->o);
+      t_write += since(tw);
       q_free.push(std::move(j->b));
     }
   });
@@ -649,7 +684,9 @@ int g3a_mem_main(int argc, char **argv) {
       continue;
     }
     std::unique_ptr<Job> j(new Job);
+    const auto ta = clk::now();
     rc = g3a_align_batch(ctx, b->n(), b->seq.data(), b->seq_off.data(), &j->o);
+    t_align += since(ta);
     if (rc) {
       std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
       status = 1;
@@ -665,6 +702,8 @@ int g3a_mem_main(int argc, char **argv) {
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::fprintf(stderr, "g3a mem: %lld reads in %.2f s (%.0f reads/s end to end)\n", total, sec,
                sec > 0 ? total / sec : 0.0);
+  std::fprintf(stderr, "g3a mem: busy time: load %.2f s, align %.2f s, format+write %.2f s\n", t_load, t_align,
+               t_write);
   if (out_fn) std::fclose(out);
   g3a_destroy(ctx);
   return status;
