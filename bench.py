@@ -39,6 +39,10 @@ import g3sa_amd  # noqa: E402
 
 ORACLE_CLI = ROOT / "oracle" / "_build" / "g3a_oracle"
 HBM_PEAK_GBS = 8000.0
+# k_seed's memory-side bytes per launch on the 100 Mbp / 2M-read workload,
+# from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE), each scaled by the
+# same counter read on a plain copy kernel of known size
+TRAFFIC_FILE = ROOT / "profiles" / "k_seed_traffic_100Mbp_2Mx150.txt"
 
 CONFIGS = {
     "c2": dict(
@@ -152,6 +156,21 @@ def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = Tr
     }
 
 
+def seed_traffic(config: str) -> int | None:
+    """Bytes per k_seed launch recorded from the counter passes, or None."""
+    if config != "c2" or not TRAFFIC_FILE.exists():
+        return None
+    kv = {}
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        k, sep, v = line.partition(":")
+        if sep:
+            kv[k.strip()] = v.strip()
+    try:
+        return int(float(kv["traffic_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
+
+
 def end_to_end(prefix: Path, fq: Path) -> dict | None:
     """`g3a mem` over the whole FASTQ (parse, align, format, write to /dev/null),
     timed by the driver from the first read parsed to the last line written."""
@@ -227,7 +246,7 @@ def main() -> int:
     seed_gbs = seed_bytes / (kms["k_seed"] * 1e-3) / 1e9 if kms["k_seed"] > 0 else 0.0
     dominant = "k_seed" if kms["k_seed"] >= ext_ms else "extension"
     roof = {"bound": "hbm", "kernel": "k_seed", "achieved": round(seed_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(seed_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(seed_gbs / HBM_PEAK_GBS, 4), "traffic": seed_traffic(args.config),
             "algorithmic_bytes": seed_bytes, "launch_ms": round(kms["k_seed"], 3)}
     if dominant != "k_seed":
         roof["dominant"] = dominant

@@ -63,8 +63,13 @@ struct g3a_ctx {
   int n_cu = 0;
   int strip_rows = 16, lane_blocks_per_cu = 8, wave_blocks_per_cu = 8;
   DevBuf ext_res[2], ext_key, ext_key2, ext_idx, ext_sorted, ext_slow, ext_ctr, ext_sort_tmp, ext_rowbuf;
-  // host staging
+  // host staging: readbacks, and page-locked copies of each batch's input
+  // and output so that the transfers run at DMA speed
   int64_t *pinned = nullptr;
+  static constexpr size_t kStage = (size_t)16 << 20;
+  uint8_t *stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {};
+  std::vector<uint8_t> out_buf;
   hipEvent_t ev[8] = {};
   // single-kernel timing: k_seed, lane engine (2 sides), one-wavefront
   // extension (2 sides or the plain form's k_extend), k_traceback
@@ -100,6 +105,44 @@ int ensure(g3a_ctx *ctx, DevBuf &b, size_t bytes) {
   size_t want = bytes + bytes / 4;  // headroom so repeated batches do not reallocate
   HIPCK(hipMalloc(&b.p, want));
   b.bytes = want;
+  return 0;
+}
+
+// Host <-> device copies of pageable memory through two page-locked
+// staging slots: the CPU copy into (or out of) one slot overlaps the DMA of
+// the other.  Both end with the stream synchronized.
+int stage_h2d(g3a_ctx *ctx, void *dev, const void *src, size_t bytes) {
+  const size_t K = g3a_ctx::kStage;
+  for (size_t c = 0, i = 0; c < bytes; c += K, ++i) {
+    const int k = (int)(i & 1);
+    const size_t m = bytes - c < K ? bytes - c : K;
+    if (i >= 2) HIPCK(hipEventSynchronize(ctx->stage_ev[k]));
+    std::memcpy(ctx->stage[k], (const uint8_t *)src + c, m);
+    HIPCK(hipMemcpyAsync((uint8_t *)dev + c, ctx->stage[k], m, hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+  }
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int stage_d2h(g3a_ctx *ctx, void *dst, const void *dev, size_t bytes) {
+  const size_t K = g3a_ctx::kStage, n = (bytes + K - 1) / K;
+  auto issue = [&](size_t i) -> int {
+    const size_t c = i * K, m = bytes - c < K ? bytes - c : K;
+    HIPCK(hipMemcpyAsync(ctx->stage[i & 1], (const uint8_t *)dev + c, m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipEventRecord(ctx->stage_ev[i & 1], ctx->stream));
+    return 0;
+  };
+  for (size_t i = 0; i < n && i < 2; ++i)
+    if (int rc = issue(i)) return rc;
+  for (size_t i = 0; i < n; ++i) {
+    const size_t c = i * K, m = bytes - c < K ? bytes - c : K;
+    HIPCK(hipEventSynchronize(ctx->stage_ev[i & 1]));
+    std::memcpy((uint8_t *)dst + c, ctx->stage[i & 1], m);
+    if (i + 2 < n)
+      if (int rc = issue(i + 2)) return rc;
+  }
+  HIPCK(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -196,12 +239,72 @@ int g3a_device_alloc(int device, const char *prefix, const g3a_opt *o, int max_r
   if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
       ctx->n_cu < 1)
     ctx->n_cu = 256;
-  if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->stage[0], g3a_ctx::kStage, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->stage[1], g3a_ctx::kStage, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->stage_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->stage_ev[1], hipEventDisableTiming) != hipSuccess) {
     g3a_destroy(ctx);
     return G3A_E_HIP;
   }
   *out = ctx;
   return G3A_OK;
+}
+
+// Workspace of each stage, grown to the batch's sizes (the same calls size it
+// at set-up from max_reads, so that a first batch of typical shape finds it
+// allocated).
+static int alloc_reads(g3a_ctx *ctx, int n) {
+  if (ensure(ctx, ctx->n_intv, (size_t)n * 4 + 4) || ensure(ctx, ctx->n_seed, (size_t)n * 8 + 8) ||
+      ensure(ctx, ctx->slow, (size_t)n * 4 + 4) || ensure(ctx, ctx->intv_base, (size_t)n * 8 + 8) ||
+      ensure(ctx, ctx->seed_off, (size_t)(n + 1) * 8) || ensure(ctx, ctx->counters, 64 * 8) ||
+      ensure(ctx, ctx->n_task, (size_t)n * 4 + 4) || ensure(ctx, ctx->task_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->n_aln, (size_t)n * 4 + 4) || ensure(ctx, ctx->aln_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->slow_list, (size_t)n * 4 + 4) || ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(ctx->max_reads + 1)) ||
+      ensure(ctx, ctx->intv, ((size_t)n * kSlotCap + 1024) * sizeof(Intv)))
+    return G3A_E_HIP;
+  if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx);
+  return ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * 2 * kLepCap * sizeof(Intv));
+}
+
+static int alloc_seeds(g3a_ctx *ctx, int64_t T_s, int n) {
+  const size_t n_nodes = (size_t)((T_s + 32 * (int64_t)(n + 1)) / 4 + 2);
+  return ensure(ctx, ctx->seeds, (size_t)T_s * sizeof(Seed)) || ensure(ctx, ctx->seeds2, (size_t)T_s * sizeof(Seed)) ||
+         ensure(ctx, ctx->ch, (size_t)T_s * sizeof(ChainTmp)) || ensure(ctx, ctx->nxt, (size_t)T_s * 4) ||
+         ensure(ctx, ctx->order, (size_t)T_s * 4) || ensure(ctx, ctx->tmpi, (size_t)T_s * 8 + 8) ||
+         ensure(ctx, ctx->w16, (size_t)T_s * 2) || ensure(ctx, ctx->kept, (size_t)T_s) ||
+         ensure(ctx, ctx->nodes, n_nodes * sizeof(KbNode)) || ensure(ctx, ctx->tasks_tmp, (size_t)T_s * sizeof(Task));
+}
+
+static int alloc_tasks(g3a_ctx *ctx, int64_t T_t) {
+  return ensure(ctx, ctx->tasks, (size_t)T_t * sizeof(Task)) || ensure(ctx, ctx->regs, (size_t)T_t * sizeof(Region)) ||
+         ensure(ctx, ctx->flag, (size_t)T_t) || ensure(ctx, ctx->sec, (size_t)T_t * 4);
+}
+
+static int alloc_alns(g3a_ctx *ctx, int64_t T_a) {
+  if (!ctx->tb_threads) ctx->tb_threads = traceback_grid_threads(ctx->n_cu);
+  return ensure(ctx, ctx->aln_reg, (size_t)T_a * sizeof(Region)) || ensure(ctx, ctx->aln_read, (size_t)T_a * 4) ||
+         ensure(ctx, ctx->aln_rank, (size_t)T_a * 4) || ensure(ctx, ctx->z_size, (size_t)T_a * 8) ||
+         ensure(ctx, ctx->cig_size, (size_t)T_a * 8) || ensure(ctx, ctx->z_off, (size_t)(T_a + 1) * 8) ||
+         ensure(ctx, ctx->cig_off, (size_t)(T_a + 1) * 8) || ensure(ctx, ctx->alns, (size_t)T_a * sizeof(Aln)) ||
+         ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(T_a + 1)) || ensure(ctx, ctx->tb_order, (size_t)T_a * 4 + 4) ||
+         ensure(ctx, ctx->tb_ctr, 64) || ensure(ctx, ctx->eh_pool, (size_t)traceback_pool_words(ctx->tb_threads) * 4);
+}
+
+static int alloc_paths(g3a_ctx *ctx, int64_t T_z, int64_t T_c) {
+  return ensure(ctx, ctx->z, (size_t)T_z + 64) || ensure(ctx, ctx->cigar, (size_t)T_c * 4 + 64);
+}
+
+// Set-up sizes for a batch of max_reads reads and max_bases bases: 12 seeds,
+// 4 extension tasks and one alignment per read, direction bytes for
+// 4 x max_bases and the CIGAR capacity of one alignment per read.
+static int reserve_workspace(g3a_ctx *ctx) {
+  const int n = ctx->max_reads;
+  const int64_t nb = ctx->max_bases;
+  if (alloc_reads(ctx, n) || alloc_seeds(ctx, 12 * (int64_t)n, n) || alloc_tasks(ctx, 4 * (int64_t)n) ||
+      alloc_alns(ctx, n) || alloc_paths(ctx, 4 * nb, 4 * nb + 4 * (int64_t)n))
+    return G3A_E_HIP;
+  return 0;
 }
 
 int g3a_memcpy_index(g3a_ctx *ctx) {
@@ -249,6 +352,7 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
     HIPCK(hipGetLastError());
     x.kmer = P<uint32_t>(ctx->kmer);
   }
+  if (int rc = reserve_workspace(ctx)) return rc;
   ctx->index_on_device = true;
   return G3A_OK;
 }
@@ -264,10 +368,12 @@ int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_
   }
   HIPCK(hipSetDevice(ctx->device));
   const int64_t nb = n_reads ? offsets[n_reads] : 0;
-  if (ensure(ctx, ctx->seq, (size_t)nb + 64) || ensure(ctx, ctx->off, (size_t)(n_reads + 1) * 4)) return G3A_E_HIP;
-  if (nb) HIPCK(hipMemcpyAsync(ctx->seq.p, seq, (size_t)nb, hipMemcpyHostToDevice, ctx->stream));
+  const size_t off_bytes = (size_t)(n_reads + 1) * 4;
+  if (ensure(ctx, ctx->seq, (size_t)nb + 64) || ensure(ctx, ctx->off, off_bytes)) return G3A_E_HIP;
+  if (int rc = stage_h2d(ctx, ctx->seq.p, seq, (size_t)nb)) return rc;
+  if (n_reads)
+    if (int rc = stage_h2d(ctx, ctx->off.p, offsets, off_bytes)) return rc;
   launch_clamp_codes(P<uint8_t>(ctx->seq), nb, ctx->stream);
-  if (n_reads) HIPCK(hipMemcpyAsync(ctx->off.p, offsets, (size_t)(n_reads + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
   HIPCK(hipStreamSynchronize(ctx->stream));
   ctx->n_reads = n_reads;
   ctx->has_input = true;
@@ -363,16 +469,9 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   const int n = ctx->n_reads;
   ctx->has_output = false;
   ctx->total_seeds = ctx->total_tasks = ctx->total_alns = ctx->total_cig = ctx->n_slow = 0;
-  if (ensure(ctx, ctx->n_intv, (size_t)n * 4 + 4) || ensure(ctx, ctx->n_seed, (size_t)n * 8 + 8) ||
-      ensure(ctx, ctx->slow, (size_t)n * 4 + 4) || ensure(ctx, ctx->intv_base, (size_t)n * 8 + 8) ||
-      ensure(ctx, ctx->seed_off, (size_t)(n + 1) * 8) || ensure(ctx, ctx->counters, 64 * 8) ||
-      ensure(ctx, ctx->n_task, (size_t)n * 4 + 4) || ensure(ctx, ctx->task_off, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->n_aln, (size_t)n * 4 + 4) || ensure(ctx, ctx->aln_off, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->slow_list, (size_t)n * 4 + 4) || ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(ctx->max_reads + 1)))
-    return G3A_E_HIP;
+  if (alloc_reads(ctx, n)) return G3A_E_HIP;
   // fast-path interval slots plus a pool that grows for slow reads
   const size_t fast_entries = (size_t)n * kSlotCap;
-  if (ensure(ctx, ctx->intv, (fast_entries + 1024) * sizeof(Intv))) return G3A_E_HIP;
   int64_t *cnt = P<int64_t>(ctx->counters);
   HIPCK(hipMemsetAsync(cnt, 0, 64 * 8, s));
   HIPCK(hipEventRecord(ctx->ev[0], s));
@@ -389,8 +488,6 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.n_seed = P<int64_t>(ctx->n_seed);
   sa.slow = P<int32_t>(ctx->slow);
   sa.fm_steps = cnt + 0;
-  if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx);
-  if (ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * 2 * kLepCap * sizeof(Intv))) return G3A_E_HIP;
   sa.lep_pool = P<Intv>(ctx->lep_pool);
   for (int &u : ctx->kev_used) u = 0;
   HIPCK(hipEventRecord(ctx->kev[0], s));
@@ -441,8 +538,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   if (readback(ctx, {P<int64_t>(ctx->seed_off) + n})) return G3A_E_HIP;
   const int64_t T_s = ctx->pinned[0];
   ctx->total_seeds = T_s;
-  if (ensure(ctx, ctx->seeds, (size_t)T_s * sizeof(Seed)) || ensure(ctx, ctx->seeds2, (size_t)T_s * sizeof(Seed)))
-    return G3A_E_HIP;
+  if (alloc_seeds(ctx, T_s, n)) return G3A_E_HIP;
   SaArgs la{};
   la.idx = ctx->idx;
   la.opt = ctx->opt;
@@ -458,12 +554,6 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   launch_sort(so, s);
   HIPCK(hipEventRecord(ctx->ev[2], s));
   // ---- chaining
-  const size_t n_nodes = (size_t)((T_s + 32 * (int64_t)(n + 1)) / 4 + 2);
-  if (ensure(ctx, ctx->ch, (size_t)T_s * sizeof(ChainTmp)) || ensure(ctx, ctx->nxt, (size_t)T_s * 4) ||
-      ensure(ctx, ctx->order, (size_t)T_s * 4) || ensure(ctx, ctx->tmpi, (size_t)T_s * 8 + 8) ||
-      ensure(ctx, ctx->w16, (size_t)T_s * 2) || ensure(ctx, ctx->kept, (size_t)T_s) ||
-      ensure(ctx, ctx->nodes, n_nodes * sizeof(KbNode)) || ensure(ctx, ctx->tasks_tmp, (size_t)T_s * sizeof(Task)))
-    return G3A_E_HIP;
   ChainArgs ca{};
   ca.idx = ctx->idx;
   ca.opt = ctx->opt;
@@ -487,9 +577,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   const int64_t T_t = ctx->pinned[0];
   if (ctx->pinned[1] & 0xffffffff) return fail(ctx, G3A_E_INTERNAL, "chain node arena overflow");
   ctx->total_tasks = T_t;
-  if (ensure(ctx, ctx->tasks, (size_t)T_t * sizeof(Task)) || ensure(ctx, ctx->regs, (size_t)T_t * sizeof(Region)) ||
-      ensure(ctx, ctx->flag, (size_t)T_t) || ensure(ctx, ctx->sec, (size_t)T_t * 4))
-    return G3A_E_HIP;
+  if (alloc_tasks(ctx, T_t)) return G3A_E_HIP;
   CompactArgs cp{n, P<int64_t>(ctx->seed_off), P<int64_t>(ctx->task_off), P<Task>(ctx->tasks_tmp), P<Task>(ctx->tasks)};
   launch_task_compact(cp, s);
   HIPCK(hipEventRecord(ctx->ev[3], s));
@@ -504,12 +592,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   if (readback(ctx, {P<int64_t>(ctx->aln_off) + n})) return G3A_E_HIP;
   const int64_t T_a = ctx->pinned[0];
   ctx->total_alns = T_a;
-  if (ensure(ctx, ctx->aln_reg, (size_t)T_a * sizeof(Region)) || ensure(ctx, ctx->aln_read, (size_t)T_a * 4) ||
-      ensure(ctx, ctx->aln_rank, (size_t)T_a * 4) || ensure(ctx, ctx->z_size, (size_t)T_a * 8) ||
-      ensure(ctx, ctx->cig_size, (size_t)T_a * 8) || ensure(ctx, ctx->z_off, (size_t)(T_a + 1) * 8) ||
-      ensure(ctx, ctx->cig_off, (size_t)(T_a + 1) * 8) || ensure(ctx, ctx->alns, (size_t)T_a * sizeof(Aln)) ||
-      ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(T_a + 1)))
-    return G3A_E_HIP;
+  if (alloc_alns(ctx, T_a)) return G3A_E_HIP;
   AlnPrepArgs ap{ctx->opt, n, P<int64_t>(ctx->task_off), P<int64_t>(ctx->aln_off), P<Region>(ctx->regs),
                  P<Region>(ctx->aln_reg), P<int32_t>(ctx->aln_read), P<int32_t>(ctx->aln_rank),
                  P<int64_t>(ctx->z_size), P<int64_t>(ctx->cig_size)};
@@ -519,7 +602,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   if (readback(ctx, {P<int64_t>(ctx->z_off) + T_a, P<int64_t>(ctx->cig_off) + T_a})) return G3A_E_HIP;
   const int64_t T_z = ctx->pinned[0], T_c = ctx->pinned[1];
   ctx->total_cig = T_c;
-  if (ensure(ctx, ctx->z, (size_t)T_z + 64) || ensure(ctx, ctx->cigar, (size_t)T_c * 4 + 64)) return G3A_E_HIP;
+  if (alloc_paths(ctx, T_z, T_c)) return G3A_E_HIP;
   TbArgs tb{};
   tb.idx = ctx->idx;
   tb.opt = ctx->opt;
@@ -534,10 +617,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   tb.z = P<uint8_t>(ctx->z);
   tb.cigar = P<uint32_t>(ctx->cigar);
   tb.alns = P<Aln>(ctx->alns);
-  if (!ctx->tb_threads) ctx->tb_threads = traceback_grid_threads(ctx->n_cu);
-  if (ensure(ctx, ctx->eh_pool, (size_t)traceback_pool_words(ctx->tb_threads) * 4)) return G3A_E_HIP;
   tb.eh_pool = P<int32_t>(ctx->eh_pool);
-  if (ensure(ctx, ctx->tb_order, (size_t)T_a * 4 + 4) || ensure(ctx, ctx->tb_ctr, 64)) return G3A_E_HIP;
   tb.tb_order = P<int32_t>(ctx->tb_order);
   tb.tb_ctr = P<int32_t>(ctx->tb_ctr);
   HIPCK(hipEventRecord(ctx->kev[10], s));
@@ -575,12 +655,22 @@ int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   out->n_reads = n;
   out->n_alns = na;
   out->aln_off = (int64_t *)std::malloc(sizeof(int64_t) * (size_t)(n + 1));
-  std::vector<Aln> alns((size_t)na);
-  std::vector<uint32_t> cig((size_t)ctx->total_cig);
-  if (n) HIPCK(hipMemcpy(out->aln_off, ctx->aln_off.p, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyDeviceToHost));
-  else out->aln_off[0] = 0;
-  if (na) HIPCK(hipMemcpy(alns.data(), ctx->alns.p, sizeof(Aln) * (size_t)na, hipMemcpyDeviceToHost));
-  if (ctx->total_cig) HIPCK(hipMemcpy(cig.data(), ctx->cigar.p, 4 * (size_t)ctx->total_cig, hipMemcpyDeviceToHost));
+  // through the staging slots into a host buffer kept by the context, then
+  // unpacked into the caller's arrays
+  const size_t off_bytes = sizeof(int64_t) * (size_t)(n + 1), aln_bytes = sizeof(Aln) * (size_t)na,
+               cig_bytes = 4 * (size_t)ctx->total_cig;
+  const size_t at_cig = (aln_bytes + 255) & ~(size_t)255;
+  if (ctx->out_buf.size() < at_cig + cig_bytes) ctx->out_buf.resize(at_cig + cig_bytes + (at_cig + cig_bytes) / 4);
+  uint8_t *hb = ctx->out_buf.data();
+  if (n) {
+    if (int rc = stage_d2h(ctx, out->aln_off, ctx->aln_off.p, off_bytes)) return rc;
+  } else out->aln_off[0] = 0;
+  if (na)
+    if (int rc = stage_d2h(ctx, hb, ctx->alns.p, aln_bytes)) return rc;
+  if (cig_bytes)
+    if (int rc = stage_d2h(ctx, hb + at_cig, ctx->cigar.p, cig_bytes)) return rc;
+  const Aln *alns = (const Aln *)hb;
+  const uint32_t *cig = (const uint32_t *)(hb + at_cig);
   const size_t m = (size_t)(na ? na : 1);
   out->rid = (int32_t *)std::malloc(4 * m);
   out->pos = (int64_t *)std::malloc(8 * m);
@@ -598,19 +688,19 @@ int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   out->n_core = (int32_t *)std::malloc(4 * m);
   out->core_off = (int64_t *)std::malloc(8 * m);
   int64_t words = 0;
-  for (int64_t i = 0; i < na; ++i) words += alns[(size_t)i].n_cigar + alns[(size_t)i].n_core;
+  for (int64_t i = 0; i < na; ++i) words += alns[i].n_cigar + alns[i].n_core;
   out->cigar = (uint32_t *)std::malloc(4 * (size_t)(words ? words : 1));
   int64_t w = 0;
   for (int64_t i = 0; i < na; ++i) {
-    const Aln &a = alns[(size_t)i];
+    const Aln &a = alns[i];
     out->rid[i] = a.rid; out->pos[i] = a.pos; out->flag[i] = a.flag; out->is_rev[i] = a.is_rev;
     out->score[i] = a.score; out->nw_score[i] = a.nw_score; out->nm[i] = a.nm;
     out->qb[i] = a.qb; out->qe[i] = a.qe; out->rb[i] = a.rb; out->re[i] = a.re;
     out->n_cigar[i] = a.n_cigar; out->cigar_off[i] = w;
-    std::memcpy(out->cigar + w, cig.data() + a.cigar_off, 4 * (size_t)a.n_cigar);
+    std::memcpy(out->cigar + w, cig + a.cigar_off, 4 * (size_t)a.n_cigar);
     w += a.n_cigar;
     out->n_core[i] = a.n_core; out->core_off[i] = w;
-    std::memcpy(out->cigar + w, cig.data() + a.core_off, 4 * (size_t)a.n_core);
+    std::memcpy(out->cigar + w, cig + a.core_off, 4 * (size_t)a.n_core);
     w += a.n_core;
   }
   out->n_cigar_words = w;
@@ -747,6 +837,10 @@ void g3a_destroy(g3a_ctx *ctx) {
   for (auto &ev : ctx->kev)
     if (ev) hipEventDestroy(ev);
   if (ctx->pinned) hipHostFree(ctx->pinned);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->stage[k]) hipHostFree(ctx->stage[k]);
+    if (ctx->stage_ev[k]) hipEventDestroy(ctx->stage_ev[k]);
+  }
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
