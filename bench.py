@@ -200,6 +200,8 @@ def main() -> int:
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end command-line run")
     ap.add_argument("--tuning", default=None, metavar="ROWS,LANE_BLOCKS",
                     help="fixed batched-form tuning instead of the autotuner (e.g. 16,16)")
+    ap.add_argument("--seed-blocks", type=int, default=None, metavar="N",
+                    help="fixed persistent seeding blocks per CU instead of the autotuner (0: occupancy maximum)")
     ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
                     help="extension form(s) to time; with both, value is the batched form's")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -227,6 +229,14 @@ def main() -> int:
     log(f"{n} reads, {int(off[-1])} bases; creating device context")
     al = g3sa_amd.Aligner(str(prefix), max_reads=n, max_bases=int(off[-1]))
     al.upload(seq, off)
+    # seeding grid: fixed, or the autotuner's pick on one aligned pass
+    if args.seed_blocks is not None:
+        al.set_seed_blocks(args.seed_blocks)
+        seed_tuning = {"seed_blocks_per_cu": args.seed_blocks}
+    else:
+        al.run()
+        seed_tuning = al.autotune_seed()
+        log(f"seeding autotuned: {seed_tuning}")
     forms = ["plain", "batched"] if args.form == "both" else [args.form]
     per_form = {}
     for form in forms:
@@ -275,6 +285,7 @@ def main() -> int:
         "extension_gcups": round(gcups, 2),
         "extension_cells": {"reference_cell_set": ref_cells, "computed": work["ext_cells"]},
         "tuning": m.get("tuning"),
+        "seed_tuning": seed_tuning,
         "forms": {f: {"reads_per_s": v["reads_per_s"], "ms_per_step": v["ms_per_step"], "kernel_ms": v["kernel_ms"],
                       "stages_ms": v["stages_ms"]} for f, v in per_form.items()},
         "roofline": roof,

@@ -48,7 +48,8 @@ struct g3a_ctx {
   bool has_input = false, has_output = false;
   DevBuf seq, off;
   DevBuf lep_pool, eh_pool, tb_order, tb_ctr;
-  int seed_threads = 0, tb_threads = 0;
+  int seed_threads = 0, tb_threads = 0;  // resident maxima (the pools are sized for them)
+  int seed_blocks_per_cu = 0;             // 0: the occupancy maximum
   // seeding
   DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
   // seeds / chains
@@ -264,7 +265,7 @@ static int alloc_reads(g3a_ctx *ctx, int n) {
       ensure(ctx, ctx->intv, ((size_t)n * kSlotCap + 1024) * sizeof(Intv)))
     return G3A_E_HIP;
   if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx);
-  return ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * 2 * kLepCap * sizeof(Intv));
+  return ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * (kLepCap + kFoundCap) * sizeof(Intv));
 }
 
 static int alloc_seeds(g3a_ctx *ctx, int64_t T_s, int n) {
@@ -491,7 +492,10 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.lep_pool = P<Intv>(ctx->lep_pool);
   for (int &u : ctx->kev_used) u = 0;
   HIPCK(hipEventRecord(ctx->kev[0], s));
-  launch_seed(sa, ctx->seed_threads, s);
+  const int seed_run = ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * 128 * ctx->n_cu < ctx->seed_threads
+                           ? ctx->seed_blocks_per_cu * 128 * ctx->n_cu
+                           : ctx->seed_threads;
+  launch_seed(sa, seed_run, s);
   HIPCK(hipEventRecord(ctx->kev[1], s));
   ctx->kev_used[0] = 1;
   launch_collect_slow(n, P<int32_t>(ctx->slow), P<int32_t>(ctx->slow_list), P<int32_t>(ctx->counters) + 2 * 8,
@@ -773,6 +777,32 @@ int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *
   ctx->form = form;
   if (strip_rows) *strip_rows = bsr;
   if (lane_blocks_per_cu) *lane_blocks_per_cu = blb;
+  if (best_ms) *best_ms = best;
+  return G3A_OK;
+}
+
+int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu) {
+  if (!ctx || blocks_per_cu < 0 || blocks_per_cu > 64) return G3A_E_ARG;
+  ctx->seed_blocks_per_cu = blocks_per_cu;
+  return G3A_OK;
+}
+
+int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, float *best_ms) {
+  if (!ctx) return G3A_E_ARG;
+  if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
+  const int opts[4] = {4, 6, 8, 0};
+  float best = -1;
+  int bb = ctx->seed_blocks_per_cu;
+  for (int b : opts) {
+    ctx->seed_blocks_per_cu = b;
+    if (int rc = g3a_bwa_align(ctx)) return rc;
+    if (best < 0 || ctx->kms[0] < best) {
+      best = ctx->kms[0];
+      bb = b;
+    }
+  }
+  ctx->seed_blocks_per_cu = bb;
+  if (blocks_per_cu) *blocks_per_cu = bb;
   if (best_ms) *best_ms = best;
   return G3A_OK;
 }

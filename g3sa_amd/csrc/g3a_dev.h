@@ -366,7 +366,7 @@ struct alignas(16) LepNarrow {
 template <class E>
 struct LepBuf {
   E *a, *f;
-  int cap;
+  int cap, cap_f;  // capacities of the LEP list a and the found list f
 };
 
 // Round 1 (preseedAndFilterV2, bwa/src/seed.cu:25-104 with
@@ -425,7 +425,7 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     }
     const int ret = E::get(prev, 0).n + 1;
     // backward search; SMEMs are found in decreasing start order
-    int nf = 0, last_start = 0;
+    int nf = 0, last_start = len + 1;  // start of the last match found (none yet)
     E *fl = buf.f;
     P last_s = 0;  // size of the last entry written to curr (read only when nc > 0)
     for (i = x0 - 1; i >= -1; --i) {
@@ -441,13 +441,14 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
           ext_ok = ok.s >= 1;
         }
         if (!ext_ok) {
-          if (nc == 0) {
-            if (nf == 0 || i + 1 < last_start) {
+          // a new start; only matches long enough to be emitted are stored
+          if (nc == 0 && i + 1 < last_start) {
+            last_start = i + 1;
+            if (pj.n - i >= o.min_seed_len) {
               I p = pj;
               p.m = i + 1;
-              if (nf == buf.cap) return false;
+              if (nf == buf.cap_f) return false;
               E::put(fl, nf++, p);
-              last_start = i + 1;
             }
           }
         } else if (nc == 0 || ok.s != last_s) {
@@ -461,11 +462,8 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
       if (nc == 0) break;
       np = nc;
     }
-    // found in decreasing start (and end) order: emit the long-enough ones by end
-    for (int j = nf - 1; j >= 0; --j) {
-      const I fj = E::get(fl, j);
-      if (fj.n - fj.m + 1 >= o.min_seed_len) out.push(fj);
-    }
+    // found in decreasing start (and end) order: emit them by end
+    for (int j = nf - 1; j >= 0; --j) out.push(E::get(fl, j));
     x0 = ret;
   }
   return true;
@@ -510,10 +508,11 @@ G3A_HD void smem_round1_per_end(const DevIndex &x, const Opt &o, const uint8_t *
 }
 
 // Round 2, reseedV2 (bwa/src/seed.cu:639-913), for one SMEM.  `leps` and
-// `seeds` are scratch lists of capacity cap.  Returns false on overflow.
+// `seeds` are scratch lists of capacities cap and cap_s.  Returns false on
+// overflow.
 template <class E>
 G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int len, const Intv &smem,
-                       E *leps, E *seeds, int cap, IntvSink &out, int64_t *steps) {
+                       E *leps, E *seeds, int cap, int cap_s, IntvSink &out, int64_t *steps) {
   using I = typename E::I;
   using P = decltype(I::s);
   const int split_len = (int)(o.min_seed_len * o.split_factor + .499);
@@ -568,7 +567,7 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
         E::put(leps, nl++, ext);
         curr = ext.s;
       } else if (!collected && lep.n - lep.m + 1 >= o.min_seed_len) {
-        if (ns == cap) return false;
+        if (ns == cap_s) return false;
         E::put(seeds, ns++, lep);
         collected = true;
       }
@@ -578,7 +577,7 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
   if (nl > 0) {
     const I l0 = E::get(leps, 0);
     if (l0.n - l0.m + 1 >= o.min_seed_len) {
-      if (ns == cap) return false;
+      if (ns == cap_s) return false;
       E::put(seeds, ns++, l0);
     }
   }

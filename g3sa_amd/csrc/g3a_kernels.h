@@ -10,7 +10,8 @@ namespace g3a {
 constexpr int kSlowLep = 400;                  // slow-path LEP list capacity
 constexpr int kSlowScratch = 5 * kSlowLep;     // Intv entries of scratch per slow read
 constexpr int kSlotCap = 64;                   // fast-path interval slots per read
-constexpr int kLepCap = 32;                    // fast-path LEP list capacity
+constexpr int kLepCap = 24;                    // fast-path LEP list capacity
+constexpr int kFoundCap = 8;                   // fast-path found-match (and reseed seed) list capacity
 
 struct SeedArgs {
   DevIndex idx;
@@ -24,7 +25,7 @@ struct SeedArgs {
   int64_t *n_seed;
   int32_t *slow;
   int64_t *fm_steps;
-  Intv *lep_pool;    // 2 * kLepCap entries per k_seed thread
+  Intv *lep_pool;    // kLepCap + kFoundCap entries per k_seed thread
 };
 
 struct SaArgs {

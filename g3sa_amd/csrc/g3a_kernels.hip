@@ -35,8 +35,8 @@ template <class E>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
-  E *la = reinterpret_cast<E *>(a.lep_pool) + (size_t)gtid * 2 * kLepCap, *lf = la + kLepCap;
-  LepBuf<E> buf{la, lf, kLepCap};
+  E *la = reinterpret_cast<E *>(a.lep_pool) + (size_t)gtid * (kLepCap + kFoundCap), *lf = la + kLepCap;
+  LepBuf<E> buf{la, lf, kLepCap, kFoundCap};
   int64_t steps = 0;
   for (int r = gtid; r < a.n_reads; r += nthreads) {
     const int off = a.off[r], len = a.off[r + 1] - off;
@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
       if (sink.n <= sink.cap) {
         for (int k = 0; ok && k < nsm; ++k) {
           Intv smem = sink.slot[k];
-          ok = reseed_one(a.idx, a.opt, q, len, smem, la, lf, kLepCap, sink, &steps);
+          ok = reseed_one(a.idx, a.opt, q, len, smem, la, lf, kLepCap, kFoundCap, sink, &steps);
         }
       } else ok = false;
       if (ok) reseed_round3<decltype(E::I::s)>(a.idx, a.opt, q, len, sink, &steps);
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
   Intv *slot = pass ? pool + pool_off[t] : sc + 3 * kSlowLep;
   const int cap = pass ? (int)(pool_off[t + 1] - pool_off[t]) : 2 * kSlowLep;
   IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};
-  LepBuf<LepWide> buf{reinterpret_cast<LepWide *>(sc), reinterpret_cast<LepWide *>(sc) + kSlowLep, kSlowLep};
+  LepBuf<LepWide> buf{reinterpret_cast<LepWide *>(sc), reinterpret_cast<LepWide *>(sc) + kSlowLep, kSlowLep, kSlowLep};
   if (!smem_round1(a.idx, a.opt, q, len, sink, buf, nullptr)) {
     sink.n = 0;
     sink.n_seed = 0;
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
   const int nsm = sink.n;
   for (int k = 0; k < nsm; ++k) {
     Intv smem = slot[k];
-    reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.f, kSlowLep, sink, nullptr);
+    reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.f, kSlowLep, kSlowLep, sink, nullptr);
   }
   reseed_round3<int64_t>(a.idx, a.opt, q, len, sink, nullptr);
   a.n_intv[r] = sink.n;

@@ -136,6 +136,17 @@ int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wav
  * for every setting; the batch's output is left as it was.  No reference
  * counterpart. */
 int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *best_ms);
+/* Persistent seeding blocks (128 threads) per compute unit; 0 restores the
+ * default, the most the kernel's occupancy allows.  Each resident thread
+ * keeps its LEP lists in a global pool slice, so fewer threads also shrink
+ * the working set the caches see.  No reference counterpart. */
+int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu);
+/* Seeding autotuner: re-aligns the batch last aligned once per seeding grid
+ * (blocks per compute unit 4, 6, 8 and the occupancy maximum), keeps the one
+ * with the fastest seeding kernel and reports it (0 = maximum) with its
+ * time.  The batch's output is identical for every grid.  No reference
+ * counterpart. */
+int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, float *best_ms);
 const char *g3a_last_error(const g3a_ctx *ctx);
 
 /* Index construction (the `g3a index` command): BWA-format files plus the
