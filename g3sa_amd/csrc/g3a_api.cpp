@@ -512,7 +512,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
     const int32_t *list = P<int32_t>(ctx->slow_list);
     for (int c0 = 0; c0 < n_slow; c0 += chunk) {
       int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
-      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), nullptr, nullptr, 0, s);
+      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), nullptr, nullptr, 0, 0, s);
     }
     launch_gather_slow_counts(n_slow, list, P<int32_t>(ctx->n_intv), P<int64_t>(ctx->slow_n), s);
     scan_i64(P<int64_t>(ctx->slow_n), P<int64_t>(ctx->slow_off), n_slow, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
@@ -532,7 +532,8 @@ int g3a_bwa_align(g3a_ctx *ctx) {
     Intv *pool_base = P<Intv>(ctx->intv) + fast_entries;
     for (int c0 = 0; c0 < n_slow; c0 += chunk) {
       int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
-      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), P<int64_t>(ctx->slow_off) + c0, pool_base, 1, s);
+      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), P<int64_t>(ctx->slow_off) + c0, pool_base, 1,
+                       n_slow <= chunk, s);
     }
     launch_set_slow_base(n_slow, list, P<int64_t>(ctx->slow_off), (int64_t)fast_entries, P<int64_t>(ctx->intv_base), s);
   }

@@ -10,7 +10,7 @@ namespace g3a {
 constexpr int kSlowLep = 400;                  // slow-path LEP list capacity
 constexpr int kSlowScratch = 5 * kSlowLep;     // Intv entries of scratch per slow read
 constexpr int kSlotCap = 64;                   // fast-path interval slots per read
-constexpr int kLepCap = 24;                    // fast-path LEP list capacity
+constexpr int kLepCap = 32;                    // fast-path LEP list capacity
 constexpr int kFoundCap = 8;                   // fast-path found-match (and reseed seed) list capacity
 
 struct SeedArgs {
@@ -158,7 +158,7 @@ struct TbArgs {
 int seed_grid_threads(int n_cu, const DevIndex &x);
 void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s);
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
-                      Intv *pool, int pass, hipStream_t s);
+                      Intv *pool, int pass, int from_scratch, hipStream_t s);
 void launch_sa_lookup(const SaArgs &a, hipStream_t s);
 void launch_sort(const SortArgs &a, hipStream_t s);
 void launch_chain(const ChainArgs &a, hipStream_t s);

@@ -73,16 +73,24 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
 
 // Slow path for the reads listed in `list`: global-memory LEP lists of
 // kSlowLep entries, which no read of <= 320 bases can outgrow.  Pass 0 counts
-// (storing only the round-1 SMEMs, which round 2 needs), pass 1 stores all
-// intervals at the pool offsets computed from the counts.
+// and keeps the first 2 * kSlowLep intervals in the read's scratch; pass 1
+// puts all intervals at the pool offsets computed from the counts, copying
+// them from the scratch when it still holds them all (`from_scratch`: every
+// slow read had its own scratch in pass 0) and searching again otherwise.
 __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *list, int n_list, Intv *scratch,
-                                                  const int64_t *pool_off, Intv *pool, int pass) {
+                                                  const int64_t *pool_off, Intv *pool, int pass, int from_scratch) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_list) return;
   const int r = list[t];
   const int off = a.off[r], len = a.off[r + 1] - off;
   const uint8_t *q = a.seq + off;
   Intv *sc = scratch + (int64_t)t * kSlowScratch;
+  if (pass && from_scratch && a.n_intv[r] <= 2 * kSlowLep) {
+    const Intv *src = sc + 3 * kSlowLep;
+    Intv *dst = pool + pool_off[t];
+    for (int k = 0, n = a.n_intv[r]; k < n; ++k) dst[k] = src[k];
+    return;
+  }
   Intv *slot = pass ? pool + pool_off[t] : sc + 3 * kSlowLep;
   const int cap = pass ? (int)(pool_off[t + 1] - pool_off[t]) : 2 * kSlowLep;
   IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};
@@ -755,10 +763,10 @@ void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s) {
   else hipLaunchKernelGGL(k_seed<LepWide>, grid, dim3(128), 0, s, a);
 }
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
-                      Intv *pool, int pass, hipStream_t s) {
+                      Intv *pool, int pass, int from_scratch, hipStream_t s) {
   if (n_list)
     hipLaunchKernelGGL(k_seed_slow, dim3(blocks(n_list, 64)), dim3(64), 0, s, a, list, n_list, scratch, pool_off, pool,
-                       pass);
+                       pass, from_scratch);
 }
 void launch_sa_lookup(const SaArgs &a, hipStream_t s) {
   if (a.total) hipLaunchKernelGGL(k_sa_lookup, dim3(blocks(a.total, 256)), dim3(256), 0, s, a);

@@ -214,3 +214,39 @@ def test_api_error_paths(ds_c1):
         lib.g3a_batch_out_free(C.byref(out))
     finally:
         lib.g3a_destroy(ctx)
+
+
+def test_slow_seeding_path_parity(built, oracle, tmp_path):
+    """Reads inside long tandem repeats grow their LEP lists past the fast
+    seeding path's capacity and are seeded again on the slow path (global
+    lists of 400 entries); the output still equals the oracle's."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(17)
+    L = 200_000
+    g = rng.integers(0, 4, L).astype(np.uint8)
+    spans = [(20_000, [0, 1, 2]), (80_000, [3, 1, 0, 2]), (150_000, [2, 2, 0, 1, 3, 3])]
+    for start, unit in spans:  # 4 kbp tandem repeats at 1% point divergence
+        rep = np.tile(np.array(unit, np.uint8), 4000 // len(unit) + 1)[:4000]
+        mut = rng.random(4000) < 0.01
+        rep[mut] = rng.integers(0, 4, int(mut.sum()))
+        g[start:start + 4000] = rep
+    fa = tmp_path / "tandem.fa"
+    fa.write_text(">chrT\n" + "\n".join("".join("ACGT"[b] for b in g[i:i + 60]) for i in range(0, L, 60)) + "\n")
+    prefix = tmp_path / "tandem"
+    g3sa_amd.build_index(str(fa), str(prefix), 8)
+    reads = []
+    for k in range(400):
+        if k % 4 == 3:
+            p = int(rng.integers(0, L - 150))
+        else:
+            s0 = spans[k % 3][0]
+            p = int(rng.integers(s0, s0 + 4000 - 150))
+        r = g[p:p + 150].copy()
+        if k % 2:
+            r = (3 - r[::-1]).astype(np.uint8)
+        sub = rng.random(150) < 0.005
+        r[sub] = rng.integers(0, 4, int(sub.sum()))
+        reads.append(r)
+    seq, off = _batch(reads)
+    got, _ = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats")
+    assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path"
