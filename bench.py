@@ -181,7 +181,7 @@ def end_to_end(prefix: Path, fq: Path) -> dict | None:
         log(f"g3a mem failed: {r.stderr[-300:]}")
         return None
     out = {"reads_per_s": float(m.group(3)), "reads": int(m.group(1)), "seconds": float(m.group(2)),
-           "what": "g3a mem -S, standard SAM to /dev/null, batches of 500k reads, index load excluded"}
+           "what": "g3a mem -S, standard SAM to /dev/null, batches of 250k reads, index load excluded"}
     b = re.search(r"busy time: load ([0-9.]+) s, align ([0-9.]+) s, format\+write ([0-9.]+) s", r.stderr)
     if b:
         out["stage_busy_s"] = {"load": float(b.group(1)), "align": float(b.group(2)), "format_write": float(b.group(3))}

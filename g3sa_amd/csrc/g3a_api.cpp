@@ -58,6 +58,7 @@ struct g3a_ctx {
   DevBuf tasks, regs, flag, sec, n_aln, aln_off;
   // alignments
   DevBuf aln_reg, aln_read, aln_rank, z_size, cig_size, z_off, cig_off, z, cigar, alns;
+  DevBuf cig_words, cig_coff, cig_out;  // compacted CIGARs for the host copy
   DevBuf scan_tmp;
   // batched-form extension
   int form = G3A_FORM_BATCHED;
@@ -305,6 +306,12 @@ static int reserve_workspace(g3a_ctx *ctx) {
   if (alloc_reads(ctx, n) || alloc_seeds(ctx, 12 * (int64_t)n, n) || alloc_tasks(ctx, 4 * (int64_t)n) ||
       alloc_alns(ctx, n) || alloc_paths(ctx, 4 * nb, 4 * nb + 4 * (int64_t)n))
     return G3A_E_HIP;
+  // host copy of the alignment records and CIGARs (32 words per read),
+  // written once here so its pages are resident before the first batch
+  if (ensure(ctx, ctx->cig_words, (size_t)n * 4 + 4) || ensure(ctx, ctx->cig_coff, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->cig_out, (size_t)n * 128 + 64))
+    return G3A_E_HIP;
+  ctx->out_buf.assign((size_t)n * (sizeof(Aln) + 128) + 256, 0);
   return 0;
 }
 
@@ -629,6 +636,17 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   launch_traceback(tb, ctx->tb_threads, ctx->n_cu, s);
   HIPCK(hipEventRecord(ctx->kev[11], s));
   ctx->kev_used[5] = 1;
+  // CIGARs packed to the words used, for the copy to the host
+  if (ensure(ctx, ctx->cig_words, (size_t)T_a * 4 + 4) || ensure(ctx, ctx->cig_coff, (size_t)(T_a + 1) * 8))
+    return G3A_E_HIP;
+  launch_cig_words(P<Aln>(ctx->alns), T_a, P<int32_t>(ctx->cig_words), s);
+  scan_i32(P<int32_t>(ctx->cig_words), P<int64_t>(ctx->cig_coff), T_a, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->cig_coff) + T_a})) return G3A_E_HIP;
+  const int64_t T_u = ctx->pinned[0];
+  if (ensure(ctx, ctx->cig_out, (size_t)T_u * 4 + 64)) return G3A_E_HIP;
+  launch_cig_gather(P<Aln>(ctx->alns), T_a, P<int64_t>(ctx->cig_coff), P<uint32_t>(ctx->cigar),
+                    P<uint32_t>(ctx->cig_out), s);
+  ctx->total_cig = T_u;
   HIPCK(hipEventRecord(ctx->ev[5], s));
   HIPCK(hipStreamSynchronize(s));
   HIPCK(hipGetLastError());
@@ -673,7 +691,7 @@ int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   if (na)
     if (int rc = stage_d2h(ctx, hb, ctx->alns.p, aln_bytes)) return rc;
   if (cig_bytes)
-    if (int rc = stage_d2h(ctx, hb + at_cig, ctx->cigar.p, cig_bytes)) return rc;
+    if (int rc = stage_d2h(ctx, hb + at_cig, ctx->cig_out.p, cig_bytes)) return rc;
   const Aln *alns = (const Aln *)hb;
   const uint32_t *cig = (const uint32_t *)(hb + at_cig);
   const size_t m = (size_t)(na ? na : 1);
@@ -859,6 +877,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->tasks_tmp, &ctx->n_task, &ctx->task_off, &ctx->tasks, &ctx->regs, &ctx->flag, &ctx->sec,
                     &ctx->n_aln, &ctx->aln_off, &ctx->aln_reg, &ctx->aln_read, &ctx->aln_rank, &ctx->z_size,
                     &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp,
+                    &ctx->cig_words, &ctx->cig_coff, &ctx->cig_out,
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
                     &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
                     &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr};

@@ -192,5 +192,7 @@ void launch_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_
 void scan_i32(const int32_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t s);
 void scan_i64(const int64_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t s);
 size_t scan_tmp_bytes(int64_t n);
+void launch_cig_words(const Aln *alns, int64_t n, int32_t *words, hipStream_t s);
+void launch_cig_gather(Aln *alns, int64_t n, const int64_t *coff, const uint32_t *cig, uint32_t *out, hipStream_t s);
 
 }  // namespace g3a

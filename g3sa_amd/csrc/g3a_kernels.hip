@@ -857,6 +857,27 @@ __global__ void k_set_slow_base(int n, const int32_t *list, const int64_t *off, 
 }
 __global__ void k_zero_first(int64_t *p) { *p = 0; }
 
+// CIGAR compaction for the host copy: the traceback writes each alignment's
+// two CIGARs into a slot sized for the worst case; the copy holds only the
+// words used, alignment by alignment (reference-format CIGAR, then the
+// global one), and the records point into it.
+__global__ void __launch_bounds__(256) k_cig_words(const Aln *alns, int64_t n, int32_t *words) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) words[i] = alns[i].n_cigar + alns[i].n_core;
+}
+__global__ void __launch_bounds__(256) k_cig_gather(Aln *alns, int64_t n, const int64_t *coff, const uint32_t *cig,
+                                                    uint32_t *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Aln a = alns[i];
+  const int64_t o = coff[i];
+  for (int k = 0; k < a.n_cigar; ++k) out[o + k] = cig[a.cigar_off + k];
+  for (int k = 0; k < a.n_core; ++k) out[o + a.n_cigar + k] = cig[a.core_off + k];
+  a.cigar_off = o;
+  a.core_off = o + a.n_cigar;
+  alns[i] = a;
+}
+
 void launch_collect_slow(int n, const int32_t *slow, int32_t *list, int32_t *count, int64_t *intv_base, int slot_cap,
                          hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_collect_slow, dim3(blocks(n, 256)), dim3(256), 0, s, n, slow, list, count, intv_base,
@@ -873,6 +894,14 @@ void launch_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_
 struct WidenI32 {
   __host__ __device__ int64_t operator()(int32_t v) const { return (int64_t)v; }
 };
+
+void launch_cig_words(const Aln *alns, int64_t n, int32_t *words, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_cig_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, alns, n, words);
+}
+void launch_cig_gather(Aln *alns, int64_t n, const int64_t *coff, const uint32_t *cig, uint32_t *out, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cig_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, alns, n, coff, cig, out);
+}
 
 size_t scan_tmp_bytes(int64_t n) {
   size_t a = 0, b = 0;

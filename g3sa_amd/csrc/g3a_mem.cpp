@@ -13,6 +13,7 @@
 #include "g3a_index.h"
 
 #include <fcntl.h>
+#include <malloc.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -551,8 +552,14 @@ bool load_ref(const char *prefix, Ref &ref) {
 }  // namespace
 
 int g3a_mem_main(int argc, char **argv) {
+  // Batch-sized buffers (parsed batches, the library's output arrays, text
+  // parts) are freed and allocated again every batch: keep them in the heap
+  // instead of mapping fresh pages each time, since page faults cost more
+  // than the work done on the pages.
+  mallopt(M_MMAP_THRESHOLD, 1 << 30);
+  mallopt(M_TRIM_THRESHOLD, 1 << 30);
   const char *out_fn = nullptr;
-  int batch = 500000, gpus = 1, threads = 8;
+  int batch = 250000, gpus = 1, threads = 8;
   bool sam = false;
   int i = 1;
   for (; i < argc && argv[i][0] == '-' && argv[i][1]; ++i) {
