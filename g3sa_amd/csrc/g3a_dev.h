@@ -308,6 +308,58 @@ struct TextReader {
   }
 };
 
+// Sequential reader of a byte array (read codes): keeps the 16-byte aligned
+// chunk holding the last byte read, so a run of neighbouring positions costs
+// one vector load per 16 bytes.  The array must be readable up to the next
+// 16-byte boundary past its end (the device read buffer has that padding).
+struct ByteReader {
+  const uint8_t *base;
+  int64_t cidx;
+  uint32_t w0, w1, w2, w3;
+  G3A_HD explicit ByteReader(const uint8_t *b) : base(b), cidx(-1), w0(0), w1(0), w2(0), w3(0) {}
+  G3A_HD int operator()(int j) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint8_t *p = base + j;
+    const uint64_t a = (uint64_t)p;
+    const int64_t ci = (int64_t)(a >> 4);
+    if (ci != cidx) {
+      cidx = ci;
+      const uint4 v = *(const uint4 *)(p - (a & 15));
+      w0 = v.x;
+      w1 = v.y;
+      w2 = v.z;
+      w3 = v.w;
+    }
+    const int k = (int)((a >> 2) & 3);
+    const uint32_t x = k == 0 ? w0 : k == 1 ? w1 : k == 2 ? w2 : w3;
+    return (int)((x >> (8 * (a & 3))) & 0xff);
+#else
+    return base[j];
+#endif
+  }
+};
+
+// Scores of one target base against query codes 0..4, packed: codes 0..3 as
+// bytes of lo, code 4 in hi.  Replaces per-cell lookups into the score
+// matrix (a kernel argument, so a per-lane index would be a memory load).
+struct ScoreRow {
+  uint32_t lo;
+  int hi;
+  G3A_HD int operator()(int q) const { return q < 4 ? (int)(int8_t)(uint8_t)(lo >> (8 * q)) : hi; }
+};
+G3A_HD ScoreRow score_row(const Opt &o, int t) {
+  ScoreRow r;
+  r.lo = 0;
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t v0 = (uint8_t)o.mat[c], v1 = (uint8_t)o.mat[5 + c], v2 = (uint8_t)o.mat[10 + c],
+                   v3 = (uint8_t)o.mat[15 + c], v4 = (uint8_t)o.mat[20 + c];
+    const uint32_t v = t == 0 ? v0 : t == 1 ? v1 : t == 2 ? v2 : t == 3 ? v3 : v4;
+    r.lo |= v << (8 * c);
+  }
+  r.hi = t == 0 ? o.mat[4] : t == 1 ? o.mat[9] : t == 2 ? o.mat[14] : t == 3 ? o.mat[19] : o.mat[24];
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Seeding
 // ---------------------------------------------------------------------------
@@ -1142,7 +1194,7 @@ G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH 
   for (; j <= qlen && j <= w + 1; ++j) eh.H(j) = eh.E(j) = MINUS_INF;
   for (int i = 0; i < tlen; ++i) {
     int32_t f = MINUS_INF, h1, beg, end, t;
-    const int tb = tt(i);
+    const ScoreRow sr = score_row(o, tt(i));
     beg = i > w ? i - w : 0;
     end = i + w + 1 < qlen ? i + w + 1 : qlen;
     h1 = beg == 0 ? -(o.o_del + o.e_del * (i + 1)) : MINUS_INF;
@@ -1152,7 +1204,7 @@ G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH 
       int32_t h, m = eh.H(j), e = eh.E(j);
       uint32_t d;
       eh.H(j) = h1;
-      m += o.mat[tb * 5 + tq(j)];
+      m += sr(tq(j));
       d = m >= e ? 0 : 1;
       h = m >= e ? m : e;
       d = h >= f ? d : 2;
@@ -1225,25 +1277,31 @@ G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *r
   uint32_t *core = cig + core_at;
   int n_core = 0, nw = 0;
   TextReader tr(x.pac, l_pac);
+  // NM as the reference's traceback counts it (aux.cu:2910-2941)
+  int n_mm = 0, n_gap = 0;
   if (w == 0) {
+    // one ungapped pass: score and mismatches together
     core[0] = (uint32_t)(lq << 4);
     n_core = 1;
-    for (int i = 0; i < lq; ++i) nw += o.mat[tr(rb + i) * 5 + query[i]];
+    ByteReader qr(query);
+    for (int i = 0; i < lq; ++i) {
+      const int tb = tr(rb + i), qc = qr(i);
+      nw += score_row(o, tb)(qc);
+      n_mm += qc != tb;
+    }
   } else {
+    // the band revisits each query byte on 2w+1 rows: plain loads (cached)
     nw = global_align(
         o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return tr(rb + i); }, w, eh, z, core,
         &n_core);
-  }
-  // NM as the reference's traceback counts it (aux.cu:2910-2941)
-  int n_mm = 0, n_gap = 0;
-  {
     TextReader trn(x.pac, l_pac);
+    ByteReader qn(query);
     int xq = 0;
     int64_t yr = 0;
     for (int k = 0; k < n_core; ++k) {
       int op = core[k] & 0xf, ln = (int)(core[k] >> 4);
       if (op == 0) {
-        for (int i = 0; i < ln; ++i) n_mm += query[xq + i] != trn(rb + yr + i);
+        for (int i = 0; i < ln; ++i) n_mm += qn(xq + i) != trn(rb + yr + i);
         xq += ln;
         yr += ln;
       } else if (op == 2) {
