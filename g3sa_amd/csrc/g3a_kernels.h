@@ -151,7 +151,7 @@ struct TbArgs {
   Aln *alns;
   int32_t *eh_pool;  // traceback_pool_words(threads) words
   int32_t *tb_order;  // n_alns: processing order
-  int32_t *tb_ctr;    // 2 counters
+  int32_t *tb_ctr;    // 8 counters: class sizes, placement cursors
 };
 
 // threads of k_seed's persistent grid (occupancy-limited, n_cu compute units)

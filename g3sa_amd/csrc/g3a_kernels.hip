@@ -681,42 +681,63 @@ __global__ void __launch_bounds__(128) k_aln_prep(AlnPrepArgs a) {
 // Alignment order for k_traceback: those that need the banded global
 // alignment (w > 0) first, the ungapped ones (w = 0) last, so that the DP
 // loops of a wavefront run with all lanes busy.
+// Traceback classes by band: 0: 2w+2 <= 16, 1: <= 32, 2: wider (w > 0),
+// 3: ungapped (w = 0).  The row buffers of class c < 2 fit an LDS ring of
+// 16 << c columns per lane, so the smaller classes run at higher occupancy.
+__device__ __forceinline__ int tb_class(const TbArgs &a, int64_t t) {
+  const Region g = a.aln_reg[t];
+  const int w = global_bw(a.opt, g.qe - g.qb, g.re - g.rb, g.score);
+  return w == 0 ? 3 : 2 * w + 2 <= 16 ? 0 : 2 * w + 2 <= 32 ? 1 : 2;
+}
+// tb_ctr[0..2]: alignments of classes 0..2
+__global__ void __launch_bounds__(256) k_tb_count(TbArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = t < a.n_alns ? tb_class(a, t) : 3;
+  for (int k = 0; k < 3; ++k) wave_reserve(c == k, a.tb_ctr + k);
+}
+// The list holds classes 0, 1, 2 in turn from the front and class 3 from the
+// back; tb_ctr[4..7] are the placement cursors.
 __global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = t < a.n_alns;
-  bool dp = false;
-  if (in) {
-    const Region g = a.aln_reg[t];
-    dp = global_bw(a.opt, g.qe - g.qb, g.re - g.rb, g.score) != 0;
+  const int c = in ? tb_class(a, t) : -1;
+  const int64_t base1 = a.tb_ctr[0], base2 = base1 + a.tb_ctr[1];
+  int at = -1;
+  for (int k = 0; k < 4; ++k) {
+    const int r = wave_reserve(c == k, a.tb_ctr + 4 + k);
+    if (c == k) at = r;
   }
-  const int f = wave_reserve(in && dp, a.tb_ctr);
-  const int b = wave_reserve(in && !dp, a.tb_ctr + 1);
-  if (in) a.tb_order[dp ? f : a.n_alns - 1 - b] = (int32_t)t;
+  if (in) {
+    const int64_t pos = c == 0 ? at : c == 1 ? base1 + at : c == 2 ? base2 + at : a.n_alns - 1 - at;
+    a.tb_order[pos] = (int32_t)t;
+  }
 }
 
-// Persistent grids over the ordered list: k_traceback_dp takes the first
-// tb_ctr[0] entries (w > 0) with its row buffers in a 64-column ring per lane
-// in LDS when 2w+2 <= 64, else in the thread's slice of the global pool
-// (2 * kQMax words, contiguous per thread); k_traceback takes the rest.
-constexpr int kTbRing = 64;
-
-__global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a) {
-  __shared__ int32_t ring_h[kTbRing * 64], ring_e[kTbRing * 64];
+// Persistent grids over the ordered list.  k_traceback_dp<RB> takes one
+// class: its row buffers are an RB-column ring per lane in LDS (lanes
+// interleaved), or for the widest class with 2w+2 > RB the thread's slice of
+// the global pool (2 * kQMax words, contiguous per thread); k_traceback takes
+// the ungapped class.
+template <int RB>
+__global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a, int cls) {
+  __shared__ int32_t ring_h[RB * 64], ring_e[RB * 64];
   const int lane = threadIdx.x;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + lane;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
   int32_t *eh_h = a.eh_pool + gtid * 2 * kQMax, *eh_e = eh_h + kQMax;
-  const int64_t n_dp = a.tb_ctr[0];
-  for (int64_t e = gtid; e < n_dp; e += nthreads) {
+  int64_t e0 = 0;
+  for (int k = 0; k < cls; ++k) e0 += a.tb_ctr[k];
+  const int64_t e1 = e0 + a.tb_ctr[cls];
+  for (int64_t e = e0 + gtid; e < e1; e += nthreads) {
     const int64_t t = a.tb_order[e];
     const int r = a.aln_read[t];
     const int off = a.off[r], l_seq = a.off[r + 1] - off;
     const Region g = a.aln_reg[t];
     const int w = global_bw(a.opt, g.qe - g.qb, g.re - g.rb, g.score);
     Aln out;
-    if (2 * w + 2 <= kTbRing)
+    if (2 * w + 2 <= RB)
       trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t],
-                         EhRing<kTbRing>{ring_h + lane, ring_e + lane, 64}, a.z + a.z_off[t], a.cigar + a.cig_off[t],
+                         EhRing<RB>{ring_h + lane, ring_e + lane, 64}, a.z + a.z_off[t], a.cigar + a.cig_off[t],
                          a.cig_off[t], out);
     else
       trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], EhArray{eh_h, eh_e},
@@ -729,7 +750,7 @@ __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
   int32_t *eh_h = a.eh_pool + gtid * 2 * kQMax, *eh_e = eh_h + kQMax;
-  for (int64_t e = a.tb_ctr[0] + gtid; e < a.n_alns; e += nthreads) {
+  for (int64_t e = a.tb_ctr[0] + a.tb_ctr[1] + a.tb_ctr[2] + gtid; e < a.n_alns; e += nthreads) {
     const int64_t t = a.tb_order[e];
     const int r = a.aln_read[t];
     const int off = a.off[r], l_seq = a.off[r + 1] - off;
@@ -821,17 +842,24 @@ int traceback_grid_threads(int n_cu) {
 int64_t traceback_pool_words(int n_threads) { return (int64_t)n_threads * 2 * kQMax; }
 void launch_traceback(const TbArgs &a, int n_threads, int n_cu, hipStream_t s) {
   if (!a.n_alns) return;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, a.tb_ctr, 2, 0);
-  hipLaunchKernelGGL(k_tb_order, dim3((unsigned)((a.n_alns + 255) / 256)), dim3(256), 0, s, a);
-  {
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, a.tb_ctr, 8, 0);
+  const dim3 g256((unsigned)((a.n_alns + 255) / 256));
+  hipLaunchKernelGGL(k_tb_count, g256, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tb_order, g256, dim3(256), 0, s, a);
+  // the pool has a slice for each of the n_threads threads
+  auto dp_blocks = [&](const void *k) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_traceback_dp, 64, 0) != hipSuccess || per_cu < 1)
-      per_cu = 4;
-    int blocks_dp = n_cu * per_cu;  // the pool has a slice for each of these threads
-    if (blocks_dp > n_threads / 64) blocks_dp = n_threads / 64;
-    if (blocks_dp < 1) blocks_dp = 1;
-    hipLaunchKernelGGL(k_traceback_dp, dim3((unsigned)blocks_dp), dim3(64), 0, s, a);
-  }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0) != hipSuccess || per_cu < 1) per_cu = 4;
+    int b = n_cu * per_cu;
+    if (b > n_threads / 64) b = n_threads / 64;
+    return b < 1 ? 1 : b;
+  };
+  hipLaunchKernelGGL(k_traceback_dp<16>, dim3((unsigned)dp_blocks((const void *)k_traceback_dp<16>)), dim3(64), 0, s,
+                     a, 0);
+  hipLaunchKernelGGL(k_traceback_dp<32>, dim3((unsigned)dp_blocks((const void *)k_traceback_dp<32>)), dim3(64), 0, s,
+                     a, 1);
+  hipLaunchKernelGGL(k_traceback_dp<64>, dim3((unsigned)dp_blocks((const void *)k_traceback_dp<64>)), dim3(64), 0, s,
+                     a, 2);
   const int64_t need = (a.n_alns + 63) / 64, have = n_threads / 64;
   hipLaunchKernelGGL(k_traceback, dim3((unsigned)(need < have ? need : have)), dim3(64), 0, s, a);
 }
