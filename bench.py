@@ -202,6 +202,8 @@ def main() -> int:
                     help="fixed batched-form tuning instead of the autotuner (e.g. 16,16)")
     ap.add_argument("--seed-blocks", type=int, default=None, metavar="N",
                     help="fixed persistent seeding blocks per CU instead of the autotuner (0: occupancy maximum)")
+    ap.add_argument("--seed-order", type=int, default=1, choices=[0, 1],
+                    help="with --seed-blocks: seed in minimizer order (1) or input order (0)")
     ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
                     help="extension form(s) to time; with both, value is the batched form's")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -232,7 +234,8 @@ def main() -> int:
     # seeding grid: fixed, or the autotuner's pick on one aligned pass
     if args.seed_blocks is not None:
         al.set_seed_blocks(args.seed_blocks)
-        seed_tuning = {"seed_blocks_per_cu": args.seed_blocks}
+        al.set_seed_order(args.seed_order)
+        seed_tuning = {"seed_blocks_per_cu": args.seed_blocks, "seed_ordered": args.seed_order}
     else:
         al.run()
         seed_tuning = al.autotune_seed()

@@ -64,7 +64,7 @@ EXPORTS = [
     "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
     "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
-    "g3a_set_seed_blocks", "g3a_autotune_seed",
+    "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order",
 ]
 
 _lib = None
@@ -96,7 +96,8 @@ def load_library() -> C.CDLL:
     lib.g3a_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     lib.g3a_set_seed_blocks.argtypes = [C.c_void_p, C.c_int]
-    lib.g3a_autotune_seed.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    lib.g3a_autotune_seed.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
+    lib.g3a_set_seed_order.argtypes = [C.c_void_p, C.c_int]
     _lib = lib
     return lib
 
@@ -171,11 +172,15 @@ class Aligner:
         """Persistent seeding blocks per compute unit (0: occupancy maximum)."""
         self._check(self.lib.g3a_set_seed_blocks(self.ctx, blocks_per_cu))
 
+    def set_seed_order(self, ordered: bool) -> None:
+        """Seed reads in minimizer order (True) or input order (False)."""
+        self._check(self.lib.g3a_set_seed_order(self.ctx, int(bool(ordered))))
+
     def autotune_seed(self) -> dict:
-        """Re-align the last batch per seeding grid and keep the fastest."""
-        b, ms = C.c_int(), C.c_float()
-        self._check(self.lib.g3a_autotune_seed(self.ctx, C.byref(b), C.byref(ms)))
-        return {"seed_blocks_per_cu": b.value, "k_seed_ms": ms.value}
+        """Re-align the last batch per seeding grid and order; keep the fastest."""
+        b, o, ms = C.c_int(), C.c_int(), C.c_float()
+        self._check(self.lib.g3a_autotune_seed(self.ctx, C.byref(b), C.byref(o), C.byref(ms)))
+        return {"seed_blocks_per_cu": b.value, "seed_ordered": o.value, "seed_stage_ms": ms.value}
 
     def _check(self, rc: int) -> None:
         if rc != G3A_OK:

@@ -42,12 +42,14 @@ struct g3a_ctx {
   int max_reads = 0;
   int64_t max_bases = 0;
   // index
-  DevBuf occ, pac, sa, ctg_off, ctg_len, ctg_alt, kmer;
+  DevBuf occ, pac, sa, ctg_off, ctg_len, ctg_alt, kmer, occn, sbc;
   // batch input
   int n_reads = 0;
   bool has_input = false, has_output = false;
   DevBuf seq, off;
   DevBuf lep_pool, eh_pool, tb_order, tb_ctr;
+  DevBuf seed_key, seed_key2, seed_idx, seed_order, seed_sort_tmp;  // seeding order
+  int seed_ordered = 0;  // seed reads in minimizer order (1) or input order (0)
   int seed_threads = 0, tb_threads = 0;  // resident maxima (the pools are sized for them)
   int seed_blocks_per_cu = 0;             // 0: the occupancy maximum
   // seeding
@@ -265,6 +267,10 @@ static int alloc_reads(g3a_ctx *ctx, int n) {
       ensure(ctx, ctx->slow_list, (size_t)n * 4 + 4) || ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(ctx->max_reads + 1)) ||
       ensure(ctx, ctx->intv, ((size_t)n * kSlotCap + 1024) * sizeof(Intv)))
     return G3A_E_HIP;
+  if (ensure(ctx, ctx->seed_key, (size_t)n * 4 + 4) || ensure(ctx, ctx->seed_key2, (size_t)n * 4 + 4) ||
+      ensure(ctx, ctx->seed_idx, (size_t)n * 4 + 4) || ensure(ctx, ctx->seed_order, (size_t)n * 4 + 4) ||
+      ensure(ctx, ctx->seed_sort_tmp, seed_sort_tmp_bytes(n)))
+    return G3A_E_HIP;
   if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx);
   return ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * (kLepCap + kFoundCap) * sizeof(Intv));
 }
@@ -353,7 +359,15 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   x.ctg_len = P<int64_t>(ctx->ctg_len);
   x.ctg_alt = P<int32_t>(ctx->ctg_alt);
   x.kmer = nullptr;
+  x.occn = x.sbc = nullptr;
   if (x.n + 1 < (1LL << 32)) {  // 32-bit interval fields
+    std::vector<uint32_t> nl, sb;
+    if (!build_occ_narrow(h, nl, sb)) return fail(ctx, G3A_E_INTERNAL, "narrow Occ layout");
+    if (ensure(ctx, ctx->occn, nl.size() * 4) || ensure(ctx, ctx->sbc, sb.size() * 4)) return G3A_E_HIP;
+    HIPCK(hipMemcpy(ctx->occn.p, nl.data(), nl.size() * 4, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->sbc.p, sb.data(), sb.size() * 4, hipMemcpyHostToDevice));
+    x.occn = P<uint32_t>(ctx->occn);
+    x.sbc = P<uint32_t>(ctx->sbc);
     if (ensure(ctx, ctx->kmer, ((size_t)3 << (2 * kKmer)) * 4)) return G3A_E_HIP;
     launch_kmer_table(x, P<uint32_t>(ctx->kmer), ctx->stream);
     HIPCK(hipStreamSynchronize(ctx->stream));
@@ -497,6 +511,14 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.slow = P<int32_t>(ctx->slow);
   sa.fm_steps = cnt + 0;
   sa.lep_pool = P<Intv>(ctx->lep_pool);
+  sa.order = nullptr;
+  if (ctx->seed_ordered && n > 1) {
+    launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(ctx->seed_key), P<int32_t>(ctx->seed_idx), s);
+    if (seed_sort(P<uint32_t>(ctx->seed_key), P<uint32_t>(ctx->seed_key2), P<int32_t>(ctx->seed_idx),
+                  P<int32_t>(ctx->seed_order), n, ctx->seed_sort_tmp.p, ctx->seed_sort_tmp.bytes, s))
+      return fail(ctx, G3A_E_HIP, "seed order sort failed");
+    sa.order = P<int32_t>(ctx->seed_order);
+  }
   for (int &u : ctx->kev_used) u = 0;
   HIPCK(hipEventRecord(ctx->kev[0], s));
   const int seed_run = ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * 128 * ctx->n_cu < ctx->seed_threads
@@ -806,22 +828,34 @@ int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu) {
   return G3A_OK;
 }
 
-int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, float *best_ms) {
+int g3a_set_seed_order(g3a_ctx *ctx, int ordered) {
+  if (!ctx || (ordered != 0 && ordered != 1)) return G3A_E_ARG;
+  ctx->seed_ordered = ordered;
+  return G3A_OK;
+}
+
+int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms) {
   if (!ctx) return G3A_E_ARG;
   if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
   const int opts[4] = {4, 6, 8, 0};
   float best = -1;
-  int bb = ctx->seed_blocks_per_cu;
-  for (int b : opts) {
-    ctx->seed_blocks_per_cu = b;
-    if (int rc = g3a_bwa_align(ctx)) return rc;
-    if (best < 0 || ctx->kms[0] < best) {
-      best = ctx->kms[0];
-      bb = b;
+  int bb = ctx->seed_blocks_per_cu, bo = ctx->seed_ordered;
+  for (int o = 0; o < 2; ++o)
+    for (int b : opts) {
+      ctx->seed_blocks_per_cu = b;
+      ctx->seed_ordered = o;
+      if (int rc = g3a_bwa_align(ctx)) return rc;
+      // the seeding stage: the order's key and sort kernels and k_seed
+      if (best < 0 || ctx->ms[0] < best) {
+        best = ctx->ms[0];
+        bb = b;
+        bo = o;
+      }
     }
-  }
   ctx->seed_blocks_per_cu = bb;
+  ctx->seed_ordered = bo;
   if (blocks_per_cu) *blocks_per_cu = bb;
+  if (ordered) *ordered = bo;
   if (best_ms) *best_ms = best;
   return G3A_OK;
 }
@@ -870,14 +904,15 @@ void g3a_destroy(g3a_ctx *ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  DevBuf *bufs[] = {&ctx->occ, &ctx->kmer, &ctx->pac, &ctx->sa, &ctx->ctg_off, &ctx->ctg_len, &ctx->ctg_alt, &ctx->seq, &ctx->off,
+  DevBuf *bufs[] = {&ctx->occ, &ctx->kmer, &ctx->occn, &ctx->sbc, &ctx->pac, &ctx->sa, &ctx->ctg_off, &ctx->ctg_len, &ctx->ctg_alt, &ctx->seq, &ctx->off,
                     &ctx->intv, &ctx->n_intv, &ctx->n_seed, &ctx->slow, &ctx->intv_base, &ctx->seed_off,
                     &ctx->counters, &ctx->slow_list, &ctx->slow_scratch, &ctx->slow_n, &ctx->slow_off, &ctx->seeds,
                     &ctx->seeds2, &ctx->ch, &ctx->nxt, &ctx->order, &ctx->tmpi, &ctx->w16, &ctx->kept, &ctx->nodes,
                     &ctx->tasks_tmp, &ctx->n_task, &ctx->task_off, &ctx->tasks, &ctx->regs, &ctx->flag, &ctx->sec,
                     &ctx->n_aln, &ctx->aln_off, &ctx->aln_reg, &ctx->aln_read, &ctx->aln_rank, &ctx->z_size,
                     &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp,
-                    &ctx->cig_words, &ctx->cig_coff, &ctx->cig_out,
+                    &ctx->cig_words, &ctx->cig_coff, &ctx->cig_out, &ctx->seed_key, &ctx->seed_key2, &ctx->seed_idx,
+                    &ctx->seed_order, &ctx->seed_sort_tmp,
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
                     &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
                     &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr};

@@ -56,6 +56,11 @@ struct DevIndex {
   // bidirectional interval (k, l, s) of every kKmer-mer, 2-bit code of the
   // first base highest; null when the text has 2^32 rows or more
   const uint32_t *kmer;
+  // texts of fewer than 2^32 rows: 16-byte lines of 32 rows (21-bit counts
+  // of bases 0-2 relative to a 2^21-row superblock, then 32 symbols) and the
+  // superblocks' four absolute counts (uint32 each); null otherwise
+  const uint32_t *occn;
+  const uint32_t *sbc;
 };
 
 constexpr int kKmer = 12;
@@ -145,16 +150,55 @@ G3A_HD void occ_at(const DevIndex &x, const OccLine &L, int64_t p, int64_t o[4])
   o[0] -= dollar;
 }
 
-// The same for a text of fewer than 2^32 rows, in 32-bit arithmetic: the
-// only superblock is the first, whose counts are 0.
-G3A_HD void occ_at(const DevIndex &x, const OccLine &L, uint32_t p, uint32_t o[4]) {
-  uint32_t in[4];
-  line_counts(L.w0, L.w1, (int)(p & 63), in);
-  const uint32_t pr = (uint32_t)x.primary;
-  const uint32_t dollar = (p >> 6) == (pr >> 6) && p >= pr;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) o[c] = L.c[c] + in[c];
-  o[0] -= dollar;
+// Narrow form (texts of fewer than 2^32 rows, 32-bit arithmetic): one
+// 16-byte line of 32 rows, plus the line's superblock counts.
+struct OccLineN {
+  uint64_t cnt, w;
+};
+G3A_HD OccLineN occn_line(const DevIndex &x, uint32_t line) {
+  OccLineN L;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint4 v = *(const uint4 *)(x.occn + ((uint64_t)line << 2));
+  L.cnt = (uint64_t)v.x | (uint64_t)v.y << 32;
+  L.w = (uint64_t)v.z | (uint64_t)v.w << 32;
+#else
+  const uint32_t *b = x.occn + ((uint64_t)line << 2);
+  L.cnt = (uint64_t)b[0] | (uint64_t)b[1] << 32;
+  L.w = (uint64_t)b[2] | (uint64_t)b[3] << 32;
+#endif
+  return L;
+}
+// Occ(p, c) for all c from p's narrow line
+G3A_HD void occ_at(const DevIndex &x, const OccLineN &L, uint32_t p, uint32_t o[4]) {
+  const int y = (int)(p & 31);
+  const uint64_t m = (~0ULL << (62 - 2 * y)) & 0x5555555555555555ULL;
+  const uint64_t hi = (L.w >> 1) & m, lo = L.w & m;
+  const uint32_t n3 = (uint32_t)G3A_POPC(hi & lo), n2 = (uint32_t)G3A_POPC(hi & ~lo),
+                 n1 = (uint32_t)G3A_POPC(lo & ~hi);
+  uint32_t s0, s1, s2, s3;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint4 sb = *(const uint4 *)(x.sbc + ((p >> 21) << 2));
+  s0 = sb.x;
+  s1 = sb.y;
+  s2 = sb.z;
+  s3 = sb.w;
+#else
+  const uint32_t *sb = x.sbc + ((p >> 21) << 2);
+  s0 = sb[0];
+  s1 = sb[1];
+  s2 = sb[2];
+  s3 = sb[3];
+#endif
+  const uint32_t r0 = (uint32_t)(L.cnt & 0x1fffff), r1 = (uint32_t)(L.cnt >> 21) & 0x1fffff,
+                 r2 = (uint32_t)(L.cnt >> 42) & 0x1fffff;
+  const uint32_t pr = (uint32_t)x.primary, ls = p & ~31u, ss = p & ~((1u << 21) - 1);
+  // the '$' row is stored as symbol 0 and counted by no base
+  const uint32_t dol_before = pr >= ss && pr < ls, dol_in = pr >= ls && pr <= p;
+  const uint32_t r3 = (ls - ss) - r0 - r1 - r2 - dol_before;
+  o[0] = s0 + r0 + ((uint32_t)y + 1 - n1 - n2 - n3) - dol_in;
+  o[1] = s1 + r1 + n1;
+  o[2] = s2 + r2 + n2;
+  o[3] = s3 + r3 + n3;
 }
 
 // Occ(p, c): occurrences of c in rows [0, p], '$' excluded
@@ -162,11 +206,16 @@ G3A_HD void occ4(const DevIndex &x, int64_t p, int64_t o[4]) { occ_at(x, occ_lin
 
 // Occ at two rows; one line fetch when both lie in the same line (the
 // common case once an interval has narrowed to a few rows)
-template <class P>
-G3A_HD void occ4_pair(const DevIndex &x, P p1, P p2, P o1[4], P o2[4]) {
-  OccLine L = occ_line(x, (int64_t)(p1 >> 6));
+G3A_HD void occ4_pair(const DevIndex &x, int64_t p1, int64_t p2, int64_t o1[4], int64_t o2[4]) {
+  OccLine L = occ_line(x, p1 >> 6);
   occ_at(x, L, p1, o1);
-  if ((p2 >> 6) != (p1 >> 6)) L = occ_line(x, (int64_t)(p2 >> 6));
+  if ((p2 >> 6) != (p1 >> 6)) L = occ_line(x, p2 >> 6);
+  occ_at(x, L, p2, o2);
+}
+G3A_HD void occ4_pair(const DevIndex &x, uint32_t p1, uint32_t p2, uint32_t o1[4], uint32_t o2[4]) {
+  OccLineN L = occn_line(x, p1 >> 5);
+  occ_at(x, L, p1, o1);
+  if ((p2 >> 5) != (p1 >> 5)) L = occn_line(x, p2 >> 5);
   occ_at(x, L, p2, o2);
 }
 
@@ -385,12 +434,14 @@ struct IntvSink {
 // Each names the interval type its searches compute in (I).
 struct LepWide {
   using I = Intv;
+  static constexpr bool kNarrow = false;
   Intv v;
   static G3A_HD Intv get(const LepWide *p, int i) { return p[i].v; }
   static G3A_HD void put(LepWide *p, int i, const Intv &x) { p[i].v = x; }
 };
 struct alignas(16) LepNarrow {
   using I = IntvN;
+  static constexpr bool kNarrow = true;
   uint32_t k, l, s;
   int16_t m, n;
   static G3A_HD IntvN get(const LepNarrow *p, int i) {

@@ -26,6 +26,7 @@ struct SeedArgs {
   int32_t *slow;
   int64_t *fm_steps;
   Intv *lep_pool;    // kLepCap + kFoundCap entries per k_seed thread
+  const int32_t *order;  // reads in processing order, or null for input order
 };
 
 struct SaArgs {
@@ -192,6 +193,10 @@ void launch_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_
 void scan_i32(const int32_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t s);
 void scan_i64(const int64_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_bytes, hipStream_t s);
 size_t scan_tmp_bytes(int64_t n);
+void launch_seed_keys(const uint8_t *seq, const int32_t *off, int n, uint32_t *key, int32_t *idx, hipStream_t s);
+size_t seed_sort_tmp_bytes(int64_t n);
+int seed_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,
+              size_t tmp_bytes, hipStream_t s);
 void launch_cig_words(const Aln *alns, int64_t n, int32_t *words, hipStream_t s);
 void launch_cig_gather(Aln *alns, int64_t n, const int64_t *coff, const uint32_t *cig, uint32_t *out, hipStream_t s);
 

@@ -31,14 +31,25 @@ namespace g3a {
 // lists live in a global pool slice of its own, contiguous per thread (a
 // divergent list access then touches one cache line; private memory would
 // scatter every entry over eight lines of the per-lane swizzle).
+// In the narrow form the superblock counts (16 B per 2^21 rows, at most
+// 32 KB) are copied to LDS, so an Occ lookup costs one 16-byte global load.
 template <class E>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
+  extern __shared__ uint4 sb_lds[];
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
+  if (E::kNarrow) {
+    const int n_sb = (int)(((uint64_t)a.idx.n + 1) >> 21) + 1;
+    const uint4 *src = (const uint4 *)a.idx.sbc;
+    for (int k = threadIdx.x; k < n_sb; k += blockDim.x) sb_lds[k] = src[k];
+    __syncthreads();
+    a.idx.sbc = (const uint32_t *)sb_lds;
+  }
   E *la = reinterpret_cast<E *>(a.lep_pool) + (size_t)gtid * (kLepCap + kFoundCap), *lf = la + kLepCap;
   LepBuf<E> buf{la, lf, kLepCap, kFoundCap};
   int64_t steps = 0;
-  for (int r = gtid; r < a.n_reads; r += nthreads) {
+  for (int i = gtid; i < a.n_reads; i += nthreads) {
+    const int r = a.order ? a.order[i] : i;
     const int off = a.off[r], len = a.off[r + 1] - off;
     const uint8_t *q = a.seq + off;
     IntvSink sink{a.intv + (int64_t)r * a.slot_cap, a.slot_cap, 0, 0, a.opt.max_occ};
@@ -69,6 +80,44 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     for (int o = 32; o >= 1; o >>= 1) steps += __shfl_xor(steps, o);
     if ((threadIdx.x & 63) == 0 && steps) atomicAdd((unsigned long long *)a.fm_steps, (unsigned long long)steps);
   }
+}
+
+// Seeding order.  Overlapping reads of one locus walk the FM-index through
+// the same intervals; given neighbouring lanes, their Occ lines are still in
+// L2 when the second read needs them.  The key of a read is the smallest
+// hash over its canonical 16-mers (both strands give the same key); sorting
+// by it brings reads that share that 16-mer together.
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+__global__ void __launch_bounds__(256) k_seed_keys(const uint8_t *seq, const int32_t *off, int n, uint32_t *key,
+                                                   int32_t *idx) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int o = off[r], len = off[r + 1] - o;
+  ByteReader q(seq + o);
+  uint32_t fw = 0, rv = 0, best = 0xffffffffu;
+  int run = 0;
+  for (int i = 0; i < len; ++i) {
+    const uint32_t c = (uint32_t)q(i);
+    if (c > 3) {
+      run = 0;
+      continue;
+    }
+    fw = fw << 2 | c;
+    rv = rv >> 2 | (3u - c) << 30;
+    if (++run >= 16) {
+      const uint32_t h = mix32(fw < rv ? fw : rv);
+      best = h < best ? h : best;
+    }
+  }
+  key[r] = best >> 8;  // 24-bit key, sorted on bits [0, 24)
+  idx[r] = r;
 }
 
 // Slow path for the reads listed in `list`: global-memory LEP lists of
@@ -769,10 +818,14 @@ static inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1
 
 // 16-byte list entries when every SA row fits 32 bits
 static bool seed_narrow(const DevIndex &x) { return x.n + 1 < (1LL << 32); }
+static size_t seed_lds_bytes(const DevIndex &x) {
+  return seed_narrow(x) ? (size_t)((((uint64_t)x.n + 1) >> 21) + 1) * 16 : 0;
+}
 int seed_grid_threads(int n_cu, const DevIndex &x) {
   int per_cu = 0;
-  hipError_t e = seed_narrow(x) ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepNarrow>, 128, 0)
-                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepWide>, 128, 0);
+  hipError_t e = seed_narrow(x)
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepNarrow>, 128, seed_lds_bytes(x))
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepWide>, 128, 0);
   if (e != hipSuccess || per_cu < 1) per_cu = 8;
   return n_cu * per_cu * 128;
 }
@@ -780,7 +833,7 @@ void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s) {
   if (!a.n_reads) return;
   const int need = blocks(a.n_reads, 128), have = n_threads / 128;
   const dim3 grid(need < have ? need : have);
-  if (seed_narrow(a.idx)) hipLaunchKernelGGL(k_seed<LepNarrow>, grid, dim3(128), 0, s, a);
+  if (seed_narrow(a.idx)) hipLaunchKernelGGL(k_seed<LepNarrow>, grid, dim3(128), seed_lds_bytes(a.idx), s, a);
   else hipLaunchKernelGGL(k_seed<LepWide>, grid, dim3(128), 0, s, a);
 }
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
@@ -922,6 +975,28 @@ void launch_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_
 struct WidenI32 {
   __host__ __device__ int64_t operator()(int32_t v) const { return (int64_t)v; }
 };
+
+void launch_seed_keys(const uint8_t *seq, const int32_t *off, int n, uint32_t *key, int32_t *idx, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_seed_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, seq, off, n, key, idx);
+}
+// Keys are the top 24 bits of the hash: finer than needed to keep each
+// 16-mer's reads together, and one radix pass cheaper than 32.  They are
+// stored shifted down and sorted on bits [0, 24): with a begin bit above 0,
+// rocPRIM 4.2's merge-sort path (the one it takes for mid-sized inputs) was
+// measured to return duplicated values.
+size_t seed_sort_tmp_bytes(int64_t n) {
+  size_t b = 0;
+  rocprim::radix_sort_pairs(nullptr, b, (const uint32_t *)nullptr, (uint32_t *)nullptr, (const int32_t *)nullptr,
+                            (int32_t *)nullptr, (size_t)(n > 0 ? n : 1), 0, 24);
+  return b + 256;
+}
+int seed_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,
+              size_t tmp_bytes, hipStream_t s) {
+  if (n <= 0) return 0;
+  return rocprim::radix_sort_pairs(tmp, tmp_bytes, key_in, key_out, val_in, val_out, (size_t)n, 0, 24, s) == hipSuccess
+             ? 0
+             : -1;
+}
 
 void launch_cig_words(const Aln *alns, int64_t n, int32_t *words, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_cig_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, alns, n, words);

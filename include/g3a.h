@@ -141,12 +141,18 @@ int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *
  * keeps its LEP lists in a global pool slice, so fewer threads also shrink
  * the working set the caches see.  No reference counterpart. */
 int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu);
-/* Seeding autotuner: re-aligns the batch last aligned once per seeding grid
- * (blocks per compute unit 4, 6, 8 and the occupancy maximum), keeps the one
- * with the fastest seeding kernel and reports it (0 = maximum) with its
- * time.  The batch's output is identical for every grid.  No reference
+/* Seeding order: 1 seeds the reads in the order of a minimizer
+ * key (the smallest hash over each read's canonical 16-mers), so that
+ * overlapping reads run in neighbouring lanes and share cached Occ lines;
+ * 0 (default) seeds them in input order.  Results are identical.  No reference
  * counterpart. */
-int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, float *best_ms);
+int g3a_set_seed_order(g3a_ctx *ctx, int ordered);
+/* Seeding autotuner: re-aligns the batch last aligned once per seeding grid
+ * (blocks per compute unit 4, 6, 8 and the occupancy maximum) and order
+ * (input, minimizer), keeps the pair with the fastest seeding stage and
+ * reports it (0 = maximum) with its time.  The batch's output is identical
+ * for every setting.  No reference counterpart. */
+int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms);
 const char *g3a_last_error(const g3a_ctx *ctx);
 
 /* Index construction (the `g3a index` command): BWA-format files plus the

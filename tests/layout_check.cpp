@@ -1,0 +1,61 @@
+// Occ over every BWT row from the device layouts (the 32-byte wide lines and
+// the 16-byte narrow lines with their superblock counts), through the same
+// lookup code the kernels use (compiled for the host), against a running
+// count of the BWT symbols.  Prints "ok <rows> <superblocks>" or the first
+// mismatch.  Built and run by tests/test_layout.py.
+#include "g3sa_amd/csrc/g3a_dev.h"
+#include "g3sa_amd/csrc/g3a_index.h"
+#include "g3sa_amd/csrc/g3a_layout.h"
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace g3a;
+
+int main(int argc, char **argv) {
+  if (argc < 2) return 2;
+  HostIndex h;
+  std::string err;
+  if (!load_index(argv[1], h, err)) {
+    std::printf("load failed: %s\n", err.c_str());
+    return 1;
+  }
+  DevIndex x;
+  std::memset(&x, 0, sizeof x);
+  std::vector<uint32_t> wide, narrow, sb;
+  if (!build_occ_lines(h, wide, x.super) || !build_occ_narrow(h, narrow, sb)) {
+    std::printf("layout failed\n");
+    return 1;
+  }
+  x.occ = wide.data();
+  x.occn = narrow.data();
+  x.sbc = sb.data();
+  x.primary = (int64_t)h.primary;
+  x.n = (int64_t)h.seq_len;
+  uint64_t cnt[4] = {0, 0, 0, 0};
+  const uint64_t rows = h.seq_len + 1;
+  for (uint64_t p = 0; p < rows; ++p) {
+    const uint8_t ch = h.bwt[(size_t)p];
+    if (ch < 4) ++cnt[ch];
+    int64_t ow[4];
+    uint32_t on[4];
+    occ4(x, (int64_t)p, ow);
+    OccLineN L = occn_line(x, (uint32_t)(p >> 5));
+    occ_at(x, L, (uint32_t)p, on);
+    for (int c = 0; c < 4; ++c)
+      if ((uint64_t)ow[c] != cnt[c] || on[c] != cnt[c]) {
+        std::printf("mismatch row %llu base %d: true %llu wide %lld narrow %u\n", (unsigned long long)p, c,
+                    (unsigned long long)cnt[c], (long long)ow[c], on[c]);
+        return 1;
+      }
+    const int bc = bwt_char(x, (int64_t)p);
+    if (bc != (p == h.primary ? 4 : ch)) {
+      std::printf("bwt_char mismatch row %llu\n", (unsigned long long)p);
+      return 1;
+    }
+  }
+  std::printf("ok %llu %zu\n", (unsigned long long)rows, sb.size() / 4);
+  return 0;
+}

@@ -40,8 +40,9 @@ def compare(gpu: g3sa_amd.Result, ref: dict, label: str) -> None:
         assert np.array_equal(gk, rk), f"{label}: global CIGAR of alignment {i}"
 
 
-# (form, tuning): the plain form and the batched form at both strip heights
-VARIANTS = [("plain", None), ("batched", (16, 8, 8)), ("batched", (32, 4, 8))]
+# (form, tuning, seeding order): the plain form and the batched form at both
+# strip heights; reads seeded in minimizer order and in input order
+VARIANTS = [("plain", None, 1), ("batched", (16, 8, 8), 1), ("batched", (32, 4, 8), 0)]
 
 
 def run_both(ds, oracle, seq, off, label, max_reads=None):
@@ -53,13 +54,14 @@ def run_both(ds, oracle, seq, off, label, max_reads=None):
         oracle.lib.orc_index_free(idx)
     with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(max_reads or n, 1),
                           max_bases=max(int(off[-1]), 1)) as al:
-        for form, tuning in VARIANTS:
-            print(f"[parity] {label}: {n} reads, form {form} {tuning}", flush=True)
+        for form, tuning, ordered in VARIANTS:
+            print(f"[parity] {label}: {n} reads, form {form} {tuning} seed order {ordered}", flush=True)
             al.set_form(form)
             if tuning:
                 al.set_tuning(*tuning)
+            al.set_seed_order(ordered)
             got = al.align(seq, off)
-            compare(got, ref, f"{label} [{form} {tuning}]")
+            compare(got, ref, f"{label} [{form} {tuning} order {ordered}]")
             # seeds resolved by the GPU equal the oracle's SA lookups
             assert got.work["n_seeds"] == ref["work"]["sa_reads"], label
     return got, ref
@@ -152,12 +154,13 @@ def test_scoring_variants_parity(ds_c1, oracle, w, b):
     finally:
         oracle.lib.orc_index_free(idx)
     with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=len(off) - 1, max_bases=int(off[-1]), opt=go) as al:
-        for form, tuning in VARIANTS:
-            print(f"[parity] scoring w={w} b={b}: form {form} {tuning}", flush=True)
+        for form, tuning, ordered in VARIANTS:
+            print(f"[parity] scoring w={w} b={b}: form {form} {tuning} seed order {ordered}", flush=True)
             al.set_form(form)
             if tuning:
                 al.set_tuning(*tuning)
-            compare(al.align(seq, off), ref, f"scoring w={w} b={b} [{form} {tuning}]")
+            al.set_seed_order(ordered)
+            compare(al.align(seq, off), ref, f"scoring w={w} b={b} [{form} {tuning} order {ordered}]")
 
 
 def test_empty_batch(ds_c1):
