@@ -489,6 +489,29 @@ __device__ __forceinline__ int wave_min_i32(int v) {
 
 // Slot of this lane in a wave-aggregated append (one atomic per wavefront).
 // Every lane of the wavefront must call it.
+// The same for a 256-thread block: one atomic per block (atomics on one
+// counter from every wavefront of a large grid serialise at the L2).  Every
+// thread of the block must call it; `sh` is 5 ints of LDS.
+__device__ __forceinline__ int block_reserve(bool pred, int32_t *ctr, int32_t *sh) {
+  const uint64_t m = __ballot(pred);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < 4; ++w) {
+      const int c = sh[w];
+      sh[w] = tot;
+      tot += c;
+    }
+    sh[4] = tot ? atomicAdd(ctr, tot) : 0;
+  }
+  __syncthreads();
+  const int r = pred ? sh[4] + sh[wave] + __popcll(m & ((1ull << lane) - 1)) : -1;
+  __syncthreads();  // sh is free for the next call
+  return r;
+}
+
 __device__ __forceinline__ int wave_reserve(bool pred, int32_t *ctr) {
   const uint64_t m = __ballot(pred);
   if (!m) return -1;
@@ -551,8 +574,9 @@ __global__ void __launch_bounds__(256) k_ext_classify(ExtSideArgs a) {
     a.key[t] = lane ? (uint32_t)st.qlen : 0u;
     a.tid[t] = (int32_t)t;
   }
-  wave_reserve(lane, a.ctr);
-  const int slot = wave_reserve(slow, a.ctr + 1);
+  __shared__ int32_t sh[5];
+  block_reserve(lane, a.ctr, sh);
+  const int slot = block_reserve(slow, a.ctr + 1, sh);
   if (slow) a.slow_list[slot] = (int32_t)t;
 }
 
@@ -742,7 +766,8 @@ __device__ __forceinline__ int tb_class(const TbArgs &a, int64_t t) {
 __global__ void __launch_bounds__(256) k_tb_count(TbArgs a) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int c = t < a.n_alns ? tb_class(a, t) : 3;
-  for (int k = 0; k < 3; ++k) wave_reserve(c == k, a.tb_ctr + k);
+  __shared__ int32_t sh[5];
+  for (int k = 0; k < 3; ++k) block_reserve(c == k, a.tb_ctr + k, sh);
 }
 // The list holds classes 0, 1, 2 in turn from the front and class 3 from the
 // back; tb_ctr[4..7] are the placement cursors.
@@ -752,8 +777,9 @@ __global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
   const int c = in ? tb_class(a, t) : -1;
   const int64_t base1 = a.tb_ctr[0], base2 = base1 + a.tb_ctr[1];
   int at = -1;
+  __shared__ int32_t sh[5];
   for (int k = 0; k < 4; ++k) {
-    const int r = wave_reserve(c == k, a.tb_ctr + 4 + k);
+    const int r = block_reserve(c == k, a.tb_ctr + 4 + k, sh);
     if (c == k) at = r;
   }
   if (in) {
