@@ -524,7 +524,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   const int seed_run = ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * 128 * ctx->n_cu < ctx->seed_threads
                            ? ctx->seed_blocks_per_cu * 128 * ctx->n_cu
                            : ctx->seed_threads;
-  launch_seed(sa, seed_run, s);
+  launch_seed(sa, seed_run, ctx->seed_blocks_per_cu, s);
   HIPCK(hipEventRecord(ctx->kev[1], s));
   ctx->kev_used[0] = 1;
   launch_collect_slow(n, P<int32_t>(ctx->slow), P<int32_t>(ctx->slow_list), P<int32_t>(ctx->counters) + 2 * 8,

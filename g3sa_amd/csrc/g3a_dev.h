@@ -466,11 +466,56 @@ struct alignas(16) LepNarrow {
 };
 
 // Scratch for the LEP list and the found list.
+// A list in a global slice of entries of type E.
 template <class E>
-struct LepBuf {
-  E *a, *f;
-  int cap, cap_f;  // capacities of the LEP list a and the found list f
+struct GlobalList {
+  using I = typename E::I;
+  E *p;
+  G3A_HD I get(int i) const { return E::get(p, i); }
+  G3A_HD void put(int i, const I &v) const { E::put(p, i, v); }
 };
+
+// A narrow list whose first `capl` entries live in LDS and the rest in a
+// global slice.  In LDS an entry is three words, k, s and m | n << 16, the
+// words of entry i at (3i + c) * stride from the lane's first word (lanes
+// interleaved).  l is not kept: these lists only hold intervals that are
+// extended backward or output, and neither reads l.
+struct SplitList {
+  using I = IntvN;
+  uint32_t *lds;
+  int stride, capl;
+  LepNarrow *g;
+  G3A_HD I get(int i) const {
+    if (i >= capl) return LepNarrow::get(g, i - capl);
+    const uint32_t *w = lds + 3 * i * stride;
+    I r;
+    r.k = w[0];
+    r.l = 0;
+    r.s = w[stride];
+    const uint32_t mn = w[2 * stride];
+    r.m = (int32_t)(mn & 0xffff);
+    r.n = (int32_t)(mn >> 16);
+    return r;
+  }
+  G3A_HD void put(int i, const I &v) const {
+    if (i >= capl) return LepNarrow::put(g, i - capl, v);
+    uint32_t *w = lds + 3 * i * stride;
+    w[0] = v.k;
+    w[stride] = v.s;
+    w[2 * stride] = (uint32_t)(v.m & 0xffff) | (uint32_t)v.n << 16;
+  }
+};
+
+// The LEP list a and the found list f of one thread, with their capacities.
+template <class LA, class LF>
+struct LepBuf {
+  using I = typename LA::I;
+  LA a;
+  LF f;
+  int cap, cap_f;
+};
+template <class E>
+using GlobalBuf = LepBuf<GlobalList<E>, GlobalList<E>>;
 
 // Round 1 (preseedAndFilterV2, bwa/src/seed.cu:25-104 with
 // preseedOnePos2Backward, seed.cu:454-531).  The reference keeps, for each
@@ -481,17 +526,17 @@ struct LepBuf {
 // then backward), which visits each read position a few times instead of
 // once per end position.  Output order is by end position, as the
 // reference's.  Returns false when a LEP list outgrows `buf`.
-template <class E>
-G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
-                        LepBuf<E> &buf, int64_t *steps) {
-  using I = typename E::I;
+template <class B>
+G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out, B &buf,
+                        int64_t *steps) {
+  using I = typename B::I;
   using P = decltype(I::s);
   int x0 = 0;
   while (x0 < len) {
     if (q[x0] > 3) { ++x0; continue; }
     // the backward pass rebuilds the list in place: entry nc is written only
     // after entry j >= nc has been read
-    E *prev = buf.a, *curr = buf.a;
+    const auto &prev = buf.a, &curr = buf.a;
     int np = 0;
     I ik;
     intv_init(x, ik, q[x0]);
@@ -505,37 +550,37 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
         if (steps) ++*steps;
         if (ok.s != ik.s) {
           if (np == buf.cap) return false;
-          E::put(prev, np++, ik);
+          prev.put(np++, ik);
           if (ok.s < 1) break;
         }
         ik = ok;
         ik.n = i;
       } else {
         if (np == buf.cap) return false;
-        E::put(prev, np++, ik);
+        prev.put(np++, ik);
         break;
       }
     }
     if (i == len) {
       if (np == buf.cap) return false;
-      E::put(prev, np++, ik);
+      prev.put(np++, ik);
     }
     // longest first
     for (int a = 0, b = np - 1; a < b; ++a, --b) {
-      const I ta = E::get(prev, a), tb = E::get(prev, b);
-      E::put(prev, a, tb);
-      E::put(prev, b, ta);
+      const I ta = prev.get(a), tb = prev.get(b);
+      prev.put(a, tb);
+      prev.put(b, ta);
     }
-    const int ret = E::get(prev, 0).n + 1;
+    const int ret = prev.get(0).n + 1;
     // backward search; SMEMs are found in decreasing start order
     int nf = 0, last_start = len + 1;  // start of the last match found (none yet)
-    E *fl = buf.f;
+    const auto &fl = buf.f;
     P last_s = 0;  // size of the last entry written to curr (read only when nc > 0)
     for (i = x0 - 1; i >= -1; --i) {
       int c = i < 0 ? -1 : (q[i] < 4 ? q[i] : -1);
       int nc = 0;
       for (int j = 0; j < np; ++j) {
-        const I pj = E::get(prev, j);
+        const I pj = prev.get(j);
         I ok;
         bool ext_ok = false;
         if (c >= 0) {
@@ -551,14 +596,14 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
               I p = pj;
               p.m = i + 1;
               if (nf == buf.cap_f) return false;
-              E::put(fl, nf++, p);
+              fl.put(nf++, p);
             }
           }
         } else if (nc == 0 || ok.s != last_s) {
           ok.m = pj.m;
           ok.n = pj.n;
           if (nc == buf.cap) return false;
-          E::put(curr, nc++, ok);
+          curr.put(nc++, ok);
           last_s = ok.s;
         }
       }
@@ -566,7 +611,7 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
       np = nc;
     }
     // found in decreasing start (and end) order: emit them by end
-    for (int j = nf - 1; j >= 0; --j) out.push(E::get(fl, j));
+    for (int j = nf - 1; j >= 0; --j) out.push(fl.get(j));
     x0 = ret;
   }
   return true;
@@ -613,10 +658,10 @@ G3A_HD void smem_round1_per_end(const DevIndex &x, const Opt &o, const uint8_t *
 // Round 2, reseedV2 (bwa/src/seed.cu:639-913), for one SMEM.  `leps` and
 // `seeds` are scratch lists of capacities cap and cap_s.  Returns false on
 // overflow.
-template <class E>
+template <class LA, class LF>
 G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int len, const Intv &smem,
-                       E *leps, E *seeds, int cap, int cap_s, IntvSink &out, int64_t *steps) {
-  using I = typename E::I;
+                       const LA &leps, const LF &seeds, int cap, int cap_s, IntvSink &out, int64_t *steps) {
+  using I = typename LA::I;
   using P = decltype(I::s);
   const int split_len = (int)(o.min_seed_len * o.split_factor + .499);
   const int slen = smem.n - smem.m + 1;
@@ -639,19 +684,19 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     seed.n = n + 1;
     if (seed.s < old.s) {
       if (nl == cap) return false;
-      E::put(leps, nl++, old);
+      leps.put(nl++, old);
       if (seed.s < min_seed_intv) break;
     }
   }
   if (seed.s >= min_seed_intv) {
     seed.n = n;
     if (nl == cap) return false;
-    E::put(leps, nl++, seed);
+    leps.put(nl++, seed);
   }
   for (int a = 0, b = nl - 1; a < b; ++a, --b) {
-    const I ta = E::get(leps, a), tb = E::get(leps, b);
-    E::put(leps, a, tb);
-    E::put(leps, b, ta);
+    const I ta = leps.get(a), tb = leps.get(b);
+    leps.put(a, tb);
+    leps.put(b, ta);
   }
   for (int m = pivot; m > 0; --m) {
     int base = q[m - 1];
@@ -661,30 +706,30 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     bool collected = false;
     nl = 0;
     for (int j = 0; j < num_old; ++j) {
-      I lep = E::get(leps, j), ext;
+      I lep = leps.get(j), ext;
       backward_ext(x, lep, base, ext);
       if (steps) ++*steps;
       if (ext.s >= curr) {
         ext.m = m - 1;
         ext.n = lep.n;
-        E::put(leps, nl++, ext);
+        leps.put(nl++, ext);
         curr = ext.s;
       } else if (!collected && lep.n - lep.m + 1 >= o.min_seed_len) {
         if (ns == cap_s) return false;
-        E::put(seeds, ns++, lep);
+        seeds.put(ns++, lep);
         collected = true;
       }
     }
     if (nl == 0) break;
   }
   if (nl > 0) {
-    const I l0 = E::get(leps, 0);
+    const I l0 = leps.get(0);
     if (l0.n - l0.m + 1 >= o.min_seed_len) {
       if (ns == cap_s) return false;
-      E::put(seeds, ns++, l0);
+      seeds.put(ns++, l0);
     }
   }
-  for (int j = 0; j < ns; ++j) out.push(E::get(seeds, j));
+  for (int j = 0; j < ns; ++j) out.push(seeds.get(j));
   return true;
 }
 

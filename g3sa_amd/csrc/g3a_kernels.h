@@ -27,6 +27,7 @@ struct SeedArgs {
   int64_t *fm_steps;
   Intv *lep_pool;    // kLepCap + kFoundCap entries per k_seed thread
   const int32_t *order;  // reads in processing order, or null for input order
+  int lep_lds;           // LEP entries per thread kept in LDS (narrow form; set by launch_seed)
 };
 
 struct SaArgs {
@@ -157,7 +158,9 @@ struct TbArgs {
 
 // threads of k_seed's persistent grid (occupancy-limited, n_cu compute units)
 int seed_grid_threads(int n_cu, const DevIndex &x);
-void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s);
+// blocks_per_cu: the grid the caller runs (0: the occupancy maximum), which
+// sets the LDS share of each block for the LEP lists
+void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_t s);
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
                       Intv *pool, int pass, int from_scratch, hipStream_t s);
 void launch_sa_lookup(const SaArgs &a, hipStream_t s);

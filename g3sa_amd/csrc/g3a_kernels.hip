@@ -32,21 +32,14 @@ namespace g3a {
 // divergent list access then touches one cache line; private memory would
 // scatter every entry over eight lines of the per-lane swizzle).
 // In the narrow form the superblock counts (16 B per 2^21 rows, at most
-// 32 KB) are copied to LDS, so an Occ lookup costs one 16-byte global load.
-template <class E>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
-  extern __shared__ uint4 sb_lds[];
-  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nthreads = gridDim.x * blockDim.x;
-  if (E::kNarrow) {
-    const int n_sb = (int)(((uint64_t)a.idx.n + 1) >> 21) + 1;
-    const uint4 *src = (const uint4 *)a.idx.sbc;
-    for (int k = threadIdx.x; k < n_sb; k += blockDim.x) sb_lds[k] = src[k];
-    __syncthreads();
-    a.idx.sbc = (const uint32_t *)sb_lds;
-  }
-  E *la = reinterpret_cast<E *>(a.lep_pool) + (size_t)gtid * (kLepCap + kFoundCap), *lf = la + kLepCap;
-  LepBuf<E> buf{la, lf, kLepCap, kFoundCap};
+// 32 KB) are copied to LDS, so an Occ lookup costs one 16-byte global load,
+// and the first a.lep_lds entries of each thread's LEP list live in LDS
+// after them (12 B per entry, lanes interleaved), the rest in the thread's
+// global slice: the list traffic then stays out of the caches that the Occ
+// lines need.
+template <class B>
+__device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nthreads) {
+  using I = typename B::I;
   int64_t steps = 0;
   for (int i = gtid; i < a.n_reads; i += nthreads) {
     const int r = a.order ? a.order[i] : i;
@@ -65,10 +58,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
       if (sink.n <= sink.cap) {
         for (int k = 0; ok && k < nsm; ++k) {
           Intv smem = sink.slot[k];
-          ok = reseed_one(a.idx, a.opt, q, len, smem, la, lf, kLepCap, kFoundCap, sink, &steps);
+          ok = reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.f, buf.cap, buf.cap_f, sink, &steps);
         }
       } else ok = false;
-      if (ok) reseed_round3<decltype(E::I::s)>(a.idx, a.opt, q, len, sink, &steps);
+      if (ok) reseed_round3<decltype(I::s)>(a.idx, a.opt, q, len, sink, &steps);
     }
     ok = ok && sink.n <= sink.cap;
     a.slow[r] = ok ? 0 : 1;
@@ -79,6 +72,28 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) steps += __shfl_xor(steps, o);
     if ((threadIdx.x & 63) == 0 && steps) atomicAdd((unsigned long long *)a.fm_steps, (unsigned long long)steps);
+  }
+}
+
+template <class E>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
+  extern __shared__ uint4 sb_lds[];
+  const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nthreads = gridDim.x * blockDim.x;
+  E *la = reinterpret_cast<E *>(a.lep_pool) + (size_t)gtid * (kLepCap + kFoundCap), *lf = la + kLepCap;
+  if constexpr (E::kNarrow) {
+    const int n_sb = (int)(((uint64_t)a.idx.n + 1) >> 21) + 1;
+    const uint4 *src = (const uint4 *)a.idx.sbc;
+    for (int k = threadIdx.x; k < n_sb; k += blockDim.x) sb_lds[k] = src[k];
+    __syncthreads();
+    a.idx.sbc = (const uint32_t *)sb_lds;
+    LepBuf<SplitList, GlobalList<E>> buf{
+        SplitList{(uint32_t *)(sb_lds + n_sb) + threadIdx.x, (int)blockDim.x, a.lep_lds, la}, GlobalList<E>{lf},
+        kLepCap, kFoundCap};
+    seed_reads(a, buf, gtid, nthreads);
+  } else {
+    GlobalBuf<E> buf{GlobalList<E>{la}, GlobalList<E>{lf}, kLepCap, kFoundCap};
+    seed_reads(a, buf, gtid, nthreads);
   }
 }
 
@@ -143,7 +158,8 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
   Intv *slot = pass ? pool + pool_off[t] : sc + 3 * kSlowLep;
   const int cap = pass ? (int)(pool_off[t + 1] - pool_off[t]) : 2 * kSlowLep;
   IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};
-  LepBuf<LepWide> buf{reinterpret_cast<LepWide *>(sc), reinterpret_cast<LepWide *>(sc) + kSlowLep, kSlowLep, kSlowLep};
+  GlobalBuf<LepWide> buf{GlobalList<LepWide>{reinterpret_cast<LepWide *>(sc)},
+                         GlobalList<LepWide>{reinterpret_cast<LepWide *>(sc) + kSlowLep}, kSlowLep, kSlowLep};
   if (!smem_round1(a.idx, a.opt, q, len, sink, buf, nullptr)) {
     sink.n = 0;
     sink.n_seed = 0;
@@ -855,12 +871,23 @@ int seed_grid_threads(int n_cu, const DevIndex &x) {
   if (e != hipSuccess || per_cu < 1) per_cu = 8;
   return n_cu * per_cu * 128;
 }
-void launch_seed(const SeedArgs &a, int n_threads, hipStream_t s) {
+void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_t s) {
   if (!a.n_reads) return;
   const int need = blocks(a.n_reads, 128), have = n_threads / 128;
   const dim3 grid(need < have ? need : have);
-  if (seed_narrow(a.idx)) hipLaunchKernelGGL(k_seed<LepNarrow>, grid, dim3(128), seed_lds_bytes(a.idx), s, a);
-  else hipLaunchKernelGGL(k_seed<LepWide>, grid, dim3(128), 0, s, a);
+  if (seed_narrow(a.idx)) {
+    // LDS of a compute unit (160 KB) shared by the blocks it holds, at most
+    // 64 KB per block
+    const size_t per_block = (size_t)160 * 1024 / (size_t)(blocks_per_cu > 0 ? blocks_per_cu : 8);
+    const size_t lim = per_block < (size_t)64 * 1024 ? per_block : (size_t)64 * 1024, sb = seed_lds_bytes(a.idx);
+    int capl = lim > sb ? (int)((lim - sb) / (128 * 12)) : 0;
+    capl = capl < kLepCap ? capl : kLepCap;
+    SeedArgs b = a;
+    b.lep_lds = capl;
+    hipLaunchKernelGGL(k_seed<LepNarrow>, grid, dim3(128), sb + (size_t)capl * 128 * 12, s, b);
+  } else {
+    hipLaunchKernelGGL(k_seed<LepWide>, grid, dim3(128), 0, s, a);
+  }
 }
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
                       Intv *pool, int pass, int from_scratch, hipStream_t s) {
