@@ -837,7 +837,7 @@ int g3a_set_seed_order(g3a_ctx *ctx, int ordered) {
 int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms) {
   if (!ctx) return G3A_E_ARG;
   if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
-  const int opts[4] = {4, 6, 8, 0};
+  const int opts[4] = {6, 8, 10, 0};
   float best = -1;
   int bb = ctx->seed_blocks_per_cu, bo = ctx->seed_ordered;
   for (int o = 0; o < 2; ++o)

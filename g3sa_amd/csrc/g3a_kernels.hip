@@ -876,12 +876,26 @@ void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_
   const int need = blocks(a.n_reads, 128), have = n_threads / 128;
   const dim3 grid(need < have ? need : have);
   if (seed_narrow(a.idx)) {
-    // LDS of a compute unit (160 KB) shared by the blocks it holds, at most
-    // 64 KB per block
-    const size_t per_block = (size_t)160 * 1024 / (size_t)(blocks_per_cu > 0 ? blocks_per_cu : 8);
-    const size_t lim = per_block < (size_t)64 * 1024 ? per_block : (size_t)64 * 1024, sb = seed_lds_bytes(a.idx);
-    int capl = lim > sb ? (int)((lim - sb) / (128 * 12)) : 0;
+    // The compute unit's LDS (160 KB, at most 64 KB per block) is shared by
+    // the blocks it is to hold; the LEP entries per thread are the most that
+    // still lets that many blocks be resident (a block that is not resident
+    // would only start when another finishes, doubling the kernel's time).
+    const size_t sb = seed_lds_bytes(a.idx);
+    int want = blocks_per_cu;
+    if (want <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&want, k_seed<LepNarrow>, 128, sb) != hipSuccess ||
+                      want < 1))
+      want = 8;
+    size_t per_block = (size_t)160 * 1024 / (size_t)want - 1024;  // slack for allocation granules
+    per_block = per_block < (size_t)64 * 1024 ? per_block : (size_t)64 * 1024;
+    int capl = per_block > sb ? (int)((per_block - sb) / (128 * 12)) : 0;
     capl = capl < kLepCap ? capl : kLepCap;
+    int resident = 0;
+    for (; capl > 0; --capl) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_seed<LepNarrow>, 128,
+                                                       sb + (size_t)capl * 128 * 12) != hipSuccess)
+        resident = 0;
+      if (resident >= want) break;
+    }
     SeedArgs b = a;
     b.lep_lds = capl;
     hipLaunchKernelGGL(k_seed<LepNarrow>, grid, dim3(128), sb + (size_t)capl * 128 * 12, s, b);

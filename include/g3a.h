@@ -137,9 +137,11 @@ int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wav
  * counterpart. */
 int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *best_ms);
 /* Persistent seeding blocks (128 threads) per compute unit; 0 restores the
- * default, the most the kernel's occupancy allows.  Each resident thread
- * keeps its LEP lists in a global pool slice, so fewer threads also shrink
- * the working set the caches see.  No reference counterpart. */
+ * default, the occupancy maximum.  Each
+ * block's share of the compute unit's LDS holds the first entries of its
+ * threads' LEP lists (the rest spill to a global slice per thread), so the
+ * grid trades latency hiding against list traffic.  No reference
+ * counterpart. */
 int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu);
 /* Seeding order: 1 seeds the reads in the order of a minimizer
  * key (the smallest hash over each read's canonical 16-mers), so that
@@ -148,10 +150,11 @@ int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu);
  * counterpart. */
 int g3a_set_seed_order(g3a_ctx *ctx, int ordered);
 /* Seeding autotuner: re-aligns the batch last aligned once per seeding grid
- * (blocks per compute unit 4, 6, 8 and the occupancy maximum) and order
- * (input, minimizer), keeps the pair with the fastest seeding stage and
- * reports it (0 = maximum) with its time.  The batch's output is identical
- * for every setting.  No reference counterpart. */
+ * (blocks per compute unit 6, 8, 10 and the occupancy maximum; each block's
+ * LDS share holds the first entries of its threads' LEP lists, so fewer
+ * blocks keep more in LDS) and order (input, minimizer), keeps the pair with
+ * the fastest seeding stage and reports it with its time.  The batch's
+ * output is identical for every setting.  No reference counterpart. */
 int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms);
 const char *g3a_last_error(const g3a_ctx *ctx);
 
