@@ -190,16 +190,20 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
         hd0 = v > 0 ? v : 0;
         Et = 0;
       }
-      // M of every row from the previous column's H (bottom-up, in place)
+      // M of every row from the previous column's H (bottom-up, in place).
+      // M = 0 when the diagonal H is 0; otherwise H + score.  Computed as
+      // min(hd + score, 8 hd): for hd >= 1 the bound 8 hd >= hd + 7 never
+      // binds, for hd = 0 it gives a value <= 0.  A negative M acts as 0:
+      // H = max(M, E, F) and E, F >= 0, and M - gap < 0 in either case.
 #pragma unroll
       for (int r = R - 1; r >= 1; --r) {
         const int hd = hc[r - 1];
         const int m = hd + (int)((W[r >> 3] >> ((r & 7) * 4)) & 15) - 8;
-        hc[r] = hd ? m : 0;
+        hc[r] = m < (hd << 3) ? m : (hd << 3);
       }
       {
         const int m = hd0 + (int)(W[0] & 15) - 8;
-        hc[0] = hd0 ? m : 0;
+        hc[0] = m < (hd0 << 3) ? m : (hd0 << 3);
         htop = Ht;
       }
       int e = Et, cmax = 0;
