@@ -40,9 +40,12 @@ def compare(gpu: g3sa_amd.Result, ref: dict, label: str) -> None:
         assert np.array_equal(gk, rk), f"{label}: global CIGAR of alignment {i}"
 
 
-# (form, tuning, seeding order): the plain form and the batched form at both
-# strip heights; reads seeded in minimizer order and in input order
-VARIANTS = [("plain", None, 1), ("batched", (16, 8, 8), 1), ("batched", (32, 4, 8), 0)]
+# (form, tuning, seeding order, seeding blocks per CU): the plain form and
+# the batched form at both strip heights; reads seeded in minimizer order and
+# in input order; seeding grids whose LDS share holds 9, 25 and the default
+# number of LEP entries per thread (the rest of a list spills to its global
+# slice)
+VARIANTS = [("plain", None, 1, 0), ("batched", (16, 8, 8), 1, 10), ("batched", (32, 4, 8), 0, 4)]
 
 
 def run_both(ds, oracle, seq, off, label, max_reads=None):
@@ -54,12 +57,14 @@ def run_both(ds, oracle, seq, off, label, max_reads=None):
         oracle.lib.orc_index_free(idx)
     with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(max_reads or n, 1),
                           max_bases=max(int(off[-1]), 1)) as al:
-        for form, tuning, ordered in VARIANTS:
-            print(f"[parity] {label}: {n} reads, form {form} {tuning} seed order {ordered}", flush=True)
+        for form, tuning, ordered, blocks in VARIANTS:
+            print(f"[parity] {label}: {n} reads, form {form} {tuning} seed order {ordered} blocks {blocks}",
+                  flush=True)
             al.set_form(form)
             if tuning:
                 al.set_tuning(*tuning)
             al.set_seed_order(ordered)
+            al.set_seed_blocks(blocks)
             got = al.align(seq, off)
             compare(got, ref, f"{label} [{form} {tuning} order {ordered}]")
             # seeds resolved by the GPU equal the oracle's SA lookups
@@ -154,12 +159,14 @@ def test_scoring_variants_parity(ds_c1, oracle, w, b):
     finally:
         oracle.lib.orc_index_free(idx)
     with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=len(off) - 1, max_bases=int(off[-1]), opt=go) as al:
-        for form, tuning, ordered in VARIANTS:
-            print(f"[parity] scoring w={w} b={b}: form {form} {tuning} seed order {ordered}", flush=True)
+        for form, tuning, ordered, blocks in VARIANTS:
+            print(f"[parity] scoring w={w} b={b}: form {form} {tuning} seed order {ordered} blocks {blocks}",
+                  flush=True)
             al.set_form(form)
             if tuning:
                 al.set_tuning(*tuning)
             al.set_seed_order(ordered)
+            al.set_seed_blocks(blocks)
             compare(al.align(seq, off), ref, f"scoring w={w} b={b} [{form} {tuning} order {ordered}]")
 
 
@@ -170,8 +177,9 @@ def test_empty_batch(ds_c1):
 
 
 def test_autotune_keeps_output(ds_c1, oracle):
-    """The autotuner reruns the extension stage under every tile shape; the
-    batch's output and every later batch stay identical to the oracle."""
+    """The autotuners rerun the extension stage under every tile shape and the
+    whole batch under every seeding grid and order; the batch's output and
+    every later batch stay identical to the oracle."""
     names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["c3"]), limit=2000)
     idx = oracle.load(ds_c1.prefix)
     try:
@@ -186,6 +194,11 @@ def test_autotune_keeps_output(ds_c1, oracle):
         assert t["strip_rows"] in (16, 32) and t["lane_blocks_per_cu"] in (2, 4, 8, 16)
         compare(al.download(), ref, "after autotune")
         compare(al.align(seq, off), ref, "next batch after autotune")
+        print("[parity] seeding autotune", flush=True)
+        st = al.autotune_seed()
+        assert st["seed_blocks_per_cu"] in (6, 8, 10, 0) and st["seed_ordered"] in (0, 1)
+        compare(al.download(), ref, "after seeding autotune")
+        compare(al.align(seq, off), ref, "next batch after seeding autotune")
 
 
 def test_api_error_paths(ds_c1):
