@@ -295,6 +295,24 @@ G3A_HD int64_t sa_value(const DevIndex &x, int64_t k, int64_t *steps) {
   return t + (x.sa32 ? (int64_t)x.sa32[i] : (int64_t)x.sa64[i]);
 }
 
+// The same on the narrow layout (texts of fewer than 2^32 rows): each LF
+// step reads the row's symbol and its Occ from one 16-byte line.
+G3A_HD int64_t sa_value_n(const DevIndex &x, uint32_t k) {
+  const uint32_t mask = (1u << x.sa_shift) - 1, pr = (uint32_t)x.primary;
+  int64_t t = 0;
+  while (k & mask) {
+    if (k == pr) return t;
+    const OccLineN L = occn_line(x, k >> 5);
+    const int c = (int)((L.w >> (62 - 2 * (k & 31))) & 3);
+    uint32_t o[4];
+    occ_at(x, L, k, o);
+    k = (uint32_t)pick4(x.count, c) - 1 + pick4(o, c);
+    ++t;
+  }
+  const uint32_t i = k >> x.sa_shift;
+  return t + (x.sa32 ? (int64_t)x.sa32[i] : (int64_t)x.sa64[i]);
+}
+
 // bns_pos2rid_gpu / bns_intv2rid_gpu (bwa/src/bntseq.cu:7-37)
 G3A_HD int pos2rid(const DevIndex &x, int64_t pos_f) {
   if (pos_f >= x.l_pac) return -1;

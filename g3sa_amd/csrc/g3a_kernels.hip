@@ -205,7 +205,8 @@ __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
   const int64_t j = local - acc;
   const int64_t step = p.s > max_occ ? p.s / max_occ : 1;
   Seed out;
-  out.rbeg = sa_value(a.idx, p.k + step * j, nullptr);
+  const int64_t row = p.k + step * j;
+  out.rbeg = a.idx.occn ? sa_value_n(a.idx, (uint32_t)row) : sa_value(a.idx, row, nullptr);
   out.qbeg = p.m;
   out.len = p.n - p.m + 1;
   out.rid = intv2rid(a.idx, out.rbeg, out.rbeg + out.len);

@@ -34,6 +34,10 @@ int main(int argc, char **argv) {
   x.sbc = sb.data();
   x.primary = (int64_t)h.primary;
   x.n = (int64_t)h.seq_len;
+  build_counts(h, x.count);
+  if (h.sa_width == 4) x.sa32 = (const uint32_t *)h.sa.data();
+  else x.sa64 = (const uint64_t *)h.sa.data();
+  x.sa_shift = log2_exact(h.sa_intv);
   uint64_t cnt[4] = {0, 0, 0, 0};
   const uint64_t rows = h.seq_len + 1;
   for (uint64_t p = 0; p < rows; ++p) {
@@ -50,6 +54,13 @@ int main(int argc, char **argv) {
                     (unsigned long long)cnt[c], (long long)ow[c], on[c]);
         return 1;
       }
+    if (p % 7 == 0) {  // SA of the row by LF walks on both layouts agree
+      const int64_t sw = sa_value(x, (int64_t)p, nullptr), sn = sa_value_n(x, (uint32_t)p);
+      if (sw != sn) {
+        std::printf("sa mismatch row %llu: wide %lld narrow %lld\n", (unsigned long long)p, (long long)sw, (long long)sn);
+        return 1;
+      }
+    }
     const int bc = bwt_char(x, (int64_t)p);
     if (bc != (p == h.primary ? 4 : ch)) {
       std::printf("bwt_char mismatch row %llu\n", (unsigned long long)p);
