@@ -22,10 +22,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -35,6 +37,7 @@
 namespace {
 
 struct Batch {
+  int64_t id = 0;  // position in the input, for ordered output
   std::string names, quals;
   std::vector<int64_t> name_off{0}, qual_off{0};
   std::vector<uint8_t> seq;
@@ -602,21 +605,33 @@ int g3a_mem_main(int argc, char **argv) {
   }
   g3a_opt opt;
   g3a_opt_init(&opt);
-  g3a_ctx *ctx = nullptr;
-  int rc = g3a_device_alloc(0, prefix, &opt, batch, (int64_t)batch * 320, &ctx);
-  if (rc) {
-    std::fprintf(stderr, "g3a mem: cannot create context (%d)\n", rc);
-    return 1;
-  }
-  if ((rc = g3a_memcpy_index(ctx))) {
-    std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
-    g3a_destroy(ctx);
-    return 1;
+  // two contexts on the GPU, each with its own stream and workspace: while
+  // one batch's kernels run, the other context moves its batch to or from
+  // the host
+  const int kCtx = 2;
+  std::vector<g3a_ctx *> ctxs;
+  auto destroy_all = [&] {
+    for (g3a_ctx *c : ctxs) g3a_destroy(c);
+  };
+  for (int k = 0; k < kCtx; ++k) {
+    g3a_ctx *c = nullptr;
+    int rc = g3a_device_alloc(0, prefix, &opt, batch, (int64_t)batch * 320, &c);
+    if (rc) {
+      std::fprintf(stderr, "g3a mem: cannot create context (%d)\n", rc);
+      destroy_all();
+      return 1;
+    }
+    ctxs.push_back(c);
+    if ((rc = g3a_memcpy_index(c))) {
+      std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(c));
+      destroy_all();
+      return 1;
+    }
   }
   FILE *out = out_fn ? std::fopen(out_fn, "wb") : stdout;
   if (!out) {
     std::fprintf(stderr, "g3a mem: cannot write %s\n", out_fn);
-    g3a_destroy(ctx);
+    destroy_all();
     return 1;
   }
   if (sam) {
@@ -629,17 +644,18 @@ int g3a_mem_main(int argc, char **argv) {
     hdr += "@PG\tID:g3a\tPN:g3a\n";
     std::fwrite(hdr.data(), 1, hdr.size(), out);
   }
-  // three stages overlap: the loader parses batch k+1 while the GPU aligns
-  // batch k and the formatter writes batch k-1, in input order
+  // the stages overlap: the loader parses batch k+2 while the aligners
+  // (one per context) align batches k+1 and k and the formatter writes
+  // batch k-1; the writer puts batches back in input order
   struct Job {
     std::unique_ptr<Batch> b;
     g3a_batch_out o;
   };
   Queue<std::unique_ptr<Batch>> q_in(2);
-  Queue<std::unique_ptr<Job>> q_out(2);
+  Queue<std::unique_ptr<Job>> q_out(4);
   // batches circulate: loader -> aligner -> writer -> loader, so their
   // buffers are allocated (and faulted in) once
-  const int kBatches = 5;
+  const int kBatches = 7;
   Queue<std::unique_ptr<Batch>> q_free((size_t)kBatches);
   for (int k = 0; k < kBatches; ++k) q_free.push(std::unique_ptr<Batch>(new Batch));
   const auto t0 = std::chrono::steady_clock::now();
@@ -648,12 +664,15 @@ int g3a_mem_main(int argc, char **argv) {
   double t_load = 0, t_align = 0, t_write = 0;  // busy time of each stage
   double t_align_first = 0;
   int n_batches = 0;
+  int64_t n_loaded = 0;
+  std::mutex stat_m;
   std::thread loader([&] {
     for (;;) {
       std::unique_ptr<Batch> b = q_free.pop();
       const auto tl = clk::now();
       b->clear();
       rd.next_batch(*b, batch, threads);
+      b->id = n_loaded++;
       t_load += since(tl);
       if (b->n() == 0) {
         q_in.push(nullptr);
@@ -664,9 +683,24 @@ int g3a_mem_main(int argc, char **argv) {
   });
   long long total = 0;
   std::thread writer([&] {
+    std::map<int64_t, std::unique_ptr<Job>> pending;  // aligned out of order
+    int64_t next_id = 0;
     for (;;) {
-      std::unique_ptr<Job> j = q_out.pop();
-      if (!j) return;
+      std::unique_ptr<Job> j;
+      auto it = pending.find(next_id);
+      if (it != pending.end()) {
+        j = std::move(it->second);
+        pending.erase(it);
+      } else {
+        j = q_out.pop();
+        if (!j) return;
+        if (j->b->id != next_id) {
+          const int64_t id = j->b->id;
+          pending[id] = std::move(j);
+          continue;
+        }
+      }
+      ++next_id;
       const auto tw = clk::now();
       const Batch &b = *j->b;
       const int nt = b.n() < 4096 ? 1 : threads;
@@ -684,29 +718,40 @@ int g3a_mem_main(int argc, char **argv) {
       q_free.push(std::move(j->b));
     }
   });
-  int status = 0;
-  for (;;) {
-    std::unique_ptr<Batch> b = q_in.pop();
-    if (!b) break;
-    if (status) {  // drain the loader after a failure
-      q_free.push(std::move(b));
-      continue;
+  std::atomic<int> status{0};
+  auto aligner = [&](g3a_ctx *ctx) {
+    for (;;) {
+      std::unique_ptr<Batch> b = q_in.pop();
+      if (!b) {
+        q_in.push(nullptr);  // for the other aligners
+        return;
+      }
+      if (status) {  // drain the loader after a failure
+        q_free.push(std::move(b));
+        continue;
+      }
+      std::unique_ptr<Job> j(new Job);
+      const auto ta = clk::now();
+      const int rc = g3a_align_batch(ctx, b->n(), b->seq.data(), b->seq_off.data(), &j->o);
+      const double dt = since(ta);
+      {
+        std::lock_guard<std::mutex> l(stat_m);
+        t_align += dt;
+        if (n_batches++ == 0) t_align_first = dt;
+      }
+      if (rc) {
+        std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
+        status = 1;
+        q_free.push(std::move(b));
+        continue;
+      }
+      j->b = std::move(b);
+      q_out.push(std::move(j));
     }
-    std::unique_ptr<Job> j(new Job);
-    const auto ta = clk::now();
-    rc = g3a_align_batch(ctx, b->n(), b->seq.data(), b->seq_off.data(), &j->o);
-    const double dt = since(ta);
-    t_align += dt;
-    if (n_batches++ == 0) t_align_first = dt;
-    if (rc) {
-      std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(ctx));
-      status = 1;
-      q_free.push(std::move(b));
-      continue;
-    }
-    j->b = std::move(b);
-    q_out.push(std::move(j));
-  }
+  };
+  std::vector<std::thread> aligners;
+  for (g3a_ctx *c : ctxs) aligners.emplace_back(aligner, c);
+  for (auto &t : aligners) t.join();
   q_out.push(nullptr);
   loader.join();
   writer.join();
@@ -718,6 +763,6 @@ int g3a_mem_main(int argc, char **argv) {
   std::fprintf(stderr, "g3a mem: %d batches; align of the first %.3f s, of the others %.3f s each\n", n_batches,
                t_align_first, n_batches > 1 ? (t_align - t_align_first) / (n_batches - 1) : 0.0);
   if (out_fn) std::fclose(out);
-  g3a_destroy(ctx);
+  destroy_all();
   return status;
 }
