@@ -14,10 +14,16 @@ alignment + finalize) over the 2M-read batch already resident in HBM; outputs
 stay in HBM.  The CPU baseline is the oracle (C, all host cores) on a bounded
 sample of the same reads.
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c2|c3|c1] [--form both|batched|plain]
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c2|c3|c1]
+                  [--form all|both|batched|plain|learned]
 
-Both extension forms give identical output; `value` is the batched form's
-rate and the plain form's is reported beside it.
+Forms (identical output): plain (one wavefront per extension task), batched
+(length-binned persistent lane engine) and learned (batched plus the
+learned-index seeding: full SA, text comparison of unique matches, model
+intervals in the last round).  `value` is the learned form's rate when it
+ran (the metric's batched + learned form), else the batched form's; the
+others are reported beside it.  The learned index is built once per context
+on the device and, like the index upload, outside the timed steps.
 """
 from __future__ import annotations
 
@@ -131,7 +137,7 @@ def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = Tr
         al.run()
         log(f"[{form}] warmup step {i + 1}/{warmup} done")
     tuned = None
-    if form == "batched" and autotune:
+    if form in ("batched", "learned") and autotune:
         if warmup == 0:
             al.run()
         tuned = al.autotune()
@@ -204,8 +210,8 @@ def main() -> int:
                     help="fixed persistent seeding blocks per CU instead of the autotuner (0: occupancy maximum)")
     ap.add_argument("--seed-order", type=int, default=1, choices=[0, 1],
                     help="with --seed-blocks: seed in minimizer order (1) or input order (0)")
-    ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
-                    help="extension form(s) to time; with both, value is the batched form's")
+    ap.add_argument("--form", default="all", choices=["all", "both", "batched", "plain", "learned"],
+                    help="form(s) to time: all = plain, batched, learned; both = plain, batched")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (torch) to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -240,22 +246,38 @@ def main() -> int:
         al.run()
         seed_tuning = al.autotune_seed()
         log(f"seeding autotuned: {seed_tuning}")
-    forms = ["plain", "batched"] if args.form == "both" else [args.form]
+    forms = {"all": ["plain", "batched", "learned"], "both": ["plain", "batched"]}.get(args.form, [args.form])
     per_form = {}
+    learned_info, learned_tuning = None, None
     for form in forms:
-        al.set_form(form)
-        if form == "batched" and args.tuning:
+        al.set_form("plain" if form == "plain" else "batched")
+        if form != "plain" and args.tuning:
             rows, lb = (int(v) for v in args.tuning.split(","))
             al.set_tuning(rows, lb, 8)
-        per_form[form] = run_form(al, n, args.steps, args.warmup, form, autotune=not args.tuning)
-    main_form = "batched" if "batched" in per_form else forms[0]
+        if form == "learned":
+            learned_info = al.set_learned(True)
+            log(f"learned index built: {learned_info}")
+            if args.seed_blocks is None:
+                al.run()
+                learned_tuning = al.autotune_seed()
+                log(f"seeding autotuned (learned form): {learned_tuning}")
+        per_form[form] = run_form(al, n, args.steps, args.warmup, form,
+                                  autotune=not args.tuning and (form == "batched" or "batched" not in per_form))
+        if form == "learned":
+            per_form[form]["learned_work"] = al.learned_stats()
+    main_form = "learned" if "learned" in per_form else ("batched" if "batched" in per_form else forms[0])
     m = per_form[main_form]
     mean, kms, work = m["stages_ms"], m["kernel_ms"], m["work"]
     # dominant kernel and its roofline (k_seed: HBM; extension: integer VALU)
     ext_ms = kms["ext_lane"] + kms["ext_wave"]
     # algorithmic bytes of k_seed: each FM step reads two 32-byte Occ lines
-    # (rows k-1 and k+s-1), plus the read's bases once
-    seed_bytes = 64 * work["fm_steps"] + int(off[-1])
+    # (rows k-1 and k+s-1), plus the read's bases once; in the learned form
+    # also one 4-byte SA entry per text comparison and 2 bits per base it
+    # covers, and per model lookup 280 bytes of root and leaf parameters and
+    # 16 eight-byte keys of window search
+    lw = m.get("learned_work") or {}
+    seed_bytes = (64 * work["fm_steps"] + int(off[-1]) + 4 * lw.get("text_runs", 0)
+                  + lw.get("text_run_bases", 0) // 4 + (280 + 128) * lw.get("lookups", 0))
     seed_gbs = seed_bytes / (kms["k_seed"] * 1e-3) / 1e9 if kms["k_seed"] > 0 else 0.0
     dominant = "k_seed" if kms["k_seed"] >= ext_ms else "extension"
     roof = {"bound": "hbm", "kernel": "k_seed", "achieved": round(seed_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -281,7 +303,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": cfg["workload"], "reads": n, "bases": int(off[-1]), "form": main_form},
+        "config": {"workload": cfg["workload"], "reads": n, "bases": int(off[-1]),
+                   "form": "batched+learned" if main_form == "learned" else main_form},
         "stages_ms": mean,
         "kernel_ms": kms,
         "work": work,
@@ -290,9 +313,17 @@ def main() -> int:
         "tuning": m.get("tuning"),
         "seed_tuning": seed_tuning,
         "forms": {f: {"reads_per_s": v["reads_per_s"], "ms_per_step": v["ms_per_step"], "kernel_ms": v["kernel_ms"],
-                      "stages_ms": v["stages_ms"]} for f, v in per_form.items()},
+                      "stages_ms": v["stages_ms"], "fm_steps": v["work"]["fm_steps"]} for f, v in per_form.items()},
         "roofline": roof,
     }
+    if learned_info is not None:
+        result["learned_index"] = {**learned_info, "seed_tuning": learned_tuning,
+                                   "work": per_form["learned"]["learned_work"]}
+        if "plain" in per_form:
+            result["learned_vs_plain"] = round(per_form["learned"]["reads_per_s"] / per_form["plain"]["reads_per_s"], 3)
+        if "batched" in per_form:
+            result["learned_vs_batched"] = round(per_form["learned"]["reads_per_s"] /
+                                                 per_form["batched"]["reads_per_s"], 3)
     al.close()
     if not args.no_e2e:
         log("end-to-end: g3a mem")

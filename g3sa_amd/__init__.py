@@ -60,11 +60,24 @@ class BatchOut(C.Structure):
     ]
 
 
+class LearnedInfo(C.Structure):
+    """g3a_learned_info (include/g3a.h)."""
+
+    _fields_ = [
+        ("rows", C.c_int64), ("distinct_keys", C.c_int64),
+        ("leaves", C.c_int32), ("hidden_units", C.c_int32), ("window_cap", C.c_int32), ("pad", C.c_int32),
+        ("model_bytes", C.c_int64), ("sa_bytes", C.c_int64), ("key_bytes", C.c_int64),
+        ("max_err", C.c_int64), ("keys_over_window", C.c_int64), ("leaves_over_window", C.c_int64),
+        ("mean_err", C.c_double), ("build_ms", C.c_float), ("pad2", C.c_float),
+    ]
+
+
 EXPORTS = [
     "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
     "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
-    "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order",
+    "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
+    "g3a_last_intervals",
 ]
 
 _lib = None
@@ -98,6 +111,9 @@ def load_library() -> C.CDLL:
     lib.g3a_set_seed_blocks.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_autotune_seed.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_seed_order.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_set_learned.argtypes = [C.c_void_p, C.c_int, C.POINTER(LearnedInfo)]
+    lib.g3a_learned_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    lib.g3a_last_intervals.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
     _lib = lib
     return lib
 
@@ -182,6 +198,30 @@ class Aligner:
         self._check(self.lib.g3a_autotune_seed(self.ctx, C.byref(b), C.byref(o), C.byref(ms)))
         return {"seed_blocks_per_cu": b.value, "seed_ordered": o.value, "seed_stage_ms": ms.value}
 
+    def set_learned(self, on: bool) -> dict:
+        """Switch seeding to the learned-index form (built on first use) or back;
+        returns the model's figures."""
+        info = LearnedInfo()
+        self._check(self.lib.g3a_set_learned(self.ctx, int(bool(on)), C.byref(info)))
+        return {k: getattr(info, k) for k, _ in LearnedInfo._fields_ if not k.startswith("pad")}
+
+    def learned_stats(self) -> dict:
+        """Learned-form work of the last batch."""
+        v = (C.c_int64 * 4)()
+        self._check(self.lib.g3a_learned_stats(self.ctx, v))
+        return dict(zip(["lookups", "window_misses", "text_runs", "text_run_bases"], list(v)))
+
+    def intervals(self) -> tuple[np.ndarray, np.ndarray]:
+        """Seeding intervals of the last batch: per-read counts and (k, s, m, n) rows."""
+        n = self.n_reads
+        counts = np.zeros(max(n, 1), dtype=np.int32)
+        total = C.c_int64()
+        self._check(self.lib.g3a_last_intervals(self.ctx, counts.ctypes.data, None, 0, C.byref(total)))
+        kmsn = np.zeros((max(total.value, 1), 4), dtype=np.int64)
+        self._check(self.lib.g3a_last_intervals(self.ctx, counts.ctypes.data, kmsn.ctypes.data, total.value,
+                                                C.byref(total)))
+        return counts[:n], kmsn[:total.value]
+
     def _check(self, rc: int) -> None:
         if rc != G3A_OK:
             raise RuntimeError(f"g3a call failed ({rc}): {self.lib.g3a_last_error(self.ctx).decode()}")
@@ -190,6 +230,7 @@ class Aligner:
         seq = np.ascontiguousarray(seq, dtype=np.uint8)
         offsets = np.ascontiguousarray(offsets, dtype=np.int32)
         self._keep = (seq, offsets)
+        self.n_reads = len(offsets) - 1
         self._check(self.lib.g3a_memcpy_input(self.ctx, len(offsets) - 1, seq.ctypes.data, offsets.ctypes.data))
 
     def run(self) -> None:

@@ -11,6 +11,7 @@
 #include "g3a_index.h"
 #include "g3a_kernels.h"
 #include "g3a_layout.h"
+#include "g3a_learned.h"
 
 #include <hip/hip_runtime.h>
 
@@ -81,6 +82,11 @@ struct g3a_ctx {
   int kev_used[6] = {0, 0, 0, 0, 0, 0};
   float kms[4] = {0, 0, 0, 0};
   // last batch results
+  // learned-index form: full SA, keys, model, build scratch
+  DevBuf li_saf, li_keys, li_root, li_leaf, li_leaf_a, li_mom, li_short, li_ctr;
+  bool li_built = false;
+  uint32_t li_leaf_w = 0;
+  g3a_learned_info li_info{};
   int64_t total_seeds = 0, total_tasks = 0, total_alns = 0, total_cig = 0, n_slow = 0;
   float ms[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -511,6 +517,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.slow = P<int32_t>(ctx->slow);
   sa.fm_steps = cnt + 0;
   sa.lep_pool = P<Intv>(ctx->lep_pool);
+  sa.li_stats = ctx->idx.li_leaf ? cnt + 32 : nullptr;
   sa.order = nullptr;
   if (ctx->seed_ordered && n > 1) {
     launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(ctx->seed_key), P<int32_t>(ctx->seed_idx), s);
@@ -834,6 +841,128 @@ int g3a_set_seed_order(g3a_ctx *ctx, int ordered) {
   return G3A_OK;
 }
 
+int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
+  if (!ctx || (on != 0 && on != 1)) return G3A_E_ARG;
+  if (!ctx->index_on_device) return fail(ctx, G3A_E_STATE, "index not on device");
+  DevIndex &x = ctx->idx;
+  if (!on) {
+    x.saf = nullptr;
+    x.keys = nullptr;
+    x.li_root = nullptr;
+    x.li_leaf = nullptr;
+    x.li_short = nullptr;
+    if (info) *info = ctx->li_info;
+    return G3A_OK;
+  }
+  if (!x.occn) return fail(ctx, G3A_E_ARG, "the learned index needs a text of fewer than 2^32 rows");
+  if (!ctx->li_built) {
+    HIPCK(hipSetDevice(ctx->device));
+    const int64_t N = x.n + 1;
+    if (ensure(ctx, ctx->li_saf, (size_t)N * 4) || ensure(ctx, ctx->li_keys, (size_t)N * 8) ||
+        ensure(ctx, ctx->li_root, ((size_t)1 << kLiRootBits) * 4 + 4) ||
+        ensure(ctx, ctx->li_leaf, sizeof(LiLeaf) * kLiLeaves) || ensure(ctx, ctx->li_leaf_a, (kLiLeaves + 1) * 4) ||
+        ensure(ctx, ctx->li_mom, sizeof(double) * 5 * kLiLeaves * (kLiHinge + 1)) ||
+        ensure(ctx, ctx->li_short, 2 * kLiShort * 4) || ensure(ctx, ctx->li_ctr, 64))
+      return G3A_E_HIP;
+    LearnedBufs b{};
+    b.saf = P<uint32_t>(ctx->li_saf);
+    b.keys = P<uint64_t>(ctx->li_keys);
+    b.root = P<uint32_t>(ctx->li_root);
+    b.leaf = P<LiLeaf>(ctx->li_leaf);
+    b.leaf_a = P<uint32_t>(ctx->li_leaf_a);
+    b.mom = P<double>(ctx->li_mom);
+    b.shortt = P<uint32_t>(ctx->li_short);
+    b.ctr = P<unsigned long long>(ctx->li_ctr);
+    b.n_samples = (int64_t)(ctx->h.sa.size() / ctx->h.sa_width);
+    LearnedBuildStats st{};
+    HIPCK(hipEventRecord(ctx->ev[6], ctx->stream));
+    const int rc = learned_build(x, b, ctx->stream, st);
+    if (rc == 2) return fail(ctx, G3A_E_HIP, "learned index build: HIP error");
+    if (rc) {
+      char m[256];
+      std::snprintf(m, sizeof m,
+                    "learned index build: check failed (rows not covered %lld, keys out of order %lld, "
+                    "bad walks %lld, short rows %lld)",
+                    (long long)st.unfilled_rows, (long long)st.order_violations, (long long)st.bad_walks,
+                    (long long)st.short_rows);
+      return fail(ctx, G3A_E_INTERNAL, m);
+    }
+    HIPCK(hipEventRecord(ctx->ev[7], ctx->stream));
+    HIPCK(hipEventSynchronize(ctx->ev[7]));
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
+    g3a_learned_info &li = ctx->li_info;
+    li.rows = N;
+    li.distinct_keys = st.distinct_keys;
+    li.leaves = kLiLeaves;
+    li.hidden_units = kLiHinge;
+    li.window_cap = kLiWindowCap;
+    li.model_bytes = (int64_t)sizeof(LiLeaf) * kLiLeaves + (((int64_t)1 << kLiRootBits) + 1) * 4 + 2 * kLiShort * 4;
+    li.sa_bytes = N * 4;
+    li.key_bytes = N * 8;
+    li.max_err = st.max_err;
+    li.keys_over_window = st.keys_over_window;
+    li.leaves_over_window = st.leaves_over_window;
+    li.mean_err = st.distinct_keys ? (double)st.err_sum / (double)st.distinct_keys : 0.0;
+    li.build_ms = ms;
+    ctx->li_leaf_w = st.leaf_w;
+    ctx->li_built = true;
+  }
+  x.saf = P<uint32_t>(ctx->li_saf);
+  x.keys = P<uint64_t>(ctx->li_keys);
+  x.li_root = P<uint32_t>(ctx->li_root);
+  x.li_leaf = P<LiLeaf>(ctx->li_leaf);
+  x.li_short = P<uint32_t>(ctx->li_short);
+  x.li_leaf_w = ctx->li_leaf_w;
+  if (info) *info = ctx->li_info;
+  return G3A_OK;
+}
+
+int g3a_learned_stats(const g3a_ctx *ctx, int64_t out[4]) {
+  if (!ctx || !out) return G3A_E_ARG;
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipMemcpy(out, (const int64_t *)ctx->counters.p + 32, 4 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return G3A_E_HIP;
+  return G3A_OK;
+}
+
+int g3a_last_intervals(const g3a_ctx *ctx, int32_t *counts, int64_t *kmsn, int64_t cap, int64_t *total) {
+  if (!ctx || !counts || !total) return G3A_E_ARG;
+  if (!ctx->has_output) return G3A_E_STATE;
+  const int n = ctx->n_reads;
+  if (n == 0) {
+    *total = 0;
+    return G3A_OK;
+  }
+  std::vector<int64_t> base((size_t)n);
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipMemcpy(counts, ctx->n_intv.p, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(base.data(), ctx->intv_base.p, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return G3A_E_HIP;
+  int64_t t = 0, extent = 0;
+  for (int r = 0; r < n; ++r) {
+    t += counts[r];
+    extent = base[r] + counts[r] > extent ? base[r] + counts[r] : extent;
+  }
+  *total = t;
+  if (!kmsn) return G3A_OK;
+  if (cap < t) return G3A_E_ARG;
+  std::vector<Intv> iv((size_t)extent);
+  if (extent && hipMemcpy(iv.data(), ctx->intv.p, (size_t)extent * sizeof(Intv), hipMemcpyDeviceToHost) != hipSuccess)
+    return G3A_E_HIP;
+  int64_t w = 0;
+  for (int r = 0; r < n; ++r)
+    for (int k = 0; k < counts[r]; ++k) {
+      const Intv &v = iv[(size_t)(base[r] + k)];
+      kmsn[4 * w] = v.k;
+      kmsn[4 * w + 1] = v.s;
+      kmsn[4 * w + 2] = v.m;
+      kmsn[4 * w + 3] = v.n;
+      ++w;
+    }
+  return G3A_OK;
+}
+
 int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms) {
   if (!ctx) return G3A_E_ARG;
   if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
@@ -915,7 +1044,8 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->seed_order, &ctx->seed_sort_tmp,
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
                     &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
-                    &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr};
+                    &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr, &ctx->li_saf, &ctx->li_keys,
+                    &ctx->li_root, &ctx->li_leaf, &ctx->li_leaf_a, &ctx->li_mom, &ctx->li_short, &ctx->li_ctr};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

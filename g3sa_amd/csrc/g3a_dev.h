@@ -33,6 +33,28 @@ struct Opt {
   int8_t mat[25];
 };
 
+// Learned-index form: a two-stage model from the 32-symbol prefix of a
+// suffix (2 bits per symbol, first symbol highest; symbols past the text end
+// are 0) to its row.  The root is a piecewise-linear interpolation of the
+// exact row of every 6-symbol prefix; it routes a key to one of kLiLeaves
+// leaves.  A leaf is a one-hidden-layer network in FP32: input x = the key's
+// offset from the leaf's first key, scaled to [0, 1]; kLiHinge ReLU units
+// relu(x - t_h) with their knots t_h at the leaf's key quantiles; output
+// c0 + c1 x + sum w_h relu(x - t_h) = row offset from the leaf's first row.
+constexpr int kLiRootBits = 12;
+constexpr int kLiLeaves = 4096;
+constexpr int kLiHinge = 30;
+constexpr int kLiWindowCap = 256;  // largest half-width of a search window
+constexpr int kLiShort = 32;       // rows whose suffix has fewer than 32 symbols ('$' row included)
+
+struct alignas(16) LiLeaf {
+  uint32_t a;     // first row routed to the leaf
+  uint32_t err;   // largest |predicted - true row| over the leaf's distinct keys
+  uint64_t kmin;  // key of row a
+  float inv_span, c0, c1, pad;
+  float t[kLiHinge], w[kLiHinge];
+};
+
 // Device view of the index.  `occ` holds one 32-byte line per 64 BWT rows:
 // four uint32 base counts of the rows before the line, relative to its
 // 2^32-row superblock (`super`), then the 64 rows' 2-bit BWT symbols in two
@@ -61,6 +83,16 @@ struct DevIndex {
   // superblocks' four absolute counts (uint32 each); null otherwise
   const uint32_t *occn;
   const uint32_t *sbc;
+  // learned-index form (all null otherwise; texts of fewer than 2^32 rows):
+  // the full suffix array, the 32-symbol key of every row, the root's
+  // 2^kLiRootBits + 1 segment rows, the leaves, and the (row, suffix length)
+  // pairs of the kLiShort short suffixes in row order
+  const uint32_t *saf;
+  const uint64_t *keys;
+  const uint32_t *li_root;
+  const LiLeaf *li_leaf;
+  const uint32_t *li_short;
+  uint32_t li_leaf_w;  // routing: leaf = predicted row / li_leaf_w
 };
 
 constexpr int kKmer = 12;
@@ -313,6 +345,206 @@ G3A_HD int64_t sa_value_n(const DevIndex &x, uint32_t k) {
   return t + (x.sa32 ? (int64_t)x.sa32[i] : (int64_t)x.sa64[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Learned-index form: text words, the model, exact windowed search
+// ---------------------------------------------------------------------------
+G3A_HD uint32_t bswap32_(uint32_t v) { return __builtin_bswap32(v); }
+
+// Forward-strand symbols [i, i + 32), first in the highest bits.  pac is
+// MSB-first and 4-byte aligned, with at least 16 bytes of padding.
+G3A_HD uint64_t fwd_word(const DevIndex &x, int64_t i) {
+  const uint64_t byte = (uint64_t)i >> 2;
+  const uint32_t *w = (const uint32_t *)(x.pac + (byte & ~(uint64_t)3));
+  const uint64_t be01 = (uint64_t)bswap32_(w[0]) << 32 | bswap32_(w[1]);
+  const uint32_t be2 = bswap32_(w[2]);
+  const int sh = (int)(byte & 3) * 8 + (int)(i & 3) * 2;  // < 32
+  return sh ? (be01 << sh) | ((uint64_t)be2 >> (32 - sh)) : be01;
+}
+
+// The 32 two-bit symbols of v in reverse order
+G3A_HD uint64_t rev_sym(uint64_t v) {
+  v = ((v >> 2) & 0x3333333333333333ULL) | ((v & 0x3333333333333333ULL) << 2);
+  v = ((v >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((v & 0x0F0F0F0F0F0F0F0FULL) << 4);
+  return __builtin_bswap64(v);
+}
+
+// Text symbols [q, q + 32) for l_pac <= q < n (reverse-complement half:
+// T[q + t] = 3 - fwd[n - 1 - q - t]); 0 past the text end.
+G3A_HD uint64_t rc_word(const DevIndex &x, int64_t q) {
+  const int64_t m = x.n - q;  // symbols left, 1..l_pac
+  const uint64_t f = m >= 32 ? fwd_word(x, m - 32) : fwd_word(x, 0) >> (2 * (32 - m));
+  uint64_t r = ~rev_sym(f);
+  if (m < 32) r &= ~0ULL << (2 * (32 - m));
+  return r;
+}
+
+// Text symbols [p, p + 32) of T = forward + reverse complement; 0 past the
+// end.  This is the key of the row whose suffix starts at p.
+G3A_HD uint64_t text_word(const DevIndex &x, int64_t p) {
+  if (p >= x.n) return 0;
+  if (p >= x.l_pac) return rc_word(x, p);
+  const int64_t f = x.l_pac - p;
+  const uint64_t w = fwd_word(x, p);
+  if (f >= 32) return w;
+  return (w & (~0ULL << (2 * (32 - f)))) | rc_word(x, x.l_pac) >> (2 * f);
+}
+
+// Root: the leaf of a key.  Rows of the 6-symbol prefix segment s are
+// [li_root[s], li_root[s + 1]); the key's place in it is the next 32 key
+// bits, interpolated in integer arithmetic (routing is exact and identical
+// wherever it runs).  Monotone in the key, so each leaf holds a row range.
+G3A_HD uint32_t li_route(const DevIndex &x, uint64_t key) {
+  const uint32_t sg = (uint32_t)(key >> (64 - kLiRootBits));
+  const uint32_t b0 = x.li_root[sg], b1 = x.li_root[sg + 1];
+  const uint64_t frac = (key << kLiRootBits) >> 32;
+  const uint64_t pred = b0 + (((uint64_t)(b1 - b0) * frac) >> 32);
+  const uint64_t j = pred / x.li_leaf_w;
+  return j < (uint64_t)kLiLeaves ? (uint32_t)j : (uint32_t)(kLiLeaves - 1);
+}
+
+G3A_HD float li_leaf_x(const LiLeaf &L, uint64_t key) {
+  const uint64_t d = key > L.kmin ? key - L.kmin : 0;
+  const float v = (float)d * L.inv_span;
+  return v < 1.f ? v : 1.f;
+}
+
+// The leaf network's output: predicted row offset from the leaf's first row
+G3A_HD float li_leaf_eval(const LiLeaf &L, float v) {
+  float y = L.c0 + L.c1 * v;
+#pragma unroll 5
+  for (int h = 0; h < kLiHinge; ++h) {
+    const float u = v - L.t[h];
+    y += L.w[h] * (u > 0.f ? u : 0.f);
+  }
+  return y;
+}
+
+G3A_HD int64_t li_predict(const LiLeaf &L, uint64_t key) {
+  float y = li_leaf_eval(L, li_leaf_x(L, key));
+  y = y > -1e9f ? (y < 5e9f ? y : 5e9f) : -1e9f;
+  return (int64_t)L.a + (int64_t)(y >= 0.f ? y + 0.5f : y - 0.5f);
+}
+
+// First row whose key is >= K, searched exactly in the window of half-width
+// min(err + 2, kLiWindowCap) around the model's prediction; -1 when the
+// window does not bracket it (the caller then takes the FM-index path).
+G3A_HD int64_t li_lower_bound(const DevIndex &x, uint64_t K) {
+  const LiLeaf &L = x.li_leaf[li_route(x, K)];
+  const int64_t N = x.n + 1;
+  const int64_t p = li_predict(L, K);
+  int64_t E = (int64_t)L.err + 2;
+  if (E > kLiWindowCap) E = kLiWindowCap;
+  int64_t lo = p - E, hi = p + E + 1;
+  if (lo < 0) lo = 0;
+  if (hi > N) hi = N;
+  if (lo >= hi) return -1;
+  if (lo > 0 && x.keys[lo - 1] >= K) return -1;
+  if (hi < N && x.keys[hi] < K) return -1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (x.keys[mid] < K) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Rows [k, k + s) of the suffixes that start with the j-symbol string W
+// (first symbol in the highest bits, the rest of the word 0; 1 <= j <= 32):
+// the interval the FM-index's backward search gives for it.  False when a
+// search window misses.
+G3A_HD bool li_range(const DevIndex &x, uint64_t W, int j, uint32_t &k, uint32_t &s) {
+  const int64_t N = x.n + 1;
+  const int64_t k0 = li_lower_bound(x, W);
+  if (k0 < 0) return false;
+  const uint64_t unit = (uint64_t)1 << (64 - 2 * j);
+  const uint64_t Wn = W + unit;  // first key past the prefix; 0 when W is all T
+  int64_t k1;
+  if (Wn == 0) {
+    k1 = N;
+  } else {
+    // gallop from k0 over rows k0 + 2^i - 1, then bisect
+    int64_t prev = k0 - 1, r = k0, step = 1;
+    while (r < N && x.keys[r] < Wn && step < 1024) {
+      prev = r;
+      step <<= 1;
+      r = k0 + step - 1;
+    }
+    if (r < N && x.keys[r] < Wn) {
+      k1 = li_lower_bound(x, Wn);
+      if (k1 < 0) return false;
+    } else {
+      int64_t lo = prev + 1, hi = r < N ? r : N;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (x.keys[mid] < Wn) lo = mid + 1;
+        else hi = mid;
+      }
+      k1 = lo;
+    }
+  }
+  // suffixes shorter than j whose zero-padded key starts with W do not
+  // start with W; they sort first in [k0, k1) ('$' is below every base)
+  int64_t kk = k0;
+  for (int t = 0; t < kLiShort; ++t) {
+    const int64_t r = x.li_short[2 * t];
+    if (r >= k0 && r < k1 && (int)x.li_short[2 * t + 1] < j) ++kk;
+  }
+  k = (uint32_t)kk;
+  s = (uint32_t)(k1 - kk);
+  return true;
+}
+
+// Full-SA construction step: from sampled row j << sa_shift, whose value
+// is known, walk back through the rows of positions p - 1, p - 2, ... by LF
+// steps, writing each, until the next sampled row or position 0 (the row
+// whose BWT symbol is '$').  Every row is written by exactly one walk.
+// False if the walk leaves the text (an inconsistent index).
+G3A_HD bool li_walk_sample(const DevIndex &x, int64_t j, uint32_t *saf) {
+  const int64_t r64 = j << x.sa_shift;
+  if (r64 > x.n) return true;
+  uint32_t r = (uint32_t)r64;
+  int64_t p = x.sa32 ? (int64_t)x.sa32[j] : (int64_t)x.sa64[j];
+  saf[r] = (uint32_t)p;
+  const uint32_t mask = (1u << x.sa_shift) - 1, pr = (uint32_t)x.primary;
+  while (r != pr) {
+    const OccLineN L = occn_line(x, r >> 5);
+    const int c = (int)((L.w >> (62 - 2 * (r & 31))) & 3);
+    uint32_t o[4];
+    occ_at(x, L, r, o);
+    r = (uint32_t)pick4(x.count, c) - 1 + pick4(o, c);
+    --p;
+    if ((r & mask) == 0) break;
+    if (p < 0 || r > (uint32_t)x.n) return false;
+    saf[r] = (uint32_t)p;
+  }
+  return true;
+}
+
+// Number of leading positions u < maxlen with q[u] a base equal to the text
+// symbol T[t + u], t + u < n: how far a match known to occur once, ending
+// just before t, continues forward.
+G3A_HD int text_run(const DevIndex &x, int64_t t, const uint8_t *q, int maxlen) {
+  const int64_t lim = x.n - t;
+  if (lim < maxlen) maxlen = (int)lim;
+  int u = 0;
+  while (u < maxlen) {
+    const int c = maxlen - u < 32 ? maxlen - u : 32;
+    uint64_t rw = 0;
+    int v = 0;
+    for (; v < c; ++v) {
+      const int b = q[u + v];
+      if (b > 3) break;
+      rw |= (uint64_t)b << (62 - 2 * v);
+    }
+    const uint64_t tw = text_word(x, t + u);
+    const uint64_t d = v ? (tw ^ rw) & (~0ULL << (64 - 2 * v)) : 0;
+    if (d) return u + (__builtin_clzll(d) >> 1);
+    u += v;
+    if (v < c) return u;
+  }
+  return u;
+}
+
 // bns_pos2rid_gpu / bns_intv2rid_gpu (bwa/src/bntseq.cu:7-37)
 G3A_HD int pos2rid(const DevIndex &x, int64_t pos_f) {
   if (pos_f >= x.l_pac) return -1;
@@ -544,9 +776,11 @@ using GlobalBuf = LepBuf<GlobalList<E>, GlobalList<E>>;
 // then backward), which visits each read position a few times instead of
 // once per end position.  Output order is by end position, as the
 // reference's.  Returns false when a LEP list outgrows `buf`.
-template <class B>
+// kLearn = false compiles the learned-index form out (the FM-only kernel
+// keeps its registers); with kLearn it is used when x.saf / x.li_leaf are set.
+template <bool kLearn = true, class B>
 G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out, B &buf,
-                        int64_t *steps) {
+                        int64_t *steps, int64_t *lc = nullptr) {
   using I = typename B::I;
   using P = decltype(I::s);
   int x0 = 0;
@@ -562,6 +796,23 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     ik.n = x0;
     int i;
     for (i = x0 + 1; i < len; ++i) {
+      if (kLearn && x.saf && ik.s == 1) {
+        // one occurrence, at saf[k]: the forward extensions keep s = 1 for
+        // as long as the read matches the text after it, so compare the two
+        // directly (only l, which nothing downstream reads, is left stale)
+        const int run = text_run(x, (int64_t)x.saf[ik.k] + (i - x0), q + i, len - i);
+        if (lc) {
+          ++lc[2];
+          lc[3] += run;
+        }
+        ik.n = i + run - 1;
+        i += run;
+        if (i < len) {
+          if (np == buf.cap) return false;
+          prev.put(np++, ik);
+        }
+        break;
+      }
       if (q[i] < 4) {
         I ok = ik;
         forward_ext(x, ok, q[i]);
@@ -755,16 +1006,45 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
 // (seed.cu:1518-1550).  Every start m extends forward at least to length
 // min_len before it can emit, so when the first kKmer bases hold no N their
 // interval is read from the k-mer table instead of kKmer-1 forward steps.
-template <class P>
+template <class P, bool kLearn = true>
 G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out,
-                          int64_t *steps) {
+                          int64_t *steps, int64_t *lc = nullptr) {
   const int min_len = o.min_seed_len + 1;
   const int64_t max_intv = o.max_mem_intv;
   const bool table = x.kmer && min_len > kKmer;
+  const bool learned = kLearn && x.li_leaf && min_len <= 32;
   int m = 0;
   while (m < len) {
     int next_m = m + 1;
     int base = q[m];
+    if (base < 4 && learned && m + min_len <= len) {
+      // the pass emits at length min_len when fewer than max_intv rows hold
+      // read[m, m + min_len): the learned index gives that interval directly
+      uint64_t W = 0;
+      int t = 0;
+      for (; t < min_len && q[m + t] < 4; ++t) W |= (uint64_t)q[m + t] << (62 - 2 * t);
+      uint32_t kk, ss;
+      if (t == min_len) {
+        if (lc) ++lc[0];
+        if (li_range(x, W, min_len, kk, ss)) {
+          if ((int64_t)ss < max_intv) {
+            if (ss > 0) {
+              IntvT<P> sd;
+              sd.k = (P)kk;
+              sd.l = 0;
+              sd.s = (P)ss;
+              sd.m = m;
+              sd.n = m + min_len - 1;
+              out.push(sd);
+            }
+            m += min_len;
+            continue;
+          }
+        } else if (lc) {
+          ++lc[1];
+        }
+      }
+    }
     if (base < 4) {
       IntvT<P> seed;
       int n0 = m + 1;

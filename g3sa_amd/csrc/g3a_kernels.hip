@@ -37,10 +37,12 @@ namespace g3a {
 // after them (12 B per entry, lanes interleaved), the rest in the thread's
 // global slice: the list traffic then stays out of the caches that the Occ
 // lines need.
-template <class B>
+template <bool L, class B>
 __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nthreads) {
   using I = typename B::I;
   int64_t steps = 0;
+  // learned-index form: range lookups, window misses, text runs, run bases
+  int64_t lc[4] = {0, 0, 0, 0};
   for (int i = gtid; i < a.n_reads; i += nthreads) {
     const int r = a.order ? a.order[i] : i;
     const int off = a.off[r], len = a.off[r + 1] - off;
@@ -52,7 +54,7 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
       a.slow[r] = 0;
       continue;
     }
-    bool ok = smem_round1(a.idx, a.opt, q, len, sink, buf, &steps);
+    bool ok = smem_round1<L>(a.idx, a.opt, q, len, sink, buf, &steps, lc);
     if (ok) {
       const int nsm = sink.n < sink.cap ? sink.n : sink.cap;
       if (sink.n <= sink.cap) {
@@ -61,7 +63,7 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
           ok = reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.f, buf.cap, buf.cap_f, sink, &steps);
         }
       } else ok = false;
-      if (ok) reseed_round3<decltype(I::s)>(a.idx, a.opt, q, len, sink, &steps);
+      if (ok) reseed_round3<decltype(I::s), L>(a.idx, a.opt, q, len, sink, &steps, lc);
     }
     ok = ok && sink.n <= sink.cap;
     a.slow[r] = ok ? 0 : 1;
@@ -73,9 +75,21 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
     for (int o = 32; o >= 1; o >>= 1) steps += __shfl_xor(steps, o);
     if ((threadIdx.x & 63) == 0 && steps) atomicAdd((unsigned long long *)a.fm_steps, (unsigned long long)steps);
   }
+  if (L && a.li_stats) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int64_t v = lc[k];
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+      if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long *)a.li_stats + k, (unsigned long long)v);
+    }
+  }
 }
 
-template <class E>
+// L: the learned-index form (text comparison of unique matches, the
+// model's intervals in the last round); a separate kernel, so that the
+// FM-only one keeps its register budget
+template <class E, bool L>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
   extern __shared__ uint4 sb_lds[];
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,10 +104,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     LepBuf<SplitList, GlobalList<E>> buf{
         SplitList{(uint32_t *)(sb_lds + n_sb) + threadIdx.x, (int)blockDim.x, a.lep_lds, la}, GlobalList<E>{lf},
         kLepCap, kFoundCap};
-    seed_reads(a, buf, gtid, nthreads);
+    seed_reads<L>(a, buf, gtid, nthreads);
   } else {
     GlobalBuf<E> buf{GlobalList<E>{la}, GlobalList<E>{lf}, kLepCap, kFoundCap};
-    seed_reads(a, buf, gtid, nthreads);
+    seed_reads<false>(a, buf, gtid, nthreads);
   }
 }
 
@@ -206,7 +220,9 @@ __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
   const int64_t step = p.s > max_occ ? p.s / max_occ : 1;
   Seed out;
   const int64_t row = p.k + step * j;
-  out.rbeg = a.idx.occn ? sa_value_n(a.idx, (uint32_t)row) : sa_value(a.idx, row, nullptr);
+  out.rbeg = a.idx.saf    ? (int64_t)a.idx.saf[row]
+             : a.idx.occn ? sa_value_n(a.idx, (uint32_t)row)
+                          : sa_value(a.idx, row, nullptr);
   out.qbeg = p.m;
   out.len = p.n - p.m + 1;
   out.rid = intv2rid(a.idx, out.rbeg, out.rbeg + out.len);
@@ -867,24 +883,25 @@ static size_t seed_lds_bytes(const DevIndex &x) {
 int seed_grid_threads(int n_cu, const DevIndex &x) {
   int per_cu = 0;
   hipError_t e = seed_narrow(x)
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepNarrow>, 128, seed_lds_bytes(x))
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepWide>, 128, 0);
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepNarrow, false>, 128,
+                                                                    seed_lds_bytes(x))
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepWide, false>, 128, 0);
   if (e != hipSuccess || per_cu < 1) per_cu = 8;
   return n_cu * per_cu * 128;
 }
 void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_t s) {
   if (!a.n_reads) return;
   const int need = blocks(a.n_reads, 128), have = n_threads / 128;
-  const dim3 grid(need < have ? need : have);
+  int nb = need < have ? need : have;
   if (seed_narrow(a.idx)) {
+    void (*kern)(SeedArgs) = a.idx.li_leaf ? k_seed<LepNarrow, true> : k_seed<LepNarrow, false>;
     // The compute unit's LDS (160 KB, at most 64 KB per block) is shared by
     // the blocks it is to hold; the LEP entries per thread are the most that
     // still lets that many blocks be resident (a block that is not resident
     // would only start when another finishes, doubling the kernel's time).
     const size_t sb = seed_lds_bytes(a.idx);
     int want = blocks_per_cu;
-    if (want <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&want, k_seed<LepNarrow>, 128, sb) != hipSuccess ||
-                      want < 1))
+    if (want <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&want, kern, 128, sb) != hipSuccess || want < 1))
       want = 8;
     size_t per_block = (size_t)160 * 1024 / (size_t)want - 1024;  // slack for allocation granules
     per_block = per_block < (size_t)64 * 1024 ? per_block : (size_t)64 * 1024;
@@ -892,16 +909,22 @@ void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_
     capl = capl < kLepCap ? capl : kLepCap;
     int resident = 0;
     for (; capl > 0; --capl) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_seed<LepNarrow>, 128,
-                                                       sb + (size_t)capl * 128 * 12) != hipSuccess)
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, 128, sb + (size_t)capl * 128 * 12) !=
+          hipSuccess)
         resident = 0;
       if (resident >= want) break;
     }
+    // no more blocks than the compute units hold at once
+    int dev = 0, n_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n_cu > 0 &&
+        resident > 0 && nb > resident * n_cu)
+      nb = resident * n_cu;
     SeedArgs b = a;
     b.lep_lds = capl;
-    hipLaunchKernelGGL(k_seed<LepNarrow>, grid, dim3(128), sb + (size_t)capl * 128 * 12, s, b);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(128), sb + (size_t)capl * 128 * 12, s, b);
   } else {
-    hipLaunchKernelGGL(k_seed<LepWide>, grid, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((k_seed<LepWide, false>), dim3(nb), dim3(128), 0, s, a);
   }
 }
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,

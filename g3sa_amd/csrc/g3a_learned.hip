@@ -1,0 +1,238 @@
+// Learned-index form: construction and training kernels.
+//
+// From the FM-index already in HBM this builds, on the device:
+//   1. the full suffix array: one walk per SA sample, backwards through the
+//      text by LF steps until the next sampled row, writing every row it
+//      passes (each row is written once; n LF steps in all);
+//   2. the key of every row: the first 32 symbols of its suffix, 2 bits
+//      each, zero past the text end (keys are non-decreasing in row order);
+//   3. the root: the first row of every 6-symbol prefix;
+//   4. the leaves: first rows (from the root's routing), knots at each
+//      leaf's key quantiles, per-segment moments of (x, row) over the
+//      distinct keys, a least-squares solve of the output layer in FP64,
+//      and the largest prediction error over the distinct keys.
+// The model is read by li_range / li_lower_bound (g3a_dev.h), whose window
+// search is exact; the error bounds only size the windows.
+#include "g3a_dev.h"
+#include "g3a_learned.h"
+#include "g3a_learned_fit.h"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace g3a {
+
+namespace {
+
+constexpr int kChunk = 16;  // rows per thread in the row-range passes
+
+__global__ void __launch_bounds__(256) k_li_fill(uint32_t *p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// 1. full SA: walk from sampled row j << sa_shift (whose value is known)
+// through the rows of positions p - 1, p - 2, ... until a sampled row or
+// position 0 (the '$' BWT row).
+__global__ void __launch_bounds__(256) k_li_full_sa(DevIndex x, uint32_t *saf, int64_t n_samples,
+                                                    unsigned long long *ctr) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_samples) return;
+  if (!li_walk_sample(x, j, saf)) atomicAdd(ctr + 7, 1ull);
+}
+
+// 2. keys, rows never written by the walks, and the short suffixes
+__global__ void __launch_bounds__(256) k_li_keys(DevIndex x, const uint32_t *saf, uint64_t *keys, uint32_t *shortt,
+                                                 unsigned long long *ctr) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > x.n) return;
+  const uint32_t p = saf[r];
+  if (p == 0xffffffffu || p > (uint32_t)x.n) {
+    atomicAdd(ctr + 0, 1ull);
+    keys[r] = 0;
+    return;
+  }
+  keys[r] = text_word(x, p);
+  if (x.n - p < kLiShort) {
+    const unsigned long long t = atomicAdd(ctr + 6, 1ull);
+    if (t < (unsigned long long)kLiShort) {
+      shortt[2 * t] = (uint32_t)r;
+      shortt[2 * t + 1] = (uint32_t)(x.n - p);
+    }
+  }
+}
+
+// order check (keys must be non-decreasing) and distinct-key count
+__global__ void __launch_bounds__(256) k_li_order(const uint64_t *keys, int64_t N, unsigned long long *ctr) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  const bool first = r == 0 || keys[r - 1] != keys[r];
+  const bool bad = r > 0 && keys[r - 1] > keys[r];
+  const unsigned long long nf = __popcll(__ballot(first)), nb = __popcll(__ballot(bad));
+  if ((threadIdx.x & 63) == 0) {
+    if (nf) atomicAdd(ctr + 2, nf);
+    if (nb) atomicAdd(ctr + 1, nb);
+  }
+}
+
+// 3. root: li_root[s] = first row whose key's 6-symbol prefix is >= s;
+// thread r writes the segments that start at row r (r = N: those after
+// the last key)
+__global__ void __launch_bounds__(256) k_li_root(const uint64_t *keys, int64_t N, uint32_t *root) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > N) return;
+  const int64_t s1 = r == N ? (int64_t)1 << kLiRootBits : (int64_t)(keys[r] >> (64 - kLiRootBits));
+  const int64_t s0 = r == 0 ? -1 : (int64_t)(keys[r - 1] >> (64 - kLiRootBits));
+  for (int64_t s = s0 + 1; s <= s1; ++s) root[s] = (uint32_t)r;
+}
+
+// leaf first rows from the routing (monotone in the key)
+__global__ void __launch_bounds__(256) k_li_bounds(DevIndex x, int64_t N, uint32_t *leaf_a) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > N) return;
+  const int64_t j1 = r == N ? kLiLeaves : (int64_t)li_route(x, x.keys[r]);
+  const int64_t j0 = r == 0 ? -1 : (int64_t)li_route(x, x.keys[r - 1]);
+  for (int64_t j = j0 + 1; j <= j1; ++j) leaf_a[j] = (uint32_t)r;
+}
+
+// leaf set-up: first row, key range and knots at the key quantiles
+__global__ void __launch_bounds__(256) k_li_leaf_init(const uint64_t *keys, const uint32_t *leaf_a, LiLeaf *leaf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= kLiLeaves) return;
+  li_leaf_init(keys, leaf_a[j], leaf_a[j + 1], leaf[j]);
+}
+
+__device__ __forceinline__ void li_flush(double *mom, int bin, const double acc[5]) {
+  if (bin < 0) return;
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    if (acc[k] != 0.0) atomicAdd(mom + 5 * (int64_t)bin + k, acc[k]);
+}
+
+// moments (count, sum x, sum x^2, sum y, sum x y) of the distinct keys per
+// (leaf, segment), y = row - first row of the leaf
+__global__ void __launch_bounds__(256) k_li_moments(DevIndex x, const uint32_t *leaf_a, double *mom, int64_t N) {
+  const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kChunk;
+  if (r0 >= N) return;
+  const int64_t r1 = r0 + kChunk < N ? r0 + kChunk : N;
+  int cur = -1;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int64_t r = r0; r < r1; ++r) {
+    const uint64_t K = x.keys[r];
+    if (r > 0 && x.keys[r - 1] == K) continue;
+    const uint32_t j = li_route(x, K);
+    const LiLeaf &L = x.li_leaf[j];
+    const int64_t a = L.a, c = (int64_t)leaf_a[j + 1] - a;
+    const int bin = (int)j * (kLiHinge + 1) + li_segment(a, c, r);
+    if (bin != cur) {
+      li_flush(mom, cur, acc);
+      cur = bin;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) acc[k] = 0.0;
+    }
+    const double v = (double)li_leaf_x(L, K), y = (double)(r - a);
+    acc[0] += 1.0;
+    acc[1] += v;
+    acc[2] += v * v;
+    acc[3] += y;
+    acc[4] += v * y;
+  }
+  li_flush(mom, cur, acc);
+}
+
+// output layer of each leaf (li_solve_leaf)
+__global__ void __launch_bounds__(64) k_li_solve(LiLeaf *leaf, const double *mom) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= kLiLeaves) return;
+  li_solve_leaf(leaf[j], mom + 5 * (int64_t)j * (kLiHinge + 1));
+}
+
+// largest prediction error per leaf over its distinct keys (the same
+// evaluation as the lookup), their sum and the keys a window would miss
+__global__ void __launch_bounds__(256) k_li_err(DevIndex x, LiLeaf *leaf, int64_t N, unsigned long long *ctr) {
+  const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kChunk;
+  if (r0 >= N) return;
+  const int64_t r1 = r0 + kChunk < N ? r0 + kChunk : N;
+  int cur = -1;
+  uint32_t emax = 0;
+  unsigned long long esum = 0, over = 0;
+  for (int64_t r = r0; r < r1; ++r) {
+    const uint64_t K = x.keys[r];
+    if (r > 0 && x.keys[r - 1] == K) continue;
+    const int j = (int)li_route(x, K);
+    const int64_t p = li_predict(x.li_leaf[j], K);
+    const int64_t d = p > r ? p - r : r - p;
+    const uint32_t e = d < 0x7fffffff ? (uint32_t)d : 0x7fffffffu;
+    if (j != cur) {
+      if (cur >= 0) atomicMax(&leaf[cur].err, emax);
+      cur = j;
+      emax = 0;
+    }
+    emax = e > emax ? e : emax;
+    esum += e;
+    over += (e + 2 > (uint32_t)kLiWindowCap) ? 1 : 0;
+  }
+  if (cur >= 0) atomicMax(&leaf[cur].err, emax);
+  if (esum) atomicAdd(ctr + 4, esum);
+  if (over) atomicAdd(ctr + 5, over);
+}
+
+inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+int learned_build(DevIndex x, const LearnedBufs &b, hipStream_t s, LearnedBuildStats &st) {
+  const int64_t N = x.n + 1;
+  if (N >= ((int64_t)1 << 32) || !x.occn) return 1;
+  unsigned long long *ctr = b.ctr;
+  if (hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), s) != hipSuccess) return 2;
+  if (hipMemsetAsync(b.mom, 0, sizeof(double) * 5 * kLiLeaves * (kLiHinge + 1), s) != hipSuccess) return 2;
+  hipLaunchKernelGGL(k_li_fill, dim3(blocks(N, 256)), dim3(256), 0, s, b.saf, N, 0xffffffffu);
+  hipLaunchKernelGGL(k_li_fill, dim3(1), dim3(256), 0, s, b.shortt, (int64_t)(2 * kLiShort), 0xffffffffu);
+  const int64_t n_samples = b.n_samples;
+  hipLaunchKernelGGL(k_li_full_sa, dim3(blocks(n_samples, 256)), dim3(256), 0, s, x, b.saf, n_samples, ctr);
+  hipLaunchKernelGGL(k_li_keys, dim3(blocks(N, 256)), dim3(256), 0, s, x, (const uint32_t *)b.saf, b.keys, b.shortt,
+                     ctr);
+  hipLaunchKernelGGL(k_li_order, dim3(blocks(N, 256)), dim3(256), 0, s, (const uint64_t *)b.keys, N, ctr);
+  hipLaunchKernelGGL(k_li_root, dim3(blocks(N + 1, 256)), dim3(256), 0, s, (const uint64_t *)b.keys, N, b.root);
+  x.saf = b.saf;
+  x.keys = b.keys;
+  x.li_root = b.root;
+  x.li_leaf = b.leaf;
+  x.li_short = b.shortt;
+  x.li_leaf_w = (uint32_t)((N + kLiLeaves - 1) / kLiLeaves);
+  hipLaunchKernelGGL(k_li_bounds, dim3(blocks(N + 1, 256)), dim3(256), 0, s, x, N, b.leaf_a);
+  hipLaunchKernelGGL(k_li_leaf_init, dim3(blocks(kLiLeaves, 256)), dim3(256), 0, s, (const uint64_t *)b.keys,
+                     (const uint32_t *)b.leaf_a, b.leaf);
+  hipLaunchKernelGGL(k_li_moments, dim3(blocks(blocks(N, kChunk), 256)), dim3(256), 0, s, x,
+                     (const uint32_t *)b.leaf_a, b.mom, N);
+  hipLaunchKernelGGL(k_li_solve, dim3(blocks(kLiLeaves, 64)), dim3(64), 0, s, b.leaf, (const double *)b.mom);
+  hipLaunchKernelGGL(k_li_err, dim3(blocks(blocks(N, kChunk), 256)), dim3(256), 0, s, x, b.leaf, N, ctr);
+  if (hipGetLastError() != hipSuccess) return 2;
+  unsigned long long c[8];
+  if (hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, s) != hipSuccess) return 2;
+  std::vector<LiLeaf> leaves(kLiLeaves);
+  if (hipMemcpyAsync(leaves.data(), b.leaf, sizeof(LiLeaf) * kLiLeaves, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return 2;
+  if (hipStreamSynchronize(s) != hipSuccess) return 2;
+  st.unfilled_rows = (int64_t)c[0];
+  st.order_violations = (int64_t)c[1];
+  st.distinct_keys = (int64_t)c[2];
+  st.err_sum = (int64_t)c[4];
+  st.keys_over_window = (int64_t)c[5];
+  st.short_rows = (int64_t)c[6];
+  st.bad_walks = (int64_t)c[7];
+  st.max_err = 0;
+  st.leaves_over_window = 0;
+  for (const LiLeaf &L : leaves) {
+    st.max_err = L.err > (uint32_t)st.max_err ? (int64_t)L.err : st.max_err;
+    st.leaves_over_window += L.err + 2 > (uint32_t)kLiWindowCap ? 1 : 0;
+  }
+  st.leaf_w = x.li_leaf_w;
+  const int64_t want_short = N < kLiShort ? N : kLiShort;
+  if (st.unfilled_rows || st.order_violations || st.bad_walks || st.short_rows != want_short) return 3;
+  return 0;
+}
+
+}  // namespace g3a

@@ -156,6 +156,39 @@ int g3a_set_seed_order(g3a_ctx *ctx, int ordered);
  * the fastest seeding stage and reports it with its time.  The batch's
  * output is identical for every setting.  No reference counterpart. */
 int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms);
+/* Learned-index form.  g3a_set_learned(ctx, 1, info) builds, on the device
+ * and once per context, the full suffix array, the 32-base key of every
+ * suffix-array row and a two-stage FP32 model from a key to its row
+ * (piecewise-linear root over the first 6 bases, 4096 leaves of one hidden
+ * layer of ReLU units), and switches seeding to it: a read's match that
+ * occurs once is extended by comparing it with the text; the last round's
+ * intervals are read from the model's prediction by an exact search of a
+ * window of at most +-256 rows, with the FM-index as the fallback on a window
+ * miss.  Output is identical to the FM-index-only path.  0 switches back
+ * (the built model is kept).  Texts of fewer than 2^32 rows.  No reference
+ * counterpart. */
+typedef struct g3a_learned_info {
+  int64_t rows, distinct_keys;
+  int32_t leaves, hidden_units, window_cap, pad;
+  int64_t model_bytes; /* root + leaves + short-suffix table */
+  int64_t sa_bytes, key_bytes;
+  int64_t max_err;            /* largest |predicted - true row| over the distinct keys */
+  int64_t keys_over_window;   /* distinct keys whose error exceeds the window */
+  int64_t leaves_over_window; /* leaves whose largest error exceeds it */
+  double mean_err;
+  float build_ms, pad2;
+} g3a_learned_info;
+int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info);
+/* Learned-form work of the last g3a_bwa_align: [0] model lookups, [1] window
+ * misses (each answered by the FM-index), [2] text comparisons of unique
+ * matches, [3] bases they covered.  No reference counterpart. */
+int g3a_learned_stats(const g3a_ctx *ctx, int64_t out[4]);
+/* The seeding stage's intervals of the last g3a_bwa_align (SMEMs, reseeded
+ * and last-round seeds, in each read's order): counts[n_reads] per read, and
+ * (k, s, m, n) per interval into kmsn (4 * cap words) unless kmsn is null;
+ * *total = their number.  For parity checks of the seeding forms.  No
+ * reference counterpart (the reference keeps them in d_aux, bwa/src/seed.cu). */
+int g3a_last_intervals(const g3a_ctx *ctx, int32_t *counts, int64_t *kmsn, int64_t cap, int64_t *total);
 const char *g3a_last_error(const g3a_ctx *ctx);
 
 /* Index construction (the `g3a index` command): BWA-format files plus the

@@ -1,0 +1,94 @@
+"""GPU parity of the learned-index form: the device-built full SA, keys and
+two-stage model, the text comparison of unique matches and the model's
+intervals in the last seeding round must give the same seeding intervals
+(k, s, m, n) as the FM-index-only path, and alignments identical to the
+oracle, on uniform, repeat-rich and tandem-repeat references."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import g3sa_amd
+from conftest import oracle_threads
+from test_gpu_parity import _batch, compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _intervals_equal(a, b, label):
+    ca, ia = a
+    cb, ib = b
+    np.testing.assert_array_equal(ca, cb, err_msg=f"{label}: intervals per read differ")
+    if not np.array_equal(ia, ib):
+        bad = np.nonzero((ia != ib).any(axis=1))[0][:5]
+        raise AssertionError(f"{label}: intervals differ at {bad.tolist()}: fm {ia[bad].tolist()} learned {ib[bad].tolist()}")
+
+
+def _run(ds, oracle, seq, off, label, forms=(("batched", 1, 0), ("batched", 0, 10), ("plain", 1, 0))):
+    idx = oracle.load(ds.prefix)
+    try:
+        ref = oracle.align(idx, seq, off, threads=oracle_threads())
+    finally:
+        oracle.lib.orc_index_free(idx)
+    with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(len(off) - 1, 1), max_bases=max(int(off[-1]), 1)) as al:
+        info = None
+        for form, ordered, blocks in forms:
+            al.set_form(form)
+            al.set_seed_order(ordered)
+            al.set_seed_blocks(blocks)
+            al.set_learned(False)
+            print(f"[learned] {label}: {len(off) - 1} reads, FM-only, form {form} order {ordered} blocks {blocks}",
+                  flush=True)
+            got_fm = al.align(seq, off)
+            iv_fm = al.intervals()
+            info = al.set_learned(True)
+            print(f"[learned] {label}: learned form; model {info}", flush=True)
+            got = al.align(seq, off)
+            iv = al.intervals()
+            st = al.learned_stats()
+            print(f"[learned] {label}: work {st}", flush=True)
+            compare(got_fm, ref, f"{label} FM-only [{form}]")
+            compare(got, ref, f"{label} learned [{form}]")
+            _intervals_equal(iv_fm, iv, f"{label} [{form}]")
+            assert got.work["n_seeds"] == ref["work"]["sa_reads"], label
+        return info, st
+
+
+def test_learned_config1(ds_c1, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["main"]))
+    info, st = _run(ds_c1, oracle, seq, off, "config1")
+    assert info["leaves"] == 4096 and info["rows"] == 2 * 1_000_000 + 1
+    assert info["distinct_keys"] > 0.99 * info["rows"]
+    assert info["max_err"] < 256 and info["keys_over_window"] == 0
+    assert info["model_bytes"] < 2 << 20
+    assert st["lookups"] > 0 and st["text_runs"] > 0 and st["window_misses"] == 0
+
+
+def test_learned_config3_style(ds_c1, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["c3"]))
+    _run(ds_c1, oracle, seq, off, "config3-style", forms=(("batched", 1, 0),))
+
+
+def test_learned_repeats(ds_rep, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_rep.reads["main"]))
+    info, st = _run(ds_rep, oracle, seq, off, "repeats", forms=(("batched", 1, 0),))
+    assert info["distinct_keys"] < info["rows"]  # repeated 32-mers share keys
+
+
+def test_learned_edge_cases(ds_c1, oracle):
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["main"]), limit=40)
+    reads = [seq[off[i]:off[i + 1]] for i in range(40)]
+    rng = np.random.default_rng(11)
+    cases = [
+        np.zeros(0, np.uint8),
+        np.full(150, 4, np.uint8),
+        reads[0][:19],
+        reads[1][:20],                                        # one last-round window exactly
+        np.concatenate([reads[2][:40], [4], reads[2][41:]]),  # N inside a unique match
+        np.concatenate([reads[6], reads[7][:170]]),           # 320 bp
+        rng.integers(0, 4, 150).astype(np.uint8),
+        np.tile(np.array([0, 1], np.uint8), 75),
+        np.full(150, 0, np.uint8),                            # poly-A: long runs of equal keys
+    ] + reads[9:40]
+    seq2, off2 = _batch(cases)
+    _run(ds_c1, oracle, seq2, off2, "edge-cases", forms=(("batched", 1, 0),))
