@@ -212,6 +212,8 @@ def main() -> int:
                     help="with --seed-blocks: seed in minimizer order (1) or input order (0)")
     ap.add_argument("--form", default="all", choices=["all", "both", "batched", "plain", "learned"],
                     help="form(s) to time: all = plain, batched, learned; both = plain, batched")
+    ap.add_argument("--learned-mode", type=int, default=1, choices=[1, 2],
+                    help="learned form: 1 = full SA and model lookups, 2 = full SA only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (torch) to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -255,7 +257,7 @@ def main() -> int:
             rows, lb = (int(v) for v in args.tuning.split(","))
             al.set_tuning(rows, lb, 8)
         if form == "learned":
-            learned_info = al.set_learned(True)
+            learned_info = al.set_learned(args.learned_mode)
             log(f"learned index built: {learned_info}")
             if args.seed_blocks is None:
                 al.run()

@@ -198,18 +198,19 @@ class Aligner:
         self._check(self.lib.g3a_autotune_seed(self.ctx, C.byref(b), C.byref(o), C.byref(ms)))
         return {"seed_blocks_per_cu": b.value, "seed_ordered": o.value, "seed_stage_ms": ms.value}
 
-    def set_learned(self, on: bool) -> dict:
-        """Switch seeding to the learned-index form (built on first use) or back;
-        returns the model's figures."""
+    def set_learned(self, on: bool | int) -> dict:
+        """Switch seeding to the learned-index form (built on first use) or back
+        (2: full SA without the model's lookups); returns the model's figures."""
         info = LearnedInfo()
-        self._check(self.lib.g3a_set_learned(self.ctx, int(bool(on)), C.byref(info)))
+        self._check(self.lib.g3a_set_learned(self.ctx, int(on), C.byref(info)))
         return {k: getattr(info, k) for k, _ in LearnedInfo._fields_ if not k.startswith("pad")}
 
     def learned_stats(self) -> dict:
         """Learned-form work of the last batch."""
-        v = (C.c_int64 * 4)()
+        v = (C.c_int64 * 6)()
         self._check(self.lib.g3a_learned_stats(self.ctx, v))
-        return dict(zip(["lookups", "window_misses", "text_runs", "text_run_bases"], list(v)))
+        return dict(zip(["lookups", "fm_answered", "text_runs", "text_run_bases", "model_narrowed",
+                         "model_window_misses"], list(v)))
 
     def intervals(self) -> tuple[np.ndarray, np.ndarray]:
         """Seeding intervals of the last batch: per-read counts and (k, s, m, n) rows."""

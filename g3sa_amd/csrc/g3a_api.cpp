@@ -517,7 +517,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.slow = P<int32_t>(ctx->slow);
   sa.fm_steps = cnt + 0;
   sa.lep_pool = P<Intv>(ctx->lep_pool);
-  sa.li_stats = ctx->idx.li_leaf ? cnt + 32 : nullptr;
+  sa.li_stats = ctx->idx.saf ? cnt + 32 : nullptr;
   sa.order = nullptr;
   if (ctx->seed_ordered && n > 1) {
     launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(ctx->seed_key), P<int32_t>(ctx->seed_idx), s);
@@ -842,7 +842,7 @@ int g3a_set_seed_order(g3a_ctx *ctx, int ordered) {
 }
 
 int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
-  if (!ctx || (on != 0 && on != 1)) return G3A_E_ARG;
+  if (!ctx || on < 0 || on > 2) return G3A_E_ARG;
   if (!ctx->index_on_device) return fail(ctx, G3A_E_STATE, "index not on device");
   DevIndex &x = ctx->idx;
   if (!on) {
@@ -911,17 +911,17 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
   x.saf = P<uint32_t>(ctx->li_saf);
   x.keys = P<uint64_t>(ctx->li_keys);
   x.li_root = P<uint32_t>(ctx->li_root);
-  x.li_leaf = P<LiLeaf>(ctx->li_leaf);
+  x.li_leaf = on == 1 ? P<LiLeaf>(ctx->li_leaf) : nullptr;  // 2: no model lookups
   x.li_short = P<uint32_t>(ctx->li_short);
   x.li_leaf_w = ctx->li_leaf_w;
   if (info) *info = ctx->li_info;
   return G3A_OK;
 }
 
-int g3a_learned_stats(const g3a_ctx *ctx, int64_t out[4]) {
+int g3a_learned_stats(const g3a_ctx *ctx, int64_t out[6]) {
   if (!ctx || !out) return G3A_E_ARG;
   if (hipSetDevice(ctx->device) != hipSuccess ||
-      hipMemcpy(out, (const int64_t *)ctx->counters.p + 32, 4 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      hipMemcpy(out, (const int64_t *)ctx->counters.p + 32, 6 * 8, hipMemcpyDeviceToHost) != hipSuccess)
     return G3A_E_HIP;
   return G3A_OK;
 }

@@ -448,46 +448,98 @@ G3A_HD int64_t li_lower_bound(const DevIndex &x, uint64_t K) {
   return lo;
 }
 
+// Exact search in [lo, hi): first row whose key is >= K, or hi
+G3A_HD int64_t li_search(const DevIndex &x, uint64_t K, int64_t lo, int64_t hi) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (x.keys[mid] < K) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Buckets of more rows than this are narrowed by the model's window
+constexpr int64_t kLiModelMin = 64;
+
 // Rows [k, k + s) of the suffixes that start with the j-symbol string W
 // (first symbol in the highest bits, the rest of the word 0; 1 <= j <= 32):
-// the interval the FM-index's backward search gives for it.  False when a
-// search window misses.
-G3A_HD bool li_range(const DevIndex &x, uint64_t W, int j, uint32_t &k, uint32_t &s) {
+// the interval the FM-index's backward search gives for it.  For j >= kKmer
+// the k-mer table's interval of W's first kKmer symbols bounds the search
+// exactly; in a bucket of more than kLiModelMin rows the model's window
+// (checked against the keys at its edges) narrows it further.  Without the
+// table the model's window alone is searched, and false is returned when it
+// misses.  lc (optional): [4] lookups the model narrowed, [5] model windows
+// that missed.
+G3A_HD bool li_range(const DevIndex &x, uint64_t W, int j, uint32_t &k, uint32_t &s, int64_t *lc = nullptr) {
   const int64_t N = x.n + 1;
-  const int64_t k0 = li_lower_bound(x, W);
-  if (k0 < 0) return false;
   const uint64_t unit = (uint64_t)1 << (64 - 2 * j);
   const uint64_t Wn = W + unit;  // first key past the prefix; 0 when W is all T
+  int64_t lo = 0, hi = N;        // lb(W) lies in [lo, hi]; rows >= hi do not start with W
+  int64_t bend = N;
+  bool bounded = false;
+  if (x.kmer && j >= kKmer) {
+    const uint32_t *e = x.kmer + 3 * (uint64_t)(W >> (64 - 2 * kKmer));
+    lo = e[0];
+    hi = lo + e[2];
+    bend = hi;
+    bounded = true;
+    if (lo == hi) {  // the first kKmer symbols do not occur
+      k = (uint32_t)lo;
+      s = 0;
+      return true;
+    }
+  }
+  if (!bounded || hi - lo > kLiModelMin) {
+    const LiLeaf &L = x.li_leaf[li_route(x, W)];
+    const int64_t p = li_predict(L, W);
+    int64_t E = (int64_t)L.err + 2;
+    if (E > kLiWindowCap) E = kLiWindowCap;
+    int64_t wl = p - E, wh = p + E + 1;
+    wl = wl < lo ? lo : wl;
+    wh = wh > hi ? hi : wh;
+    const bool ok = wl < wh && (wl == lo || x.keys[wl - 1] < W) && (wh == hi || x.keys[wh] >= W);
+    if (ok) {
+      lo = wl;
+      hi = wh;
+      if (lc) ++lc[4];
+    } else {
+      if (lc) ++lc[5];
+      if (!bounded) return false;
+    }
+  }
+  const int64_t k0 = li_search(x, W, lo, hi);
+  // end of the run: gallop from k0 over rows k0 + 2^i - 1, then bisect
   int64_t k1;
+  const int64_t end = bend;
   if (Wn == 0) {
-    k1 = N;
+    k1 = end;
   } else {
-    // gallop from k0 over rows k0 + 2^i - 1, then bisect
     int64_t prev = k0 - 1, r = k0, step = 1;
-    while (r < N && x.keys[r] < Wn && step < 1024) {
+    while (r < end && x.keys[r] < Wn && step < 1024) {
       prev = r;
       step <<= 1;
       r = k0 + step - 1;
     }
-    if (r < N && x.keys[r] < Wn) {
-      k1 = li_lower_bound(x, Wn);
-      if (k1 < 0) return false;
-    } else {
-      int64_t lo = prev + 1, hi = r < N ? r : N;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (x.keys[mid] < Wn) lo = mid + 1;
-        else hi = mid;
+    if (r < end && x.keys[r] < Wn) {
+      if (bounded) {
+        k1 = li_search(x, Wn, r + 1, end);
+      } else {
+        k1 = li_lower_bound(x, Wn);
+        if (k1 < 0) return false;
       }
-      k1 = lo;
+    } else {
+      k1 = li_search(x, Wn, prev + 1, r < end ? r : end);
     }
   }
   // suffixes shorter than j whose zero-padded key starts with W do not
-  // start with W; they sort first in [k0, k1) ('$' is below every base)
+  // start with W; they sort first in [k0, k1) ('$' is below every base).
+  // Their padding makes W's last symbol A, so only then is the table read.
   int64_t kk = k0;
-  for (int t = 0; t < kLiShort; ++t) {
-    const int64_t r = x.li_short[2 * t];
-    if (r >= k0 && r < k1 && (int)x.li_short[2 * t + 1] < j) ++kk;
+  if (((W >> (64 - 2 * j)) & 3) == 0 && k1 > k0) {
+    for (int t = 0; t < kLiShort; ++t) {
+      const int64_t r = x.li_short[2 * t];
+      if (r >= k0 && r < k1 && (int)x.li_short[2 * t + 1] < j) ++kk;
+    }
   }
   k = (uint32_t)kk;
   s = (uint32_t)(k1 - kk);
@@ -794,8 +846,27 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     intv_init(x, ik, q[x0]);
     ik.m = x0;
     ik.n = x0;
-    int i;
-    for (i = x0 + 1; i < len; ++i) {
+    int i = x0 + 1;
+    if (kLearn && x0 == 0 && x.kmer && len >= kKmer) {
+      // the pass from 0 has no backward extensions: only its longest match
+      // is used, so it may start from the k-mer table's interval of
+      // read[0, kKmer) when that occurs (the shorter prefixes' entries
+      // would never be read)
+      uint32_t code = 0;
+      int t = 0;
+      for (; t < kKmer && q[t] < 4; ++t) code = code << 2 | q[t];
+      if (t == kKmer) {
+        const uint32_t *e = x.kmer + 3 * (uint64_t)code;
+        if (e[2] > 0) {
+          ik.k = (P)e[0];
+          ik.l = (P)e[1];
+          ik.s = (P)e[2];
+          ik.n = kKmer - 1;
+          i = kKmer;
+        }
+      }
+    }
+    for (; i < len; ++i) {
       if (kLearn && x.saf && ik.s == 1) {
         // one occurrence, at saf[k]: the forward extensions keep s = 1 for
         // as long as the read matches the text after it, so compare the two
@@ -1026,7 +1097,7 @@ G3A_HD void reseed_round3(const DevIndex &x, const Opt &o, const uint8_t *q, int
       uint32_t kk, ss;
       if (t == min_len) {
         if (lc) ++lc[0];
-        if (li_range(x, W, min_len, kk, ss)) {
+        if (li_range(x, W, min_len, kk, ss, lc)) {
           if ((int64_t)ss < max_intv) {
             if (ss > 0) {
               IntvT<P> sd;

@@ -28,7 +28,7 @@ struct SeedArgs {
   Intv *lep_pool;    // kLepCap + kFoundCap entries per k_seed thread
   const int32_t *order;  // reads in processing order, or null for input order
   int lep_lds;           // LEP entries per thread kept in LDS (narrow form; set by launch_seed)
-  int64_t *li_stats;     // learned-index form counters (4), or null
+  int64_t *li_stats;     // learned-index form counters (6), or null
 };
 
 struct SaArgs {

@@ -41,8 +41,9 @@ template <bool L, class B>
 __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nthreads) {
   using I = typename B::I;
   int64_t steps = 0;
-  // learned-index form: range lookups, window misses, text runs, run bases
-  int64_t lc[4] = {0, 0, 0, 0};
+  // learned-index form: range lookups, lookups answered by the FM-index,
+  // text runs, run bases, lookups the model narrowed, model windows missed
+  int64_t lc[6] = {0, 0, 0, 0, 0, 0};
   for (int i = gtid; i < a.n_reads; i += nthreads) {
     const int r = a.order ? a.order[i] : i;
     const int off = a.off[r], len = a.off[r + 1] - off;
@@ -77,7 +78,7 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
   }
   if (L && a.li_stats) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 6; ++k) {
       int64_t v = lc[k];
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -90,7 +91,7 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
 // model's intervals in the last round); a separate kernel, so that the
 // FM-only one keeps its register budget
 template <class E, bool L>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_seed(SeedArgs a) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(L ? 5 : 4))) k_seed(SeedArgs a) {
   extern __shared__ uint4 sb_lds[];
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
@@ -894,7 +895,7 @@ void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_
   const int need = blocks(a.n_reads, 128), have = n_threads / 128;
   int nb = need < have ? need : have;
   if (seed_narrow(a.idx)) {
-    void (*kern)(SeedArgs) = a.idx.li_leaf ? k_seed<LepNarrow, true> : k_seed<LepNarrow, false>;
+    void (*kern)(SeedArgs) = a.idx.saf ? k_seed<LepNarrow, true> : k_seed<LepNarrow, false>;
     // The compute unit's LDS (160 KB, at most 64 KB per block) is shared by
     // the blocks it is to hold; the LEP entries per thread are the most that
     // still lets that many blocks be resident (a block that is not resident

@@ -162,11 +162,14 @@ int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *bes
  * (piecewise-linear root over the first 6 bases, 4096 leaves of one hidden
  * layer of ReLU units), and switches seeding to it: a read's match that
  * occurs once is extended by comparing it with the text; the last round's
- * intervals are read from the model's prediction by an exact search of a
- * window of at most +-256 rows, with the FM-index as the fallback on a window
- * miss.  Output is identical to the FM-index-only path.  0 switches back
- * (the built model is kept).  Texts of fewer than 2^32 rows.  No reference
- * counterpart. */
+ * intervals are read from the sorted keys by an exact search inside the
+ * 12-mer's interval (the k-mer table), narrowed in large buckets to the
+ * model's window of at most +-256 rows around its prediction (checked at its
+ * edges; a miss searches the whole bucket), with the FM-index as the
+ * fallback where no bucket is known.  Output is identical to the FM-index-only path.  0 switches back
+ * (the built model is kept); 2 uses the full suffix array (text comparison,
+ * SA lookup) without the model's lookups, to measure what they add.  Texts
+ * of fewer than 2^32 rows.  No reference counterpart. */
 typedef struct g3a_learned_info {
   int64_t rows, distinct_keys;
   int32_t leaves, hidden_units, window_cap, pad;
@@ -179,10 +182,12 @@ typedef struct g3a_learned_info {
   float build_ms, pad2;
 } g3a_learned_info;
 int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info);
-/* Learned-form work of the last g3a_bwa_align: [0] model lookups, [1] window
- * misses (each answered by the FM-index), [2] text comparisons of unique
- * matches, [3] bases they covered.  No reference counterpart. */
-int g3a_learned_stats(const g3a_ctx *ctx, int64_t out[4]);
+/* Learned-form work of the last g3a_bwa_align: [0] interval lookups,
+ * [1] lookups answered by the FM-index instead, [2] text comparisons of
+ * unique matches, [3] bases they covered, [4] lookups whose search the
+ * model's window narrowed, [5] model windows that missed (the lookup then
+ * searched its whole k-mer bucket).  No reference counterpart. */
+int g3a_learned_stats(const g3a_ctx *ctx, int64_t out[6]);
 /* The seeding stage's intervals of the last g3a_bwa_align (SMEMs, reseeded
  * and last-round seeds, in each read's order): counts[n_reads] per read, and
  * (k, s, m, n) per interval into kmsn (4 * cap words) unless kmsn is null;

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace g3a;
@@ -184,8 +185,9 @@ int main(int argc, char **argv) {
   }
 
   std::mt19937_64 rng(12345);
+  int64_t rmiss_long = 0;  // range misses of strings of kKmer symbols or more
   auto check_lookups = [&](bool trained, int64_t &misses, int64_t &ranges, int64_t &range_miss) -> int {
-    misses = ranges = range_miss = 0;
+    misses = ranges = range_miss = rmiss_long = 0;
     for (int64_t r = 0; r < N; ++r) {
       if (r && M.keys[r - 1] == M.keys[r]) continue;
       const int64_t lb = li_lower_bound(x, M.keys[r]);
@@ -220,14 +222,30 @@ int main(int argc, char **argv) {
       ++ranges;
       if (!li_range(x, W, j, k2, s2)) {
         ++range_miss;
+        rmiss_long += j >= kKmer;
         continue;
       }
       if (s1 != s2 || (s1 && k1 != k2)) return fail("range", i, j, (long long)s1 - (long long)s2);
     }
     return 0;
   };
-  int64_t miss_t, ranges_t, rmiss_t, miss_z, ranges_z, rmiss_z;
+  int64_t miss_t, ranges_t, rmiss_t, miss_z, ranges_z, rmiss_z, miss_b, ranges_b, rmiss_b;
+  // the model's windows alone (no k-mer table)
   if (check_lookups(true, miss_t, ranges_t, rmiss_t)) return 1;
+  // with the k-mer table, as the product runs: strings of kKmer symbols or
+  // more are searched inside their k-mer's interval and never miss
+  std::vector<uint32_t> kmer_table((size_t)3 << (2 * kKmer));
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < 8; ++t)
+      th.emplace_back([&, t] {
+        for (uint32_t c = t; c < (1u << (2 * kKmer)); c += 8) kmer_interval(x, c, &kmer_table[3 * (size_t)c]);
+      });
+    for (auto &t : th) t.join();
+  }
+  x.kmer = kmer_table.data();
+  if (check_lookups(true, miss_b, ranges_b, rmiss_b)) return 1;
+  if (rmiss_long) return fail("bucketed miss", rmiss_long, 0, 0);
 
   // seeding of simulated reads, plain and learned
   Opt o;
@@ -242,7 +260,7 @@ int main(int argc, char **argv) {
   xp.li_root = nullptr;
   xp.li_leaf = nullptr;
   xp.li_short = nullptr;
-  int64_t lc[4] = {0, 0, 0, 0}, n_iv = 0, steps_p = 0, steps_l = 0;
+  int64_t lc[6] = {0, 0, 0, 0, 0, 0}, n_iv = 0, steps_p = 0, steps_l = 0;
   for (int i = 0; i < n_reads; ++i) {
     const int len = 70 + (int)(rng() % 181);
     std::vector<uint8_t> q((size_t)len);
@@ -278,14 +296,16 @@ int main(int argc, char **argv) {
     L.err = 0;
   }
   if (check_lookups(false, miss_z, ranges_z, rmiss_z)) return 1;
+  if (rmiss_long) return fail("zeroed bucketed miss", rmiss_long, 0, 0);
 
   std::printf("ok rows=%lld distinct=%lld max_err=%lld mean_err=%.2f over_window=%lld lb_miss=%lld "
               "range_checked=%lld range_miss=%lld zeroed_lb_miss=%lld zeroed_range_miss=%lld reads=%d "
               "intervals=%lld lookups=%lld lookup_miss=%lld text_runs=%lld run_bases=%lld steps_plain=%lld "
-              "steps_learned=%lld\n",
+              "steps_learned=%lld model_narrowed=%lld model_missed=%lld bucketed_range_miss=%lld\n",
               (long long)N, (long long)distinct, (long long)emax, distinct ? (double)esum / distinct : 0.0,
               (long long)over, (long long)miss_t, (long long)ranges_t, (long long)rmiss_t, (long long)miss_z,
               (long long)rmiss_z, n_reads, (long long)n_iv, (long long)lc[0], (long long)lc[1], (long long)lc[2],
-              (long long)lc[3], (long long)steps_p, (long long)steps_l);
+              (long long)lc[3], (long long)steps_p, (long long)steps_l, (long long)lc[4], (long long)lc[5],
+              (long long)rmiss_b);
   return 0;
 }

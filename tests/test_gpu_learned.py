@@ -24,7 +24,7 @@ def _intervals_equal(a, b, label):
         raise AssertionError(f"{label}: intervals differ at {bad.tolist()}: fm {ia[bad].tolist()} learned {ib[bad].tolist()}")
 
 
-def _run(ds, oracle, seq, off, label, forms=(("batched", 1, 0), ("batched", 0, 10), ("plain", 1, 0))):
+def _run(ds, oracle, seq, off, label, forms=(("batched", 1, 0), ("batched", 0, 10), ("plain", 1, 0)), modes=(1,)):
     idx = oracle.load(ds.prefix)
     try:
         ref = oracle.align(idx, seq, off, threads=oracle_threads())
@@ -41,27 +41,30 @@ def _run(ds, oracle, seq, off, label, forms=(("batched", 1, 0), ("batched", 0, 1
                   flush=True)
             got_fm = al.align(seq, off)
             iv_fm = al.intervals()
-            info = al.set_learned(True)
-            print(f"[learned] {label}: learned form; model {info}", flush=True)
-            got = al.align(seq, off)
-            iv = al.intervals()
-            st = al.learned_stats()
-            print(f"[learned] {label}: work {st}", flush=True)
             compare(got_fm, ref, f"{label} FM-only [{form}]")
-            compare(got, ref, f"{label} learned [{form}]")
-            _intervals_equal(iv_fm, iv, f"{label} [{form}]")
-            assert got.work["n_seeds"] == ref["work"]["sa_reads"], label
+            for mode in modes:
+                info = al.set_learned(mode)
+                print(f"[learned] {label}: learned form (mode {mode}); model {info}", flush=True)
+                got = al.align(seq, off)
+                iv = al.intervals()
+                st = al.learned_stats()
+                print(f"[learned] {label}: work {st}", flush=True)
+                compare(got, ref, f"{label} learned mode {mode} [{form}]")
+                _intervals_equal(iv_fm, iv, f"{label} mode {mode} [{form}]")
+                assert got.work["n_seeds"] == ref["work"]["sa_reads"], label
+            al.set_learned(1)
+            st = al.learned_stats() if modes[-1] == 1 else st
         return info, st
 
 
 def test_learned_config1(ds_c1, oracle):
     names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["main"]))
-    info, st = _run(ds_c1, oracle, seq, off, "config1")
+    info, st = _run(ds_c1, oracle, seq, off, "config1", modes=(2, 1))
     assert info["leaves"] == 4096 and info["rows"] == 2 * 1_000_000 + 1
     assert info["distinct_keys"] > 0.99 * info["rows"]
     assert info["max_err"] < 256 and info["keys_over_window"] == 0
     assert info["model_bytes"] < 2 << 20
-    assert st["lookups"] > 0 and st["text_runs"] > 0 and st["window_misses"] == 0
+    assert st["lookups"] > 0 and st["text_runs"] > 0 and st["fm_answered"] == 0
 
 
 def test_learned_config3_style(ds_c1, oracle):

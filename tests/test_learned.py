@@ -16,7 +16,7 @@ from conftest import ROOT, gen
 @pytest.fixture(scope="module")
 def checker(tmp_path_factory):
     out = tmp_path_factory.mktemp("learned") / "learned_check"
-    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", str(ROOT), "-o", str(out),
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", str(ROOT), "-o", str(out),
                     str(ROOT / "tests" / "learned_check.cpp"), str(ROOT / "g3sa_amd" / "csrc" / "index_io.cpp")],
                    check=True, capture_output=True)
     return out
