@@ -48,7 +48,8 @@ HBM_PEAK_GBS = 8000.0
 # k_seed's memory-side bytes per launch on the 100 Mbp / 2M-read workload,
 # from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE), each scaled by the
 # same counter read on a plain copy kernel of known size
-TRAFFIC_FILE = ROOT / "profiles" / "k_seed_traffic_100Mbp_2Mx150.txt"
+TRAFFIC_FILES = {"batched": ROOT / "profiles" / "k_seed_traffic_100Mbp_2Mx150.txt",
+                 "learned": ROOT / "profiles" / "k_seed_learned_traffic_100Mbp_2Mx150.txt"}
 
 CONFIGS = {
     "c2": dict(
@@ -162,12 +163,13 @@ def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = Tr
     }
 
 
-def seed_traffic(config: str) -> int | None:
+def seed_traffic(config: str, form: str) -> int | None:
     """Bytes per k_seed launch recorded from the counter passes, or None."""
-    if config != "c2" or not TRAFFIC_FILE.exists():
+    f = TRAFFIC_FILES.get(form)
+    if config != "c2" or f is None or not f.exists():
         return None
     kv = {}
-    for line in TRAFFIC_FILE.read_text().splitlines():
+    for line in f.read_text().splitlines():
         k, sep, v = line.partition(":")
         if sep:
             kv[k.strip()] = v.strip()
@@ -177,17 +179,18 @@ def seed_traffic(config: str) -> int | None:
         return None
 
 
-def end_to_end(prefix: Path, fq: Path) -> dict | None:
+def end_to_end(prefix: Path, fq: Path, learned: bool) -> dict | None:
     """`g3a mem` over the whole FASTQ (parse, align, format, write to /dev/null),
     timed by the driver from the first read parsed to the last line written."""
-    r = subprocess.run([str(g3sa_amd.CLI_PATH), "mem", "-S", "-t", "16", "-o", "/dev/null", str(prefix), str(fq)],
-                       capture_output=True, text=True)
+    cmd = [str(g3sa_amd.CLI_PATH), "mem", "-S", "-t", "16", "-o", "/dev/null"] + (["--learned"] if learned else [])
+    r = subprocess.run(cmd + [str(prefix), str(fq)], capture_output=True, text=True)
     m = re.search(r"(\d+) reads in ([0-9.]+) s \(([0-9.]+) reads/s end to end\)", r.stderr)
     if r.returncode != 0 or not m:
         log(f"g3a mem failed: {r.stderr[-300:]}")
         return None
     out = {"reads_per_s": float(m.group(3)), "reads": int(m.group(1)), "seconds": float(m.group(2)),
-           "what": "g3a mem -S, standard SAM to /dev/null, batches of 250k reads, index load excluded"}
+           "what": "g3a mem -S" + (" --learned" if learned else "") + ", standard SAM to /dev/null, batches of "
+                   "250k reads, index load (and learned-index build) excluded"}
     b = re.search(r"busy time: load ([0-9.]+) s, align ([0-9.]+) s, format\+write ([0-9.]+) s", r.stderr)
     if b:
         out["stage_busy_s"] = {"load": float(b.group(1)), "align": float(b.group(2)), "format_write": float(b.group(3))}
@@ -283,7 +286,7 @@ def main() -> int:
     seed_gbs = seed_bytes / (kms["k_seed"] * 1e-3) / 1e9 if kms["k_seed"] > 0 else 0.0
     dominant = "k_seed" if kms["k_seed"] >= ext_ms else "extension"
     roof = {"bound": "hbm", "kernel": "k_seed", "achieved": round(seed_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(seed_gbs / HBM_PEAK_GBS, 4), "traffic": seed_traffic(args.config),
+            "frac": round(seed_gbs / HBM_PEAK_GBS, 4), "traffic": seed_traffic(args.config, main_form),
             "algorithmic_bytes": seed_bytes, "launch_ms": round(kms["k_seed"], 3)}
     if dominant != "k_seed":
         roof["dominant"] = dominant
@@ -329,7 +332,7 @@ def main() -> int:
     al.close()
     if not args.no_e2e:
         log("end-to-end: g3a mem")
-        result["end_to_end"] = end_to_end(prefix, fq)
+        result["end_to_end"] = end_to_end(prefix, fq, main_form == "learned")
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(prefix, fq, min(args.cpu_sample, n))
     print(json.dumps(result), flush=True)

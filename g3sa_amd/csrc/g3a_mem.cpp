@@ -7,8 +7,9 @@
 // or, with -S, standard SAM.  Output is in input order.
 //
 // Options: -o FILE output, -Z N reads per batch, -S standard SAM, -g N GPUs
-// (1 only), -t N formatting threads; -b -v -i -z -l -p -F -L -m are accepted
-// for compatibility with the reference's command line and ignored.
+// (1 only), -t N formatting threads, --learned the learned-index seeding form
+// (same output); -b -v -i -z -l -p -F -L -m are accepted for compatibility
+// with the reference's command line and ignored.
 #include "../../include/g3a.h"
 #include "g3a_index.h"
 
@@ -563,7 +564,7 @@ int g3a_mem_main(int argc, char **argv) {
   mallopt(M_TRIM_THRESHOLD, 1 << 30);
   const char *out_fn = nullptr;
   int batch = 250000, gpus = 1, threads = 8;
-  bool sam = false;
+  bool sam = false, learned = false;
   int i = 1;
   for (; i < argc && argv[i][0] == '-' && argv[i][1]; ++i) {
     const char *a = argv[i];
@@ -573,6 +574,7 @@ int g3a_mem_main(int argc, char **argv) {
     else if (!std::strcmp(a, "-g")) { const char *v = val(); gpus = v ? std::atoi(v) : gpus; }
     else if (!std::strcmp(a, "-t")) { const char *v = val(); threads = v ? std::atoi(v) : threads; }
     else if (!std::strcmp(a, "-S")) sam = true;
+    else if (!std::strcmp(a, "--learned")) learned = true;
     else if (!std::strcmp(a, "-b")) {}
     else if (!std::strcmp(a, "-v") || !std::strcmp(a, "-i") || !std::strcmp(a, "-z") || !std::strcmp(a, "-l") ||
              !std::strcmp(a, "-p") || !std::strcmp(a, "-F") || !std::strcmp(a, "-L") || !std::strcmp(a, "-m"))
@@ -583,7 +585,8 @@ int g3a_mem_main(int argc, char **argv) {
     }
   }
   if (i + 2 > argc) {
-    std::fprintf(stderr, "usage: g3a mem [-o out] [-Z reads_per_batch] [-S] [-t threads] <prefix> <reads.fq>\n");
+    std::fprintf(stderr,
+                 "usage: g3a mem [-o out] [-Z reads_per_batch] [-S] [-t threads] [--learned] <prefix> <reads.fq>\n");
     return 1;
   }
   if (gpus != 1) {
@@ -626,6 +629,20 @@ int g3a_mem_main(int argc, char **argv) {
       std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(c));
       destroy_all();
       return 1;
+    }
+    if (learned) {
+      g3a_learned_info li;
+      if ((rc = g3a_set_learned(c, 1, &li))) {
+        std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(c));
+        destroy_all();
+        return 1;
+      }
+      if (k == 0)
+        std::fprintf(stderr,
+                     "g3a mem: learned index: %d leaves x %d hidden units, model %lld bytes, max error %lld rows "
+                     "(mean %.1f), %lld keys beyond the %d-row window, built in %.0f ms\n",
+                     li.leaves, li.hidden_units, (long long)li.model_bytes, (long long)li.max_err, li.mean_err,
+                     (long long)li.keys_over_window, li.window_cap, li.build_ms);
     }
   }
   FILE *out = out_fn ? std::fopen(out_fn, "wb") : stdout;

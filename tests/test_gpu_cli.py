@@ -57,6 +57,9 @@ def test_cli_byte_identical(ds_c1, tmp_path, mode):
     print(f"[cli] {mode}: 997-byte input chunks", flush=True)
     got = gpu_out(ds_c1, fq, flags + ["-Z", "1500"], tmp_path, f"{mode}_chunks", chunk=997)
     assert got == ref, first_diff(got, ref)
+    print(f"[cli] {mode}: learned-index form", flush=True)
+    got = gpu_out(ds_c1, fq, flags + ["-Z", "1500", "--learned"], tmp_path, f"{mode}_learned")
+    assert got == ref, first_diff(got, ref)
 
 
 def test_cli_repeats_sam(ds_rep, tmp_path):
@@ -65,5 +68,8 @@ def test_cli_repeats_sam(ds_rep, tmp_path):
     print("[cli] repeats: sam", flush=True)
     got = gpu_out(ds_rep, fq, ["-S", "-Z", "5000"], tmp_path, "rep_sam")
     assert got == ref, first_diff(got, ref)
+    print("[cli] repeats: sam, learned-index form", flush=True)
+    got_l = gpu_out(ds_rep, fq, ["-S", "-Z", "5000", "--learned"], tmp_path, "rep_sam_learned")
+    assert got_l == ref, first_diff(got_l, ref)
     lines = [ln for ln in got.split(b"\n") if ln and not ln.startswith(b"@")]
     assert len(lines) >= 12000
