@@ -858,7 +858,7 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
   if (!ctx->li_built) {
     HIPCK(hipSetDevice(ctx->device));
     const int64_t N = x.n + 1;
-    if (ensure(ctx, ctx->li_saf, (size_t)N * 4) || ensure(ctx, ctx->li_keys, (size_t)N * 8) ||
+    if (ensure(ctx, ctx->li_saf, (size_t)N * 4) || ensure(ctx, ctx->li_keys, (size_t)(N + 16) * 8) ||
         ensure(ctx, ctx->li_root, ((size_t)1 << kLiRootBits) * 4 + 4) ||
         ensure(ctx, ctx->li_leaf, sizeof(LiLeaf) * kLiLeaves) || ensure(ctx, ctx->li_leaf_a, (kLiLeaves + 1) * 4) ||
         ensure(ctx, ctx->li_mom, sizeof(double) * 5 * kLiLeaves * (kLiHinge + 1)) ||

@@ -461,6 +461,23 @@ G3A_HD int64_t li_search(const DevIndex &x, uint64_t K, int64_t lo, int64_t hi) 
 // Buckets of more rows than this are narrowed by the model's window
 constexpr int64_t kLiModelMin = 64;
 
+// Rows [k0, k1) hold the keys that start with the j-symbol string W.
+// Suffixes shorter than j whose zero-padded key starts with W do not start
+// with W; they sort first in [k0, k1) ('$' is below every base).  Their
+// padding makes W's last symbol A, so only then is their table read.
+G3A_HD bool li_finish(const DevIndex &x, uint64_t W, int j, int64_t k0, int64_t k1, uint32_t &k, uint32_t &s) {
+  int64_t kk = k0;
+  if (((W >> (64 - 2 * j)) & 3) == 0 && k1 > k0) {
+    for (int t = 0; t < kLiShort; ++t) {
+      const int64_t r = x.li_short[2 * t];
+      if (r >= k0 && r < k1 && (int)x.li_short[2 * t + 1] < j) ++kk;
+    }
+  }
+  k = (uint32_t)kk;
+  s = (uint32_t)(k1 - kk);
+  return true;
+}
+
 // Rows [k, k + s) of the suffixes that start with the j-symbol string W
 // (first symbol in the highest bits, the rest of the word 0; 1 <= j <= 32):
 // the interval the FM-index's backward search gives for it.  For j >= kKmer
@@ -531,19 +548,7 @@ G3A_HD bool li_range(const DevIndex &x, uint64_t W, int j, uint32_t &k, uint32_t
       k1 = li_search(x, Wn, prev + 1, r < end ? r : end);
     }
   }
-  // suffixes shorter than j whose zero-padded key starts with W do not
-  // start with W; they sort first in [k0, k1) ('$' is below every base).
-  // Their padding makes W's last symbol A, so only then is the table read.
-  int64_t kk = k0;
-  if (((W >> (64 - 2 * j)) & 3) == 0 && k1 > k0) {
-    for (int t = 0; t < kLiShort; ++t) {
-      const int64_t r = x.li_short[2 * t];
-      if (r >= k0 && r < k1 && (int)x.li_short[2 * t + 1] < j) ++kk;
-    }
-  }
-  k = (uint32_t)kk;
-  s = (uint32_t)(k1 - kk);
-  return true;
+  return li_finish(x, W, j, k0, k1, k, s);
 }
 
 // Full-SA construction step: from sampled row j << sa_shift, whose value

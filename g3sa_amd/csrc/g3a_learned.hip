@@ -63,13 +63,21 @@ __global__ void __launch_bounds__(256) k_li_keys(DevIndex x, const uint32_t *saf
   }
 }
 
-// order check (keys must be non-decreasing) and distinct-key count
+// order check (keys must be non-decreasing) and distinct-key count: a
+// grid-stride loop with per-thread counts, one atomic per wavefront at the
+// end (an atomic per wavefront per 64 rows serialises on the two counters)
 __global__ void __launch_bounds__(256) k_li_order(const uint64_t *keys, int64_t N, unsigned long long *ctr) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= N) return;
-  const bool first = r == 0 || keys[r - 1] != keys[r];
-  const bool bad = r > 0 && keys[r - 1] > keys[r];
-  const unsigned long long nf = __popcll(__ballot(first)), nb = __popcll(__ballot(bad));
+  unsigned long long nf = 0, nb = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[r], kp = r > 0 ? keys[r - 1] : 0;
+    nf += (r == 0 || kp != k) ? 1 : 0;
+    nb += (r > 0 && kp > k) ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    nf += __shfl_xor(nf, o);
+    nb += __shfl_xor(nb, o);
+  }
   if ((threadIdx.x & 63) == 0) {
     if (nf) atomicAdd(ctr + 2, nf);
     if (nb) atomicAdd(ctr + 1, nb);
@@ -194,7 +202,7 @@ int learned_build(DevIndex x, const LearnedBufs &b, hipStream_t s, LearnedBuildS
   hipLaunchKernelGGL(k_li_full_sa, dim3(blocks(n_samples, 256)), dim3(256), 0, s, x, b.saf, n_samples, ctr);
   hipLaunchKernelGGL(k_li_keys, dim3(blocks(N, 256)), dim3(256), 0, s, x, (const uint32_t *)b.saf, b.keys, b.shortt,
                      ctr);
-  hipLaunchKernelGGL(k_li_order, dim3(blocks(N, 256)), dim3(256), 0, s, (const uint64_t *)b.keys, N, ctr);
+  hipLaunchKernelGGL(k_li_order, dim3(4096), dim3(256), 0, s, (const uint64_t *)b.keys, N, ctr);
   hipLaunchKernelGGL(k_li_root, dim3(blocks(N + 1, 256)), dim3(256), 0, s, (const uint64_t *)b.keys, N, b.root);
   x.saf = b.saf;
   x.keys = b.keys;

@@ -120,7 +120,7 @@ int main(int argc, char **argv) {
     const uint64_t c = (h.pac[(size_t)(f >> 2)] >> ((~f & 3) << 1)) & 3;
     return p < x.l_pac ? c : 3 - c;
   };
-  M.keys.resize((size_t)N);
+  M.keys.resize((size_t)N + 16);  // padded as on the device
   for (int64_t r = 0; r < N; ++r) {
     const int64_t p = M.saf[r];
     uint64_t w = 0;
@@ -203,7 +203,7 @@ int main(int argc, char **argv) {
       uint64_t K = rng();
       if (i & 1) K = M.keys[rng() % N] + (rng() % 3) - 1;  // near a text key
       const int64_t lb = li_lower_bound(x, K);
-      const int64_t want = std::lower_bound(M.keys.begin(), M.keys.end(), K) - M.keys.begin();
+      const int64_t want = std::lower_bound(M.keys.begin(), M.keys.begin() + N, K) - M.keys.begin();
       if (lb >= 0 && lb != want) return fail("lb random", i, lb, want);
     }
     for (int i = 0; i < 200000; ++i) {
