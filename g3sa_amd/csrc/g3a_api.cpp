@@ -48,7 +48,7 @@ struct g3a_ctx {
   int n_reads = 0;
   bool has_input = false, has_output = false;
   DevBuf seq, off;
-  DevBuf lep_pool, eh_pool, tb_order, tb_ctr;
+  DevBuf lep_pool, eh_pool, tb_order, tb_ctr, tb_nw;
   DevBuf seed_key, seed_key2, seed_idx, seed_order, seed_sort_tmp;  // seeding order
   int seed_ordered = 0;  // seed reads in minimizer order (1) or input order (0)
   int seed_threads = 0, tb_threads = 0;  // resident maxima (the pools are sized for them)
@@ -302,6 +302,7 @@ static int alloc_alns(g3a_ctx *ctx, int64_t T_a) {
          ensure(ctx, ctx->cig_size, (size_t)T_a * 8) || ensure(ctx, ctx->z_off, (size_t)(T_a + 1) * 8) ||
          ensure(ctx, ctx->cig_off, (size_t)(T_a + 1) * 8) || ensure(ctx, ctx->alns, (size_t)T_a * sizeof(Aln)) ||
          ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(T_a + 1)) || ensure(ctx, ctx->tb_order, (size_t)T_a * 4 + 4) ||
+         ensure(ctx, ctx->tb_nw, (size_t)T_a * 4 + 4) ||
          ensure(ctx, ctx->tb_ctr, 64) || ensure(ctx, ctx->eh_pool, (size_t)traceback_pool_words(ctx->tb_threads) * 4);
 }
 
@@ -661,6 +662,7 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   tb.eh_pool = P<int32_t>(ctx->eh_pool);
   tb.tb_order = P<int32_t>(ctx->tb_order);
   tb.tb_ctr = P<int32_t>(ctx->tb_ctr);
+  tb.tb_nw = P<int32_t>(ctx->tb_nw);
   HIPCK(hipEventRecord(ctx->kev[10], s));
   launch_traceback(tb, ctx->tb_threads, ctx->n_cu, s);
   HIPCK(hipEventRecord(ctx->kev[11], s));
@@ -1044,7 +1046,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->seed_order, &ctx->seed_sort_tmp,
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
                     &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
-                    &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr, &ctx->li_saf, &ctx->li_keys,
+                    &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr, &ctx->tb_nw, &ctx->li_saf, &ctx->li_keys,
                     &ctx->li_root, &ctx->li_leaf, &ctx->li_leaf_a, &ctx->li_mom, &ctx->li_short, &ctx->li_ctr};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)

@@ -1649,11 +1649,11 @@ G3A_HD int z_stride(int qlen, int w) {
   return (n_col + 3) & ~3;
 }
 
-// ksw_global2 (bwa/src/ksw.cu:706-812, DR-9): banded global alignment with
-// traceback.  The direction bytes of a row are written four at a time.
+// ksw_global2 (bwa/src/ksw.cu:706-812, DR-9): banded global alignment.
+// global_dp fills the direction bytes (a row's written four at a time) and
+// returns the score; global_walk turns them into the CIGAR.
 template <class Q, class T, class EH>
-G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH eh, uint8_t *z, uint32_t *cigar,
-                        int *n_cigar_out) {
+G3A_HD int global_dp(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH eh, uint8_t *z) {
   const int MINUS_INF = -0x40000000;
   const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
   const int zs = z_stride(qlen, w);
@@ -1700,7 +1700,13 @@ G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH 
     eh.H(end) = h1;
     eh.E(end) = MINUS_INF;
   }
-  int score = eh.H(qlen);
+  return eh.H(qlen);
+}
+
+// The traceback of global_dp's direction bytes: CIGAR (BAM encoding) into
+// cigar, returns its operation count
+G3A_HD int global_walk(int qlen, int tlen, int w, const uint8_t *z, uint32_t *cigar) {
+  const int zs = z_stride(qlen, w);
   int n = 0, which = 0, i = tlen - 1, k = (i + w + 1 < qlen ? i + w + 1 : qlen) - 1;
   while (i >= 0 && k >= 0) {
     which = z[(int64_t)i * zs + (k - (i > w ? i - w : 0))] >> (which << 1) & 3;
@@ -1711,8 +1717,7 @@ G3A_HD int global_align(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH 
   if (i >= 0) n = push_cigar(cigar, n, 2, i + 1);
   if (k >= 0) n = push_cigar(cigar, n, 1, k + 1);
   for (int a = 0; a < n >> 1; ++a) { uint32_t tmp = cigar[a]; cigar[a] = cigar[n - 1 - a]; cigar[n - 1 - a] = tmp; }
-  *n_cigar_out = n;
-  return score;
+  return n;
 }
 
 // Final alignment record (traceback + finalize, bwa/src/aux.cu:2874-3059).
@@ -1734,9 +1739,12 @@ G3A_HD int64_t aln_z_size(int lq, int64_t rlen, int w) {
 // kept region of a read of length l_seq.  `read` is the read, `cig` has
 // aln_cigar_cap() words (core at cig + lq + rlen + 4 - ... see below), z has
 // aln_z_size() bytes, eh_h/eh_e lq+1 entries.
+// pre_nw: when not null, the direction bytes and the score (*pre_nw) of the
+// global alignment are already computed (k_tb_wave_dp) and only the walk runs.
 template <class EH>
 G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *read, int l_seq, const Region &g,
-                               int ai, EH eh, uint8_t *z, uint32_t *cig, int64_t cig_base, Aln &a) {
+                               int ai, EH eh, uint8_t *z, uint32_t *cig, int64_t cig_base, Aln &a,
+                               const int32_t *pre_nw = nullptr) {
   const int64_t l_pac = x.l_pac;
   const int lq = g.qe - g.qb;
   const int64_t rlen = g.re - g.rb;
@@ -1761,9 +1769,10 @@ G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *r
     }
   } else {
     // the band revisits each query byte on 2w+1 rows: plain loads (cached)
-    nw = global_align(
-        o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return tr(rb + i); }, w, eh, z, core,
-        &n_core);
+    nw = pre_nw ? *pre_nw
+                : global_dp(o, lq, [&](int j) { return (int)query[j]; }, (int)rlen, [&](int i) { return tr(rb + i); },
+                            w, eh, z);
+    n_core = global_walk(lq, (int)rlen, w, z, core);
     TextReader trn(x.pac, l_pac);
     ByteReader qn(query);
     int xq = 0;

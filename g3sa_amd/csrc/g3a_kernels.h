@@ -155,6 +155,7 @@ struct TbArgs {
   int32_t *eh_pool;  // traceback_pool_words(threads) words
   int32_t *tb_order;  // n_alns: processing order
   int32_t *tb_ctr;    // 8 counters: class sizes, placement cursors
+  int32_t *tb_nw;     // n_alns: global-alignment scores from k_tb_wave_dp
 };
 
 // threads of k_seed's persistent grid (occupancy-limited, n_cu compute units)

@@ -844,7 +844,10 @@ __global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a, int cls) {
     const Region g = a.aln_reg[t];
     const int w = global_bw(a.opt, g.qe - g.qb, g.re - g.rb, g.score);
     Aln out;
-    if (2 * w + 2 <= RB)
+    if (RB == 64 && 2 * w + 1 <= 128)  // direction bytes and score from k_tb_wave_dp
+      trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], EhArray{eh_h, eh_e},
+                         a.z + a.z_off[t], a.cigar + a.cig_off[t], a.cig_off[t], out, a.tb_nw + t);
+    else if (2 * w + 2 <= RB)
       trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t],
                          EhRing<RB>{ring_h + lane, ring_e + lane, 64}, a.z + a.z_off[t], a.cigar + a.cig_off[t],
                          a.cig_off[t], out);
@@ -852,6 +855,111 @@ __global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a, int cls) {
       trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], EhArray{eh_h, eh_e},
                          a.z + a.z_off[t], a.cigar + a.cig_off[t], a.cig_off[t], out);
     a.alns[t] = out;
+  }
+}
+
+// Banded global DP (global_dp's recurrence, ksw_global2, DR-9) of the
+// widest class's alignments with 64 (NH - 1) < 2w+1 <= 64 NH band columns:
+// one wavefront per alignment, lane L on columns beg + L + 64 h (h < NH) of
+// each row.  F depends only on the row's M values, so a row is one max-plus
+// prefix scan over the columns (F_c = max(-inf - c e, max_{c' < c} M_c' - o
+// - e (c - c'))), carried from one 64-column half to the next; E and the
+// next row's H come from neighbouring columns, shifted by the band's start.
+// Writes the direction bytes and the score (tb_nw) that k_traceback_dp<64>
+// then walks; identical to global_dp on every cell.
+template <int NH>
+__device__ __forceinline__ int col_get(const int (&v)[NH], int c) {
+  int r = __shfl(v[0], c & 63);
+  if (NH > 1) {
+    const int r1 = __shfl(v[NH - 1], c & 63);
+    r = c >= 64 ? r1 : r;
+  }
+  return r;
+}
+template <int NH>
+__global__ void __launch_bounds__(256) k_tb_wave_dp(TbArgs a) {
+  const int L = threadIdx.x & 63;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t e0 = (int64_t)a.tb_ctr[0] + a.tb_ctr[1], e1 = e0 + a.tb_ctr[2];
+  const Opt &o = a.opt;
+  const int MINUS_INF = -0x40000000;
+  const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
+  for (int64_t e = e0 + wid; e < e1; e += nwaves) {
+    const int64_t t = a.tb_order[e];
+    const Region g = a.aln_reg[t];
+    const int qlen = g.qe - g.qb, tlen = (int)(g.re - g.rb);
+    const int w = global_bw(o, qlen, tlen, g.score);
+    if (2 * w + 1 > 64 * NH || 2 * w + 1 <= 64 * (NH - 1)) continue;  // another launch's alignment
+    const int r = a.aln_read[t];
+    const uint8_t *query = a.seq + a.off[r] + g.qb;
+    uint8_t *z = a.z + a.z_off[t];
+    const int zs = z_stride(qlen, w);
+    TextReader tr(a.idx.pac, a.idx.l_pac);
+    int hp[NH], ep[NH], beg_p = 0, n_p = 0;
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      hp[h] = 0;
+      ep[h] = MINUS_INF;
+    }
+    for (int i = 0; i < tlen; ++i) {
+      const int beg = i > w ? i - w : 0, end = i + w + 1 < qlen ? i + w + 1 : qlen, n = end - beg;
+      const ScoreRow sr = score_row(o, tr(g.rb + i));
+      int m[NH], pm[NH], ein[NH];
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        const int c = 64 * h + L, j = beg + c;
+        const bool act = c < n;
+        const int qj = act ? (int)query[j] : 4;
+        int hd;
+        if (i == 0) {
+          hd = j == 0 ? 0 : -(o.o_ins + o.e_ins * j);
+          ein[h] = MINUS_INF;
+        } else {
+          const int delta = beg - beg_p, kp = c - 1 + delta, ke = c + delta;
+          const int hs = col_get<NH>(hp, kp < 0 ? 0 : kp), es = col_get<NH>(ep, ke > 64 * NH - 1 ? 64 * NH - 1 : ke);
+          hd = kp < 0 ? (beg_p == 0 ? -(o.o_del + o.e_del * i) : MINUS_INF) : hs;
+          ein[h] = ke >= n_p ? MINUS_INF : es;
+        }
+        m[h] = hd + sr(qj);
+        pm[h] = act ? m[h] - oe_ins + o.e_ins * (c + 1) : MINUS_INF;
+      }
+      // inclusive max scan of each half, the second carried by the first's total
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const int u = __shfl_up(pm[h], d);
+          pm[h] = L >= d && u > pm[h] ? u : pm[h];
+        }
+      }
+      int carry = MINUS_INF;
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        pm[h] = carry > pm[h] ? carry : pm[h];
+        int ex = __shfl_up(pm[h], 1);
+        ex = L == 0 ? carry : ex;
+        ex = ex < MINUS_INF ? MINUS_INF : ex;
+        carry = __shfl(pm[h], 63);
+        const int c = 64 * h + L;
+        const int f = ex - c * o.e_ins;
+        int dir = m[h] >= ein[h] ? 0 : 1, hh = m[h] >= ein[h] ? m[h] : ein[h];
+        dir = hh >= f ? dir : 2;
+        hh = hh >= f ? hh : f;
+        int tt = m[h] - oe_del, e2 = ein[h] - o.e_del;
+        dir |= e2 > tt ? 4 : 0;
+        e2 = e2 > tt ? e2 : tt;
+        tt = m[h] - oe_ins;
+        dir |= f - o.e_ins > tt ? 32 : 0;
+        if (c < n) z[(int64_t)i * zs + c] = (uint8_t)dir;
+        hp[h] = hh;
+        ep[h] = e2;
+      }
+      beg_p = beg;
+      n_p = n;
+    }
+    const int score = col_get<NH>(hp, qlen - 1 - beg_p);
+    if (L == 0) a.tb_nw[t] = score;
   }
 }
 
@@ -1003,6 +1111,9 @@ void launch_traceback(const TbArgs &a, int n_threads, int n_cu, hipStream_t s) {
                      a, 0);
   hipLaunchKernelGGL(k_traceback_dp<32>, dim3((unsigned)dp_blocks((const void *)k_traceback_dp<32>)), dim3(64), 0, s,
                      a, 1);
+  // the widest class: band DP by wavefronts (2w+1 <= 64 and <= 128), then walks
+  hipLaunchKernelGGL(k_tb_wave_dp<1>, dim3((unsigned)(n_cu * 8)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tb_wave_dp<2>, dim3((unsigned)(n_cu * 8)), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_traceback_dp<64>, dim3((unsigned)dp_blocks((const void *)k_traceback_dp<64>)), dim3(64), 0, s,
                      a, 2);
   const int64_t need = (a.n_alns + 63) / 64, have = n_threads / 64;

@@ -266,3 +266,44 @@ def test_slow_seeding_path_parity(built, oracle, tmp_path):
     seq, off = _batch(reads)
     got, _ = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats")
     assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path"
+
+
+def test_wide_band_global_alignment_parity(built, oracle, tmp_path):
+    """Reads of 250 bases carrying one 15-60 base insertion or deletion: their
+    global alignments have bands of 2w+1 from about 40 up to 127 columns,
+    computed by the one-wavefront DP (one and two columns per lane) and
+    walked per thread; the CIGARs must equal the oracle's."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(23)
+    L = 300_000
+    g = rng.integers(0, 4, L).astype(np.uint8)
+    fa = tmp_path / "wide.fa"
+    fa.write_text(">chrW\n" + "\n".join("".join("ACGT"[b] for b in g[i:i + 60]) for i in range(0, L, 60)) + "\n")
+    prefix = tmp_path / "wide"
+    g3sa_amd.build_index(str(fa), str(prefix), 8)
+    reads = []
+    for k in range(600):
+        gap = 15 + k % 46
+        p = int(rng.integers(0, L - 400))
+        cut = 90 + int(rng.integers(0, 70))
+        if k % 2:  # deletion from the read
+            r = np.concatenate([g[p:p + cut], g[p + cut + gap:p + 250 + gap]])
+        else:      # insertion into the read
+            r = np.concatenate([g[p:p + cut], rng.integers(0, 4, gap).astype(np.uint8), g[p + cut:p + 250 - gap]])
+        r = r[:250].copy()
+        if k % 4 >= 2:
+            r = (3 - r[::-1]).astype(np.uint8)
+        sub = rng.random(len(r)) < 0.005
+        r[sub] = rng.integers(0, 4, int(sub.sum()))
+        reads.append(r)
+    seq, off = _batch(reads)
+    run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "wide-bands")
+    # the oracle's alignments include bands wider than 64 columns
+    idx = oracle.load(prefix)
+    try:
+        ref = oracle.align(idx, seq, off, threads=oracle_threads())
+    finally:
+        oracle.lib.orc_index_free(idx)
+    lq = ref["qe"] - ref["qb"]
+    rl = ref["re"] - ref["rb"]
+    assert int(np.abs(lq - rl).max()) >= 32, "no alignment with a band over 64 columns"
