@@ -95,3 +95,37 @@ def test_learned_edge_cases(ds_c1, oracle):
     ] + reads[9:40]
     seq2, off2 = _batch(cases)
     _run(ds_c1, oracle, seq2, off2, "edge-cases", forms=(("batched", 1, 0),))
+
+
+@pytest.mark.parametrize("min_seed_len", [9, 11, 15, 26, 31])
+def test_learned_seed_lengths(ds_rep, oracle, min_seed_len):
+    """The last round's window is min_seed_len + 1 bases: below 12 the model's
+    window alone is searched (a miss goes to the FM-index), from 12 on inside
+    the 12-mer bucket, and above 32 the round runs on the FM-index; seeding
+    intervals and alignments stay identical."""
+    import ctypes as C
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds_rep.reads["main"]), limit=3000)
+    oo = type(oracle.opt)()
+    C.pointer(oo)[0] = oracle.opt
+    oo.min_seed_len = min_seed_len
+    go = g3sa_amd.default_opt()
+    go.min_seed_len = min_seed_len
+    idx = oracle.load(ds_rep.prefix)
+    try:
+        ref = oracle.align(idx, seq, off, threads=oracle_threads(), opt=oo)
+    finally:
+        oracle.lib.orc_index_free(idx)
+    with g3sa_amd.Aligner(str(ds_rep.prefix), max_reads=len(off) - 1, max_bases=int(off[-1]), opt=go) as al:
+        print(f"[learned] min_seed_len {min_seed_len}: FM-only", flush=True)
+        got_fm = al.align(seq, off)
+        iv_fm = al.intervals()
+        compare(got_fm, ref, f"min_seed_len {min_seed_len} FM-only")
+        al.set_learned(True)
+        print(f"[learned] min_seed_len {min_seed_len}: learned", flush=True)
+        got = al.align(seq, off)
+        iv = al.intervals()
+        st = al.learned_stats()
+        print(f"[learned] min_seed_len {min_seed_len}: work {st}", flush=True)
+        compare(got, ref, f"min_seed_len {min_seed_len} learned")
+        _intervals_equal(iv_fm, iv, f"min_seed_len {min_seed_len}")
+        assert (st["lookups"] > 0) == (min_seed_len + 1 <= 32)
