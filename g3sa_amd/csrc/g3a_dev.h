@@ -1359,7 +1359,11 @@ G3A_HD int chain_build(const DevIndex &x, const Opt &o, const Seed *s, int ns, C
   // in-order traversal (__kb_traverse, kbtree_CUDA.cu:121-144); each frame
   // is (node, next key index, whether child `index` was already visited)
   int k = 0;
-  if (nch) {
+  if (nch && !nodes[t.root].internal) {
+    // one leaf (most reads): its keys in order
+    const KbNode &xn = nodes[t.root];
+    for (int i = 0; i < xn.n; ++i) order[k++] = xn.key[i];
+  } else if (nch) {
     int sn[40], si[40], sd[40], sp = 0;
     sn[0] = t.root; si[0] = 0; sd[0] = 0;
     while (sp >= 0) {
