@@ -77,7 +77,7 @@ EXPORTS = [
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
     "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
-    "g3a_last_intervals",
+    "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed",
 ]
 
 _lib = None
@@ -111,6 +111,8 @@ def load_library() -> C.CDLL:
     lib.g3a_set_seed_blocks.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_autotune_seed.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_seed_order.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_set_ext_packed.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_ext_packed.argtypes = [C.c_void_p]
     lib.g3a_set_learned.argtypes = [C.c_void_p, C.c_int, C.POINTER(LearnedInfo)]
     lib.g3a_learned_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     lib.g3a_last_intervals.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
@@ -179,7 +181,12 @@ class Aligner:
         """Pick the batched form's fastest tile shape on the last aligned batch."""
         sr, lb, ms = C.c_int(), C.c_int(), C.c_float()
         self._check(self.lib.g3a_autotune(self.ctx, C.byref(sr), C.byref(lb), C.byref(ms)))
-        return {"strip_rows": sr.value, "lane_blocks_per_cu": lb.value, "extension_ms": ms.value}
+        return {"strip_rows": sr.value, "lane_blocks_per_cu": lb.value, "packed": self.lib.g3a_ext_packed(self.ctx),
+                "extension_ms": ms.value}
+
+    def set_ext_packed(self, packed: bool) -> None:
+        """Lane engine with two strip rows per register (True) or one cell per lane."""
+        self._check(self.lib.g3a_set_ext_packed(self.ctx, int(bool(packed))))
 
     def set_tuning(self, strip_rows: int, lane_blocks_per_cu: int, wave_blocks_per_cu: int) -> None:
         self._check(self.lib.g3a_set_tuning(self.ctx, strip_rows, lane_blocks_per_cu, wave_blocks_per_cu))

@@ -67,6 +67,7 @@ struct g3a_ctx {
   int form = G3A_FORM_BATCHED;
   int n_cu = 0;
   int strip_rows = 16, lane_blocks_per_cu = 8, wave_blocks_per_cu = 8;
+  int ext_packed = 1;  // lane engine: two strip rows per register in plain columns
   DevBuf ext_res[2], ext_key, ext_key2, ext_idx, ext_sorted, ext_slow, ext_ctr, ext_sort_tmp, ext_rowbuf;
   // host staging: readbacks, and page-locked copies of each batch's input
   // and output so that the transfers run at DMA speed
@@ -477,7 +478,7 @@ int run_extension(g3a_ctx *ctx, int64_t T_t, int64_t *cells) {
                    P<int32_t>(ctx->ext_sorted), T_t, ctx->ext_sort_tmp.p, ctx->ext_sort_tmp.bytes, s))
         return fail(ctx, G3A_E_HIP, "extension task sort failed");
       HIPCK(hipEventRecord(ctx->kev[2 + 2 * side], s));
-      launch_ext_lane(ea, ctx->strip_rows, lane_blocks, s);
+      launch_ext_lane(ea, ctx->strip_rows, ctx->ext_packed, lane_blocks, s);
       HIPCK(hipEventRecord(ctx->kev[3 + 2 * side], s));
       launch_ext_wave(ea, wave_blocks, s);
       HIPCK(hipEventRecord(ctx->kev[7 + 2 * side], s));
@@ -791,16 +792,18 @@ int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *
   if (!ctx) return G3A_E_ARG;
   if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
   HIPCK(hipSetDevice(ctx->device));
-  const int form = ctx->form, sr0 = ctx->strip_rows, lb0 = ctx->lane_blocks_per_cu;
+  const int form = ctx->form, sr0 = ctx->strip_rows, lb0 = ctx->lane_blocks_per_cu, pk0 = ctx->ext_packed;
   ctx->form = G3A_FORM_BATCHED;
   int64_t *cells = P<int64_t>(ctx->counters) + 24;
   int64_t saved = 0;
   HIPCK(hipMemcpy(&saved, cells, 8, hipMemcpyDeviceToHost));
   const int srs[2] = {16, 32}, lbs[4] = {2, 4, 8, 16};
   float best = -1;
-  int bsr = sr0, blb = lb0;
+  int bsr = sr0, blb = lb0, bpk = pk0;
+  for (int pk = 0; pk < 2; ++pk)
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 4; ++b) {
+      ctx->ext_packed = pk;
       ctx->strip_rows = srs[a];
       ctx->lane_blocks_per_cu = lbs[b];
       float t = 0;
@@ -816,10 +819,12 @@ int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *
         best = t;
         bsr = srs[a];
         blb = lbs[b];
+        bpk = pk;
       }
     }
   ctx->strip_rows = bsr;
   ctx->lane_blocks_per_cu = blb;
+  ctx->ext_packed = bpk;
   // leave the regions and counters as the aligned batch had them
   if (int rc = run_extension(ctx, ctx->total_tasks, cells)) return rc;
   HIPCK(hipMemcpyAsync(cells, &saved, 8, hipMemcpyHostToDevice, ctx->stream));
@@ -1006,6 +1011,14 @@ int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wav
   ctx->wave_blocks_per_cu = wave_blocks_per_cu;
   return G3A_OK;
 }
+
+int g3a_set_ext_packed(g3a_ctx *ctx, int packed) {
+  if (!ctx || (packed != 0 && packed != 1)) return G3A_E_ARG;
+  ctx->ext_packed = packed;
+  return G3A_OK;
+}
+
+int g3a_ext_packed(const g3a_ctx *ctx) { return ctx ? ctx->ext_packed : -1; }
 
 int g3a_last_stats(const g3a_ctx *ctx, float ms[6], int64_t work[6]) {
   if (!ctx || !ms || !work) return G3A_E_ARG;

@@ -49,6 +49,68 @@ G3A_HD int max3i(int a, int b, int c) {
 }
 constexpr int kExtPad = 5;             // query/target code of padding cells
 
+// Two 16-bit lanes of one 32-bit register: VOP3P packed integer
+// instructions on the device (v_pk_add_u16, v_pk_max_i16, v_pk_sub_u16 with
+// clamp, ...), a pair of int16 on the host.  Signed unless named otherwise.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef short g3a_s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short g3a_u16x2 __attribute__((ext_vector_type(2)));
+struct P2 {
+  g3a_s16x2 v;
+};
+G3A_HD P2 p2(int lo, int hi) {
+  P2 r;
+  r.v = (g3a_s16x2){(short)lo, (short)hi};
+  return r;
+}
+G3A_HD int p2_lo(P2 a) { return a.v.x; }
+G3A_HD int p2_hi(P2 a) { return a.v.y; }
+G3A_HD P2 p2_add(P2 a, P2 b) { P2 r; r.v = a.v + b.v; return r; }
+G3A_HD P2 p2_sub(P2 a, P2 b) { P2 r; r.v = a.v - b.v; return r; }
+G3A_HD P2 p2_max(P2 a, P2 b) { P2 r; r.v = __builtin_elementwise_max(a.v, b.v); return r; }
+G3A_HD P2 p2_min(P2 a, P2 b) { P2 r; r.v = __builtin_elementwise_min(a.v, b.v); return r; }
+// max(a - b, 0) for a, b >= 0 (unsigned saturating subtract)
+G3A_HD P2 p2_subsat(P2 a, P2 b) {
+  const g3a_u16x2 ua = __builtin_bit_cast(g3a_u16x2, a.v), ub = __builtin_bit_cast(g3a_u16x2, b.v);
+  P2 r;
+  r.v = __builtin_bit_cast(g3a_s16x2, __builtin_elementwise_sub_sat(ua, ub));
+  return r;
+}
+G3A_HD P2 p2_shl3(P2 a) {
+  P2 r;
+  r.v = __builtin_bit_cast(g3a_s16x2, __builtin_bit_cast(g3a_u16x2, a.v) << (g3a_u16x2){3, 3});
+  return r;
+}
+// the signed 4-bit field q (0..3) of each 16-bit half of x
+G3A_HD P2 p2_nibble(uint32_t x, int q) {
+  const unsigned short sh = (unsigned short)(12 - 4 * q);
+  const g3a_u16x2 u = __builtin_bit_cast(g3a_u16x2, x) << (g3a_u16x2){sh, sh};
+  P2 r;
+  r.v = __builtin_bit_cast(g3a_s16x2, u) >> (g3a_s16x2){12, 12};
+  return r;
+}
+#else
+struct P2 {
+  int16_t lo, hi;
+};
+G3A_HD P2 p2(int lo, int hi) { return P2{(int16_t)lo, (int16_t)hi}; }
+G3A_HD int p2_lo(P2 a) { return a.lo; }
+G3A_HD int p2_hi(P2 a) { return a.hi; }
+G3A_HD P2 p2_add(P2 a, P2 b) { return p2(a.lo + b.lo, a.hi + b.hi); }
+G3A_HD P2 p2_sub(P2 a, P2 b) { return p2(a.lo - b.lo, a.hi - b.hi); }
+G3A_HD P2 p2_max(P2 a, P2 b) { return p2(a.lo > b.lo ? a.lo : b.lo, a.hi > b.hi ? a.hi : b.hi); }
+G3A_HD P2 p2_min(P2 a, P2 b) { return p2(a.lo < b.lo ? a.lo : b.lo, a.hi < b.hi ? a.hi : b.hi); }
+G3A_HD P2 p2_subsat(P2 a, P2 b) {
+  const int l = (uint16_t)a.lo - (uint16_t)b.lo, h = (uint16_t)a.hi - (uint16_t)b.hi;
+  return p2(l > 0 ? l : 0, h > 0 ? h : 0);
+}
+G3A_HD P2 p2_shl3(P2 a) { return p2((int16_t)(uint16_t)((uint16_t)a.lo << 3), (int16_t)(uint16_t)((uint16_t)a.hi << 3)); }
+G3A_HD P2 p2_nibble(uint32_t x, int q) {
+  const int lo = (int)((x >> (4 * q)) & 15), hi = (int)((x >> (16 + 4 * q)) & 15);
+  return p2(lo >= 8 ? lo - 16 : lo, hi >= 8 ? hi - 16 : hi);
+}
+#endif
+
 // Band width after clipping by max_ins/max_del (ksw.cu:840-851).
 G3A_HD int ext_band_w(const Opt &o, int qlen, int end_bonus) {
   int max_ = 0;
@@ -98,7 +160,10 @@ struct ExtLaneOut {
 //   rowbuf: column j of this lane at rowbuf[j * rs]
 //   wt: score words of query code c at wt[c * cs + k], k < R/8
 //   SYM: o_del + e_del == o_ins + e_ins (one M - o - e per cell)
-template <int R, bool SYM, class QB, class TB>
+//   PK: the plain columns run with rows r and r + R/2 of a strip packed in
+//     one register (16-bit lanes), the lower half one column behind the
+//     upper, so that each packed operation computes two cells
+template <int R, bool SYM, bool PK = false, class QB, class TB>
 G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w, int h0, int qmax, int w_min,
                            int w_max, uint32_t *rowbuf, int rs, uint32_t *wt, int cs) {
   static_assert(R == 16 || R == 32, "strip height");
@@ -249,6 +314,167 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
       const int hl = hc[R - 1], ub = hl + ms * (qlen - 1 - j);
       if (hl > 0 && ub > u) u = ub;
     };
+    // The best cell of column j from its two halves' maxima: cmax_u over rows
+    // [0, R/2) with the smallest such row rmin_u (computed when it could be
+    // needed), cmax_l over the rest, lower(r) the value of row R/2 + r; and
+    // the bound from its last row hl.  (The packed columns' bookkeeping.)
+    auto finish = [&](int jj, int cmax_u, int rmin_u, int cmax_l, auto lower, int hl) {
+      constexpr int H = R / 2;
+      const int cmax = cmax_u > cmax_l ? cmax_u : cmax_l;
+      if (cmax > best || (cmax == best && bi >= i0)) {
+        int rmin = rmin_u;
+        if (cmax_u != cmax) {
+          rmin = R - 1;
+#pragma unroll
+          for (int r = H - 1; r >= 0; --r)
+            if (lower(r) == cmax) rmin = H + r;
+        }
+        const int ci = i0 + rmin;
+        if (cmax > best || ci <= bi) {
+          best = cmax;
+          bi = ci;
+          bj = jj;
+        }
+      }
+      const int ub = hl + ms * (qlen - 1 - jj);
+      if (hl > 0 && ub > u) u = ub;
+    };
+    // one plain cell of row r (32-bit), as in column()
+    auto plain_cell = [&](int r, int &e, int &cm) {
+      const int m = hc[r], f = F[r];
+      const int h = max3i(m, e, f);
+      const int td = m - oe_del, ti = SYM ? td : m - oe_ins;
+      int e2 = e - o.e_del;
+      e2 = e2 > td ? e2 : td;
+      e2 = e2 > 0 ? e2 : 0;
+      int f2 = f - o.e_ins;
+      f2 = f2 > ti ? f2 : ti;
+      f2 = f2 > 0 ? f2 : 0;
+      hc[r] = h;
+      F[r] = f2;
+      e = e2;
+      cm = cm > h ? cm : h;
+    };
+    // Plain columns [c0, c1), c1 - c0 >= 2, with rows r and r + H (H = R/2)
+    // in one register: the upper half at column c, the lower half at c - 1.
+    // The lower half's diagonal and E inputs at row H come from the upper
+    // half's last row one and zero columns earlier; both chains (E down the
+    // rows, F along the columns) keep their order in each half.
+    auto packed_run = [&](int c0, int c1) {
+      constexpr int H = R / 2, NG = H / 4;
+      const P2 oeP = p2(oe_del, oe_del), oiP = p2(oe_ins, oe_ins), edP = p2(o.e_del, o.e_del),
+               eiP = p2(o.e_ins, o.e_ins);
+      // prologue: the upper rows of column c0
+      int hul = hc[H - 1];  // upper row H-1, one column before the lower half's column
+      uint32_t wp[NW];
+      uint32_t qcp;
+      int e_mid, cmax_u = 0, rmin_u = R - 1;
+      {
+        const uint32_t word = rowbuf[(int64_t)c0 * rs];
+        const int Ht = (int)(word & 0xfff), Et = (int)((word >> 12) & 0xfff);
+        qcp = word >> 24;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) wp[k] = wt[qcp * cs + k];
+#pragma unroll
+        for (int r = H - 1; r >= 1; --r) {
+          const int hd = hc[r - 1];
+          const int m = hd + (int)((wp[r >> 3] >> ((r & 7) * 4)) & 15) - 8;
+          hc[r] = m < (hd << 3) ? m : (hd << 3);
+        }
+        {
+          const int m = htop + (int)(wp[0] & 15) - 8;
+          hc[0] = m < (htop << 3) ? m : (htop << 3);
+          htop = Ht;
+        }
+        int e = Et;
+#pragma unroll
+        for (int r = 0; r < H; ++r) plain_cell(r, e, cmax_u);
+        e_mid = e;
+        if (cmax_u > best || (cmax_u == best && bi >= i0)) {
+#pragma unroll
+          for (int r = H - 1; r >= 0; --r)
+            if (hc[r] == cmax_u) rmin_u = r;
+        }
+      }
+      P2 hP[H], fP[H];
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        hP[r] = p2(hc[r], hc[r + H]);
+        fP[r] = p2(F[r], F[r + H]);
+      }
+      for (int c = c0 + 1; c < c1; ++c) {
+        const uint32_t word = rowbuf[(int64_t)c * rs];
+        const int Ht = (int)(word & 0xfff), Et = (int)((word >> 12) & 0xfff);
+        const uint32_t qc = word >> 24;
+        uint32_t wc[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) wc[k] = wt[qc * cs + k];
+        // 4-row groups: upper rows' fields (column c) in the low half, lower
+        // rows' (column c-1) in the high half, as signed 4-bit values
+        uint32_t X[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+          const int sh = 16 * (g & 1);
+          const uint32_t lo = (wc[g >> 1] >> sh) & 0xffffu, hi = (wp[H / 8 + (g >> 1)] >> sh) & 0xffffu;
+          X[g] = (lo | hi << 16) ^ 0x88888888u;
+        }
+        const int nxt = p2_lo(hP[H - 1]);
+#pragma unroll
+        for (int r = H - 1; r >= 0; --r) {
+          const P2 hd = r ? hP[r - 1] : p2(htop, hul);
+          const P2 m = p2_add(hd, p2_nibble(X[r >> 2], r & 3));
+          hP[r] = p2_min(m, p2_shl3(hd));
+        }
+        htop = Ht;
+        hul = nxt;
+        P2 eP = p2(Et, e_mid), cm = p2(0, 0);
+#pragma unroll
+        for (int r = 0; r < H; ++r) {
+          const P2 m = hP[r], f = fP[r];
+          const P2 h = p2_max(p2_max(m, eP), f);
+          const P2 td = p2_sub(m, oeP), ti = SYM ? td : p2_sub(m, oiP);
+          eP = p2_max(p2_subsat(eP, edP), td);
+          fP[r] = p2_max(p2_subsat(f, eiP), ti);
+          hP[r] = h;
+          cm = p2_max(cm, h);
+        }
+        e_mid = p2_lo(eP);
+        // column c-1 is complete
+        const int hl = p2_hi(hP[H - 1]);
+        rowbuf[(int64_t)(c - 1) * rs] = (uint32_t)hl | ((uint32_t)p2_hi(eP) << 12) | (qcp << 24);
+        finish(c - 1, cmax_u, rmin_u, p2_hi(cm), [&](int r) { return p2_hi(hP[r]); }, hl);
+        // the upper rows of column c
+        cmax_u = p2_lo(cm);
+        rmin_u = R - 1;
+        if (cmax_u > best || (cmax_u == best && bi >= i0)) {
+#pragma unroll
+          for (int r = H - 1; r >= 0; --r)
+            if (p2_lo(hP[r]) == cmax_u) rmin_u = r;
+        }
+#pragma unroll
+        for (int k = 0; k < NW; ++k) wp[k] = wc[k];
+        qcp = qc;
+      }
+      // epilogue: the lower rows of column c1-1
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        hc[r] = p2_lo(hP[r]);
+        hc[r + H] = p2_hi(hP[r]);
+        F[r] = p2_lo(fP[r]);
+        F[r + H] = p2_hi(fP[r]);
+      }
+#pragma unroll
+      for (int r = R - 1; r >= H; --r) {
+        const int hd = r > H ? hc[r - 1] : hul;
+        const int m = hd + (int)((wp[r >> 3] >> ((r & 7) * 4)) & 15) - 8;
+        hc[r] = m < (hd << 3) ? m : (hd << 3);
+      }
+      int e = e_mid, cmax_l = 0;
+#pragma unroll
+      for (int r = H; r < R; ++r) plain_cell(r, e, cmax_l);
+      rowbuf[(int64_t)(c1 - 1) * rs] = (uint32_t)hc[R - 1] | ((uint32_t)e << 12) | (qcp << 24);
+      finish(c1 - 1, cmax_u, rmin_u, cmax_l, [&](int r) { return hc[H + r]; }, hc[R - 1]);
+    };
     // columns [ju0, ju1) need no rule for any lane of the wavefront
     int ju0 = jfull > jstart ? jfull : jstart;
     if (i0 + w_min + 1 < 30 - i0 && ju0 < 31 - i0) ju0 = 31 - i0;
@@ -259,6 +485,10 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
     if (ju1 < ju0) ju1 = ju0;
     int j = jstart;
     for (; j < ju0; ++j) column(j, std::true_type());
+    if (PK && ju1 - j >= 2) {
+      packed_run(j, ju1);
+      j = ju1;
+    }
     for (; j < ju1; ++j) column(j, std::false_type());
     for (; j < qmax; ++j) column(j, std::true_type());
 #pragma unroll

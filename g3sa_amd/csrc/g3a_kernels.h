@@ -171,7 +171,8 @@ void launch_chain(const ChainArgs &a, hipStream_t s);
 void launch_task_compact(const CompactArgs &a, hipStream_t s);
 void launch_extend(const ExtArgs &a, hipStream_t s);
 void launch_ext_classify(const ExtSideArgs &a, hipStream_t s);
-void launch_ext_lane(const ExtSideArgs &a, int strip_rows, int n_blocks, hipStream_t s);
+// packed: plain columns with two strip rows per register (16-bit lanes)
+void launch_ext_lane(const ExtSideArgs &a, int strip_rows, int packed, int n_blocks, hipStream_t s);
 void launch_ext_wave(const ExtSideArgs &a, int n_blocks, hipStream_t s);
 void launch_ext_finish(const ExtFinishArgs &a, hipStream_t s);
 void launch_fill_i32(int32_t *p, int n, int32_t v, hipStream_t s);

@@ -614,7 +614,7 @@ __global__ void __launch_bounds__(256) k_ext_classify(ExtSideArgs a) {
   if (slow) a.slow_list[slot] = (int32_t)t;
 }
 
-template <int R, bool SYM>
+template <int R, bool SYM, bool PK>
 __global__ void __launch_bounds__(64) k_ext_lane(ExtSideArgs a) {
   constexpr int NW = R / 8;
   __shared__ uint32_t wt[6 * 64 * NW];
@@ -649,12 +649,12 @@ __global__ void __launch_bounds__(64) k_ext_lane(ExtSideArgs a) {
       const int64_t rbeg = st.rbeg, rend = st.rend;
       ExtLaneOut r;
       if (a.side == 0)
-        r = ext_lane<R, SYM>(
+        r = ext_lane<R, SYM, PK>(
             a.opt, st.qlen, [=](int j) { return (int)rd[qbeg - 1 - j]; }, st.tlen,
             [=](int i) { return text_base_p(pac, l_pac, rbeg - 1 - i); }, w, st.h0, qmax, wmin, wmax, rb, 64,
             wt + lane * NW, 64 * NW);
       else
-        r = ext_lane<R, SYM>(
+        r = ext_lane<R, SYM, PK>(
             a.opt, st.qlen, [=](int j) { return (int)rd[qend + j]; }, st.tlen,
             [=](int i) { return text_base_p(pac, l_pac, rend + i); }, w, st.h0, qmax, wmin, wmax, rb, 64,
             wt + lane * NW, 64 * NW);
@@ -1060,16 +1060,17 @@ void launch_extend(const ExtArgs &a, hipStream_t s) {
 void launch_ext_classify(const ExtSideArgs &a, hipStream_t s) {
   if (a.n_tasks) hipLaunchKernelGGL(k_ext_classify, dim3(blocks(a.n_tasks, 256)), dim3(256), 0, s, a);
 }
-void launch_ext_lane(const ExtSideArgs &a, int strip_rows, int n_blocks, hipStream_t s) {
+void launch_ext_lane(const ExtSideArgs &a, int strip_rows, int packed, int n_blocks, hipStream_t s) {
   if (!a.n_tasks) return;
   const bool sym = a.opt.o_del + a.opt.e_del == a.opt.o_ins + a.opt.e_ins;
-  if (strip_rows == 32) {
-    if (sym) hipLaunchKernelGGL((k_ext_lane<32, true>), dim3(n_blocks), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((k_ext_lane<32, false>), dim3(n_blocks), dim3(64), 0, s, a);
-  } else {
-    if (sym) hipLaunchKernelGGL((k_ext_lane<16, true>), dim3(n_blocks), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((k_ext_lane<16, false>), dim3(n_blocks), dim3(64), 0, s, a);
-  }
+  void (*k)(ExtSideArgs);
+  if (strip_rows == 32)
+    k = sym ? (packed ? k_ext_lane<32, true, true> : k_ext_lane<32, true, false>)
+            : (packed ? k_ext_lane<32, false, true> : k_ext_lane<32, false, false>);
+  else
+    k = sym ? (packed ? k_ext_lane<16, true, true> : k_ext_lane<16, true, false>)
+            : (packed ? k_ext_lane<16, false, true> : k_ext_lane<16, false, false>);
+  hipLaunchKernelGGL(k, dim3(n_blocks), dim3(64), 0, s, a);
 }
 void launch_ext_wave(const ExtSideArgs &a, int n_blocks, hipStream_t s) {
   if (a.n_tasks) hipLaunchKernelGGL(k_ext_wave, dim3(n_blocks), dim3(64), 0, s, a);

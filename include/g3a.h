@@ -129,10 +129,18 @@ int g3a_set_form(g3a_ctx *ctx, int form);
  * persistent blocks per compute unit for the lane and wavefront kernels.
  * No reference counterpart. */
 int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wave_blocks_per_cu);
+/* Lane engine arithmetic: 1 (default) runs the plain columns with two strip
+ * rows per register in 16-bit lanes (packed integer instructions, the lower
+ * half one column behind the upper), 0 with one cell per 32-bit lane.
+ * Identical results.  g3a_ext_packed returns the current setting.  No
+ * reference counterpart. */
+int g3a_set_ext_packed(g3a_ctx *ctx, int packed);
+int g3a_ext_packed(const g3a_ctx *ctx);
 /* Autotuner of the batched form: on the batch last aligned (its tasks are
- * still on the device), time the extension stage for each strip height
- * (16, 32) and persistent lane blocks per compute unit (2, 4, 8, 16), keep
- * the fastest setting on the context and report it.  Results are identical
+ * still on the device), time the extension stage for each lane arithmetic
+ * (packed or not), strip height (16, 32) and persistent lane blocks per
+ * compute unit (2, 4, 8, 16), keep the fastest setting on the context and
+ * report it (the arithmetic through g3a_ext_packed).  Results are identical
  * for every setting; the batch's output is left as it was.  No reference
  * counterpart. */
 int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *best_ms);

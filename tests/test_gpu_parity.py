@@ -40,12 +40,14 @@ def compare(gpu: g3sa_amd.Result, ref: dict, label: str) -> None:
         assert np.array_equal(gk, rk), f"{label}: global CIGAR of alignment {i}"
 
 
-# (form, tuning, seeding order, seeding blocks per CU): the plain form and
-# the batched form at both strip heights; reads seeded in minimizer order and
-# in input order; seeding grids whose LDS share holds 9, 25 and the default
-# number of LEP entries per thread (the rest of a list spills to its global
-# slice)
-VARIANTS = [("plain", None, 1, 0), ("batched", (16, 8, 8), 1, 10), ("batched", (32, 4, 8), 0, 4)]
+# (form, tuning, seeding order, seeding blocks per CU, packed lane
+# arithmetic): the plain form and the batched form at both strip heights
+# with two strip rows per register, and at 16 rows with one cell per lane;
+# reads seeded in minimizer order and in input order; seeding grids whose LDS
+# share holds 9, 25 and the default number of LEP entries per thread (the
+# rest of a list spills to its global slice)
+VARIANTS = [("plain", None, 1, 0, 1), ("batched", (16, 8, 8), 1, 10, 1), ("batched", (32, 4, 8), 0, 4, 1),
+            ("batched", (16, 8, 8), 0, 0, 0)]
 
 
 def run_both(ds, oracle, seq, off, label, max_reads=None):
@@ -57,14 +59,15 @@ def run_both(ds, oracle, seq, off, label, max_reads=None):
         oracle.lib.orc_index_free(idx)
     with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(max_reads or n, 1),
                           max_bases=max(int(off[-1]), 1)) as al:
-        for form, tuning, ordered, blocks in VARIANTS:
-            print(f"[parity] {label}: {n} reads, form {form} {tuning} seed order {ordered} blocks {blocks}",
-                  flush=True)
+        for form, tuning, ordered, blocks, packed in VARIANTS:
+            print(f"[parity] {label}: {n} reads, form {form} {tuning} seed order {ordered} blocks {blocks} "
+                  f"packed {packed}", flush=True)
             al.set_form(form)
             if tuning:
                 al.set_tuning(*tuning)
             al.set_seed_order(ordered)
             al.set_seed_blocks(blocks)
+            al.set_ext_packed(packed)
             got = al.align(seq, off)
             compare(got, ref, f"{label} [{form} {tuning} order {ordered}]")
             # seeds resolved by the GPU equal the oracle's SA lookups
@@ -159,14 +162,15 @@ def test_scoring_variants_parity(ds_c1, oracle, w, b):
     finally:
         oracle.lib.orc_index_free(idx)
     with g3sa_amd.Aligner(str(ds_c1.prefix), max_reads=len(off) - 1, max_bases=int(off[-1]), opt=go) as al:
-        for form, tuning, ordered, blocks in VARIANTS:
-            print(f"[parity] scoring w={w} b={b}: form {form} {tuning} seed order {ordered} blocks {blocks}",
-                  flush=True)
+        for form, tuning, ordered, blocks, packed in VARIANTS:
+            print(f"[parity] scoring w={w} b={b}: form {form} {tuning} seed order {ordered} blocks {blocks} "
+                  f"packed {packed}", flush=True)
             al.set_form(form)
             if tuning:
                 al.set_tuning(*tuning)
             al.set_seed_order(ordered)
             al.set_seed_blocks(blocks)
+            al.set_ext_packed(packed)
             compare(al.align(seq, off), ref, f"scoring w={w} b={b} [{form} {tuning} order {ordered}]")
 
 
@@ -191,7 +195,7 @@ def test_autotune_keeps_output(ds_c1, oracle):
         al.run()
         print("[parity] autotune", flush=True)
         t = al.autotune()
-        assert t["strip_rows"] in (16, 32) and t["lane_blocks_per_cu"] in (2, 4, 8, 16)
+        assert t["strip_rows"] in (16, 32) and t["lane_blocks_per_cu"] in (2, 4, 8, 16) and t["packed"] in (0, 1)
         compare(al.download(), ref, "after autotune")
         compare(al.align(seq, off), ref, "next batch after autotune")
         print("[parity] seeding autotune", flush=True)
