@@ -292,6 +292,46 @@ G3A_HD void backward_ext(const DevIndex &x, const IntvT<P> &in, int base, IntvT<
   out.s = pick4(s, base);
 }
 
+// Occ(p, c) of one base c from p's narrow line: one popcount of the rows
+// [line start, p] holding c ('$' is stored as 0 and not counted)
+G3A_HD uint32_t occ1_at(const DevIndex &x, const OccLineN &L, uint32_t p, int c) {
+  const int y = (int)(p & 31);
+  const uint64_t m = (~0ULL << (62 - 2 * y)) & 0x5555555555555555ULL;
+  const uint64_t hi = (L.w >> 1) & m, lo = L.w & m;
+  const uint64_t sel = c == 3 ? hi & lo : c == 2 ? hi & ~lo : c == 1 ? lo & ~hi : m & ~hi & ~lo;
+  const uint32_t n = (uint32_t)G3A_POPC(sel);
+  uint32_t sc;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint4 sb = *(const uint4 *)(x.sbc + ((p >> 21) << 2));
+  sc = c == 0 ? sb.x : c == 1 ? sb.y : c == 2 ? sb.z : sb.w;
+#else
+  sc = x.sbc[((p >> 21) << 2) + c];
+#endif
+  const uint32_t r0 = (uint32_t)(L.cnt & 0x1fffff), r1 = (uint32_t)(L.cnt >> 21) & 0x1fffff,
+                 r2 = (uint32_t)(L.cnt >> 42) & 0x1fffff;
+  const uint32_t pr = (uint32_t)x.primary, ls = p & ~31u, ss = p & ~((1u << 21) - 1);
+  const uint32_t dol_before = pr >= ss && pr < ls, dol_in = pr >= ls && pr <= p;
+  const uint32_t rc = c == 0 ? r0 : c == 1 ? r1 : c == 2 ? r2 : (ls - ss) - r0 - r1 - r2 - dol_before;
+  return sc + rc + n - (c == 0 ? dol_in : 0u);
+}
+
+// Backward extension by `base` of k and s only (l is left as it was): for
+// the backward phases, whose intervals are only extended backward again or
+// output, and neither reads l.  Narrow layout: one base's count per row.
+G3A_HD void backward_ext_ks(const DevIndex &x, const IntvT<uint32_t> &in, int base, IntvT<uint32_t> &out) {
+  const uint32_t p1 = in.k - 1, p2 = in.k + in.s - 1;
+  OccLineN L = occn_line(x, p1 >> 5);
+  const uint32_t a = occ1_at(x, L, p1, base);
+  if ((p2 >> 5) != (p1 >> 5)) L = occn_line(x, p2 >> 5);
+  const uint32_t b = occ1_at(x, L, p2, base);
+  out.k = (uint32_t)pick4(x.count, base) + a;
+  out.l = in.l;
+  out.s = b - a;
+}
+G3A_HD void backward_ext_ks(const DevIndex &x, const IntvT<int64_t> &in, int base, IntvT<int64_t> &out) {
+  backward_ext(x, in, base, out);
+}
+
 // backwardExtBackward (fmindex.cu:162-180): k and s only
 G3A_HD void backward_ext_b(const DevIndex &x, Intv &io, int base) {
   int64_t oa[4], ob[4];
@@ -929,7 +969,7 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
         I ok;
         bool ext_ok = false;
         if (c >= 0) {
-          backward_ext(x, pj, c, ok);
+          backward_ext_ks(x, pj, c, ok);
           if (steps) ++*steps;
           ext_ok = ok.s >= 1;
         }
@@ -1052,7 +1092,7 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     nl = 0;
     for (int j = 0; j < num_old; ++j) {
       I lep = leps.get(j), ext;
-      backward_ext(x, lep, base, ext);
+      backward_ext_ks(x, lep, base, ext);
       if (steps) ++*steps;
       if (ext.s >= curr) {
         ext.m = m - 1;
