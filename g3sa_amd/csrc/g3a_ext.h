@@ -171,11 +171,11 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
   constexpr int NW = R / 8;
   const int oe_del = o.o_del + o.e_del, oe_ins = o.o_ins + o.e_ins;
   const int ms = mat_max(o);
-  // score + 8 of target code t against query codes 0..4, 4 bits each
+  // signed 4-bit scores of target code t against query codes 0..4
   uint32_t rwt[5];
   for (int t = 0; t < 5; ++t) {
     uint32_t v = 0;
-    for (int c = 0; c < 5; ++c) v |= (uint32_t)(o.mat[t * 5 + c] + 8) << (4 * c);
+    for (int c = 0; c < 5; ++c) v |= (uint32_t)(o.mat[t * 5 + c] & 15) << (4 * c);
     rwt[t] = v;
   }
   const uint32_t rw0 = rwt[0], rw1 = rwt[1], rw2 = rwt[2], rw3 = rwt[3], rw4 = rwt[4];
@@ -190,19 +190,19 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
   int64_t cells = 0;
   int i0 = 0;
   for (;;) {
-    // 4-bit fields (score + 8) of the strip's rows against each query code;
+    // signed 4-bit scores of the strip's rows against each query code;
     // padding rows and columns score -8, below any real score
     {
       uint32_t wl[6][NW];
 #pragma unroll
       for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int k = 0; k < NW; ++k) wl[c][k] = 0;
+        for (int k = 0; k < NW; ++k) wl[c][k] = c == 5 ? 0x88888888u : 0u;  // column padding: -8
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int i = i0 + r;
         const int tc = i < tlen ? tb(i) : kExtPad;
-        uint32_t rw = tc == 0 ? rw0 : tc == 1 ? rw1 : tc == 2 ? rw2 : tc == 3 ? rw3 : tc == 4 ? rw4 : 0u;
+        uint32_t rw = tc == 0 ? rw0 : tc == 1 ? rw1 : tc == 2 ? rw2 : tc == 3 ? rw3 : tc == 4 ? rw4 : 0x88888u;
 #pragma unroll
         for (int c = 0; c < 5; ++c) wl[c][r >> 3] |= ((rw >> (4 * c)) & 15u) << ((r & 7) * 4);
       }
@@ -263,11 +263,11 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
 #pragma unroll
       for (int r = R - 1; r >= 1; --r) {
         const int hd = hc[r - 1];
-        const int m = hd + (int)((W[r >> 3] >> ((r & 7) * 4)) & 15) - 8;
+        const int m = hd + ((int)(W[r >> 3] << (28 - (r & 7) * 4)) >> 28);
         hc[r] = m < (hd << 3) ? m : (hd << 3);
       }
       {
-        const int m = hd0 + (int)(W[0] & 15) - 8;
+        const int m = hd0 + ((int)(W[0] << 28) >> 28);
         hc[0] = m < (hd0 << 3) ? m : (hd0 << 3);
         htop = Ht;
       }
@@ -378,11 +378,11 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
 #pragma unroll
         for (int r = H - 1; r >= 1; --r) {
           const int hd = hc[r - 1];
-          const int m = hd + (int)((wp[r >> 3] >> ((r & 7) * 4)) & 15) - 8;
+          const int m = hd + ((int)(wp[r >> 3] << (28 - (r & 7) * 4)) >> 28);
           hc[r] = m < (hd << 3) ? m : (hd << 3);
         }
         {
-          const int m = htop + (int)(wp[0] & 15) - 8;
+          const int m = htop + ((int)(wp[0] << 28) >> 28);
           hc[0] = m < (htop << 3) ? m : (htop << 3);
           htop = Ht;
         }
@@ -416,7 +416,7 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
         for (int g = 0; g < NG; ++g) {
           const int sh = 16 * (g & 1);
           const uint32_t lo = (wc[g >> 1] >> sh) & 0xffffu, hi = (wp[H / 8 + (g >> 1)] >> sh) & 0xffffu;
-          X[g] = (lo | hi << 16) ^ 0x88888888u;
+          X[g] = lo | hi << 16;
         }
         const int nxt = p2_lo(hP[H - 1]);
 #pragma unroll
@@ -466,7 +466,7 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
 #pragma unroll
       for (int r = R - 1; r >= H; --r) {
         const int hd = r > H ? hc[r - 1] : hul;
-        const int m = hd + (int)((wp[r >> 3] >> ((r & 7) * 4)) & 15) - 8;
+        const int m = hd + ((int)(wp[r >> 3] << (28 - (r & 7) * 4)) >> 28);
         hc[r] = m < (hd << 3) ? m : (hd << 3);
       }
       int e = e_mid, cmax_l = 0;
