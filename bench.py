@@ -60,6 +60,14 @@ CONFIGS = {
         reads=["--n", "2000000", "--paired", "--len", "150", "--insert-mean", "400", "--insert-sd", "50",
                "--sub-start", "0.005", "--sub-end", "0.015", "--indel", "0.0005", "--seed", "12"],
         ref_len=100_000_000, n_reads=2_000_000),
+    "c4": dict(
+        workload="bwa-mem SE, 1 Gbp 24-contig synthetic reference (20% repeats), 5M pairs 2x150 bp as 10M reads",
+        ref=["--len", "1000000008", "--contigs", "24", "--seed", "21", "--gc-mean", "0.41", "--gc-sd", "0.05",
+             "--repeat-frac", "0.2", "--families", "200", "--fam-min", "300", "--fam-max", "6000",
+             "--div-min", "0.05", "--div-max", "0.15"],
+        reads=["--n", "10000000", "--paired", "--len", "150", "--insert-mean", "400", "--insert-sd", "50",
+               "--sub", "0.01", "--indel", "0.0005", "--seed", "22"],
+        ref_len=1_000_000_008, n_reads=10_000_000),
     "c3": dict(
         workload="bwa-mem SE, 1 Mbp uniform synthetic reference, 2M reads of 70-250 bp (5% at 8% divergence "
                  "with a 20-40 bp random tail)",

@@ -38,6 +38,9 @@ struct HostIndex {
 };
 
 bool build_index(const char *fasta, const char *prefix, int dense_intv, std::string &err);
+// Progress lines of build_index (phase, recursion level of the suffix sort,
+// length sorted) go to fn; null (the default) keeps the build silent.
+void set_build_progress(void (*fn)(const char *what, int level, int64_t n));
 bool write_bns(const std::string &prefix, const std::vector<std::string> &names,
                const std::vector<std::string> &annos, const std::vector<std::vector<uint8_t>> &seqs,
                std::vector<uint8_t> &fwd, std::string &err);

@@ -5,12 +5,20 @@
 // reference's `g3.exe mem` flags (bwa/src/fastmap.cpp:443-484).
 #include "g3a_index.h"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 
 int g3a_mem_main(int argc, char **argv);
+
+static std::chrono::steady_clock::time_point g_t0;
+static void index_progress(const char *what, int level, int64_t n) {
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - g_t0).count();
+  std::fprintf(stderr, "g3a index: %7.1f s  %s (level %d, %lld symbols)\n", t, what, level, (long long)n);
+  std::fflush(stderr);
+}
 
 static int index_main(int argc, char **argv) {
   int intv = 4;
@@ -26,6 +34,8 @@ static int index_main(int argc, char **argv) {
   const char *fa = argv[i];
   std::string prefix = i + 1 < argc ? argv[i + 1] : fa;
   std::string err;
+  g_t0 = std::chrono::steady_clock::now();
+  g3a::set_build_progress(index_progress);
   if (!g3a::build_index(fa, prefix.c_str(), intv, err)) {
     std::fprintf(stderr, "g3a index: %s\n", err.c_str());
     return 1;

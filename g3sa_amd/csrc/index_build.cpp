@@ -277,15 +277,21 @@ bool build_fm_from_fwd(const std::string &prefix, const std::vector<uint8_t> &fw
     text[(size_t)(2 * L - 1 - i)] = (uint8_t)(4 - fwd[(size_t)i]);  // complement, symbol = (3-b)+1
   }
   text[(size_t)(2 * L)] = 0;
+  bool ok;
   if (text.size() < 0xffffffffULL) {
     std::vector<uint32_t> SA;
     build_sa(text, SA);
-    return emit_fm(prefix, text, SA, dense_intv, err);
+    ok = emit_fm(prefix, text, SA, dense_intv, err);
+  } else {
+    std::vector<uint64_t> SA;
+    build_sa(text, SA);
+    ok = emit_fm(prefix, text, SA, dense_intv, err);
   }
-  std::vector<uint64_t> SA;
-  build_sa(text, SA);
-  return emit_fm(prefix, text, SA, dense_intv, err);
+  if (ok && sais_progress) sais_progress(".bwt .sa .g3sa .counts written", 0, (int64_t)text.size());
+  return ok;
 }
+
+void set_build_progress(void (*fn)(const char *what, int level, int64_t n)) { sais_progress = fn; }
 
 bool build_index(const char *fasta, const char *prefix, int dense_intv, std::string &err) {
   if (dense_intv < 1 || dense_intv > 32 || (dense_intv & (dense_intv - 1))) {
@@ -301,8 +307,10 @@ bool build_index(const char *fasta, const char *prefix, int dense_intv, std::str
     annos.push_back(c.anno);
     seqs.push_back(std::move(c.seq));
   }
+  if (sais_progress) sais_progress("reference read", 0, 0);
   std::vector<uint8_t> fwd;
   if (!write_bns(prefix, names, annos, seqs, fwd, err)) return false;
+  if (sais_progress) sais_progress(".pac .ann .amb written", 0, (int64_t)fwd.size());
   return build_fm_from_fwd(prefix, fwd, dense_intv, err);
 }
 

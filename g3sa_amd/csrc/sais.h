@@ -9,6 +9,10 @@
 
 namespace g3a {
 
+// Progress hook of the index build (null: silent): phase, recursion level,
+// length of the string being sorted.
+inline void (*sais_progress)(const char *what, int level, int64_t n) = nullptr;
+
 template <typename I>
 struct SaisTypes {
   std::vector<uint8_t> bits;  // 1 = S-type
@@ -33,13 +37,14 @@ static void sais_buckets(const C *s, I n, I K, std::vector<I> &bkt, bool end) {
 }
 
 template <typename C, typename I>
-static void sais_induce(const C *s, I *SA, I n, I K, const SaisTypes<I> &t, std::vector<I> &bkt) {
+static void sais_induce(const C *s, I *SA, I n, I K, const SaisTypes<I> &t, std::vector<I> &bkt, int level) {
   const I EMPTY = (I)~(I)0;
   sais_buckets(s, n, K, bkt, false);
   for (I i = 0; i < n; ++i) {
     I p = SA[i];
     if (p != EMPTY && p > 0 && !t.s((size_t)(p - 1))) SA[bkt[(size_t)s[p - 1]]++] = p - 1;
   }
+  if (sais_progress) sais_progress("L-type suffixes placed", level, (int64_t)n);
   sais_buckets(s, n, K, bkt, true);
   for (I i = n; i-- > 0;) {
     I p = SA[i];
@@ -49,7 +54,7 @@ static void sais_induce(const C *s, I *SA, I n, I K, const SaisTypes<I> &t, std:
 
 // SA of s[0..n); s[n-1] == 0 is the unique minimum.
 template <typename C, typename I>
-void sais(const C *s, I *SA, I n, I K) {
+void sais(const C *s, I *SA, I n, I K, int level = 0) {
   const I EMPTY = (I)~(I)0;
   SaisTypes<I> t((size_t)n);
   t.set((size_t)(n - 1), true);
@@ -61,7 +66,8 @@ void sais(const C *s, I *SA, I n, I K) {
   for (I i = 0; i < n; ++i) SA[i] = EMPTY;
   for (I i = 1; i < n; ++i)
     if (t.lms((size_t)i)) SA[--bkt[(size_t)s[i]]] = i;
-  sais_induce(s, SA, n, K, t, bkt);
+  sais_induce(s, SA, n, K, t, bkt, level);
+  if (sais_progress) sais_progress("LMS substrings sorted", level, (int64_t)n);
   I n1 = 0;
   for (I i = 0; i < n; ++i)
     if (t.lms((size_t)SA[i])) SA[n1++] = SA[i];
@@ -85,8 +91,9 @@ void sais(const C *s, I *SA, I n, I K) {
     if (SA[i] != EMPTY) SA[--j] = SA[i];
   // stage 2: sort the reduced string (recursively if names repeat)
   I *s1 = SA + n - n1, *SA1 = SA;
+  if (sais_progress) sais_progress("LMS substrings named", level, (int64_t)n);
   if (name < n1) {
-    sais<I, I>(s1, SA1, n1, name);
+    sais<I, I>(s1, SA1, n1, name, level + 1);
   } else {
     for (I i = 0; i < n1; ++i) SA1[s1[i]] = i;
   }
@@ -101,7 +108,8 @@ void sais(const C *s, I *SA, I n, I K) {
     SA[i] = EMPTY;
     SA[--bkt[(size_t)s[j]]] = j;
   }
-  sais_induce(s, SA, n, K, t, bkt);
+  sais_induce(s, SA, n, K, t, bkt, level);
+  if (sais_progress) sais_progress("suffixes induced", level, (int64_t)n);
 }
 
 }  // namespace g3a
