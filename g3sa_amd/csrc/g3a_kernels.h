@@ -10,8 +10,8 @@ namespace g3a {
 constexpr int kSlowLep = 400;                  // slow-path LEP list capacity
 constexpr int kSlowScratch = 5 * kSlowLep;     // Intv entries of scratch per slow read
 constexpr int kSlotCap = 64;                   // fast-path interval slots per read
-constexpr int kLepCap = 32;                    // fast-path LEP list capacity
-constexpr int kFoundCap = 8;                   // fast-path found-match (and reseed seed) list capacity
+constexpr int kLepCap = 64;                    // fast-path LEP list capacity
+constexpr int kFoundCap = 16;                  // fast-path found-match (and reseed seed) list capacity
 
 struct SeedArgs {
   DevIndex idx;
