@@ -12,7 +12,9 @@ One step = one pass of the full device pipeline (seeding, SA lookup, seed
 sort, chaining + chain filter, banded extension, region filter, global
 alignment + finalize) over the 2M-read batch already resident in HBM; outputs
 stay in HBM.  The CPU baseline is the oracle (C, all host cores) on a bounded
-sample of the same reads.
+sample of the same reads; its SAM for that sample is compared byte for byte
+with `g3a mem -S` on the same reads at the workload's full index size
+(`parity_sample` in the output line).
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c2|c3|c1]
                   [--form all|both|batched|plain|learned]
@@ -122,7 +124,9 @@ def load_reads(fq: Path):
     return table[buf[idx]], off.astype(np.int32)
 
 
-def cpu_baseline(prefix: Path, fq: Path, sample: int) -> dict | None:
+def cpu_baseline(prefix: Path, fq: Path, sample: int, sam_out: Path | None = None) -> dict | None:
+    """The oracle on the first `sample` reads; its timer covers the alignment
+    alone, so writing its SAM to `sam_out` (for the parity sample) is not timed."""
     if not ORACLE_CLI.exists():
         return None
     try:
@@ -131,14 +135,43 @@ def cpu_baseline(prefix: Path, fq: Path, sample: int) -> dict | None:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
     log(f"CPU baseline: oracle on {sample} reads, {cores} threads")
-    r = subprocess.run([str(ORACLE_CLI), "-t", str(cores), "-n", str(sample), "-o", "/dev/null", str(prefix), str(fq)],
-                       capture_output=True, text=True)
+    r = subprocess.run([str(ORACLE_CLI), "-t", str(cores), "-n", str(sample), "-m", "sam", "-o",
+                        str(sam_out) if sam_out else "/dev/null", str(prefix), str(fq)], capture_output=True, text=True)
     m = re.search(r"oracle_reads_per_s ([0-9.]+) threads (\d+) reads (\d+) seconds ([0-9.]+)", r.stderr)
     if r.returncode != 0 or not m:
         log(f"oracle failed: {r.stderr[-500:]}")
         return None
     return {"value": round(float(m.group(1)), 1), "unit": "reads/s", "cores": int(m.group(2)), "kind": "port",
             "sample": f"first {m.group(3)} reads of the same FASTQ, alignment only ({float(m.group(4)):.1f} s)"}
+
+
+def parity_sample(prefix: Path, fq: Path, sample: int, oracle_sam: Path, learned: bool) -> dict:
+    """Byte comparison, at the workload's full index size, of `g3a mem -S` on
+    the first `sample` reads against the oracle's SAM for the same reads."""
+    head = fq.with_name(f"{fq.stem}_head{sample}.fq")
+    if not head.exists():
+        with open(fq, "rb") as src, open(head, "wb") as dst:
+            for i, line in enumerate(src):
+                if i >= 4 * sample:
+                    break
+                dst.write(line)
+    got = oracle_sam.with_name(oracle_sam.stem + "_g3a.sam")
+    cmd = [str(g3sa_amd.CLI_PATH), "mem", "-S", "-t", "16", "-o", str(got)] + (["--learned"] if learned else [])
+    r = subprocess.run(cmd + [str(prefix), str(head)], capture_output=True, text=True)
+    out = {"reads": sample, "what": "g3a mem -S" + (" --learned" if learned else "") + " vs the oracle's SAM"}
+    if r.returncode != 0:
+        log(f"g3a mem failed on the parity sample: {r.stderr[-300:]}")
+        return {**out, "identical": False, "error": "g3a mem failed"}
+    a, b = got.read_bytes(), oracle_sam.read_bytes()
+    out.update(identical=a == b, bytes=len(b), lines=b.count(b"\n"))
+    if a != b:
+        la, lb = a.split(b"\n"), b.split(b"\n")
+        i = next((k for k in range(min(len(la), len(lb))) if la[k] != lb[k]), min(len(la), len(lb)))
+        out["first_difference"] = {"line": i, "g3a": la[i][:200].decode(errors="replace") if i < len(la) else None,
+                                   "oracle": lb[i][:200].decode(errors="replace") if i < len(lb) else None}
+        log(f"PARITY SAMPLE DIFFERS at line {i}")
+    got.unlink()
+    return out
 
 
 def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = True) -> dict:
@@ -342,7 +375,13 @@ def main() -> int:
         log("end-to-end: g3a mem")
         result["end_to_end"] = end_to_end(prefix, fq, main_form == "learned")
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(prefix, fq, min(args.cpu_sample, n))
+        sample = min(args.cpu_sample, n)
+        oracle_sam = Path(args.data) / f"{args.config}_oracle_{sample}.sam"
+        result["cpu_baseline"] = cpu_baseline(prefix, fq, sample, oracle_sam)
+        if result["cpu_baseline"] is not None:
+            log(f"parity sample: g3a mem vs the oracle on {sample} reads")
+            result["parity_sample"] = parity_sample(prefix, fq, sample, oracle_sam, main_form == "learned")
+            oracle_sam.unlink()
     print(json.dumps(result), flush=True)
     return 0
 
