@@ -244,11 +244,12 @@ G3A_HD void occ4_pair(const DevIndex &x, int64_t p1, int64_t p2, int64_t o1[4], 
   if ((p2 >> 6) != (p1 >> 6)) L = occ_line(x, p2 >> 6);
   occ_at(x, L, p2, o2);
 }
+// Narrow form: both lines are requested before either is used, so that when
+// they differ the two fetches overlap (the same line twice is a cache hit)
 G3A_HD void occ4_pair(const DevIndex &x, uint32_t p1, uint32_t p2, uint32_t o1[4], uint32_t o2[4]) {
-  OccLineN L = occn_line(x, p1 >> 5);
-  occ_at(x, L, p1, o1);
-  if ((p2 >> 5) != (p1 >> 5)) L = occn_line(x, p2 >> 5);
-  occ_at(x, L, p2, o2);
+  const OccLineN L1 = occn_line(x, p1 >> 5), L2 = occn_line(x, p2 >> 5);
+  occ_at(x, L1, p1, o1);
+  occ_at(x, L2, p2, o2);
 }
 
 G3A_HD int64_t occ1(const DevIndex &x, int64_t p, int c) {
@@ -320,10 +321,9 @@ G3A_HD uint32_t occ1_at(const DevIndex &x, const OccLineN &L, uint32_t p, int c)
 // output, and neither reads l.  Narrow layout: one base's count per row.
 G3A_HD void backward_ext_ks(const DevIndex &x, const IntvT<uint32_t> &in, int base, IntvT<uint32_t> &out) {
   const uint32_t p1 = in.k - 1, p2 = in.k + in.s - 1;
-  OccLineN L = occn_line(x, p1 >> 5);
-  const uint32_t a = occ1_at(x, L, p1, base);
-  if ((p2 >> 5) != (p1 >> 5)) L = occn_line(x, p2 >> 5);
-  const uint32_t b = occ1_at(x, L, p2, base);
+  const OccLineN L1 = occn_line(x, p1 >> 5), L2 = occn_line(x, p2 >> 5);  // overlapping fetches
+  const uint32_t a = occ1_at(x, L1, p1, base);
+  const uint32_t b = occ1_at(x, L2, p2, base);
   out.k = (uint32_t)pick4(x.count, base) + a;
   out.l = in.l;
   out.s = b - a;
