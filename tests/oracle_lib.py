@@ -49,10 +49,11 @@ class OrcBatch(C.Structure):
 
 
 class Oracle:
-    def __init__(self):
-        if not LIB.exists():
-            raise RuntimeError(f"{LIB} missing: build the oracle (make -C oracle)")
-        lib = C.CDLL(str(LIB))
+    def __init__(self, lib_path: Path | None = None):
+        path = Path(lib_path) if lib_path is not None else LIB
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: build the oracle (make -C oracle)")
+        lib = C.CDLL(str(path))
         lib.orc_opt_init.argtypes = [C.POINTER(OrcOpt)]
         lib.orc_index_load.argtypes = [C.c_char_p]
         lib.orc_index_load.restype = C.c_void_p
