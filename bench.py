@@ -1,31 +1,35 @@
 #!/usr/bin/env python3
-"""Benchmark: reads/s of the BWA-MEM hot path on one MI355X.
+"""Benchmark: reads/s of the BWA-MEM hot path on one MI355X, SAM included.
 
-Workload (the metric's configuration): a seeded 100 Mbp reference in 5 contigs
-(GC per 10 kb window ~ N(0.41, 0.05), 15% interspersed repeats from 200
-families of 300 bp-6 kbp at 5-15% divergence) and 1M pairs of 2x150 bp reads
-(insert N(400, 50), substitution rate 0.5%->1.5% along the read, 0.05% indels)
-aligned as 2M single-end reads with BWA default scoring.  Everything is
-generated deterministically on the machine (synthetic data).
+Workload (default, the metric's configuration): a seeded 1 Gbp reference in
+24 contigs (GC per 10 kb window ~ N(0.41, 0.05), 20% interspersed repeats
+from 200 families of 300 bp-6 kbp at 5-15% divergence) and 5M pairs of
+2x150 bp reads at 1% error aligned as 10M single-end reads with BWA default
+scoring.  Everything is generated deterministically on the machine
+(synthetic data); the index is built on the GPU (`g3a index -d 0`).
 
-One step = one pass of the full device pipeline (seeding, SA lookup, seed
-sort, chaining + chain filter, banded extension, region filter, global
-alignment + finalize) over the 2M-read batch already resident in HBM; outputs
-stay in HBM.  The CPU baseline is the oracle (C, all host cores) on a bounded
-sample of the same reads; its SAM for that sample is compared byte for byte
-with `g3a mem -S` on the same reads at the workload's full index size
-(`parity_sample` in the output line).
+One step = one batch of 10M reads, resident in HBM as 2-bit codes, names and
+base qualities, taken through the whole device pipeline (seeding, SA lookup,
+seed sort, chaining + chain filter, banded extension, region filter, global
+alignment + finalize) and formatted on the device into SAM records with
+MAPQ, NM, MD and AS (the last SAM byte in HBM).  `value` is reads per second
+of such steps.  The same steps through PCIe (reads in from the host, SAM out
+to the host) are reported as `pcie_inclusive`; the `g3a mem` command line over
+the FASTQ file as `end_to_end`.  The CPU baseline is the oracle (C, the box's
+16 host threads) on a bounded sample of the same reads, alignment plus record
+formatting; its SAM for that sample is compared byte for byte with `g3a mem
+-S` on the same reads at the workload's full index size (`parity_sample`).
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c2|c3|c1]
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config c4|c2|c3|c1]
                   [--form all|both|batched|plain|learned]
 
 Forms (identical output): plain (one wavefront per extension task), batched
 (length-binned persistent lane engine) and learned (batched plus the
-learned-index seeding: full SA, text comparison of unique matches, model
-intervals in the last round).  `value` is the learned form's rate when it
-ran (the metric's batched + learned form), else the batched form's; the
-others are reported beside it.  The learned index is built once per context
-on the device and, like the index upload, outside the timed steps.
+learned-index seeding).  `value` is the learned form's rate when it ran (the
+metric's batched + learned form), else the batched form's; the others are
+reported beside it on the same definition.  The learned index is built once
+per context on the device and, like the index upload, outside the timed
+steps.
 """
 from __future__ import annotations
 
@@ -47,11 +51,13 @@ import g3sa_amd  # noqa: E402
 
 ORACLE_CLI = ROOT / "oracle" / "_build" / "g3a_oracle"
 HBM_PEAK_GBS = 8000.0
-# k_seed's memory-side bytes per launch on the 100 Mbp / 2M-read workload,
-# from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE), each scaled by the
-# same counter read on a plain copy kernel of known size
-TRAFFIC_FILES = {"batched": ROOT / "profiles" / "k_seed_traffic_100Mbp_2Mx150.txt",
-                 "learned": ROOT / "profiles" / "k_seed_learned_traffic_100Mbp_2Mx150.txt"}
+# k_seed's memory-side bytes per launch, from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE), each scaled by the same counter read on a plain
+# copy kernel of known size; one file per form and workload
+TRAFFIC_FILES = {("c2", "batched"): "k_seed_traffic_100Mbp_2Mx150.txt",
+                 ("c2", "learned"): "k_seed_learned_traffic_100Mbp_2Mx150.txt",
+                 ("c4", "batched"): "k_seed_traffic_1Gbp_10Mx150.txt",
+                 ("c4", "learned"): "k_seed_learned_traffic_1Gbp_10Mx150.txt"}
 
 CONFIGS = {
     "c2": dict(
@@ -89,44 +95,76 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def prepare(cfg_name: str, data_dir: Path) -> tuple[Path, Path]:
+def prepare(cfg_name: str, data_dir: Path) -> tuple[Path, Path, dict]:
     cfg = CONFIGS[cfg_name]
     data_dir.mkdir(parents=True, exist_ok=True)
     fa, prefix, fq = data_dir / f"{cfg_name}.fa", data_dir / cfg_name, data_dir / f"{cfg_name}.fq"
+    info = {}
     if not fa.exists():
         log(f"generating reference ({cfg['ref_len']} bp)")
-        subprocess.run([str(g3sa_amd.GEN_PATH), "ref", "-o", str(fa), *cfg["ref"]], check=True)
-    if not (data_dir / f"{cfg_name}.g3sa").exists():
-        log("building index (SA-IS on the host)")
         t0 = time.time()
-        subprocess.run([str(g3sa_amd.CLI_PATH), "index", "-s", "4", str(fa), str(prefix)], check=True)
-        log(f"index built in {time.time() - t0:.1f} s")
+        subprocess.run([str(g3sa_amd.GEN_PATH), "ref", "-o", str(fa), *cfg["ref"]], check=True)
+        info["reference_gen_s"] = round(time.time() - t0, 1)
+    if not (data_dir / f"{cfg_name}.g3sa").exists():
+        log("building index on the GPU (g3a index -d 0)")
+        t0 = time.time()
+        subprocess.run([str(g3sa_amd.CLI_PATH), "index", "-d", "0", "-s", "4", str(fa), str(prefix)], check=True)
+        info["index_build_s"] = round(time.time() - t0, 1)
+        log(f"index built in {info['index_build_s']} s")
     if not fq.exists():
         log(f"generating {cfg['n_reads']} reads")
+        t0 = time.time()
         subprocess.run([str(g3sa_amd.GEN_PATH), "reads", "-r", str(fa), "-o", str(fq), *cfg["reads"]], check=True)
-    return prefix, fq
+        info["reads_gen_s"] = round(time.time() - t0, 1)
+    return prefix, fq, info
+
+
+def _ranges(buf: np.ndarray, lo: np.ndarray, hi: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """The bytes buf[lo[i]:hi[i]] concatenated, and their int64 offsets."""
+    lens = (hi - lo).astype(np.int64)
+    off = np.zeros(len(lens) + 1, dtype=np.int64)
+    np.cumsum(lens, out=off[1:])
+    idx = np.repeat(lo.astype(np.int64) - off[:-1], lens)
+    idx += np.arange(off[-1], dtype=np.int64)
+    return buf[idx], off
 
 
 def load_reads(fq: Path):
-    """Vectorised FASTQ parse (4-line records): 2-bit codes and offsets."""
+    """Vectorised FASTQ parse (4-line records) with the reference's rules:
+    2-bit codes and int32 offsets; names (up to the first space) and
+    qualities with int64 offsets."""
     buf = np.fromfile(fq, dtype=np.uint8)
     nl = np.flatnonzero(buf == 10)
     starts = np.concatenate([[0], nl[:-1] + 1])
-    seq_lo, seq_hi = starts[1::4], nl[1::4]
-    lens = (seq_hi - seq_lo).astype(np.int64)
-    off = np.zeros(len(lens) + 1, dtype=np.int64)
-    np.cumsum(lens, out=off[1:])
-    idx = np.repeat(seq_lo - off[:-1], lens) + np.arange(off[-1])
+    h_lo, h_hi = starts[0::4] + 1, nl[0::4]
+    sp = np.flatnonzero(buf == 32)
+    k = np.searchsorted(sp, h_lo)
+    cand = np.where(k < len(sp), sp[np.minimum(k, max(len(sp) - 1, 0))] if len(sp) else h_hi, h_hi)
+    n_hi = np.minimum(cand, h_hi)
+    names, name_off = _ranges(buf, h_lo, n_hi)
+    quals, qual_off = _ranges(buf, starts[3::4], nl[3::4])
+    codes, off = _ranges(buf, starts[1::4], nl[1::4])
     table = np.full(256, 4, dtype=np.uint8)
     for i, ch in enumerate(b"ACGT"):
         table[ch] = i
         table[ch + 32] = i
-    return table[buf[idx]], off.astype(np.int32)
+    return table[codes], off.astype(np.int32), names.tobytes(), name_off, quals.tobytes(), qual_off
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(prefix: Path, fq: Path, sample: int, sam_out: Path | None = None) -> dict | None:
-    """The oracle on the first `sample` reads; its timer covers the alignment
-    alone, so writing its SAM to `sam_out` (for the parity sample) is not timed."""
+    """The oracle on the first `sample` reads: alignment on the host threads
+    the box gives one GPU (16 at most), then SAM record formatting; writing
+    its SAM to `sam_out` (for the parity sample) is not timed."""
     if not ORACLE_CLI.exists():
         return None
     try:
@@ -138,11 +176,15 @@ def cpu_baseline(prefix: Path, fq: Path, sample: int, sam_out: Path | None = Non
     r = subprocess.run([str(ORACLE_CLI), "-t", str(cores), "-n", str(sample), "-m", "sam", "-o",
                         str(sam_out) if sam_out else "/dev/null", str(prefix), str(fq)], capture_output=True, text=True)
     m = re.search(r"oracle_reads_per_s ([0-9.]+) threads (\d+) reads (\d+) seconds ([0-9.]+)", r.stderr)
-    if r.returncode != 0 or not m:
+    f = re.search(r"oracle_format_seconds ([0-9.]+)", r.stderr)
+    if r.returncode != 0 or not m or not f:
         log(f"oracle failed: {r.stderr[-500:]}")
         return None
-    return {"value": round(float(m.group(1)), 1), "unit": "reads/s", "cores": int(m.group(2)), "kind": "port",
-            "sample": f"first {m.group(3)} reads of the same FASTQ, alignment only ({float(m.group(4)):.1f} s)"}
+    reads, t_aln, t_fmt = int(m.group(3)), float(m.group(4)), float(f.group(1))
+    return {"value": round(reads / (t_aln + t_fmt), 1), "unit": "reads/s", "cores": int(m.group(2)), "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"first {reads} reads of the same FASTQ: alignment on {m.group(2)} threads ({t_aln:.1f} s) "
+                      f"+ SAM record formatting on one ({t_fmt:.2f} s)"}
 
 
 def parity_sample(prefix: Path, fq: Path, sample: int, oracle_sam: Path, learned: bool) -> dict:
@@ -175,8 +217,10 @@ def parity_sample(prefix: Path, fq: Path, sample: int, oracle_sam: Path, learned
 
 
 def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = True) -> dict:
+    """Steps of align + device SAM formatting on the resident batch."""
     for i in range(warmup):
         al.run()
+        al.format(sam=True)
         log(f"[{form}] warmup step {i + 1}/{warmup} done")
     tuned = None
     if form in ("batched", "learned") and autotune:
@@ -184,19 +228,27 @@ def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = Tr
             al.run()
         tuned = al.autotune()
         log(f"[{form}] autotuned: {tuned}")
-    stage_ms, kern_ms = [], []
-    st = None
+    stage_ms, kern_ms, fmt_s = [], [], []
+    st, text_bytes = None, 0
     t0 = time.perf_counter()
     for i in range(steps):
         al.run()  # returns after the stream is synchronised
+        tf = time.perf_counter()
+        text_bytes = al.format(sam=True)  # records in HBM; returns after the stream is synchronised
+        fmt_s.append(time.perf_counter() - tf)
         st = al.stats()
         stage_ms.append(st["ms"])
         kern_ms.append(st["kernel_ms"])
-        log(f"[{form}] step {i + 1}/{steps}: {st['ms']['total']:.1f} ms device")
+        log(f"[{form}] step {i + 1}/{steps}: {st['ms']['total']:.1f} ms device pipeline, "
+            f"{1e3 * fmt_s[-1]:.1f} ms SAM formatting")
     elapsed = time.perf_counter() - t0
+    dev_ms = float(np.mean([s["total"] for s in stage_ms]))
     return {
         "reads_per_s": round(n * steps / elapsed, 1),
         "ms_per_step": round(1000.0 * elapsed / steps, 3),
+        "device_pipeline_reads_per_s": round(n / (dev_ms * 1e-3), 1),
+        "sam_format_ms": round(1e3 * float(np.mean(fmt_s)), 3),
+        "sam_bytes": text_bytes,
         "stages_ms": {k: round(float(np.mean([s[k] for s in stage_ms])), 3) for k in stage_ms[0]},
         "kernel_ms": {k: round(float(np.mean([s[k] for s in kern_ms])), 3) for k in kern_ms[0]},
         "work": st["work"],
@@ -204,10 +256,61 @@ def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = Tr
     }
 
 
+def pcie_inclusive(al, batch, steps: int) -> dict:
+    """The same step with the batch coming from host memory and the SAM
+    records going back to it (reads in over PCIe, text out)."""
+    seq, off, names, name_off, quals, qual_off = batch
+    n = len(off) - 1
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        al.upload(seq, off)
+        al.upload_text(names, name_off, quals, qual_off)
+        al.run()
+        al.format(sam=True)
+        al.text()
+    el = time.perf_counter() - t0
+    return {"reads_per_s": round(n * steps / el, 1), "ms_per_step": round(1e3 * el / steps, 3),
+            "what": "host codes, names and qualities in, align, SAM records formatted on the device, SAM text out"}
+
+
+def gather_ceiling() -> dict | None:
+    """Random-access ceiling for seeding: torch gathers of random 16-byte and
+    64-byte rows of a 1 GiB table (the Occ lines of a 1 Gbp index are 16-byte
+    rows of a 1 GB table), useful bytes per second."""
+    try:
+        import torch
+    except Exception:
+        return None
+    out = {}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    for row in (16, 64):
+        words = row // 4
+        rows = (1 << 30) // row
+        table = torch.zeros(rows, words, dtype=torch.int32, device="cuda")
+        idx = torch.randint(0, rows, (1 << 26,), device="cuda", generator=g)
+        for _ in range(2):
+            torch.index_select(table, 0, idx)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            torch.index_select(table, 0, idx)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        out[f"{row}B_rows_GBps"] = round(len(idx) * row / (ms * 1e-3) / 1e9, 1)
+        del table, idx
+    torch.cuda.empty_cache()
+    out["what"] = "torch.index_select of 2^26 random rows from a 1 GiB table; useful bytes per second"
+    return out
+
+
 def seed_traffic(config: str, form: str) -> int | None:
     """Bytes per k_seed launch recorded from the counter passes, or None."""
-    f = TRAFFIC_FILES.get(form)
-    if config != "c2" or f is None or not f.exists():
+    name = TRAFFIC_FILES.get((config, form))
+    f = ROOT / "profiles" / name if name else None
+    if f is None or not f.exists():
         return None
     kv = {}
     for line in f.read_text().splitlines():
@@ -230,11 +333,13 @@ def end_to_end(prefix: Path, fq: Path, learned: bool) -> dict | None:
         log(f"g3a mem failed: {r.stderr[-300:]}")
         return None
     out = {"reads_per_s": float(m.group(3)), "reads": int(m.group(1)), "seconds": float(m.group(2)),
-           "what": "g3a mem -S" + (" --learned" if learned else "") + ", standard SAM to /dev/null, batches of "
-                   "250k reads, index load (and learned-index build) excluded"}
-    b = re.search(r"busy time: load ([0-9.]+) s, align ([0-9.]+) s, format\+write ([0-9.]+) s", r.stderr)
+           "what": "g3a mem -S" + (" --learned" if learned else "") + ": FASTQ parsed on the host, SAM formatted on "
+                   "the device, written to /dev/null, batches of 250k reads, index load (and learned-index build) "
+                   "excluded"}
+    b = re.search(r"busy time: load ([0-9.]+) s, align\+format ([0-9.]+) s, write ([0-9.]+) s", r.stderr)
     if b:
-        out["stage_busy_s"] = {"load": float(b.group(1)), "align": float(b.group(2)), "format_write": float(b.group(3))}
+        out["stage_busy_s"] = {"load": float(b.group(1)), "align_format": float(b.group(2)),
+                               "write": float(b.group(3))}
     return out
 
 
@@ -243,7 +348,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--data", default=os.environ.get("G3A_DATA", "/tmp/g3a_bench"))
     ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -276,13 +381,17 @@ def main() -> int:
               "(reads are independent: see DESIGN.md for how the path shards)", file=sys.stderr)
         return 2
     cfg = CONFIGS[args.config]
-    prefix, fq = prepare(args.config, Path(args.data))
+    prefix, fq, prep = prepare(args.config, Path(args.data))
     log("loading reads")
-    seq, off = load_reads(fq)
+    batch = load_reads(fq)
+    seq, off = batch[0], batch[1]
     n = len(off) - 1
     log(f"{n} reads, {int(off[-1])} bases; creating device context")
+    t0 = time.time()
     al = g3sa_amd.Aligner(str(prefix), max_reads=n, max_bases=int(off[-1]))
+    prep["index_load_upload_s"] = round(time.time() - t0, 1)
     al.upload(seq, off)
+    al.upload_text(*batch[2:])
     # seeding grid: fixed, or the autotuner's pick on one aligned pass
     if args.seed_blocks is not None:
         al.set_seed_blocks(args.seed_blocks)
@@ -313,24 +422,33 @@ def main() -> int:
             per_form[form]["learned_work"] = al.learned_stats()
     main_form = "learned" if "learned" in per_form else ("batched" if "batched" in per_form else forms[0])
     m = per_form[main_form]
+    log("PCIe-inclusive pass")
+    pcie = pcie_inclusive(al, batch, 1)
     mean, kms, work = m["stages_ms"], m["kernel_ms"], m["work"]
     # dominant kernel and its roofline (k_seed: HBM; extension: integer VALU)
     ext_ms = kms["ext_lane"] + kms["ext_wave"]
-    # algorithmic bytes of k_seed: each FM step reads two 32-byte Occ lines
-    # (rows k-1 and k+s-1), plus the read's bases once; in the learned form
-    # also one 4-byte SA entry per text comparison and 2 bits per base it
-    # covers, and per model lookup 280 bytes of root and leaf parameters and
-    # 16 eight-byte keys of window search
+    # algorithmic bytes of k_seed: each FM step reads the Occ lines of rows
+    # k-1 and k+s-1: two 16-byte lines below 2^32 text rows (two 32-byte
+    # lines above), plus the read's bases once; in the learned form also one
+    # 4-byte SA entry per text comparison and 2 bits per base it covers, and
+    # per model lookup 280 bytes of root and leaf parameters and 16
+    # eight-byte keys of window search
+    narrow = 2 * cfg["ref_len"] + 1 < (1 << 32)
+    step_bytes = 32 if narrow else 64
     lw = m.get("learned_work") or {}
-    seed_bytes = (64 * work["fm_steps"] + int(off[-1]) + 4 * lw.get("text_runs", 0)
+    seed_bytes = (step_bytes * work["fm_steps"] + int(off[-1]) + 4 * lw.get("text_runs", 0)
                   + lw.get("text_run_bases", 0) // 4 + (280 + 128) * lw.get("lookups", 0))
     seed_gbs = seed_bytes / (kms["k_seed"] * 1e-3) / 1e9 if kms["k_seed"] > 0 else 0.0
     dominant = "k_seed" if kms["k_seed"] >= ext_ms else "extension"
     roof = {"bound": "hbm", "kernel": "k_seed", "achieved": round(seed_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(seed_gbs / HBM_PEAK_GBS, 4), "traffic": seed_traffic(args.config, main_form),
-            "algorithmic_bytes": seed_bytes, "launch_ms": round(kms["k_seed"], 3)}
+            "algorithmic_bytes": seed_bytes, "bytes_per_fm_step": step_bytes, "launch_ms": round(kms["k_seed"], 3)}
     if dominant != "k_seed":
         roof["dominant"] = dominant
+    log("random-gather ceiling")
+    ceiling = gather_ceiling()
+    if ceiling:
+        roof["random_gather_ceiling_GBps"] = ceiling
     # cells of the reference's extension cell set; the batched form counts
     # only the cells it computes (pruning skips the rest), so take the plain
     # form's count when it ran
@@ -350,7 +468,12 @@ def main() -> int:
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "reads": n, "bases": int(off[-1]),
-                   "form": "batched+learned" if main_form == "learned" else main_form},
+                   "form": "batched+learned" if main_form == "learned" else main_form,
+                   "step": "resident reads -> alignments -> SAM records with MAPQ/NM/MD/AS in HBM"},
+        "device_pipeline_reads_per_s": m["device_pipeline_reads_per_s"],
+        "sam_format_ms": m["sam_format_ms"],
+        "sam_bytes": m["sam_bytes"],
+        "pcie_inclusive": pcie,
         "stages_ms": mean,
         "kernel_ms": kms,
         "work": work,
@@ -358,9 +481,12 @@ def main() -> int:
         "extension_cells": {"reference_cell_set": ref_cells, "computed": work["ext_cells"]},
         "tuning": m.get("tuning"),
         "seed_tuning": seed_tuning,
-        "forms": {f: {"reads_per_s": v["reads_per_s"], "ms_per_step": v["ms_per_step"], "kernel_ms": v["kernel_ms"],
+        "forms": {f: {"reads_per_s": v["reads_per_s"], "ms_per_step": v["ms_per_step"],
+                      "device_pipeline_reads_per_s": v["device_pipeline_reads_per_s"],
+                      "sam_format_ms": v["sam_format_ms"], "kernel_ms": v["kernel_ms"],
                       "stages_ms": v["stages_ms"], "fm_steps": v["work"]["fm_steps"]} for f, v in per_form.items()},
         "roofline": roof,
+        "prepare": prep,
     }
     if learned_info is not None:
         result["learned_index"] = {**learned_info, "seed_tuning": learned_tuning,
@@ -371,6 +497,7 @@ def main() -> int:
             result["learned_vs_batched"] = round(per_form["learned"]["reads_per_s"] /
                                                  per_form["batched"]["reads_per_s"], 3)
     al.close()
+    del batch, seq, off
     if not args.no_e2e:
         log("end-to-end: g3a mem")
         result["end_to_end"] = end_to_end(prefix, fq, main_form == "learned")

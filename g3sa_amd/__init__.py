@@ -57,6 +57,8 @@ class BatchOut(C.Structure):
         ("ms_extend", C.c_float), ("ms_traceback", C.c_float), ("ms_total", C.c_float),
         ("n_seeds", C.c_int64), ("n_tasks", C.c_int64), ("ext_cells", C.c_int64),
         ("fm_steps", C.c_int64), ("n_slow_reads", C.c_int64),
+        ("mapq", C.POINTER(C.c_int32)), ("md_off", C.POINTER(C.c_int64)), ("md", C.POINTER(C.c_char)),
+        ("n_md_bytes", C.c_int64),
     ]
 
 
@@ -77,7 +79,8 @@ EXPORTS = [
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
     "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
-    "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed",
+    "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed", "g3a_build_index_dev", "g3a_memcpy_text",
+    "g3a_format_records", "g3a_copy_text",
 ]
 
 _lib = None
@@ -103,6 +106,10 @@ def load_library() -> C.CDLL:
     lib.g3a_last_error.argtypes = [C.c_void_p]
     lib.g3a_last_error.restype = C.c_char_p
     lib.g3a_build_index.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+    lib.g3a_build_index_dev.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+    lib.g3a_memcpy_text.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.g3a_format_records.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+    lib.g3a_copy_text.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     lib.g3a_last_stats.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int64)]
     lib.g3a_set_form.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_kernel_ms.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
@@ -134,6 +141,15 @@ def build_index(fasta: str, prefix: str, sa_intv: int = 4) -> None:
         raise RuntimeError(f"g3a_build_index failed ({rc}): {err.value.decode()}")
 
 
+def build_index_dev(fasta: str, prefix: str, sa_intv: int = 4, device: int = 0) -> None:
+    """The same index files, with the suffix sort and BWT built on a HIP device."""
+    lib = load_library()
+    err = C.create_string_buffer(512)
+    rc = lib.g3a_build_index_dev(device, str(fasta).encode(), str(prefix).encode(), sa_intv, err, 512)
+    if rc != G3A_OK:
+        raise RuntimeError(f"g3a_build_index_dev failed ({rc}): {err.value.decode()}")
+
+
 class Result:
     """Alignments of one batch as numpy arrays (copied out of g3a_batch_out)."""
 
@@ -153,6 +169,10 @@ class Result:
         self.timing = {k: getattr(out, k) for k in
                        ["ms_seed", "ms_sa", "ms_chain", "ms_extend", "ms_traceback", "ms_total"]}
         self.work = {k: getattr(out, k) for k in ["n_seeds", "n_tasks", "ext_cells", "fm_steps", "n_slow_reads"]}
+        self.mapq = np.ctypeslib.as_array(out.mapq, shape=(max(na, 1),))[:na].copy()
+        md_off = np.ctypeslib.as_array(out.md_off, shape=(na + 1,)).copy()
+        blob = C.string_at(out.md, out.n_md_bytes)
+        self.md = [blob[md_off[i]:md_off[i + 1] - 1].decode() for i in range(na)]
 
 
 class Aligner:
@@ -263,6 +283,29 @@ class Aligner:
         keys_k = ["k_seed", "ext_lane", "ext_wave", "k_traceback"]
         return {"ms": dict(zip(keys_ms, list(ms))), "work": dict(zip(keys_w, list(work))),
                 "kernel_ms": dict(zip(keys_k, list(kms)))}
+
+    def upload_text(self, names: bytes, name_off: np.ndarray, quals: bytes, qual_off: np.ndarray) -> None:
+        """Names and qualities of the resident batch, for the device formatter."""
+        name_off = np.ascontiguousarray(name_off, dtype=np.int64)
+        qual_off = np.ascontiguousarray(qual_off, dtype=np.int64)
+        nb = np.frombuffer(names, dtype=np.uint8) if len(names) else np.zeros(1, np.uint8)
+        qb = np.frombuffer(quals, dtype=np.uint8) if len(quals) else np.zeros(1, np.uint8)
+        self._keep_text = (nb, qb, name_off, qual_off)
+        self._check(self.lib.g3a_memcpy_text(self.ctx, len(name_off) - 1, nb.ctypes.data, name_off.ctypes.data,
+                                             qb.ctypes.data, qual_off.ctypes.data))
+
+    def format(self, sam: bool = True) -> int:
+        """Format the last batch's records on the device; returns their bytes."""
+        n = C.c_int64()
+        self._check(self.lib.g3a_format_records(self.ctx, 1 if sam else 0, C.byref(n)))
+        self._text_bytes = n.value
+        return n.value
+
+    def text(self) -> bytes:
+        """Copy the records of the last format() to the host."""
+        buf = C.create_string_buffer(max(self._text_bytes, 1))
+        self._check(self.lib.g3a_copy_text(self.ctx, buf, self._text_bytes))
+        return buf.raw[:self._text_bytes]
 
     def align(self, seq: np.ndarray, offsets: np.ndarray) -> Result:
         self.upload(seq, offsets)

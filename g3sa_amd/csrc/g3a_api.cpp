@@ -44,6 +44,13 @@ struct g3a_ctx {
   int64_t max_bases = 0;
   // index
   DevBuf occ, pac, sa, ctg_off, ctg_len, ctg_alt, kmer, occn, sbc;
+  DevBuf ctg_names, ctg_name_off, log_tab;  // for the record formatter
+  // batch text (names, qualities) and formatted records
+  bool has_text = false;
+  int text_n = -1;
+  DevBuf names, name_off, quals, qual_off, txt_len, txt_off, txt;
+  int64_t txt_bytes = -1;
+  DevBuf mapq, md_len, md_off, md;  // per alignment, for g3a_memcpy_output
   // batch input
   int n_reads = 0;
   bool has_input = false, has_output = false;
@@ -207,6 +214,19 @@ int g3a_build_index(const char *fasta, const char *prefix, int sa_intv, char *er
   return e == "null argument" ? G3A_E_ARG : G3A_E_INDEX;
 }
 
+int g3a_build_index_dev(int device, const char *fasta, const char *prefix, int sa_intv, char *err, int err_len) {
+  std::string e;
+  int rc = G3A_E_INDEX;
+  if (!fasta || !prefix || device < 0) {
+    e = "invalid argument";
+    rc = G3A_E_ARG;
+  } else if (build_index(fasta, prefix, sa_intv, e, device)) {
+    return G3A_OK;
+  }
+  if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", e.c_str());
+  return rc;
+}
+
 int g3a_device_alloc(int device, const char *prefix, const g3a_opt *o, int max_reads, int64_t max_bases,
                      g3a_ctx **out) {
   if (!out || !prefix || max_reads < 1 || max_bases < 1) return G3A_E_ARG;
@@ -333,11 +353,39 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   if (!ctx) return G3A_E_ARG;
   HIPCK(hipSetDevice(ctx->device));
   const HostIndex &h = ctx->h;
-  std::vector<uint32_t> lines;
   DevIndex &x = ctx->idx;
-  if (!build_occ_lines(h, lines, x.super)) return fail(ctx, G3A_E_INDEX, "index larger than 2^34 rows");
-  if (ensure(ctx, ctx->occ, lines.size() * 4)) return G3A_E_HIP;
-  HIPCK(hipMemcpy(ctx->occ.p, lines.data(), lines.size() * 4, hipMemcpyHostToDevice));
+  const uint64_t rows = h.seq_len + 1;
+  if (rows > (4ULL << 32)) return fail(ctx, G3A_E_INDEX, "index larger than 2^34 rows");
+  const bool narrow = rows < (1ULL << 32);
+  // Occ lines, built on the device from the BWA .bwt body (k_occ_lines)
+  {
+    DevBuf words;
+    if (ensure(ctx, words, h.bwt_words.size() * 4)) return G3A_E_HIP;
+    if (hipMemcpy(words.p, h.bwt_words.data(), h.bwt_words.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      free_buf(words);
+      return fail(ctx, G3A_E_HIP, "copy of the .bwt body to the device failed");
+    }
+    const size_t wide_bytes = (size_t)((rows >> 6) + 1) * 32, narrow_bytes = (size_t)((rows >> 5) + 1) * 16,
+                 sb_bytes = (size_t)((rows >> kNarrowSbShift) + 1) * 16;
+    if (ensure(ctx, ctx->occ, wide_bytes)) { free_buf(words); return G3A_E_HIP; }
+    if (narrow && (ensure(ctx, ctx->occn, narrow_bytes) || ensure(ctx, ctx->sbc, sb_bytes))) {
+      free_buf(words);
+      return G3A_E_HIP;
+    }
+    launch_occ_lines(P<uint32_t>(words), h.seq_len, h.primary, P<uint32_t>(ctx->occ),
+                     narrow ? P<uint32_t>(ctx->occn) : nullptr, narrow ? P<uint32_t>(ctx->sbc) : nullptr,
+                     ctx->stream);
+    const hipError_t e1 = hipStreamSynchronize(ctx->stream), e2 = hipGetLastError();
+    free_buf(words);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+      return fail(ctx, G3A_E_HIP, std::string("Occ line build: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    // counts before each 2^32-row superblock of the wide lines
+    for (uint64_t sb = 0; sb < 4; ++sb) {
+      uint64_t c[4] = {0, 0, 0, 0};
+      if ((sb << 32) < rows) bwa_occ_rows(h.bwt_words.data(), h.primary, sb << 32, c);
+      for (int k = 0; k < 4; ++k) x.super[sb][k] = (int64_t)c[k];
+    }
+  }
   // 16 zero bytes of padding: word-wise readers may read past the last base
   if (ensure(ctx, ctx->pac, h.pac.size() + 16)) return G3A_E_HIP;
   HIPCK(hipMemset(ctx->pac.p, 0, h.pac.size() + 16));
@@ -353,6 +401,22 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   HIPCK(hipMemcpy(ctx->ctg_off.p, co.data(), co.size() * 8, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(ctx->ctg_len.p, cl.data(), cl.size() * 8, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(ctx->ctg_alt.p, ca.data(), ca.size() * 4, hipMemcpyHostToDevice));
+  {  // contig names and the MAPQ log table, for the record formatter
+    std::string all;
+    std::vector<int64_t> noff(1, 0);
+    for (auto &c : h.anns) {
+      all += c.name;
+      noff.push_back((int64_t)all.size());
+    }
+    std::vector<float> lt(kLogTab);
+    for (int i = 0; i < kLogTab; ++i) lt[(size_t)i] = logf((float)i);
+    if (ensure(ctx, ctx->ctg_names, all.size() + 1) || ensure(ctx, ctx->ctg_name_off, noff.size() * 8) ||
+        ensure(ctx, ctx->log_tab, lt.size() * 4))
+      return G3A_E_HIP;
+    HIPCK(hipMemcpy(ctx->ctg_names.p, all.c_str(), all.size() + 1, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->ctg_name_off.p, noff.data(), noff.size() * 8, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->log_tab.p, lt.data(), lt.size() * 4, hipMemcpyHostToDevice));
+  }
   x.occ = P<uint32_t>(ctx->occ);
   build_counts(h, x.count);
   x.primary = (int64_t)h.primary;
@@ -368,12 +432,7 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   x.ctg_alt = P<int32_t>(ctx->ctg_alt);
   x.kmer = nullptr;
   x.occn = x.sbc = nullptr;
-  if (x.n + 1 < (1LL << 32)) {  // 32-bit interval fields
-    std::vector<uint32_t> nl, sb;
-    if (!build_occ_narrow(h, nl, sb)) return fail(ctx, G3A_E_INTERNAL, "narrow Occ layout");
-    if (ensure(ctx, ctx->occn, nl.size() * 4) || ensure(ctx, ctx->sbc, sb.size() * 4)) return G3A_E_HIP;
-    HIPCK(hipMemcpy(ctx->occn.p, nl.data(), nl.size() * 4, hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy(ctx->sbc.p, sb.data(), sb.size() * 4, hipMemcpyHostToDevice));
+  if (narrow) {  // 32-bit interval fields
     x.occn = P<uint32_t>(ctx->occn);
     x.sbc = P<uint32_t>(ctx->sbc);
     if (ensure(ctx, ctx->kmer, ((size_t)3 << (2 * kKmer)) * 4)) return G3A_E_HIP;
@@ -408,6 +467,39 @@ int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_
   ctx->n_reads = n_reads;
   ctx->has_input = true;
   ctx->has_output = false;
+  ctx->has_text = false;
+  ctx->txt_bytes = -1;
+  return G3A_OK;
+}
+
+int g3a_memcpy_text(g3a_ctx *ctx, int n_reads, const char *names, const int64_t *name_off, const char *quals,
+                    const int64_t *qual_off) {
+  if (!ctx || n_reads < 0 || (n_reads > 0 && (!names || !name_off || !quals || !qual_off))) return G3A_E_ARG;
+  if (!ctx->has_input || n_reads != ctx->n_reads)
+    return fail(ctx, G3A_E_STATE, "g3a_memcpy_text needs the batch of the last g3a_memcpy_input (same n_reads)");
+  for (int i = 0; i < n_reads; ++i)
+    if (name_off[i + 1] < name_off[i] || qual_off[i + 1] < qual_off[i])
+      return fail(ctx, G3A_E_ARG, "name and quality offsets must be non-decreasing");
+  if (n_reads > 0 && (name_off[0] != 0 || qual_off[0] != 0))
+    return fail(ctx, G3A_E_ARG, "name_off[0] and qual_off[0] must be 0");
+  HIPCK(hipSetDevice(ctx->device));
+  const int64_t nb = n_reads ? name_off[n_reads] : 0, qb = n_reads ? qual_off[n_reads] : 0;
+  const size_t ob = (size_t)(n_reads + 1) * 8;
+  if (ensure(ctx, ctx->names, (size_t)nb + 16) || ensure(ctx, ctx->quals, (size_t)qb + 16) ||
+      ensure(ctx, ctx->name_off, ob) || ensure(ctx, ctx->qual_off, ob))
+    return G3A_E_HIP;
+  if (n_reads > 0) {
+    if (int rc = stage_h2d(ctx, ctx->names.p, names, (size_t)nb)) return rc;
+    if (int rc = stage_h2d(ctx, ctx->quals.p, quals, (size_t)qb)) return rc;
+    if (int rc = stage_h2d(ctx, ctx->name_off.p, name_off, ob)) return rc;
+    if (int rc = stage_h2d(ctx, ctx->qual_off.p, qual_off, ob)) return rc;
+  } else {
+    HIPCK(hipMemsetAsync(ctx->name_off.p, 0, 8, ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->qual_off.p, 0, 8, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->has_text = true;
+  ctx->text_n = n_reads;
   return G3A_OK;
 }
 
@@ -700,6 +792,66 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   return G3A_OK;
 }
 
+static TextArgs text_args(g3a_ctx *ctx, int mode) {
+  TextArgs t{};
+  t.mode = mode;
+  t.n_reads = ctx->n_reads;
+  t.seq = P<uint8_t>(ctx->seq);
+  t.off = P<int32_t>(ctx->off);
+  t.names = P<char>(ctx->names);
+  t.name_off = P<int64_t>(ctx->name_off);
+  t.quals = P<char>(ctx->quals);
+  t.qual_off = P<int64_t>(ctx->qual_off);
+  t.ctg_names = P<char>(ctx->ctg_names);
+  t.ctg_name_off = P<int64_t>(ctx->ctg_name_off);
+  t.ctg_off = P<int64_t>(ctx->ctg_off);
+  t.pac = ctx->idx.pac;
+  t.l_pac = ctx->idx.l_pac;
+  t.aln_off = P<int64_t>(ctx->aln_off);
+  t.alns = P<Aln>(ctx->alns);
+  t.cigar = P<uint32_t>(ctx->cig_out);  // the compacted CIGARs (k_cig_gather)
+  t.log_tab = P<float>(ctx->log_tab);
+  t.a = ctx->opt.a;
+  t.b = ctx->opt.b;
+  t.min_seed_len = ctx->opt.min_seed_len;
+  return t;
+}
+
+int g3a_format_records(g3a_ctx *ctx, int mode, int64_t *bytes) {
+  if (!ctx || (mode != 0 && mode != 1)) return G3A_E_ARG;
+  if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "no alignment results");
+  if (!ctx->has_text || ctx->text_n != ctx->n_reads)
+    return fail(ctx, G3A_E_STATE, "no names and qualities for this batch (g3a_memcpy_text)");
+  HIPCK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int n = ctx->n_reads;
+  if (ensure(ctx, ctx->txt_len, (size_t)n * 8 + 8) || ensure(ctx, ctx->txt_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(n + 1)))
+    return G3A_E_HIP;
+  const TextArgs t = text_args(ctx, mode);
+  launch_text_len(t, P<int64_t>(ctx->txt_len), s);
+  scan_i64(P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->txt_off) + n})) return G3A_E_HIP;
+  const int64_t total = ctx->pinned[0];
+  if (ensure(ctx, ctx->txt, (size_t)total + 16)) return G3A_E_HIP;
+  launch_text_write(t, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), s);
+  HIPCK(hipStreamSynchronize(s));
+  HIPCK(hipGetLastError());
+  ctx->txt_bytes = total;
+  if (bytes) *bytes = total;
+  return G3A_OK;
+}
+
+int g3a_copy_text(g3a_ctx *ctx, char *dst, int64_t cap) {
+  if (!ctx || (!dst && cap > 0)) return G3A_E_ARG;
+  if (ctx->txt_bytes < 0) return fail(ctx, G3A_E_STATE, "no formatted records (g3a_format_records)");
+  if (cap < ctx->txt_bytes) return fail(ctx, G3A_E_ARG, "text buffer smaller than the formatted records");
+  HIPCK(hipSetDevice(ctx->device));
+  if (ctx->txt_bytes > 0)
+    if (int rc = stage_d2h(ctx, dst, ctx->txt.p, (size_t)ctx->txt_bytes)) return rc;
+  return G3A_OK;
+}
+
 int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   if (!ctx || !out) return G3A_E_ARG;
   if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "no alignment results");
@@ -759,6 +911,34 @@ int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
     w += a.n_core;
   }
   out->n_cigar_words = w;
+  // MAPQ and MD strings, computed on the device
+  out->mapq = (int32_t *)std::malloc(4 * m);
+  out->md_off = (int64_t *)std::malloc(8 * (m + 1));
+  out->md_off[0] = 0;
+  out->n_md_bytes = 0;
+  if (na) {
+    if (ensure(ctx, ctx->mapq, (size_t)na * 4) || ensure(ctx, ctx->md_len, (size_t)na * 8) ||
+        ensure(ctx, ctx->md_off, (size_t)(na + 1) * 8) || ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(na + 1)))
+      return G3A_E_HIP;
+    const TextArgs t = text_args(ctx, 1);
+    launch_aln_mapq_md(t, na, P<int32_t>(ctx->aln_read), P<int32_t>(ctx->mapq), P<int64_t>(ctx->md_len),
+                       ctx->stream);
+    scan_i64(P<int64_t>(ctx->md_len), P<int64_t>(ctx->md_off), na, ctx->scan_tmp.p, ctx->scan_tmp.bytes,
+             ctx->stream);
+    if (readback(ctx, {P<int64_t>(ctx->md_off) + na})) return G3A_E_HIP;
+    const int64_t mdb = ctx->pinned[0];
+    if (ensure(ctx, ctx->md, (size_t)mdb + 16)) return G3A_E_HIP;
+    launch_aln_md_write(t, na, P<int32_t>(ctx->aln_read), P<int64_t>(ctx->md_off), P<char>(ctx->md), ctx->stream);
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    HIPCK(hipGetLastError());
+    out->md = (char *)std::malloc((size_t)mdb + 1);
+    if (int rc = stage_d2h(ctx, out->mapq, ctx->mapq.p, (size_t)na * 4)) return rc;
+    if (int rc = stage_d2h(ctx, out->md_off, ctx->md_off.p, (size_t)(na + 1) * 8)) return rc;
+    if (int rc = stage_d2h(ctx, out->md, ctx->md.p, (size_t)mdb)) return rc;
+    out->n_md_bytes = mdb;
+  } else {
+    out->md = (char *)std::malloc(1);
+  }
   out->ms_seed = ctx->ms[0]; out->ms_sa = ctx->ms[1]; out->ms_chain = ctx->ms[2];
   out->ms_extend = ctx->ms[3]; out->ms_traceback = ctx->ms[4]; out->ms_total = ctx->ms[5];
   int64_t c[4] = {0, 0, 0, 0};
@@ -1040,7 +1220,7 @@ void g3a_batch_out_free(g3a_batch_out *o) {
   std::free(o->aln_off); std::free(o->rid); std::free(o->pos); std::free(o->flag); std::free(o->is_rev);
   std::free(o->score); std::free(o->nw_score); std::free(o->nm); std::free(o->qb); std::free(o->qe);
   std::free(o->rb); std::free(o->re); std::free(o->n_cigar); std::free(o->cigar_off); std::free(o->n_core);
-  std::free(o->core_off); std::free(o->cigar);
+  std::free(o->core_off); std::free(o->cigar); std::free(o->mapq); std::free(o->md_off); std::free(o->md);
   std::memset(o, 0, sizeof *o);
 }
 
@@ -1060,7 +1240,10 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
                     &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
                     &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr, &ctx->tb_nw, &ctx->li_saf, &ctx->li_keys,
-                    &ctx->li_root, &ctx->li_leaf, &ctx->li_leaf_a, &ctx->li_mom, &ctx->li_short, &ctx->li_ctr};
+                    &ctx->li_root, &ctx->li_leaf, &ctx->li_leaf_a, &ctx->li_mom, &ctx->li_short, &ctx->li_ctr,
+                    &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,
+                    &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->mapq, &ctx->md_len,
+                    &ctx->md_off, &ctx->md};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

@@ -157,6 +157,86 @@ G3A_HD void line_counts(uint64_t w0, uint64_t w1, int y, uint32_t o[4]) {
   o[0] = (uint32_t)(y + 1) - n1 - n2 - n3;
 }
 
+// The BWA .bwt body (bwt.c:385-419 layout): per 128 positions of the BWT
+// string ('$' row left out), four uint64 counts of the positions before the
+// block (eight uint32 words), then eight words of 16 symbols, the first
+// symbol in the highest bits.  Row r is string position r below the '$' row
+// `primary` and r - 1 above it.
+// Counts of each base at string positions [0, k).
+G3A_HD void bwa_occ_before(const uint32_t *W, uint64_t k, uint64_t o[4]) {
+  const uint32_t *blk = W + (k >> 7) * 16;
+#pragma unroll
+  for (int x = 0; x < 4; ++x) o[x] = (uint64_t)blk[2 * x] | (uint64_t)blk[2 * x + 1] << 32;
+  uint32_t rem = (uint32_t)(k & 127);
+  for (int j = 0; rem > 0; ++j) {
+    const uint32_t w = blk[8 + j], t = rem < 16 ? rem : 16;
+    const uint32_t m = t == 16 ? 0x55555555u : 0x55555555u & ~(0xffffffffu >> (2 * t));
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const uint32_t d = w ^ (0x55555555u * (uint32_t)x);  // symbols equal to x become 00
+      o[x] += (uint64_t)G3A_POPC((uint64_t)(~(d | d >> 1) & m));
+    }
+    rem -= t;
+  }
+}
+// Base of string position k.
+G3A_HD uint32_t bwa_sym(const uint32_t *W, uint64_t k) {
+  return (W[(k >> 7) * 16 + 8 + ((k & 127) >> 4)] >> ((15 - (k & 15)) << 1)) & 3u;
+}
+// Counts of each base in rows [0, r) ('$' not counted).
+G3A_HD void bwa_occ_rows(const uint32_t *W, uint64_t primary, uint64_t r, uint64_t o[4]) {
+  bwa_occ_before(W, r <= primary ? r : r - 1, o);
+}
+
+constexpr int kNarrowSbShift = 21;  // narrow lines count relative to 2^21-row superblocks
+
+// Line L of the Occ tables from the BWA .bwt body W (rows 0..n).  Lines
+// 0 .. rows/64 are the wide lines (64 rows, 32 B: four uint32 counts
+// relative to the line's 2^32-row superblock, then the symbols as two
+// uint64 words, first row highest); when `narrow` is set, lines
+// rows/64 + 1 .. are the narrow lines (32 rows, 16 B: counts of bases 0-2
+// relative to the 2^21-row superblock at bits 0-20, 21-41, 42-62, then the
+// symbols), and the first narrow line of each superblock writes its four
+// absolute counts to sbc.  The '$' row is stored as 0 and never counted.
+G3A_HD void occ_line_from_bwa(const uint32_t *W, uint64_t n, uint64_t primary, uint64_t L, uint32_t *wide,
+                              uint32_t *narrow, uint32_t *sbc) {
+  const uint64_t rows = n + 1, n_wide = (rows >> 6) + 1;
+  const bool is_wide = L < n_wide;
+  const uint64_t line = is_wide ? L : L - n_wide, span = is_wide ? 64 : 32;
+  const uint64_t r0 = line * span;
+  const uint64_t sb0 = is_wide ? r0 & ~0xffffffffULL : r0 & ~((1ULL << kNarrowSbShift) - 1);
+  uint64_t c[4], b[4];
+  bwa_occ_rows(W, primary, r0, c);
+  bwa_occ_rows(W, primary, sb0, b);
+  uint64_t w[2] = {0, 0};
+  for (uint64_t y = 0; y < span; ++y) {
+    const uint64_t r = r0 + y;
+    if (r >= rows) break;
+    if (r == primary) continue;
+    const uint64_t ch = bwa_sym(W, r < primary ? r : r - 1);
+    w[y >> 5] |= ch << (62 - 2 * (y & 31));
+  }
+  if (is_wide) {
+    uint32_t *o = wide + line * 8;
+    for (int x = 0; x < 4; ++x) o[x] = (uint32_t)(c[x] - b[x]);
+    o[4] = (uint32_t)w[0];
+    o[5] = (uint32_t)(w[0] >> 32);
+    o[6] = (uint32_t)w[1];
+    o[7] = (uint32_t)(w[1] >> 32);
+  } else {
+    const uint64_t packed = (c[0] - b[0]) | (c[1] - b[1]) << 21 | (c[2] - b[2]) << 42;
+    uint32_t *o = narrow + line * 4;
+    o[0] = (uint32_t)packed;
+    o[1] = (uint32_t)(packed >> 32);
+    o[2] = (uint32_t)w[0];
+    o[3] = (uint32_t)(w[0] >> 32);
+    if (r0 == sb0) {
+      uint32_t *q = sbc + (r0 >> kNarrowSbShift) * 4;
+      for (int x = 0; x < 4; ++x) q[x] = (uint32_t)c[x];
+    }
+  }
+}
+
 // One Occ line: the four counts and the two symbol words.
 struct OccLine {
   uint32_t c[4];

@@ -2,6 +2,7 @@
 #pragma once
 #include "g3a_dev.h"
 #include "g3a_ext.h"
+#include "g3a_text.h"
 
 #include <hip/hip_runtime.h>
 
@@ -179,6 +180,18 @@ void launch_fill_i32(int32_t *p, int n, int32_t v, hipStream_t s);
 void launch_clamp_codes(uint8_t *seq, int64_t n, hipStream_t s);
 // fills the 3 * 4^kKmer words of the k-mer interval table
 void launch_kmer_table(const DevIndex &x, uint32_t *table, hipStream_t s);
+// record formatting (g3a_text.hip): per-read text lengths, then the text at
+// txt_off; per-alignment MAPQ and MD-string lengths, then the MD strings
+void launch_text_len(const TextArgs &t, int64_t *len, hipStream_t s);
+void launch_text_write(const TextArgs &t, const int64_t *txt_off, char *txt, hipStream_t s);
+void launch_aln_mapq_md(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, int32_t *mapq, int64_t *md_len,
+                        hipStream_t s);
+void launch_aln_md_write(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, const int64_t *md_off,
+                         char *md, hipStream_t s);
+// Occ lines (wide, and narrow with superblock counts when `narrow` is not
+// null) from the BWA .bwt body, rows 0..n
+void launch_occ_lines(const uint32_t *W, uint64_t n, uint64_t primary, uint32_t *wide, uint32_t *narrow,
+                      uint32_t *sbc, hipStream_t s);
 // descending radix sort of (key < 512, value) pairs
 size_t ext_sort_tmp_bytes(int64_t n);
 int ext_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,

@@ -742,6 +742,24 @@ void launch_kmer_table(const DevIndex &x, uint32_t *table, hipStream_t s) {
   hipLaunchKernelGGL(k_kmer_table, dim3((1u << (2 * kKmer)) / 256), dim3(256), 0, s, x, table);
 }
 
+// Occ lines built on the device from the BWA .bwt body: thread per line
+// (occ_line_from_bwa).
+__global__ void __launch_bounds__(256) k_occ_lines(const uint32_t *W, uint64_t n, uint64_t primary, uint32_t *wide,
+                                                   uint32_t *narrow, uint32_t *sbc) {
+  const uint64_t rows = n + 1;
+  const uint64_t n_lines = (rows >> 6) + 1 + (narrow ? (rows >> 5) + 1 : 0);
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t L = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; L < n_lines; L += stride)
+    occ_line_from_bwa(W, n, primary, L, wide, narrow, sbc);
+}
+void launch_occ_lines(const uint32_t *W, uint64_t n, uint64_t primary, uint32_t *wide, uint32_t *narrow,
+                      uint32_t *sbc, hipStream_t s) {
+  const uint64_t lines = ((n + 1) >> 6) + 1 + (narrow ? ((n + 1) >> 5) + 1 : 0);
+  const uint64_t blocks = (lines + 255) / 256;
+  hipLaunchKernelGGL(k_occ_lines, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, W, n, primary,
+                     wide, narrow, sbc);
+}
+
 // Input codes above 4 (anything but A, C, G, T) become 4.
 __global__ void k_clamp_codes(uint8_t *seq, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -1,5 +1,6 @@
 // Host construction of the device-resident index layout from a loaded index.
 #pragma once
+#include "g3a_dev.h"
 #include "g3a_index.h"
 
 #include <cstdint>
@@ -55,7 +56,6 @@ inline bool build_occ_lines(const HostIndex &h, std::vector<uint32_t> &lines, in
 // rows' symbols at 2 bits each as one uint64 (row y at bits 63-2y and 62-2y),
 // the '$' row stored as 0 and never counted.  sb holds four uint32 per
 // superblock: each base's count in the rows before the superblock.
-constexpr int kNarrowSbShift = 21;
 inline bool build_occ_narrow(const HostIndex &h, std::vector<uint32_t> &lines, std::vector<uint32_t> &sb) {
   const uint64_t rows = h.seq_len + 1;
   if (rows >= (1ULL << 32)) return false;

@@ -1,5 +1,5 @@
 // g3a: command-line driver.
-//   g3a index [-s SA_INTV] <ref.fa> [prefix]
+//   g3a index [-s SA_INTV] [-d DEVICE] <ref.fa> [prefix]
 //   g3a mem   [options] <prefix> <reads.fq>        (see g3a_mem.cpp)
 // `index` writes the BWA-format index plus the dense SA sample; `mem` keeps the
 // reference's `g3.exe mem` flags (bwa/src/fastmap.cpp:443-484).
@@ -21,14 +21,16 @@ static void index_progress(const char *what, int level, int64_t n) {
 }
 
 static int index_main(int argc, char **argv) {
-  int intv = 4;
+  int intv = 4, device = -1;
   int i = 1;
   for (; i < argc && argv[i][0] == '-'; ++i) {
     if (!std::strcmp(argv[i], "-s") && i + 1 < argc) intv = std::atoi(argv[++i]);
+    else if (!std::strcmp(argv[i], "-d") && i + 1 < argc) device = std::atoi(argv[++i]);
     else { std::fprintf(stderr, "g3a index: unknown option %s\n", argv[i]); return 1; }
   }
   if (i >= argc) {
-    std::fprintf(stderr, "usage: g3a index [-s SA_INTV] <ref.fa> [prefix]\n");
+    std::fprintf(stderr, "usage: g3a index [-s SA_INTV] [-d DEVICE] <ref.fa> [prefix]\n"
+                         "  -d DEVICE  build the suffix array and BWT on that HIP device (default: host SA-IS)\n");
     return 1;
   }
   const char *fa = argv[i];
@@ -36,7 +38,8 @@ static int index_main(int argc, char **argv) {
   std::string err;
   g_t0 = std::chrono::steady_clock::now();
   g3a::set_build_progress(index_progress);
-  if (!g3a::build_index(fa, prefix.c_str(), intv, err)) {
+  g3a::device_build_progress = index_progress;
+  if (!g3a::build_index(fa, prefix.c_str(), intv, err, device)) {
     std::fprintf(stderr, "g3a index: %s\n", err.c_str());
     return 1;
   }

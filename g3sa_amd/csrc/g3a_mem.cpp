@@ -1,13 +1,14 @@
 // `g3a mem`: host driver keeping the reference's `g3.exe mem` command line
 // (bwa/src/fastmap.cpp:416-566).  Reads FASTQ with the reference's field
 // rules (bwa/src/pipeline.cpp:111-227), aligns batches through the C ABI and
-// writes either the reference's records (pipeline.cpp:287-351):
+// writes, formatted on the device, either the reference's records
+// (pipeline.cpp:287-351):
 //     NAME RNAME POS CIGAR SEQ QUAL        (POS 0-based)
 //     NAME * 0 * SEQ QUAL                  (no alignment)
 // or, with -S, standard SAM.  Output is in input order.
 //
 // Options: -o FILE output, -Z N reads per batch, -S standard SAM, -g N GPUs
-// (1 only), -t N formatting threads, --learned the learned-index seeding form
+// (1 only), -t N parsing threads, --learned the learned-index seeding form
 // (same output); -b -v -i -z -l -p -F -L -m are accepted for compatibility
 // with the reference's command line and ignored.
 #include "../../include/g3a.h"
@@ -43,6 +44,8 @@ struct Batch {
   std::vector<int64_t> name_off{0}, qual_off{0};
   std::vector<uint8_t> seq;
   std::vector<int32_t> seq_off{0};
+  std::vector<char> text;  // the batch's records, formatted on the device
+  int64_t text_bytes = 0;
   int n() const { return (int)seq_off.size() - 1; }
   void clear() {
     names.clear(); quals.clear(); seq.clear();
@@ -192,19 +195,22 @@ struct FastqReader {
     return true;
   }
 
-  // Up to max_reads records into b, parsed by `threads` threads: a pass over
+  // Up to max_reads records and max_bases bases into b (a record longer
+  // than max_bases is taken alone), parsed by `threads` threads: a pass over
   // the buffer finds the records' line ends, then the records found are split
   // into equal parts whose lengths are summed and whose fields are copied in
   // parallel; the buffer is refilled until the batch is full or the file ends.
-  void next_batch(Batch &b, int max_reads, int threads) {
+  void next_batch(Batch &b, int max_reads, int64_t max_bases, int threads) {
     struct Rec {
       size_t h, s, p, q, e;  // header, sequence, separator, quality starts; end
       size_t name_end;
     };
     std::vector<Rec> recs;
+    bool full = false;
     for (;;) {
       recs.clear();
       const int want = max_reads - b.n();
+      int64_t bases = (int64_t)b.seq.size();
       while ((int)recs.size() < want && pos < len) {
         size_t ends[4], i = pos;
         int k = 0;
@@ -215,6 +221,12 @@ struct FastqReader {
           i = ends[k] + 1;
         }
         if (k < 4) break;  // cut by the end of the buffer
+        const int64_t sl = (int64_t)(ends[1] - ends[0] - 1);
+        if (bases + sl > max_bases && b.n() + recs.size() > 0) {
+          full = true;
+          break;
+        }
+        bases += sl;
         Rec r;
         r.h = pos + 1;
         r.s = ends[0] + 1;
@@ -226,13 +238,13 @@ struct FastqReader {
         while (pos < len && data()[pos] != '@') ++pos;
       }
       copy_records(b, recs, threads);
-      if (b.n() >= max_reads) return;
+      if (b.n() >= max_reads || full) return;
       if (eof) break;
       fill();
       skip_to_record();
     }
     // the file's last record without a final newline
-    while (b.n() < max_reads && next(b)) {}
+    while (b.n() < max_reads && (int64_t)b.seq.size() < max_bases && next(b)) {}
   }
 
  private:
@@ -329,187 +341,12 @@ void put_int(std::string &s, long long v) {
   s.append(p, (size_t)(e - p));
 }
 
-// len bases of sq as letters: forward, or reverse-complemented
-void put_seq(std::string &s, const uint8_t *sq, int len, bool rc) {
-  const size_t at = s.size();
-  s.resize(at + (size_t)len);
-  char *d = &s[at];
-  if (!rc)
-    for (int k = 0; k < len; ++k) d[k] = "ACGTN"[sq[k]];
-  else
-    for (int k = 0; k < len; ++k) d[k] = "TGCAN"[sq[len - 1 - k]];
-}
-
-void put_rev(std::string &s, const char *q, size_t n) {
-  const size_t at = s.size();
-  s.resize(at + n);
-  char *d = &s[at];
-  for (size_t k = 0; k < n; ++k) d[k] = q[n - 1 - k];
-}
-
-// mem_approx_mapq_se (bwa/src/aux.cu:676-700) with sub = csub = sub_n = 0 and
-// frac_rep = 0, evaluated on the host.
-int mapq_se(const g3a_opt &o, int score, int qb, int qe, int64_t rb, int64_t re) {
-  int mapq, l, sub = o.min_seed_len * o.a;
-  if (sub >= score) return 0;
-  l = qe - qb > re - rb ? qe - qb : (int)(re - rb);
-  double identity = 1. - (double)(l * o.a - score) / (o.a + o.b) / l;
-  if (score == 0) mapq = 0;
-  else {
-    const int coef_len = 50, coef_fac = (int)std::log(50.0);
-    double tmp = l < coef_len ? 1. : coef_fac / logf((float)l);
-    tmp *= identity * identity;
-    mapq = (int)(6.02 * (score - sub) / o.a * tmp * tmp + .499);
-  }
-  if (mapq > 60) mapq = 60;
-  if (mapq < 0) mapq = 0;
-  return mapq;
-}
-
 struct Ref {
   std::vector<std::string> names;
   std::vector<int64_t> offs, lens;
   std::vector<uint8_t> pac;
   int64_t l_pac = 0;
-  int base(int64_t l) const { return (pac[(size_t)(l >> 2)] >> ((~l & 3) << 1)) & 3; }
 };
-
-void format_range(const Batch &b, const g3a_batch_out &o, const Ref &ref, const g3a_opt &opt, bool sam, int r0, int r1,
-                  std::string &s) {
-  static const char kOp[] = "MIDSH";
-  std::vector<uint32_t> core;
-  std::string md;
-  {
-    const int64_t na = o.aln_off[r1] - o.aln_off[r0], nr = r1 - r0;
-    const int64_t bytes = (b.name_off[(size_t)r1] - b.name_off[(size_t)r0]) + 2 * (b.qual_off[(size_t)r1] -
-                                                                                    b.qual_off[(size_t)r0]);
-    s.reserve(s.size() + (size_t)((bytes / (nr ? nr : 1) + 96) * (na + nr)));
-  }
-  for (int q = r0; q < r1; ++q) {
-    const char *name = b.names.data() + b.name_off[(size_t)q];
-    const size_t nl = (size_t)(b.name_off[(size_t)q + 1] - b.name_off[(size_t)q]);
-    const char *qual = b.quals.data() + b.qual_off[(size_t)q];
-    const size_t ql = (size_t)(b.qual_off[(size_t)q + 1] - b.qual_off[(size_t)q]);
-    const uint8_t *sq = b.seq.data() + b.seq_off[(size_t)q];
-    const int len = b.seq_off[(size_t)q + 1] - b.seq_off[(size_t)q];
-    const int64_t a0 = o.aln_off[q], a1 = o.aln_off[q + 1];
-    if (!sam) {
-      if (a1 <= a0) {
-        s.append(name, nl);
-        s.append(" * 0 * ");
-        put_seq(s, sq, len, false);
-        s.push_back(' ');
-        s.append(qual, ql);
-        s.push_back('\n');
-        continue;
-      }
-      for (int64_t a = a0; a < a1; ++a) {
-        s.append(name, nl);
-        s.push_back(' ');
-        s.append(ref.names[(size_t)o.rid[a]]);
-        s.push_back(' ');
-        put_int(s, o.pos[a]);
-        s.push_back(' ');
-        const uint32_t *cg = o.cigar + o.cigar_off[a];
-        for (int k = 0; k < o.n_cigar[a]; ++k) {
-          put_int(s, cg[k] >> 4);
-          s.push_back(kOp[cg[k] & 0xf]);
-        }
-        s.push_back(' ');
-        put_seq(s, sq, len, false);
-        s.push_back(' ');
-        s.append(qual, ql);
-        s.push_back('\n');
-      }
-      continue;
-    }
-    if (a1 <= a0) {
-      s.append(name, nl);
-      s.append("\t4\t*\t0\t0\t*\t*\t0\t0\t");
-      put_seq(s, sq, len, false);
-      s.push_back('\t');
-      s.append(qual, ql);
-      s.push_back('\n');
-      continue;
-    }
-    for (int64_t a = a0; a < a1; ++a) {
-      const bool rev = o.rb[a] >= ref.l_pac;
-      const int nc = o.n_core[a];
-      const uint32_t *c0 = o.cigar + o.core_off[a];
-      core.assign(c0, c0 + nc);
-      if (rev)
-        for (int i = 0, j = nc - 1; i < j; ++i, --j) std::swap(core[(size_t)i], core[(size_t)j]);
-      int64_t fpos = rev ? (ref.l_pac << 1) - o.re[a] : o.rb[a];
-      int beg = 0, endc = nc;
-      if (endc > 0 && (core[0] & 0xf) == 2) { fpos += core[0] >> 4; beg = 1; }
-      if (endc > beg && (core[(size_t)endc - 1] & 0xf) == 2) --endc;
-      const int qb = o.qb[a], qe = o.qe[a];
-      const int clip_l = rev ? len - qe : qb, clip_r = rev ? qb : len - qe;
-      md.clear();
-      int nm = 0, run = 0, xq = 0;
-      int64_t yr = fpos;
-      for (int k = beg; k < endc; ++k) {
-        const int op = core[(size_t)k] & 0xf, ln = (int)(core[(size_t)k] >> 4);
-        if (op == 0) {
-          for (int t = 0; t < ln; ++t) {
-            const int qi = xq + t;
-            uint8_t c = rev ? sq[qe - 1 - qi] : sq[qb + qi];
-            if (rev && c < 4) c = (uint8_t)(3 - c);
-            const int rbase = ref.base(yr + t);
-            if (c != rbase) {
-              put_int(md, run);
-              md.push_back("ACGT"[rbase]);
-              run = 0;
-              ++nm;
-            } else ++run;
-          }
-          xq += ln;
-          yr += ln;
-        } else if (op == 1) {
-          xq += ln;
-          nm += ln;
-        } else if (op == 2) {
-          put_int(md, run);
-          md.push_back('^');
-          for (int t = 0; t < ln; ++t) md.push_back("ACGT"[ref.base(yr + t)]);
-          run = 0;
-          yr += ln;
-          nm += ln;
-        }
-      }
-      put_int(md, run);
-      const int flag = (rev ? 0x10 : 0) | (a > a0 ? 0x800 : 0);
-      s.append(name, nl);
-      s.push_back('\t');
-      put_int(s, flag);
-      s.push_back('\t');
-      s.append(ref.names[(size_t)o.rid[a]]);
-      s.push_back('\t');
-      put_int(s, fpos - ref.offs[(size_t)o.rid[a]] + 1);
-      s.push_back('\t');
-      put_int(s, mapq_se(opt, o.score[a], qb, qe, o.rb[a], o.re[a]));
-      s.push_back('\t');
-      if (clip_l) { put_int(s, clip_l); s.push_back('S'); }
-      for (int k = beg; k < endc; ++k) {
-        put_int(s, core[(size_t)k] >> 4);
-        s.push_back(kOp[core[(size_t)k] & 0xf]);
-      }
-      if (clip_r) { put_int(s, clip_r); s.push_back('S'); }
-      s.append("\t*\t0\t0\t");
-      put_seq(s, sq, len, rev);
-      s.push_back('\t');
-      if (rev) put_rev(s, qual, ql);
-      else s.append(qual, ql);
-      s.append("\tNM:i:");
-      put_int(s, nm);
-      s.append("\tMD:Z:");
-      s.append(md);
-      s.append("\tAS:i:");
-      put_int(s, o.score[a]);
-      s.push_back('\n');
-    }
-  }
-}
 
 bool load_ref(const char *prefix, Ref &ref) {
   g3a::HostIndex h;
@@ -595,6 +432,9 @@ int g3a_mem_main(int argc, char **argv) {
   }
   if (batch < 1) batch = 1;
   if (threads < 1) threads = 1;
+  // a batch closes at `batch` reads or 320 bases per read on average; a
+  // longer read (left unmapped, as reads over 320 bases are) goes alone
+  const int64_t batch_bases = (int64_t)batch * 320, ctx_bases = batch_bases + (1 << 20);
   const char *prefix = argv[i], *fq = argv[i + 1];
   Ref ref;
   if (!load_ref(prefix, ref)) {
@@ -618,7 +458,7 @@ int g3a_mem_main(int argc, char **argv) {
   };
   for (int k = 0; k < kCtx; ++k) {
     g3a_ctx *c = nullptr;
-    int rc = g3a_device_alloc(0, prefix, &opt, batch, (int64_t)batch * 320, &c);
+    int rc = g3a_device_alloc(0, prefix, &opt, batch, ctx_bases, &c);
     if (rc) {
       std::fprintf(stderr, "g3a mem: cannot create context (%d)\n", rc);
       destroy_all();
@@ -666,7 +506,6 @@ int g3a_mem_main(int argc, char **argv) {
   // batch k-1; the writer puts batches back in input order
   struct Job {
     std::unique_ptr<Batch> b;
-    g3a_batch_out o;
   };
   Queue<std::unique_ptr<Batch>> q_in(2);
   Queue<std::unique_ptr<Job>> q_out(4);
@@ -688,7 +527,7 @@ int g3a_mem_main(int argc, char **argv) {
       std::unique_ptr<Batch> b = q_free.pop();
       const auto tl = clk::now();
       b->clear();
-      rd.next_batch(*b, batch, threads);
+      rd.next_batch(*b, batch, batch_bases, threads);
       b->id = n_loaded++;
       t_load += since(tl);
       if (b->n() == 0) {
@@ -720,17 +559,8 @@ int g3a_mem_main(int argc, char **argv) {
       ++next_id;
       const auto tw = clk::now();
       const Batch &b = *j->b;
-      const int nt = b.n() < 4096 ? 1 : threads;
-      std::vector<std::string> parts((size_t)nt);
-      std::vector<std::thread> th;
-      for (int t = 0; t < nt; ++t) {
-        int r0 = (int)((int64_t)b.n() * t / nt), r1 = (int)((int64_t)b.n() * (t + 1) / nt);
-        th.emplace_back([&, t, r0, r1] { format_range(b, j->o, ref, opt, sam, r0, r1, parts[(size_t)t]); });
-      }
-      for (auto &x : th) x.join();
-      for (auto &p : parts) std::fwrite(p.data(), 1, p.size(), out);
+      std::fwrite(b.text.data(), 1, (size_t)b.text_bytes, out);
       total += b.n();
-      g3a_batch_out_free(&j->o);
       t_write += since(tw);
       q_free.push(std::move(j->b));
     }
@@ -749,7 +579,17 @@ int g3a_mem_main(int argc, char **argv) {
       }
       std::unique_ptr<Job> j(new Job);
       const auto ta = clk::now();
-      const int rc = g3a_align_batch(ctx, b->n(), b->seq.data(), b->seq_off.data(), &j->o);
+      // input, names and qualities to the device; align; format the
+      // records there; copy the text back
+      int rc = g3a_memcpy_input(ctx, b->n(), b->seq.data(), b->seq_off.data());
+      if (!rc) rc = g3a_memcpy_text(ctx, b->n(), b->names.data(), b->name_off.data(), b->quals.data(),
+                                    b->qual_off.data());
+      if (!rc) rc = g3a_bwa_align(ctx);
+      if (!rc) rc = g3a_format_records(ctx, sam ? 1 : 0, &b->text_bytes);
+      if (!rc) {
+        if ((int64_t)b->text.size() < b->text_bytes) b->text.resize((size_t)(b->text_bytes + b->text_bytes / 8));
+        rc = g3a_copy_text(ctx, b->text.data(), (int64_t)b->text.size());
+      }
       const double dt = since(ta);
       {
         std::lock_guard<std::mutex> l(stat_m);
@@ -775,7 +615,7 @@ int g3a_mem_main(int argc, char **argv) {
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::fprintf(stderr, "g3a mem: %lld reads in %.2f s (%.0f reads/s end to end)\n", total, sec,
                sec > 0 ? total / sec : 0.0);
-  std::fprintf(stderr, "g3a mem: busy time: load %.2f s, align %.2f s, format+write %.2f s\n", t_load, t_align,
+  std::fprintf(stderr, "g3a mem: busy time: load %.2f s, align+format %.2f s, write %.2f s\n", t_load, t_align,
                t_write);
   std::fprintf(stderr, "g3a mem: %d batches; align of the first %.3f s, of the others %.3f s each\n", n_batches,
                t_align_first, n_batches > 1 ? (t_align - t_align_first) / (n_batches - 1) : 0.0);

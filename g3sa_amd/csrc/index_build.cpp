@@ -92,120 +92,119 @@ bool build_sa(const std::vector<uint8_t> &text, std::vector<I> &SA) {
 }
 
 template <typename I>
-bool emit_fm(const std::string &prefix, const std::vector<uint8_t> &text, const std::vector<I> &SA,
-             int dense_intv, std::string &err) {
+void fm_files_from_sa(const std::vector<uint8_t> &text, const std::vector<I> &SA, int dense_intv, FmFiles &f) {
   const uint64_t n = text.size() - 1;  // text length without sentinel (= 2 * l_pac)
+  f.n = n;
   // BWT rows 0..n; row 0 is the suffix "$".  Occ counts exclude '$'.
-  uint64_t primary = 0, L2[5] = {0, 0, 0, 0, 0};
   uint64_t cnt[4] = {0, 0, 0, 0};
   for (uint64_t i = 0; i < n; ++i) ++cnt[text[i] - 1];
-  L2[1] = cnt[0]; L2[2] = L2[1] + cnt[1]; L2[3] = L2[2] + cnt[2]; L2[4] = L2[3] + cnt[3];
+  f.L2[0] = 0;
+  for (int x = 0; x < 4; ++x) f.L2[x + 1] = f.L2[x] + cnt[x];
   // BWA packs the BWT without the '$' row: position k of the string is row
   // k (k < primary) or row k+1 (k >= primary).
   const uint64_t n_occ = (n + 127) / 128 + 1;
   const uint64_t raw_words = (n + 15) / 16;
   const uint64_t bwt_words = raw_words + n_occ * 8;
-  std::vector<uint32_t> bwt((size_t)bwt_words, 0);
+  f.bwt.assign((size_t)bwt_words, 0);
   uint64_t c[4] = {0, 0, 0, 0};
   uint64_t k = 0, w = 0;
   uint32_t cur = 0;
   for (uint64_t row = 0; row <= n; ++row) {
     uint64_t sa = (uint64_t)SA[(size_t)row];
-    if (sa == 0) { primary = row; continue; }
+    if (sa == 0) { f.primary = row; continue; }
     uint32_t b = (uint32_t)(text[(size_t)(sa - 1)] - 1);
     if (k % 128 == 0) {
-      std::memcpy(&bwt[(size_t)w], c, sizeof c);
+      std::memcpy(&f.bwt[(size_t)w], c, sizeof c);
       w += 8;
     }
     cur |= b << ((15 - (k & 15)) << 1);
-    if ((k & 15) == 15) { bwt[(size_t)w++] = cur; cur = 0; }
+    if ((k & 15) == 15) { f.bwt[(size_t)w++] = cur; cur = 0; }
     ++c[b];
     ++k;
   }
-  if (k & 15) bwt[(size_t)w++] = cur;
-  std::memcpy(&bwt[(size_t)w], c, sizeof c);
-  w += 8;
-  if (w != bwt_words) { err = "internal error: bwt size mismatch"; return false; }
+  if (k & 15) f.bwt[(size_t)w++] = cur;
+  std::memcpy(&f.bwt[(size_t)w], c, sizeof c);
+  // BWA sampled SA: SA[32*i] for i in [1, n_sa)
+  const uint64_t n_sa = (n + 32) / 32;
+  f.sa.resize((size_t)(n_sa - 1));
+  for (uint64_t i = 1; i < n_sa; ++i) f.sa[(size_t)(i - 1)] = (uint64_t)SA[(size_t)(i * 32)];
+  // bwa-mem2 .counts: cumulative count[5], cumulative count2[17]
+  int64_t cnt1[5] = {0, 0, 0, 0, 0}, cnt2[17];
+  std::memset(cnt2, 0, sizeof cnt2);
+  for (uint64_t i = 0; i < n; ++i) ++cnt1[text[i] - 1];
+  for (uint64_t i = 0; i + 1 < n; ++i) ++cnt2[(text[i] - 1) * 4 + (text[i + 1] - 1)];
+  int64_t acc = 0;
+  for (int i = 0; i < 5; ++i) { f.cnt1[i] = acc; acc += cnt1[i]; }
+  acc = 0;
+  for (int i = 0; i < 17; ++i) { f.cnt2[i] = acc; acc += cnt2[i]; }
+  // dense SA sample for the device: SA[intv*i] for i in [0, ceil((n+1)/intv))
+  f.g3_intv = (uint32_t)dense_intv;
+  f.g3_width = n + 1 < 0xffffffffULL ? 4 : 8;
+  f.g3_count = (n + 1 + (uint64_t)dense_intv - 1) / (uint64_t)dense_intv;
+  f.g3sa.resize((size_t)(f.g3_count * f.g3_width));
+  for (uint64_t i = 0; i < f.g3_count; ++i) {
+    const uint64_t v = (uint64_t)SA[(size_t)(i * f.g3_intv)];
+    if (f.g3_width == 4) {
+      const uint32_t v32 = (uint32_t)v;
+      std::memcpy(&f.g3sa[(size_t)(i * 4)], &v32, 4);
+    } else {
+      std::memcpy(&f.g3sa[(size_t)(i * 8)], &v, 8);
+    }
+  }
+}
+
+}  // namespace
+
+bool write_fm_files(const std::string &prefix, const FmFiles &f, std::string &err) {
+  const uint64_t n = f.n;
   {
     std::string fn = prefix + ".bwt";
     FILE *fp = std::fopen(fn.c_str(), "wb");
     if (!fp) { err = "cannot write " + fn; return false; }
-    std::fwrite(&primary, 8, 1, fp);
-    std::fwrite(L2 + 1, 8, 4, fp);
-    std::fwrite(bwt.data(), 4, (size_t)bwt_words, fp);
-    std::fclose(fp);
+    bool ok = std::fwrite(&f.primary, 8, 1, fp) == 1 && std::fwrite(f.L2 + 1, 8, 4, fp) == 4 &&
+              std::fwrite(f.bwt.data(), 4, f.bwt.size(), fp) == f.bwt.size();
+    ok = std::fclose(fp) == 0 && ok;
+    if (!ok) { err = "short write to " + fn; return false; }
   }
-  {  // BWA sampled SA: header then SA[32*i] for i in [1, n_sa)
-    const uint64_t intv = 32, n_sa = (n + intv) / intv;
+  {  // BWA .sa: header then SA[32*i] for i in [1, n_sa)
+    const uint64_t intv = 32;
     std::string fn = prefix + ".sa";
     FILE *fp = std::fopen(fn.c_str(), "wb");
     if (!fp) { err = "cannot write " + fn; return false; }
-    std::fwrite(&primary, 8, 1, fp);
-    std::fwrite(L2 + 1, 8, 4, fp);
-    std::fwrite(&intv, 8, 1, fp);
-    std::fwrite(&n, 8, 1, fp);
-    std::vector<uint64_t> buf;
-    buf.reserve(1 << 20);
-    for (uint64_t i = 1; i < n_sa; ++i) {
-      buf.push_back((uint64_t)SA[(size_t)(i * intv)]);
-      if (buf.size() == (1u << 20)) { std::fwrite(buf.data(), 8, buf.size(), fp); buf.clear(); }
-    }
-    std::fwrite(buf.data(), 8, buf.size(), fp);
-    std::fclose(fp);
+    bool ok = std::fwrite(&f.primary, 8, 1, fp) == 1 && std::fwrite(f.L2 + 1, 8, 4, fp) == 4 &&
+              std::fwrite(&intv, 8, 1, fp) == 1 && std::fwrite(&n, 8, 1, fp) == 1 &&
+              std::fwrite(f.sa.data(), 8, f.sa.size(), fp) == f.sa.size();
+    ok = std::fclose(fp) == 0 && ok;
+    if (!ok) { err = "short write to " + fn; return false; }
   }
   {  // bwa-mem2 .counts: n+1, cumulative count[5], cumulative count2[17]
-    int64_t cnt1[5] = {0, 0, 0, 0, 0}, cnt2[17];
-    std::memset(cnt2, 0, sizeof cnt2);
-    for (uint64_t i = 0; i < n; ++i) ++cnt1[text[i] - 1];
-    for (uint64_t i = 0; i + 1 < n; ++i) ++cnt2[(text[i] - 1) * 4 + (text[i + 1] - 1)];
-    int64_t acc = 0;
-    for (int i = 0; i < 5; ++i) { int64_t v = cnt1[i]; cnt1[i] = acc; acc += v; }
-    acc = 0;
-    for (int i = 0; i < 17; ++i) { int64_t v = cnt2[i]; cnt2[i] = acc; acc += v; }
-    int64_t rows = (int64_t)n + 1;
+    const int64_t rows = (int64_t)n + 1;
     std::string fn = prefix + ".counts";
     FILE *fp = std::fopen(fn.c_str(), "wb");
     if (!fp) { err = "cannot write " + fn; return false; }
-    std::fwrite(&rows, 8, 1, fp);
-    std::fwrite(cnt1, 8, 5, fp);
-    std::fwrite(cnt2, 8, 17, fp);
-    std::fclose(fp);
+    bool ok = std::fwrite(&rows, 8, 1, fp) == 1 && std::fwrite(f.cnt1, 8, 5, fp) == 5 &&
+              std::fwrite(f.cnt2, 8, 17, fp) == 17;
+    ok = std::fclose(fp) == 0 && ok;
+    if (!ok) { err = "short write to " + fn; return false; }
   }
-  {  // dense SA sample for the device: SA[intv*i] for i in [0, ceil((n+1)/intv))
+  {
     G3SaHeader h;
     std::memset(&h, 0, sizeof h);
     std::memcpy(h.magic, "G3SA0001", 8);
     h.rows = n + 1;
-    h.intv = (uint32_t)dense_intv;
-    h.width = n + 1 < 0xffffffffULL ? 4 : 8;
-    h.count = (n + 1 + (uint64_t)dense_intv - 1) / (uint64_t)dense_intv;
+    h.intv = f.g3_intv;
+    h.width = f.g3_width;
+    h.count = f.g3_count;
     std::string fn = prefix + ".g3sa";
     FILE *fp = std::fopen(fn.c_str(), "wb");
     if (!fp) { err = "cannot write " + fn; return false; }
-    std::fwrite(&h, sizeof h, 1, fp);
-    if (h.width == 4) {
-      std::vector<uint32_t> buf;
-      buf.reserve(1 << 20);
-      for (uint64_t i = 0; i < h.count; ++i) {
-        buf.push_back((uint32_t)SA[(size_t)(i * h.intv)]);
-        if (buf.size() == (1u << 20)) { std::fwrite(buf.data(), 4, buf.size(), fp); buf.clear(); }
-      }
-      std::fwrite(buf.data(), 4, buf.size(), fp);
-    } else {
-      std::vector<uint64_t> buf;
-      buf.reserve(1 << 20);
-      for (uint64_t i = 0; i < h.count; ++i) {
-        buf.push_back((uint64_t)SA[(size_t)(i * h.intv)]);
-        if (buf.size() == (1u << 20)) { std::fwrite(buf.data(), 8, buf.size(), fp); buf.clear(); }
-      }
-      std::fwrite(buf.data(), 8, buf.size(), fp);
-    }
-    std::fclose(fp);
+    bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1 &&
+              std::fwrite(f.g3sa.data(), 1, f.g3sa.size(), fp) == f.g3sa.size();
+    ok = std::fclose(fp) == 0 && ok;
+    if (!ok) { err = "short write to " + fn; return false; }
   }
   return true;
 }
-
-}  // namespace
 
 bool write_bns(const std::string &prefix, const std::vector<std::string> &names,
                const std::vector<std::string> &annos, const std::vector<std::vector<uint8_t>> &seqs,
@@ -268,32 +267,37 @@ bool write_bns(const std::string &prefix, const std::vector<std::string> &names,
 }
 
 bool build_fm_from_fwd(const std::string &prefix, const std::vector<uint8_t> &fwd, int dense_intv,
-                       std::string &err) {
+                       std::string &err, int device) {
   const uint64_t L = fwd.size();
   if (L == 0) { err = "empty reference"; return false; }
-  std::vector<uint8_t> text((size_t)(2 * L + 1));
-  for (uint64_t i = 0; i < L; ++i) {
-    text[(size_t)i] = (uint8_t)(fwd[(size_t)i] + 1);
-    text[(size_t)(2 * L - 1 - i)] = (uint8_t)(4 - fwd[(size_t)i]);  // complement, symbol = (3-b)+1
-  }
-  text[(size_t)(2 * L)] = 0;
-  bool ok;
-  if (text.size() < 0xffffffffULL) {
-    std::vector<uint32_t> SA;
-    build_sa(text, SA);
-    ok = emit_fm(prefix, text, SA, dense_intv, err);
+  FmFiles f;
+  if (device >= 0) {
+    if (!build_fm_device(device, fwd, dense_intv, f, err)) return false;
   } else {
-    std::vector<uint64_t> SA;
-    build_sa(text, SA);
-    ok = emit_fm(prefix, text, SA, dense_intv, err);
+    std::vector<uint8_t> text((size_t)(2 * L + 1));
+    for (uint64_t i = 0; i < L; ++i) {
+      text[(size_t)i] = (uint8_t)(fwd[(size_t)i] + 1);
+      text[(size_t)(2 * L - 1 - i)] = (uint8_t)(4 - fwd[(size_t)i]);  // complement, symbol = (3-b)+1
+    }
+    text[(size_t)(2 * L)] = 0;
+    if (text.size() < 0xffffffffULL) {
+      std::vector<uint32_t> SA;
+      build_sa(text, SA);
+      fm_files_from_sa(text, SA, dense_intv, f);
+    } else {
+      std::vector<uint64_t> SA;
+      build_sa(text, SA);
+      fm_files_from_sa(text, SA, dense_intv, f);
+    }
   }
-  if (ok && sais_progress) sais_progress(".bwt .sa .g3sa .counts written", 0, (int64_t)text.size());
+  const bool ok = write_fm_files(prefix, f, err);
+  if (ok && sais_progress) sais_progress(".bwt .sa .g3sa .counts written", 0, (int64_t)(2 * L + 1));
   return ok;
 }
 
 void set_build_progress(void (*fn)(const char *what, int level, int64_t n)) { sais_progress = fn; }
 
-bool build_index(const char *fasta, const char *prefix, int dense_intv, std::string &err) {
+bool build_index(const char *fasta, const char *prefix, int dense_intv, std::string &err, int device) {
   if (dense_intv < 1 || dense_intv > 32 || (dense_intv & (dense_intv - 1))) {
     err = "SA sample interval must be a power of two in [1, 32]";
     return false;
@@ -311,7 +315,7 @@ bool build_index(const char *fasta, const char *prefix, int dense_intv, std::str
   std::vector<uint8_t> fwd;
   if (!write_bns(prefix, names, annos, seqs, fwd, err)) return false;
   if (sais_progress) sais_progress(".pac .ann .amb written", 0, (int64_t)fwd.size());
-  return build_fm_from_fwd(prefix, fwd, dense_intv, err);
+  return build_fm_from_fwd(prefix, fwd, dense_intv, err, device);
 }
 
 }  // namespace g3a

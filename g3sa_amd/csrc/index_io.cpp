@@ -72,31 +72,35 @@ bool load_index(const char *prefix_c, HostIndex &idx, std::string &err) {
   if (!load_ann(prefix, idx, err)) return false;
   if (!read_file(prefix + ".pac", idx.pac, err)) return false;
   if ((int64_t)idx.pac.size() < idx.l_pac / 4 + 1) { err = "pac shorter than l_pac"; return false; }
-  // .bwt
-  std::vector<uint8_t> raw;
-  if (!read_file(prefix + ".bwt", raw, err)) return false;
-  if (raw.size() < 40) { err = "bwt file too short"; return false; }
-  uint64_t hdr[5];
-  std::memcpy(hdr, raw.data(), 40);
-  idx.primary = hdr[0];
-  idx.L2[0] = 0;
-  for (int i = 0; i < 4; ++i) idx.L2[i + 1] = hdr[i + 1];
-  idx.seq_len = idx.L2[4];
-  const uint64_t n = idx.seq_len;
-  if (n != (uint64_t)(2 * idx.l_pac)) { err = "bwt length is not 2*l_pac"; return false; }
-  const uint32_t *w = (const uint32_t *)(raw.data() + 40);
-  const uint64_t n_words = (raw.size() - 40) / 4;
-  const uint64_t n_occ = (n + 127) / 128 + 1;
-  if (n_words != (n + 15) / 16 + n_occ * 8) { err = "bwt size inconsistent with its header"; return false; }
-  idx.bwt.assign((size_t)(n + 1), 0);
-  for (uint64_t k = 0; k < n; ++k) {
-    // string position k: 128-base block k>>7 holds 8 words of counts then 8 words of bases
-    uint64_t word = (k >> 7) * 16 + 8 + ((k & 127) >> 4);
-    uint32_t b = (w[word] >> ((~k & 15) << 1)) & 3;
-    uint64_t row = k < idx.primary ? k : k + 1;
-    idx.bwt[(size_t)row] = (uint8_t)b;
+  // .bwt: header, then the body read straight into idx.bwt_words
+  {
+    const std::string fn = prefix + ".bwt";
+    FILE *fp = std::fopen(fn.c_str(), "rb");
+    if (!fp) { err = "cannot open " + fn; return false; }
+    std::fseek(fp, 0, SEEK_END);
+    const long sz = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    uint64_t hdr[5];
+    if (sz < 40 || std::fread(hdr, 8, 5, fp) != 5) { std::fclose(fp); err = "bwt file too short"; return false; }
+    idx.primary = hdr[0];
+    idx.L2[0] = 0;
+    for (int i = 0; i < 4; ++i) idx.L2[i + 1] = hdr[i + 1];
+    idx.seq_len = idx.L2[4];
+    const uint64_t n = idx.seq_len;
+    const uint64_t n_words = (uint64_t)(sz - 40) / 4;
+    const uint64_t n_occ = (n + 127) / 128 + 1;
+    if (n != (uint64_t)(2 * idx.l_pac) || n_words != (n + 15) / 16 + n_occ * 8) {
+      std::fclose(fp);
+      err = n != (uint64_t)(2 * idx.l_pac) ? "bwt length is not 2*l_pac" : "bwt size inconsistent with its header";
+      return false;
+    }
+    idx.bwt_words.resize((size_t)n_words);
+    const size_t got = std::fread(idx.bwt_words.data(), 4, (size_t)n_words, fp);
+    std::fclose(fp);
+    if (got != n_words) { err = "short read of " + fn; return false; }
+    idx.bwt.clear();
   }
-  idx.bwt[(size_t)idx.primary] = 4;
+  const uint64_t n = idx.seq_len;
   // Prefer the dense sample; fall back to the BWA 32-row sample.
   std::vector<uint8_t> g3;
   std::string e2;
@@ -132,6 +136,20 @@ bool load_index(const char *prefix_c, HostIndex &idx, std::string &err) {
     std::memcpy(idx.sa.data() + 8, sa.data() + 56, (size_t)((n_sa - 1) * 8));
   }
   return true;
+}
+
+void unpack_bwt(HostIndex &idx) {
+  const uint64_t n = idx.seq_len;
+  const uint32_t *w = idx.bwt_words.data();
+  idx.bwt.assign((size_t)(n + 1), 0);
+  for (uint64_t k = 0; k < n; ++k) {
+    // string position k: 128-base block k>>7 holds 8 words of counts then 8 words of bases
+    uint64_t word = (k >> 7) * 16 + 8 + ((k & 127) >> 4);
+    uint32_t b = (w[word] >> ((~k & 15) << 1)) & 3;
+    uint64_t row = k < idx.primary ? k : k + 1;
+    idx.bwt[(size_t)row] = (uint8_t)b;
+  }
+  idx.bwt[(size_t)idx.primary] = 4;
 }
 
 }  // namespace g3a

@@ -79,6 +79,14 @@ typedef struct {
   float ms_seed, ms_sa, ms_chain, ms_extend, ms_traceback, ms_total;
   /* work counters of this batch */
   int64_t n_seeds, n_tasks, ext_cells, fm_steps, n_slow_reads;
+  /* MAPQ of each alignment (mem_approx_mapq_se, bwa/src/aux.cu:676-700, with
+   * sub = min_seed_len * a and frac_rep = 0), and its MD string (aux.cu:2908-
+   * 2941, in genome orientation as written to SAM): alignment i's string is
+   * md + md_off[i], zero-terminated; md_off has n_alns + 1 entries. */
+  int32_t *mapq;
+  int64_t *md_off;
+  char *md;
+  int64_t n_md_bytes;
 } g3a_batch_out;
 
 void g3a_opt_init(g3a_opt *opt);
@@ -97,6 +105,22 @@ int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_
 int g3a_bwa_align(g3a_ctx *ctx);
 /* memcpy_output: copy the results of the last g3a_bwa_align to the host. */
 int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out);
+/* Names and base qualities of the batch of the last g3a_memcpy_input (same
+ * n_reads), for the device record formatter: names concatenated with
+ * name_off[n_reads+1], qualities with qual_off[n_reads+1].  No reference
+ * counterpart: the reference keeps them on the host and formats there
+ * (bwa/src/pipeline.cpp:287-351). */
+int g3a_memcpy_text(g3a_ctx *ctx, int n_reads, const char *names, const int64_t *name_off, const char *quals,
+                    const int64_t *qual_off);
+/* Formats the records of the last g3a_bwa_align on the device, in read
+ * order: mode 0 the reference's record lines (NAME RNAME POS0 CIGAR SEQ QUAL,
+ * or NAME * 0 * SEQ QUAL when unmapped; bwa/src/pipeline.cpp:287-351), mode 1
+ * SAM records without the header (FLAG, 1-based POS, MAPQ, CIGAR and SEQ/QUAL
+ * in genome orientation, NM, MD and AS tags).  *bytes = the text's length.
+ * The text stays in HBM until the next batch; g3a_copy_text copies it out. */
+int g3a_format_records(g3a_ctx *ctx, int mode, int64_t *bytes);
+/* Copies the formatted records to dst (cap >= their length). */
+int g3a_copy_text(g3a_ctx *ctx, char *dst, int64_t cap);
 /* memcpy_input + bwa_align + memcpy_output */
 int g3a_align_batch(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t *offsets, g3a_batch_out *out);
 void g3a_batch_out_free(g3a_batch_out *out);
@@ -207,6 +231,12 @@ const char *g3a_last_error(const g3a_ctx *ctx);
 /* Index construction (the `g3a index` command): BWA-format files plus the
  * dense suffix-array sample with interval sa_intv (power of two, 1..32). */
 int g3a_build_index(const char *fasta, const char *prefix, int sa_intv, char *err, int err_len);
+/* The same files, with the suffix array, BWT, samples and count tables
+ * built on HIP device `device` (prefix doubling over radix sorts in HBM);
+ * byte-identical to g3a_build_index.  The reference builds its index on the
+ * host only (bwa/src/FMI_search.cpp:232-420 build_index, bwa/src/bwt.c:62-84);
+ * this entry point has no counterpart there. */
+int g3a_build_index_dev(int device, const char *fasta, const char *prefix, int sa_intv, char *err, int err_len);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,9 @@
  * pipeline.cpp:90-107), aligns with orc_align_batch and writes either the
  * reference's text records (pipeline.cpp:287-351) or standard SAM.  Timing of
  * the alignment alone (index load and parsing excluded) goes to stderr as
- * "oracle_reads_per_s <value> threads <t> reads <n> seconds <s>".
+ * "oracle_reads_per_s <value> threads <t> reads <n> seconds <s>", and of the
+ * record formatting (one thread, output writing excluded) as
+ * "oracle_format_seconds <s>".
  */
 #define _GNU_SOURCE
 #include "oracle.h"
@@ -155,6 +157,7 @@ int main(int argc, char **argv) {
           (long long)b.ext_cells, (long long)b.tb_cells);
   str_t s = {0, 0, 0};
   const int64_t l_pac = orc_index_l_pac(idx);
+  clock_gettime(CLOCK_MONOTONIC, &t0);
   if (sam) {
     sputs(&s, "@HD\tVN:1.6\tSO:unsorted\n");
     for (int c = 0; c < orc_index_n_contigs(idx); ++c) {
@@ -263,6 +266,9 @@ int main(int argc, char **argv) {
       free(md.s);
     }
   }
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  fprintf(stderr, "oracle_format_seconds %.6f\n",
+          (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
   FILE *out = out_fn ? fopen(out_fn, "wb") : stdout;
   if (!out) { fprintf(stderr, "g3a_oracle: cannot write %s\n", out_fn); return 1; }
   fwrite(s.s, 1, s.n, out);

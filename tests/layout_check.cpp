@@ -1,8 +1,10 @@
 // Occ over every BWT row from the device layouts (the 32-byte wide lines and
 // the 16-byte narrow lines with their superblock counts), through the same
 // lookup code the kernels use (compiled for the host), against a running
-// count of the BWT symbols.  Prints "ok <rows> <superblocks>" or the first
-// mismatch.  Built and run by tests/test_layout.py.
+// count of the BWT symbols; and the lines that the device builds from the
+// BWA .bwt body (occ_line_from_bwa, the body of k_occ_lines) against the host
+// construction, word for word.  Prints "ok <rows> <superblocks>" or the
+// first mismatch.  Built and run by tests/test_layout.py.
 #include "g3sa_amd/csrc/g3a_dev.h"
 #include "g3sa_amd/csrc/g3a_index.h"
 #include "g3sa_amd/csrc/g3a_layout.h"
@@ -22,12 +24,24 @@ int main(int argc, char **argv) {
     std::printf("load failed: %s\n", err.c_str());
     return 1;
   }
+  unpack_bwt(h);
   DevIndex x;
   std::memset(&x, 0, sizeof x);
   std::vector<uint32_t> wide, narrow, sb;
   if (!build_occ_lines(h, wide, x.super) || !build_occ_narrow(h, narrow, sb)) {
     std::printf("layout failed\n");
     return 1;
+  }
+  {  // the device builder's lines, word for word
+    std::vector<uint32_t> dw(wide.size(), 0xdeadbeef), dn(narrow.size(), 0xdeadbeef), ds(sb.size(), 0xdeadbeef);
+    const uint64_t rows = h.seq_len + 1, n_lines = (rows >> 6) + 1 + (rows >> 5) + 1;
+    for (uint64_t L = 0; L < n_lines; ++L)
+      occ_line_from_bwa(h.bwt_words.data(), h.seq_len, h.primary, L, dw.data(), dn.data(), ds.data());
+    if (dw != wide || dn != narrow || ds != sb) {
+      std::printf("device-built lines differ (wide %d narrow %d superblocks %d)\n", dw != wide, dn != narrow,
+                  ds != sb);
+      return 1;
+    }
   }
   x.occ = wide.data();
   x.occn = narrow.data();

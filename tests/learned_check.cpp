@@ -83,6 +83,7 @@ int main(int argc, char **argv) {
     std::printf("load failed: %s\n", err.c_str());
     return 1;
   }
+  unpack_bwt(h);
   h.pac.resize(h.pac.size() + 32, 0);
   DevIndex x;
   std::memset(&x, 0, sizeof x);

@@ -38,3 +38,20 @@ def test_parity_sample_plumbing(built, tmp_path, monkeypatch):
     sam.write_bytes(b"\n".join(lines))
     got = bench.parity_sample(prefix, fq, 1000, sam, False)
     assert not got["identical"] and got["first_difference"]["line"] == k, got
+
+
+def test_bench_fastq_parse_matches_reader(built, tmp_path):
+    """bench.load_reads (vectorised) against g3sa_amd.read_fastq on names
+    with and without comments, N bases and lower case."""
+    fa, fq = tmp_path / "r.fa", tmp_path / "r.fq"
+    gen("ref", "-o", str(fa), "--len", "100000", "--contigs", "1", "--seed", "5")
+    gen("reads", "-r", str(fa), "-o", str(fq), "--n", "300", "--len-min", "70", "--len-max", "250", "--seed", "6")
+    recs = fq.read_bytes().split(b"\n")
+    recs[0] = b"@plain_name"                    # no comment
+    recs[5] = recs[5][:10] + b"nN" + recs[5][12:]  # N bases, lower case
+    fq.write_bytes(b"\n".join(recs))
+    codes, off, names, name_off, quals, qual_off = bench.load_reads(fq)
+    r_names, r_seq, r_off, r_quals = g3sa_amd.read_fastq(str(fq))
+    assert list(off) == list(r_off) and bytes(codes) == bytes(r_seq)
+    assert [names[name_off[i]:name_off[i + 1]] for i in range(len(r_names))] == r_names
+    assert [quals[qual_off[i]:qual_off[i + 1]] for i in range(len(r_quals))] == r_quals

@@ -73,3 +73,20 @@ def test_cli_repeats_sam(ds_rep, tmp_path):
     assert got_l == ref, first_diff(got_l, ref)
     lines = [ln for ln in got.split(b"\n") if ln and not ln.startswith(b"@")]
     assert len(lines) >= 12000
+
+
+def test_cli_long_read_alone_in_batch(ds_c1, tmp_path):
+    """A read over 320 bases with -Z 1 (and in a batch of 3): the batch is not
+    refused, and the read comes out unmapped as the oracle writes it."""
+    src = ds_c1.reads["main"].read_bytes().split(b"\n")
+    recs = [src[i:i + 4] for i in range(0, 24, 4)]
+    long_seq = recs[1][1] + recs[2][1] + recs[3][1][:100]
+    recs[2] = [b"@long_read x", long_seq, b"+", b"I" * len(long_seq)]
+    fq = tmp_path / "long.fq"
+    fq.write_bytes(b"".join(b"\n".join(r) + b"\n" for r in recs))
+    for mode, flags in [("compat", []), ("sam", ["-S"])]:
+        ref = oracle_out(ds_c1, fq, mode, tmp_path)
+        for z in ("1", "3"):
+            print(f"[cli] long read: {mode} -Z {z}", flush=True)
+            got = gpu_out(ds_c1, fq, flags + ["-Z", z], tmp_path, f"long_{mode}_{z}")
+            assert got == ref, first_diff(got, ref)

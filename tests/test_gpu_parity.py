@@ -311,3 +311,57 @@ def test_wide_band_global_alignment_parity(built, oracle, tmp_path):
     lq = ref["qe"] - ref["qb"]
     rl = ref["re"] - ref["rb"]
     assert int(np.abs(lq - rl).max()) >= 32, "no alignment with a band over 64 columns"
+
+
+def _oracle_sam_records(ds, fq, tmp_path):
+    """The oracle command line's SAM for fq: header and mapped records."""
+    import subprocess
+    from conftest import ROOT
+    out = tmp_path / "oracle.sam"
+    subprocess.run([str(ROOT / "oracle" / "_build" / "g3a_oracle"), "-t", str(oracle_threads()), "-m", "sam", "-o",
+                    str(out), str(ds.prefix), str(fq)], check=True, capture_output=True)
+    return out.read_bytes()
+
+
+def test_abi_mapq_md_and_device_records(ds_rep, oracle, tmp_path):
+    """MAPQ and MD at the C ABI (g3a_batch_out) against the oracle's SAM, and
+    the device-formatted records (g3a_memcpy_text + g3a_format_records) byte
+    for byte against the oracle's command line in both formats."""
+    fq = ds_rep.reads["main"]
+    names, seq, off, quals = g3sa_amd.read_fastq(str(fq))
+    sam = _oracle_sam_records(ds_rep, fq, tmp_path)
+    recs = [ln.split(b"\t") for ln in sam.split(b"\n") if ln and not ln.startswith(b"@")]
+    mapped = [r for r in recs if r[2] != b"*"]
+    md_ref = [next(f[5:].decode() for f in r if f.startswith(b"MD:Z:")) for r in mapped]
+    mapq_ref = np.array([int(r[4]) for r in mapped], dtype=np.int32)
+    n = len(off) - 1
+    name_off = np.zeros(n + 1, dtype=np.int64)
+    name_off[1:] = np.cumsum([len(x) for x in names])
+    qual_off = np.zeros(n + 1, dtype=np.int64)
+    qual_off[1:] = np.cumsum([len(x) for x in quals])
+    with g3sa_amd.Aligner(str(ds_rep.prefix), max_reads=n, max_bases=int(off[-1])) as al:
+        print(f"[parity] ABI mapq/md: {n} reads", flush=True)
+        got = al.align(seq, off)
+        assert got.n_alns == len(mapped)
+        np.testing.assert_array_equal(got.mapq, mapq_ref, err_msg="MAPQ differs from the oracle's")
+        bad = [i for i in range(len(md_ref)) if got.md[i] != md_ref[i]]
+        assert not bad, f"MD differs at alignments {bad[:5]}: {[(got.md[i], md_ref[i]) for i in bad[:3]]}"
+        assert len(set(mapq_ref.tolist())) > 3  # MAPQ is not constant on a repeat-rich text
+        al.upload_text(b"".join(names), name_off, b"".join(quals), qual_off)
+        for mode in ("sam", "compat"):
+            print(f"[parity] device records: {mode}", flush=True)
+            nb = al.format(sam=mode == "sam")
+            text = al.text()
+            assert len(text) == nb
+            ref = sam if mode == "sam" else _oracle_compat(ds_rep, fq, tmp_path)
+            body = b"".join(ln + b"\n" for ln in ref.split(b"\n") if ln and not ln.startswith(b"@"))
+            assert text == body, f"{mode}: device records differ from the oracle's"
+
+
+def _oracle_compat(ds, fq, tmp_path):
+    import subprocess
+    from conftest import ROOT
+    out = tmp_path / "oracle.txt"
+    subprocess.run([str(ROOT / "oracle" / "_build" / "g3a_oracle"), "-t", str(oracle_threads()), "-m", "compat",
+                    "-o", str(out), str(ds.prefix), str(fq)], check=True, capture_output=True)
+    return out.read_bytes()
