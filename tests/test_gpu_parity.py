@@ -323,11 +323,15 @@ def _oracle_sam_records(ds, fq, tmp_path):
     return out.read_bytes()
 
 
-def test_abi_mapq_md_and_device_records(ds_rep, oracle, tmp_path):
+@pytest.mark.parametrize("which", ["rep", "c3"])
+def test_abi_mapq_md_and_device_records(ds_rep, ds_c1, oracle, tmp_path, which):
     """MAPQ and MD at the C ABI (g3a_batch_out) against the oracle's SAM, and
     the device-formatted records (g3a_memcpy_text + g3a_format_records) byte
-    for byte against the oracle's command line in both formats."""
-    fq = ds_rep.reads["main"]
+    for byte against the oracle's command line in both formats: paired reads
+    on a repeat-rich text, and variable-length reads with divergent tails
+    (whose short, low-scoring alignments give MAPQ below 60)."""
+    ds_rep = ds_rep if which == "rep" else ds_c1
+    fq = ds_rep.reads["main" if which == "rep" else "c3"]
     names, seq, off, quals = g3sa_amd.read_fastq(str(fq))
     sam = _oracle_sam_records(ds_rep, fq, tmp_path)
     recs = [ln.split(b"\t") for ln in sam.split(b"\n") if ln and not ln.startswith(b"@")]
@@ -346,7 +350,8 @@ def test_abi_mapq_md_and_device_records(ds_rep, oracle, tmp_path):
         np.testing.assert_array_equal(got.mapq, mapq_ref, err_msg="MAPQ differs from the oracle's")
         bad = [i for i in range(len(md_ref)) if got.md[i] != md_ref[i]]
         assert not bad, f"MD differs at alignments {bad[:5]}: {[(got.md[i], md_ref[i]) for i in bad[:3]]}"
-        assert len(set(mapq_ref.tolist())) > 3  # MAPQ is not constant on a repeat-rich text
+        if which == "c3":
+            assert len(set(mapq_ref.tolist())) > 3, "the oracle's MAPQ takes few values here"
         al.upload_text(b"".join(names), name_off, b"".join(quals), qual_off)
         for mode in ("sam", "compat"):
             print(f"[parity] device records: {mode}", flush=True)
