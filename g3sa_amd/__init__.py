@@ -80,7 +80,7 @@ EXPORTS = [
     "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
     "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed", "g3a_build_index_dev", "g3a_memcpy_text",
-    "g3a_format_records", "g3a_copy_text",
+    "g3a_format_records", "g3a_copy_text", "g3a_bwa_align_noalloc",
 ]
 
 _lib = None
@@ -99,6 +99,7 @@ def load_library() -> C.CDLL:
     lib.g3a_memcpy_index.argtypes = [C.c_void_p]
     lib.g3a_memcpy_input.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     lib.g3a_bwa_align.argtypes = [C.c_void_p]
+    lib.g3a_bwa_align_noalloc.argtypes = [C.c_void_p]
     lib.g3a_memcpy_output.argtypes = [C.c_void_p, C.POINTER(BatchOut)]
     lib.g3a_align_batch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(BatchOut)]
     lib.g3a_batch_out_free.argtypes = [C.POINTER(BatchOut)]
@@ -263,6 +264,10 @@ class Aligner:
 
     def run(self) -> None:
         self._check(self.lib.g3a_bwa_align(self.ctx))
+
+    def run_noalloc(self) -> None:
+        """The allocation-free entry point on the resident batch."""
+        self._check(self.lib.g3a_bwa_align_noalloc(self.ctx))
 
     def download(self) -> Result:
         out = BatchOut()

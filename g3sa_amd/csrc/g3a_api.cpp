@@ -15,22 +15,59 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace g3a;
+
+// The allocation-free entry point's workspace: one array per device that
+// the library runs on, held until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the current
+// device (scratch_address); the array's name is a host-side stand-in.
+static __device__ __align__(256) uint8_t g3a_noalloc_scratch[G3A_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool view = false;  // a region of the static scratch array, not an allocation
 };
 
 }  // namespace
+
+// Per-batch workspace of the pipeline.  A context owns one (grown as
+// batches need, kept across batches); the allocation-free entry point runs
+// on a second one whose buffers are views into the device's static scratch
+// array.
+struct Work {
+  DevBuf lep_pool, eh_pool, tb_order, tb_ctr, tb_nw;
+  DevBuf seed_key, seed_key2, seed_idx, seed_order, seed_sort_tmp;  // seeding order
+  // seeding
+  DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
+  // seeds / chains
+  DevBuf seeds, seeds2, ch, nxt, order, tmpi, w16, kept, nodes, tasks_tmp, n_task, task_off;
+  // tasks / regions
+  DevBuf tasks, regs, flag, sec, n_aln, aln_off;
+  // alignments
+  DevBuf aln_reg, aln_read, aln_rank, z_size, cig_size, z_off, cig_off, z, cigar, alns;
+  DevBuf cig_words, cig_coff, cig_out;  // compacted CIGARs for the host copy
+  DevBuf scan_tmp;
+  // batched-form extension
+  DevBuf ext_res[2], ext_key, ext_key2, ext_idx, ext_sorted, ext_slow, ext_ctr, ext_sort_tmp, ext_rowbuf;
+};
+
+// A bump allocator over the static scratch array, active during one
+// allocation-free call.
+struct ScratchArena {
+  uint8_t *base = nullptr;
+  size_t cap = 0, used = 0;
+};
 
 struct g3a_ctx {
   int device = 0;
@@ -55,27 +92,23 @@ struct g3a_ctx {
   int n_reads = 0;
   bool has_input = false, has_output = false;
   DevBuf seq, off;
-  DevBuf lep_pool, eh_pool, tb_order, tb_ctr, tb_nw;
-  DevBuf seed_key, seed_key2, seed_idx, seed_order, seed_sort_tmp;  // seeding order
   int seed_ordered = 0;  // seed reads in minimizer order (1) or input order (0)
   int seed_threads = 0, tb_threads = 0;  // resident maxima (the pools are sized for them)
   int seed_blocks_per_cu = 0;             // 0: the occupancy maximum
-  // seeding
-  DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
-  // seeds / chains
-  DevBuf seeds, seeds2, ch, nxt, order, tmpi, w16, kept, nodes, tasks_tmp, n_task, task_off;
-  // tasks / regions
-  DevBuf tasks, regs, flag, sec, n_aln, aln_off;
-  // alignments
-  DevBuf aln_reg, aln_read, aln_rank, z_size, cig_size, z_off, cig_off, z, cigar, alns;
-  DevBuf cig_words, cig_coff, cig_out;  // compacted CIGARs for the host copy
-  DevBuf scan_tmp;
+  // per-batch workspace: `own`, or `scratch_work` during and after an
+  // allocation-free call (its views stay valid for g3a_memcpy_output)
+  Work own, scratch_work;
+  Work *w = &own;
+  ScratchArena *arena = nullptr;  // set while an allocation-free call runs
+  uint8_t *scratch = nullptr;     // this device's static scratch array
+  DevBuf fmt_scan_tmp;            // scan storage of the formatter and output copy
+  // grids of the running call (sizes of the per-thread pools)
+  int run_seed_threads = 0, run_tb_threads = 0, run_lane_blocks = 0, run_slow_chunk = 16384;
   // batched-form extension
   int form = G3A_FORM_BATCHED;
   int n_cu = 0;
   int strip_rows = 16, lane_blocks_per_cu = 8, wave_blocks_per_cu = 8;
   int ext_packed = 1;  // lane engine: two strip rows per register in plain columns
-  DevBuf ext_res[2], ext_key, ext_key2, ext_idx, ext_sorted, ext_slow, ext_ctr, ext_sort_tmp, ext_rowbuf;
   // host staging: readbacks, and page-locked copies of each batch's input
   // and output so that the transfers run at DMA speed
   int64_t *pinned = nullptr;
@@ -115,6 +148,20 @@ int fail(g3a_ctx *c, int code, const std::string &msg) {
 int ensure(g3a_ctx *ctx, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 64;
   if (b.bytes >= bytes) return 0;
+  if (ctx->arena) {  // allocation-free call: carve a view out of the scratch
+    ScratchArena &a = *ctx->arena;
+    const size_t at = (a.used + 255) & ~(size_t)255;
+    if (at > a.cap || bytes > a.cap - at)
+      return fail(ctx, G3A_E_ARG,
+                  "the batch needs more than the " + std::to_string(a.cap >> 20) +
+                      " MiB scratch of the allocation-free entry point (use g3a_bwa_align)");
+    b.p = a.base + at;
+    b.bytes = bytes;
+    b.view = true;
+    a.used = at + bytes;
+    return 0;
+  }
+  if (b.view) return fail(ctx, G3A_E_INTERNAL, "scratch view grown outside an allocation-free call");
   if (b.p) {
     HIPCK(hipFree(b.p));
     b.p = nullptr;
@@ -170,9 +217,26 @@ T *P(DevBuf &b) {
 }
 
 void free_buf(DevBuf &b) {
-  if (b.p) hipFree(b.p);
-  b.p = nullptr;
-  b.bytes = 0;
+  if (b.p && !b.view) hipFree(b.p);
+  b = DevBuf{};
+}
+
+// Device address of g3a_noalloc_scratch on `device` (current), looked up
+// once per device and kept: each device holds its own copy of the array.
+std::mutex g_scratch_m;
+uint8_t *g_scratch_addr[64] = {};
+
+hipError_t scratch_address(int device, uint8_t **out) {
+  if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> l(g_scratch_m);
+  if (!g_scratch_addr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(g3a_noalloc_scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = static_cast<uint8_t *>(ptr);
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
 }
 
 }  // namespace
@@ -278,6 +342,12 @@ int g3a_device_alloc(int device, const char *prefix, const g3a_opt *o, int max_r
     g3a_destroy(ctx);
     return G3A_E_HIP;
   }
+  // the allocation-free entry point's scratch array on this device (set
+  // up here, with the device current)
+  if (scratch_address(device, &ctx->scratch) != hipSuccess) {
+    g3a_destroy(ctx);
+    return G3A_E_HIP;
+  }
   *out = ctx;
   return G3A_OK;
 }
@@ -285,65 +355,89 @@ int g3a_device_alloc(int device, const char *prefix, const g3a_opt *o, int max_r
 // Workspace of each stage, grown to the batch's sizes (the same calls size it
 // at set-up from max_reads, so that a first batch of typical shape finds it
 // allocated).
-static int alloc_reads(g3a_ctx *ctx, int n) {
-  if (ensure(ctx, ctx->n_intv, (size_t)n * 4 + 4) || ensure(ctx, ctx->n_seed, (size_t)n * 8 + 8) ||
-      ensure(ctx, ctx->slow, (size_t)n * 4 + 4) || ensure(ctx, ctx->intv_base, (size_t)n * 8 + 8) ||
-      ensure(ctx, ctx->seed_off, (size_t)(n + 1) * 8) || ensure(ctx, ctx->counters, 64 * 8) ||
-      ensure(ctx, ctx->n_task, (size_t)n * 4 + 4) || ensure(ctx, ctx->task_off, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->n_aln, (size_t)n * 4 + 4) || ensure(ctx, ctx->aln_off, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->slow_list, (size_t)n * 4 + 4) || ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(ctx->max_reads + 1)) ||
-      ensure(ctx, ctx->intv, ((size_t)n * kSlotCap + 1024) * sizeof(Intv)))
-    return G3A_E_HIP;
-  if (ensure(ctx, ctx->seed_key, (size_t)n * 4 + 4) || ensure(ctx, ctx->seed_key2, (size_t)n * 4 + 4) ||
-      ensure(ctx, ctx->seed_idx, (size_t)n * 4 + 4) || ensure(ctx, ctx->seed_order, (size_t)n * 4 + 4) ||
-      ensure(ctx, ctx->seed_sort_tmp, seed_sort_tmp_bytes(n)))
-    return G3A_E_HIP;
+// Grids of a call on n reads: the resident maxima for g3a_bwa_align; for the
+// allocation-free call no more seeding threads than reads, and smaller
+// traceback and extension grids, so that their per-thread pools fit the
+// scratch array.
+static void set_grids(g3a_ctx *ctx, int n, bool noalloc) {
   if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx);
-  return ensure(ctx, ctx->lep_pool, (size_t)ctx->seed_threads * (kLepCap + kFoundCap) * sizeof(Intv));
+  if (!ctx->tb_threads) ctx->tb_threads = traceback_grid_threads(ctx->n_cu);
+  ctx->run_seed_threads = ctx->seed_threads;
+  ctx->run_tb_threads = ctx->tb_threads;
+  ctx->run_lane_blocks = ctx->n_cu * ctx->lane_blocks_per_cu;
+  ctx->run_slow_chunk = 16384;
+  if (noalloc) {
+    const int need = ((n + 127) / 128) * 128;
+    ctx->run_seed_threads = std::max(128, std::min(ctx->seed_threads, need));
+    ctx->run_tb_threads = std::min(ctx->tb_threads, 4096);
+    ctx->run_lane_blocks = std::min(ctx->run_lane_blocks, 64);
+    ctx->run_slow_chunk = 16;
+  }
+}
+
+static int alloc_fail(const g3a_ctx *ctx) { return ctx->arena ? G3A_E_ARG : G3A_E_HIP; }
+
+static int alloc_reads(g3a_ctx *ctx, int n) {
+  if (ensure(ctx, ctx->w->n_intv, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->n_seed, (size_t)n * 8 + 8) ||
+      ensure(ctx, ctx->w->slow, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->intv_base, (size_t)n * 8 + 8) ||
+      ensure(ctx, ctx->w->seed_off, (size_t)(n + 1) * 8) || ensure(ctx, ctx->w->counters, 64 * 8) ||
+      ensure(ctx, ctx->w->n_task, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->task_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->w->n_aln, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->aln_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->w->slow_list, (size_t)n * 4 + 4) ||
+      ensure(ctx, ctx->w->scan_tmp, scan_tmp_bytes((ctx->arena ? n : ctx->max_reads) + 1)) ||
+      ensure(ctx, ctx->w->intv, ((size_t)n * kSlotCap + 1024) * sizeof(Intv)))
+    return G3A_E_HIP;
+  if (ensure(ctx, ctx->w->seed_key, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->seed_key2, (size_t)n * 4 + 4) ||
+      ensure(ctx, ctx->w->seed_idx, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->seed_order, (size_t)n * 4 + 4) ||
+      ensure(ctx, ctx->w->seed_sort_tmp, seed_sort_tmp_bytes(n)))
+    return G3A_E_HIP;
+  return ensure(ctx, ctx->w->lep_pool, (size_t)ctx->run_seed_threads * (kLepCap + kFoundCap) * sizeof(Intv));
 }
 
 static int alloc_seeds(g3a_ctx *ctx, int64_t T_s, int n) {
   const size_t n_nodes = (size_t)((T_s + 32 * (int64_t)(n + 1)) / 4 + 2);
-  return ensure(ctx, ctx->seeds, (size_t)T_s * sizeof(Seed)) || ensure(ctx, ctx->seeds2, (size_t)T_s * sizeof(Seed)) ||
-         ensure(ctx, ctx->ch, (size_t)T_s * sizeof(ChainTmp)) || ensure(ctx, ctx->nxt, (size_t)T_s * 4) ||
-         ensure(ctx, ctx->order, (size_t)T_s * 4) || ensure(ctx, ctx->tmpi, (size_t)T_s * 8 + 8) ||
-         ensure(ctx, ctx->w16, (size_t)T_s * 2) || ensure(ctx, ctx->kept, (size_t)T_s) ||
-         ensure(ctx, ctx->nodes, n_nodes * sizeof(KbNode)) || ensure(ctx, ctx->tasks_tmp, (size_t)T_s * sizeof(Task));
+  return ensure(ctx, ctx->w->seeds, (size_t)T_s * sizeof(Seed)) || ensure(ctx, ctx->w->seeds2, (size_t)T_s * sizeof(Seed)) ||
+         ensure(ctx, ctx->w->ch, (size_t)T_s * sizeof(ChainTmp)) || ensure(ctx, ctx->w->nxt, (size_t)T_s * 4) ||
+         ensure(ctx, ctx->w->order, (size_t)T_s * 4) || ensure(ctx, ctx->w->tmpi, (size_t)T_s * 8 + 8) ||
+         ensure(ctx, ctx->w->w16, (size_t)T_s * 2) || ensure(ctx, ctx->w->kept, (size_t)T_s) ||
+         ensure(ctx, ctx->w->nodes, n_nodes * sizeof(KbNode)) || ensure(ctx, ctx->w->tasks_tmp, (size_t)T_s * sizeof(Task));
 }
 
 static int alloc_tasks(g3a_ctx *ctx, int64_t T_t) {
-  return ensure(ctx, ctx->tasks, (size_t)T_t * sizeof(Task)) || ensure(ctx, ctx->regs, (size_t)T_t * sizeof(Region)) ||
-         ensure(ctx, ctx->flag, (size_t)T_t) || ensure(ctx, ctx->sec, (size_t)T_t * 4);
+  return ensure(ctx, ctx->w->tasks, (size_t)T_t * sizeof(Task)) || ensure(ctx, ctx->w->regs, (size_t)T_t * sizeof(Region)) ||
+         ensure(ctx, ctx->w->flag, (size_t)T_t) || ensure(ctx, ctx->w->sec, (size_t)T_t * 4);
 }
 
 static int alloc_alns(g3a_ctx *ctx, int64_t T_a) {
-  if (!ctx->tb_threads) ctx->tb_threads = traceback_grid_threads(ctx->n_cu);
-  return ensure(ctx, ctx->aln_reg, (size_t)T_a * sizeof(Region)) || ensure(ctx, ctx->aln_read, (size_t)T_a * 4) ||
-         ensure(ctx, ctx->aln_rank, (size_t)T_a * 4) || ensure(ctx, ctx->z_size, (size_t)T_a * 8) ||
-         ensure(ctx, ctx->cig_size, (size_t)T_a * 8) || ensure(ctx, ctx->z_off, (size_t)(T_a + 1) * 8) ||
-         ensure(ctx, ctx->cig_off, (size_t)(T_a + 1) * 8) || ensure(ctx, ctx->alns, (size_t)T_a * sizeof(Aln)) ||
-         ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(T_a + 1)) || ensure(ctx, ctx->tb_order, (size_t)T_a * 4 + 4) ||
-         ensure(ctx, ctx->tb_nw, (size_t)T_a * 4 + 4) ||
-         ensure(ctx, ctx->tb_ctr, 64) || ensure(ctx, ctx->eh_pool, (size_t)traceback_pool_words(ctx->tb_threads) * 4);
+  return ensure(ctx, ctx->w->aln_reg, (size_t)T_a * sizeof(Region)) || ensure(ctx, ctx->w->aln_read, (size_t)T_a * 4) ||
+         ensure(ctx, ctx->w->aln_rank, (size_t)T_a * 4) || ensure(ctx, ctx->w->z_size, (size_t)T_a * 8) ||
+         ensure(ctx, ctx->w->cig_size, (size_t)T_a * 8) || ensure(ctx, ctx->w->z_off, (size_t)(T_a + 1) * 8) ||
+         ensure(ctx, ctx->w->cig_off, (size_t)(T_a + 1) * 8) || ensure(ctx, ctx->w->alns, (size_t)T_a * sizeof(Aln)) ||
+         ensure(ctx, ctx->w->scan_tmp, scan_tmp_bytes(T_a + 1)) || ensure(ctx, ctx->w->tb_order, (size_t)T_a * 4 + 4) ||
+         ensure(ctx, ctx->w->tb_nw, (size_t)T_a * 4 + 4) ||
+         ensure(ctx, ctx->w->tb_ctr, 64) || ensure(ctx, ctx->w->eh_pool, (size_t)traceback_pool_words(ctx->run_tb_threads) * 4);
 }
 
 static int alloc_paths(g3a_ctx *ctx, int64_t T_z, int64_t T_c) {
-  return ensure(ctx, ctx->z, (size_t)T_z + 64) || ensure(ctx, ctx->cigar, (size_t)T_c * 4 + 64);
+  return ensure(ctx, ctx->w->z, (size_t)T_z + 64) || ensure(ctx, ctx->w->cigar, (size_t)T_c * 4 + 64);
 }
 
 // Set-up sizes for a batch of max_reads reads and max_bases bases: 12 seeds,
 // 4 extension tasks and one alignment per read, direction bytes for
 // 4 x max_bases and the CIGAR capacity of one alignment per read.
+static void set_grids(g3a_ctx *ctx, int n, bool noalloc);
+
 static int reserve_workspace(g3a_ctx *ctx) {
   const int n = ctx->max_reads;
+  set_grids(ctx, n, false);
   const int64_t nb = ctx->max_bases;
   if (alloc_reads(ctx, n) || alloc_seeds(ctx, 12 * (int64_t)n, n) || alloc_tasks(ctx, 4 * (int64_t)n) ||
       alloc_alns(ctx, n) || alloc_paths(ctx, 4 * nb, 4 * nb + 4 * (int64_t)n))
     return G3A_E_HIP;
   // host copy of the alignment records and CIGARs (32 words per read),
   // written once here so its pages are resident before the first batch
-  if (ensure(ctx, ctx->cig_words, (size_t)n * 4 + 4) || ensure(ctx, ctx->cig_coff, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->cig_out, (size_t)n * 128 + 64))
+  if (ensure(ctx, ctx->w->cig_words, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->cig_coff, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->w->cig_out, (size_t)n * 128 + 64))
     return G3A_E_HIP;
   ctx->out_buf.assign((size_t)n * (sizeof(Aln) + 128) + 256, 0);
   return 0;
@@ -518,7 +612,7 @@ int readback(g3a_ctx *ctx, std::initializer_list<const int64_t *> src) {
 
 }  // namespace
 
-static // The extension stage over the T_t tasks in ctx->tasks, into ctx->regs.
+static // The extension stage over the T_t tasks in ctx->w->tasks, into ctx->w->regs.
 int run_extension(g3a_ctx *ctx, int64_t T_t, int64_t *cells) {
   hipStream_t s = ctx->stream;
   if (ctx->form == G3A_FORM_PLAIN) {
@@ -528,24 +622,25 @@ int run_extension(g3a_ctx *ctx, int64_t T_t, int64_t *cells) {
     ea.n_tasks = T_t;
     ea.seq = P<uint8_t>(ctx->seq);
     ea.off = P<int32_t>(ctx->off);
-    ea.tasks = P<Task>(ctx->tasks);
-    ea.regs = P<Region>(ctx->regs);
+    ea.tasks = P<Task>(ctx->w->tasks);
+    ea.regs = P<Region>(ctx->w->regs);
     ea.cells = cells;
     HIPCK(hipEventRecord(ctx->kev[3], s));
     launch_extend(ea, s);
     HIPCK(hipEventRecord(ctx->kev[7], s));
     ctx->kev_used[3] = 1;
   } else if (T_t > 0) {
-    const int lane_blocks = ctx->n_cu * ctx->lane_blocks_per_cu, wave_blocks = ctx->n_cu * ctx->wave_blocks_per_cu;
-    if (ensure(ctx, ctx->ext_res[0], (size_t)T_t * sizeof(ExtRes)) ||
-        ensure(ctx, ctx->ext_res[1], (size_t)T_t * sizeof(ExtRes)) || ensure(ctx, ctx->ext_key, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->ext_key2, (size_t)T_t * 4) || ensure(ctx, ctx->ext_idx, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->ext_sorted, (size_t)T_t * 4) || ensure(ctx, ctx->ext_slow, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->ext_ctr, 2 * 4 * sizeof(int32_t)) ||
-        ensure(ctx, ctx->ext_sort_tmp, ext_sort_tmp_bytes(T_t)) ||
-        ensure(ctx, ctx->ext_rowbuf, (size_t)lane_blocks * kExtQCap * 64 * 4))
-      return G3A_E_HIP;
-    launch_fill_i32(P<int32_t>(ctx->ext_ctr), 8, 0, s);
+    const int lane_blocks = ctx->arena ? ctx->run_lane_blocks : ctx->n_cu * ctx->lane_blocks_per_cu,
+              wave_blocks = ctx->n_cu * ctx->wave_blocks_per_cu;
+    if (ensure(ctx, ctx->w->ext_res[0], (size_t)T_t * sizeof(ExtRes)) ||
+        ensure(ctx, ctx->w->ext_res[1], (size_t)T_t * sizeof(ExtRes)) || ensure(ctx, ctx->w->ext_key, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->w->ext_key2, (size_t)T_t * 4) || ensure(ctx, ctx->w->ext_idx, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->w->ext_sorted, (size_t)T_t * 4) || ensure(ctx, ctx->w->ext_slow, (size_t)T_t * 4) ||
+        ensure(ctx, ctx->w->ext_ctr, 2 * 4 * sizeof(int32_t)) ||
+        ensure(ctx, ctx->w->ext_sort_tmp, ext_sort_tmp_bytes(T_t)) ||
+        ensure(ctx, ctx->w->ext_rowbuf, (size_t)lane_blocks * kExtQCap * 64 * 4))
+      return alloc_fail(ctx);
+    launch_fill_i32(P<int32_t>(ctx->w->ext_ctr), 8, 0, s);
     for (int side = 0; side < 2; ++side) {
       ExtSideArgs ea{};
       ea.idx = ctx->idx;
@@ -555,19 +650,19 @@ int run_extension(g3a_ctx *ctx, int64_t T_t, int64_t *cells) {
       ea.n_tasks = T_t;
       ea.seq = P<uint8_t>(ctx->seq);
       ea.off = P<int32_t>(ctx->off);
-      ea.tasks = P<Task>(ctx->tasks);
-      ea.left = P<ExtRes>(ctx->ext_res[0]);
-      ea.res = P<ExtRes>(ctx->ext_res[side]);
-      ea.key = P<uint32_t>(ctx->ext_key);
-      ea.tid = P<int32_t>(ctx->ext_idx);
-      ea.sorted = P<int32_t>(ctx->ext_sorted);
-      ea.slow_list = P<int32_t>(ctx->ext_slow);
-      ea.ctr = P<int32_t>(ctx->ext_ctr) + 4 * side;
-      ea.rowbuf = P<uint32_t>(ctx->ext_rowbuf);
+      ea.tasks = P<Task>(ctx->w->tasks);
+      ea.left = P<ExtRes>(ctx->w->ext_res[0]);
+      ea.res = P<ExtRes>(ctx->w->ext_res[side]);
+      ea.key = P<uint32_t>(ctx->w->ext_key);
+      ea.tid = P<int32_t>(ctx->w->ext_idx);
+      ea.sorted = P<int32_t>(ctx->w->ext_sorted);
+      ea.slow_list = P<int32_t>(ctx->w->ext_slow);
+      ea.ctr = P<int32_t>(ctx->w->ext_ctr) + 4 * side;
+      ea.rowbuf = P<uint32_t>(ctx->w->ext_rowbuf);
       ea.cells = cells;
       launch_ext_classify(ea, s);
-      if (ext_sort(P<uint32_t>(ctx->ext_key), P<uint32_t>(ctx->ext_key2), P<int32_t>(ctx->ext_idx),
-                   P<int32_t>(ctx->ext_sorted), T_t, ctx->ext_sort_tmp.p, ctx->ext_sort_tmp.bytes, s))
+      if (ext_sort(P<uint32_t>(ctx->w->ext_key), P<uint32_t>(ctx->w->ext_key2), P<int32_t>(ctx->w->ext_idx),
+                   P<int32_t>(ctx->w->ext_sorted), T_t, ctx->w->ext_sort_tmp.p, ctx->w->ext_sort_tmp.bytes, s))
         return fail(ctx, G3A_E_HIP, "extension task sort failed");
       HIPCK(hipEventRecord(ctx->kev[2 + 2 * side], s));
       launch_ext_lane(ea, ctx->strip_rows, ctx->ext_packed, lane_blocks, s);
@@ -576,26 +671,24 @@ int run_extension(g3a_ctx *ctx, int64_t T_t, int64_t *cells) {
       HIPCK(hipEventRecord(ctx->kev[7 + 2 * side], s));
       ctx->kev_used[1 + side] = 1;
     }
-    ExtFinishArgs fa{T_t, P<Task>(ctx->tasks), P<ExtRes>(ctx->ext_res[0]), P<ExtRes>(ctx->ext_res[1]),
-                     P<Region>(ctx->regs)};
+    ExtFinishArgs fa{T_t, P<Task>(ctx->w->tasks), P<ExtRes>(ctx->w->ext_res[0]), P<ExtRes>(ctx->w->ext_res[1]),
+                     P<Region>(ctx->w->regs)};
     launch_ext_finish(fa, s);
   }
   return 0;
 }
 
-int g3a_bwa_align(g3a_ctx *ctx) {
-  if (!ctx) return G3A_E_ARG;
-  if (!ctx->index_on_device || !ctx->has_input) return fail(ctx, G3A_E_STATE, "index or input not on device");
-  HIPCK(hipSetDevice(ctx->device));
+// The pipeline on the resident batch, with ctx->w as its workspace.
+static int align_batch(g3a_ctx *ctx) {
   hipStream_t s = ctx->stream;
   const int n = ctx->n_reads;
   ctx->has_output = false;
   ctx->total_seeds = ctx->total_tasks = ctx->total_alns = ctx->total_cig = ctx->n_slow = 0;
-  if (alloc_reads(ctx, n)) return G3A_E_HIP;
+  if (alloc_reads(ctx, n)) return alloc_fail(ctx);
   // fast-path interval slots plus a pool that grows for slow reads
   const size_t fast_entries = (size_t)n * kSlotCap;
-  int64_t *cnt = P<int64_t>(ctx->counters);
-  HIPCK(hipMemsetAsync(cnt, 0, 64 * 8, s));
+  int64_t *cnt = P<int64_t>(ctx->w->counters);
+  launch_fill_i32(P<int32_t>(ctx->w->counters), 128, 0, s);  // a kernel, not a memset node
   HIPCK(hipEventRecord(ctx->ev[0], s));
   // ---- seeding
   SeedArgs sa{};
@@ -604,88 +697,88 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   sa.n_reads = n;
   sa.seq = P<uint8_t>(ctx->seq);
   sa.off = P<int32_t>(ctx->off);
-  sa.intv = P<Intv>(ctx->intv);
+  sa.intv = P<Intv>(ctx->w->intv);
   sa.slot_cap = kSlotCap;
-  sa.n_intv = P<int32_t>(ctx->n_intv);
-  sa.n_seed = P<int64_t>(ctx->n_seed);
-  sa.slow = P<int32_t>(ctx->slow);
+  sa.n_intv = P<int32_t>(ctx->w->n_intv);
+  sa.n_seed = P<int64_t>(ctx->w->n_seed);
+  sa.slow = P<int32_t>(ctx->w->slow);
   sa.fm_steps = cnt + 0;
-  sa.lep_pool = P<Intv>(ctx->lep_pool);
+  sa.lep_pool = P<Intv>(ctx->w->lep_pool);
   sa.li_stats = ctx->idx.saf ? cnt + 32 : nullptr;
   sa.order = nullptr;
   if (ctx->seed_ordered && n > 1) {
-    launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(ctx->seed_key), P<int32_t>(ctx->seed_idx), s);
-    if (seed_sort(P<uint32_t>(ctx->seed_key), P<uint32_t>(ctx->seed_key2), P<int32_t>(ctx->seed_idx),
-                  P<int32_t>(ctx->seed_order), n, ctx->seed_sort_tmp.p, ctx->seed_sort_tmp.bytes, s))
+    launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(ctx->w->seed_key), P<int32_t>(ctx->w->seed_idx), s);
+    if (seed_sort(P<uint32_t>(ctx->w->seed_key), P<uint32_t>(ctx->w->seed_key2), P<int32_t>(ctx->w->seed_idx),
+                  P<int32_t>(ctx->w->seed_order), n, ctx->w->seed_sort_tmp.p, ctx->w->seed_sort_tmp.bytes, s))
       return fail(ctx, G3A_E_HIP, "seed order sort failed");
-    sa.order = P<int32_t>(ctx->seed_order);
+    sa.order = P<int32_t>(ctx->w->seed_order);
   }
   for (int &u : ctx->kev_used) u = 0;
   HIPCK(hipEventRecord(ctx->kev[0], s));
-  const int seed_run = ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * 128 * ctx->n_cu < ctx->seed_threads
+  const int seed_run = ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * 128 * ctx->n_cu < ctx->run_seed_threads
                            ? ctx->seed_blocks_per_cu * 128 * ctx->n_cu
-                           : ctx->seed_threads;
+                           : ctx->run_seed_threads;
   launch_seed(sa, seed_run, ctx->seed_blocks_per_cu, s);
   HIPCK(hipEventRecord(ctx->kev[1], s));
   ctx->kev_used[0] = 1;
-  launch_collect_slow(n, P<int32_t>(ctx->slow), P<int32_t>(ctx->slow_list), P<int32_t>(ctx->counters) + 2 * 8,
-                      P<int64_t>(ctx->intv_base), kSlotCap, s);
+  launch_collect_slow(n, P<int32_t>(ctx->w->slow), P<int32_t>(ctx->w->slow_list), P<int32_t>(ctx->w->counters) + 2 * 8,
+                      P<int64_t>(ctx->w->intv_base), kSlotCap, s);
   if (readback(ctx, {cnt + 8})) return G3A_E_HIP;
   const int n_slow = (int)(ctx->pinned[0] & 0xffffffff);
   ctx->n_slow = n_slow;
   if (n_slow > 0) {
     // counts, then pool offsets, then the intervals themselves
-    const int chunk = 16384;
-    if (ensure(ctx, ctx->slow_scratch, (size_t)chunk * kSlowScratch * sizeof(Intv)) ||
-        ensure(ctx, ctx->slow_off, (size_t)(n_slow + 1) * 8) || ensure(ctx, ctx->slow_n, (size_t)n_slow * 8 + 8))
-      return G3A_E_HIP;
-    const int32_t *list = P<int32_t>(ctx->slow_list);
+    const int chunk = ctx->run_slow_chunk;
+    if (ensure(ctx, ctx->w->slow_scratch, (size_t)chunk * kSlowScratch * sizeof(Intv)) ||
+        ensure(ctx, ctx->w->slow_off, (size_t)(n_slow + 1) * 8) || ensure(ctx, ctx->w->slow_n, (size_t)n_slow * 8 + 8))
+      return alloc_fail(ctx);
+    const int32_t *list = P<int32_t>(ctx->w->slow_list);
     for (int c0 = 0; c0 < n_slow; c0 += chunk) {
       int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
-      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), nullptr, nullptr, 0, 0, s);
+      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->w->slow_scratch), nullptr, nullptr, 0, 0, s);
     }
-    launch_gather_slow_counts(n_slow, list, P<int32_t>(ctx->n_intv), P<int64_t>(ctx->slow_n), s);
-    scan_i64(P<int64_t>(ctx->slow_n), P<int64_t>(ctx->slow_off), n_slow, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
-    if (readback(ctx, {P<int64_t>(ctx->slow_off) + n_slow})) return G3A_E_HIP;
+    launch_gather_slow_counts(n_slow, list, P<int32_t>(ctx->w->n_intv), P<int64_t>(ctx->w->slow_n), s);
+    scan_i64(P<int64_t>(ctx->w->slow_n), P<int64_t>(ctx->w->slow_off), n_slow, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
+    if (readback(ctx, {P<int64_t>(ctx->w->slow_off) + n_slow})) return G3A_E_HIP;
     const int64_t pool = ctx->pinned[0];
     // grow the interval buffer, keeping the fast slots
     const size_t need = (fast_entries + (size_t)pool + 1024) * sizeof(Intv);
-    if (ctx->intv.bytes < need) {
+    if (ctx->w->intv.bytes < need) {
       DevBuf nb;
-      if (ensure(ctx, nb, need)) return G3A_E_HIP;
-      HIPCK(hipMemcpyAsync(nb.p, ctx->intv.p, fast_entries * sizeof(Intv), hipMemcpyDeviceToDevice, s));
+      if (ensure(ctx, nb, need)) return alloc_fail(ctx);
+      HIPCK(hipMemcpyAsync(nb.p, ctx->w->intv.p, fast_entries * sizeof(Intv), hipMemcpyDeviceToDevice, s));
       HIPCK(hipStreamSynchronize(s));
-      free_buf(ctx->intv);
-      ctx->intv = nb;
-      sa.intv = P<Intv>(ctx->intv);
+      free_buf(ctx->w->intv);
+      ctx->w->intv = nb;
+      sa.intv = P<Intv>(ctx->w->intv);
     }
-    Intv *pool_base = P<Intv>(ctx->intv) + fast_entries;
+    Intv *pool_base = P<Intv>(ctx->w->intv) + fast_entries;
     for (int c0 = 0; c0 < n_slow; c0 += chunk) {
       int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
-      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->slow_scratch), P<int64_t>(ctx->slow_off) + c0, pool_base, 1,
+      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->w->slow_scratch), P<int64_t>(ctx->w->slow_off) + c0, pool_base, 1,
                        n_slow <= chunk, s);
     }
-    launch_set_slow_base(n_slow, list, P<int64_t>(ctx->slow_off), (int64_t)fast_entries, P<int64_t>(ctx->intv_base), s);
+    launch_set_slow_base(n_slow, list, P<int64_t>(ctx->w->slow_off), (int64_t)fast_entries, P<int64_t>(ctx->w->intv_base), s);
   }
   HIPCK(hipEventRecord(ctx->ev[1], s));
   // ---- SA lookup and seed sort
-  scan_i64(P<int64_t>(ctx->n_seed), P<int64_t>(ctx->seed_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->seed_off) + n})) return G3A_E_HIP;
+  scan_i64(P<int64_t>(ctx->w->n_seed), P<int64_t>(ctx->w->seed_off), n, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->w->seed_off) + n})) return G3A_E_HIP;
   const int64_t T_s = ctx->pinned[0];
   ctx->total_seeds = T_s;
-  if (alloc_seeds(ctx, T_s, n)) return G3A_E_HIP;
+  if (alloc_seeds(ctx, T_s, n)) return alloc_fail(ctx);
   SaArgs la{};
   la.idx = ctx->idx;
   la.opt = ctx->opt;
   la.n_reads = n;
   la.total = T_s;
-  la.seed_off = P<int64_t>(ctx->seed_off);
-  la.intv = P<Intv>(ctx->intv);
-  la.intv_base = P<int64_t>(ctx->intv_base);
-  la.n_intv = P<int32_t>(ctx->n_intv);
-  la.seeds = P<Seed>(ctx->seeds);
+  la.seed_off = P<int64_t>(ctx->w->seed_off);
+  la.intv = P<Intv>(ctx->w->intv);
+  la.intv_base = P<int64_t>(ctx->w->intv_base);
+  la.n_intv = P<int32_t>(ctx->w->n_intv);
+  la.seeds = P<Seed>(ctx->w->seeds);
   launch_sa_lookup(la, s);
-  SortArgs so{n, P<int64_t>(ctx->seed_off), P<Seed>(ctx->seeds), P<Seed>(ctx->seeds2)};
+  SortArgs so{n, P<int64_t>(ctx->w->seed_off), P<Seed>(ctx->w->seeds), P<Seed>(ctx->w->seeds2)};
   launch_sort(so, s);
   HIPCK(hipEventRecord(ctx->ev[2], s));
   // ---- chaining
@@ -694,82 +787,82 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   ca.opt = ctx->opt;
   ca.n_reads = n;
   ca.off = P<int32_t>(ctx->off);
-  ca.seed_off = P<int64_t>(ctx->seed_off);
-  ca.seeds = P<Seed>(ctx->seeds);
-  ca.ch = P<ChainTmp>(ctx->ch);
-  ca.nxt = P<int32_t>(ctx->nxt);
-  ca.order = P<int32_t>(ctx->order);
-  ca.tmpi = P<int32_t>(ctx->tmpi);
-  ca.w16 = P<uint16_t>(ctx->w16);
-  ca.kept = P<uint8_t>(ctx->kept);
-  ca.nodes = P<KbNode>(ctx->nodes);
-  ca.tasks_tmp = P<Task>(ctx->tasks_tmp);
-  ca.n_task = P<int32_t>(ctx->n_task);
-  ca.error = P<int32_t>(ctx->counters) + 2 * 16;
+  ca.seed_off = P<int64_t>(ctx->w->seed_off);
+  ca.seeds = P<Seed>(ctx->w->seeds);
+  ca.ch = P<ChainTmp>(ctx->w->ch);
+  ca.nxt = P<int32_t>(ctx->w->nxt);
+  ca.order = P<int32_t>(ctx->w->order);
+  ca.tmpi = P<int32_t>(ctx->w->tmpi);
+  ca.w16 = P<uint16_t>(ctx->w->w16);
+  ca.kept = P<uint8_t>(ctx->w->kept);
+  ca.nodes = P<KbNode>(ctx->w->nodes);
+  ca.tasks_tmp = P<Task>(ctx->w->tasks_tmp);
+  ca.n_task = P<int32_t>(ctx->w->n_task);
+  ca.error = P<int32_t>(ctx->w->counters) + 2 * 16;
   launch_chain(ca, s);
-  scan_i32(P<int32_t>(ctx->n_task), P<int64_t>(ctx->task_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->task_off) + n, cnt + 16})) return G3A_E_HIP;
+  scan_i32(P<int32_t>(ctx->w->n_task), P<int64_t>(ctx->w->task_off), n, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->w->task_off) + n, cnt + 16})) return G3A_E_HIP;
   const int64_t T_t = ctx->pinned[0];
   if (ctx->pinned[1] & 0xffffffff) return fail(ctx, G3A_E_INTERNAL, "chain node arena overflow");
   ctx->total_tasks = T_t;
-  if (alloc_tasks(ctx, T_t)) return G3A_E_HIP;
-  CompactArgs cp{n, P<int64_t>(ctx->seed_off), P<int64_t>(ctx->task_off), P<Task>(ctx->tasks_tmp), P<Task>(ctx->tasks)};
+  if (alloc_tasks(ctx, T_t)) return alloc_fail(ctx);
+  CompactArgs cp{n, P<int64_t>(ctx->w->seed_off), P<int64_t>(ctx->w->task_off), P<Task>(ctx->w->tasks_tmp), P<Task>(ctx->w->tasks)};
   launch_task_compact(cp, s);
   HIPCK(hipEventRecord(ctx->ev[3], s));
   // ---- extension
   if (int rc = run_extension(ctx, T_t, cnt + 24)) return rc;
   HIPCK(hipEventRecord(ctx->ev[4], s));
   // ---- region filter, traceback, finalize
-  RegionArgs ra{ctx->opt, n, P<int64_t>(ctx->task_off), P<Region>(ctx->regs), P<uint8_t>(ctx->flag),
-                P<int32_t>(ctx->sec), P<int32_t>(ctx->n_aln)};
+  RegionArgs ra{ctx->opt, n, P<int64_t>(ctx->w->task_off), P<Region>(ctx->w->regs), P<uint8_t>(ctx->w->flag),
+                P<int32_t>(ctx->w->sec), P<int32_t>(ctx->w->n_aln)};
   launch_region_filter(ra, s);
-  scan_i32(P<int32_t>(ctx->n_aln), P<int64_t>(ctx->aln_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->aln_off) + n})) return G3A_E_HIP;
+  scan_i32(P<int32_t>(ctx->w->n_aln), P<int64_t>(ctx->w->aln_off), n, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->w->aln_off) + n})) return G3A_E_HIP;
   const int64_t T_a = ctx->pinned[0];
   ctx->total_alns = T_a;
-  if (alloc_alns(ctx, T_a)) return G3A_E_HIP;
-  AlnPrepArgs ap{ctx->opt, n, P<int64_t>(ctx->task_off), P<int64_t>(ctx->aln_off), P<Region>(ctx->regs),
-                 P<Region>(ctx->aln_reg), P<int32_t>(ctx->aln_read), P<int32_t>(ctx->aln_rank),
-                 P<int64_t>(ctx->z_size), P<int64_t>(ctx->cig_size)};
+  if (alloc_alns(ctx, T_a)) return alloc_fail(ctx);
+  AlnPrepArgs ap{ctx->opt, n, P<int64_t>(ctx->w->task_off), P<int64_t>(ctx->w->aln_off), P<Region>(ctx->w->regs),
+                 P<Region>(ctx->w->aln_reg), P<int32_t>(ctx->w->aln_read), P<int32_t>(ctx->w->aln_rank),
+                 P<int64_t>(ctx->w->z_size), P<int64_t>(ctx->w->cig_size)};
   launch_aln_prep(ap, s);
-  scan_i64(P<int64_t>(ctx->z_size), P<int64_t>(ctx->z_off), T_a, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
-  scan_i64(P<int64_t>(ctx->cig_size), P<int64_t>(ctx->cig_off), T_a, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->z_off) + T_a, P<int64_t>(ctx->cig_off) + T_a})) return G3A_E_HIP;
+  scan_i64(P<int64_t>(ctx->w->z_size), P<int64_t>(ctx->w->z_off), T_a, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
+  scan_i64(P<int64_t>(ctx->w->cig_size), P<int64_t>(ctx->w->cig_off), T_a, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->w->z_off) + T_a, P<int64_t>(ctx->w->cig_off) + T_a})) return G3A_E_HIP;
   const int64_t T_z = ctx->pinned[0], T_c = ctx->pinned[1];
   ctx->total_cig = T_c;
-  if (alloc_paths(ctx, T_z, T_c)) return G3A_E_HIP;
+  if (alloc_paths(ctx, T_z, T_c)) return alloc_fail(ctx);
   TbArgs tb{};
   tb.idx = ctx->idx;
   tb.opt = ctx->opt;
   tb.n_alns = T_a;
   tb.seq = P<uint8_t>(ctx->seq);
   tb.off = P<int32_t>(ctx->off);
-  tb.aln_reg = P<Region>(ctx->aln_reg);
-  tb.aln_read = P<int32_t>(ctx->aln_read);
-  tb.aln_rank = P<int32_t>(ctx->aln_rank);
-  tb.z_off = P<int64_t>(ctx->z_off);
-  tb.cig_off = P<int64_t>(ctx->cig_off);
-  tb.z = P<uint8_t>(ctx->z);
-  tb.cigar = P<uint32_t>(ctx->cigar);
-  tb.alns = P<Aln>(ctx->alns);
-  tb.eh_pool = P<int32_t>(ctx->eh_pool);
-  tb.tb_order = P<int32_t>(ctx->tb_order);
-  tb.tb_ctr = P<int32_t>(ctx->tb_ctr);
-  tb.tb_nw = P<int32_t>(ctx->tb_nw);
+  tb.aln_reg = P<Region>(ctx->w->aln_reg);
+  tb.aln_read = P<int32_t>(ctx->w->aln_read);
+  tb.aln_rank = P<int32_t>(ctx->w->aln_rank);
+  tb.z_off = P<int64_t>(ctx->w->z_off);
+  tb.cig_off = P<int64_t>(ctx->w->cig_off);
+  tb.z = P<uint8_t>(ctx->w->z);
+  tb.cigar = P<uint32_t>(ctx->w->cigar);
+  tb.alns = P<Aln>(ctx->w->alns);
+  tb.eh_pool = P<int32_t>(ctx->w->eh_pool);
+  tb.tb_order = P<int32_t>(ctx->w->tb_order);
+  tb.tb_ctr = P<int32_t>(ctx->w->tb_ctr);
+  tb.tb_nw = P<int32_t>(ctx->w->tb_nw);
   HIPCK(hipEventRecord(ctx->kev[10], s));
-  launch_traceback(tb, ctx->tb_threads, ctx->n_cu, s);
+  launch_traceback(tb, ctx->run_tb_threads, ctx->n_cu, s);
   HIPCK(hipEventRecord(ctx->kev[11], s));
   ctx->kev_used[5] = 1;
   // CIGARs packed to the words used, for the copy to the host
-  if (ensure(ctx, ctx->cig_words, (size_t)T_a * 4 + 4) || ensure(ctx, ctx->cig_coff, (size_t)(T_a + 1) * 8))
-    return G3A_E_HIP;
-  launch_cig_words(P<Aln>(ctx->alns), T_a, P<int32_t>(ctx->cig_words), s);
-  scan_i32(P<int32_t>(ctx->cig_words), P<int64_t>(ctx->cig_coff), T_a, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->cig_coff) + T_a})) return G3A_E_HIP;
+  if (ensure(ctx, ctx->w->cig_words, (size_t)T_a * 4 + 4) || ensure(ctx, ctx->w->cig_coff, (size_t)(T_a + 1) * 8))
+    return alloc_fail(ctx);
+  launch_cig_words(P<Aln>(ctx->w->alns), T_a, P<int32_t>(ctx->w->cig_words), s);
+  scan_i32(P<int32_t>(ctx->w->cig_words), P<int64_t>(ctx->w->cig_coff), T_a, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
+  if (readback(ctx, {P<int64_t>(ctx->w->cig_coff) + T_a})) return G3A_E_HIP;
   const int64_t T_u = ctx->pinned[0];
-  if (ensure(ctx, ctx->cig_out, (size_t)T_u * 4 + 64)) return G3A_E_HIP;
-  launch_cig_gather(P<Aln>(ctx->alns), T_a, P<int64_t>(ctx->cig_coff), P<uint32_t>(ctx->cigar),
-                    P<uint32_t>(ctx->cig_out), s);
+  if (ensure(ctx, ctx->w->cig_out, (size_t)T_u * 4 + 64)) return alloc_fail(ctx);
+  launch_cig_gather(P<Aln>(ctx->w->alns), T_a, P<int64_t>(ctx->w->cig_coff), P<uint32_t>(ctx->w->cigar),
+                    P<uint32_t>(ctx->w->cig_out), s);
   ctx->total_cig = T_u;
   HIPCK(hipEventRecord(ctx->ev[5], s));
   HIPCK(hipStreamSynchronize(s));
@@ -792,6 +885,38 @@ int g3a_bwa_align(g3a_ctx *ctx) {
   return G3A_OK;
 }
 
+int g3a_bwa_align(g3a_ctx *ctx) {
+  if (!ctx) return G3A_E_ARG;
+  if (!ctx->index_on_device || !ctx->has_input) return fail(ctx, G3A_E_STATE, "index or input not on device");
+  HIPCK(hipSetDevice(ctx->device));
+  ctx->w = &ctx->own;
+  set_grids(ctx, ctx->n_reads, false);
+  return align_batch(ctx);
+}
+
+int g3a_bwa_align_noalloc(g3a_ctx *ctx) {
+  if (!ctx) return G3A_E_ARG;
+  if (!ctx->index_on_device || !ctx->has_input) return fail(ctx, G3A_E_STATE, "index or input not on device");
+  if (ctx->n_reads > G3A_NOALLOC_MAX_READS)
+    return fail(ctx, G3A_E_ARG, "the allocation-free entry point takes at most " +
+                                    std::to_string(G3A_NOALLOC_MAX_READS) + " reads per batch");
+  if (!ctx->scratch) return fail(ctx, G3A_E_STATE, "no scratch array on this device");
+  HIPCK(hipSetDevice(ctx->device));
+  // the previous contents of the scratch array are not read: every buffer
+  // the pipeline expects to start at zero is cleared by a kernel
+  ScratchArena arena;
+  arena.base = ctx->scratch;
+  arena.cap = (size_t)G3A_NOALLOC_SCRATCH_BYTES;
+  ctx->scratch_work = Work{};
+  ctx->w = &ctx->scratch_work;
+  ctx->arena = &arena;
+  set_grids(ctx, ctx->n_reads, true);
+  const int rc = align_batch(ctx);
+  ctx->arena = nullptr;
+  if (rc) ctx->has_output = false;
+  return rc;
+}
+
 static TextArgs text_args(g3a_ctx *ctx, int mode) {
   TextArgs t{};
   t.mode = mode;
@@ -807,9 +932,9 @@ static TextArgs text_args(g3a_ctx *ctx, int mode) {
   t.ctg_off = P<int64_t>(ctx->ctg_off);
   t.pac = ctx->idx.pac;
   t.l_pac = ctx->idx.l_pac;
-  t.aln_off = P<int64_t>(ctx->aln_off);
-  t.alns = P<Aln>(ctx->alns);
-  t.cigar = P<uint32_t>(ctx->cig_out);  // the compacted CIGARs (k_cig_gather)
+  t.aln_off = P<int64_t>(ctx->w->aln_off);
+  t.alns = P<Aln>(ctx->w->alns);
+  t.cigar = P<uint32_t>(ctx->w->cig_out);  // the compacted CIGARs (k_cig_gather)
   t.log_tab = P<float>(ctx->log_tab);
   t.a = ctx->opt.a;
   t.b = ctx->opt.b;
@@ -826,11 +951,11 @@ int g3a_format_records(g3a_ctx *ctx, int mode, int64_t *bytes) {
   hipStream_t s = ctx->stream;
   const int n = ctx->n_reads;
   if (ensure(ctx, ctx->txt_len, (size_t)n * 8 + 8) || ensure(ctx, ctx->txt_off, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(n + 1)))
+      ensure(ctx, ctx->fmt_scan_tmp, scan_tmp_bytes(n + 1)))
     return G3A_E_HIP;
   const TextArgs t = text_args(ctx, mode);
   launch_text_len(t, P<int64_t>(ctx->txt_len), s);
-  scan_i64(P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_off), n, ctx->scan_tmp.p, ctx->scan_tmp.bytes, s);
+  scan_i64(P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_off), n, ctx->fmt_scan_tmp.p, ctx->fmt_scan_tmp.bytes, s);
   if (readback(ctx, {P<int64_t>(ctx->txt_off) + n})) return G3A_E_HIP;
   const int64_t total = ctx->pinned[0];
   if (ensure(ctx, ctx->txt, (size_t)total + 16)) return G3A_E_HIP;
@@ -870,12 +995,12 @@ int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   if (ctx->out_buf.size() < at_cig + cig_bytes) ctx->out_buf.resize(at_cig + cig_bytes + (at_cig + cig_bytes) / 4);
   uint8_t *hb = ctx->out_buf.data();
   if (n) {
-    if (int rc = stage_d2h(ctx, out->aln_off, ctx->aln_off.p, off_bytes)) return rc;
+    if (int rc = stage_d2h(ctx, out->aln_off, ctx->w->aln_off.p, off_bytes)) return rc;
   } else out->aln_off[0] = 0;
   if (na)
-    if (int rc = stage_d2h(ctx, hb, ctx->alns.p, aln_bytes)) return rc;
+    if (int rc = stage_d2h(ctx, hb, ctx->w->alns.p, aln_bytes)) return rc;
   if (cig_bytes)
-    if (int rc = stage_d2h(ctx, hb + at_cig, ctx->cig_out.p, cig_bytes)) return rc;
+    if (int rc = stage_d2h(ctx, hb + at_cig, ctx->w->cig_out.p, cig_bytes)) return rc;
   const Aln *alns = (const Aln *)hb;
   const uint32_t *cig = (const uint32_t *)(hb + at_cig);
   const size_t m = (size_t)(na ? na : 1);
@@ -918,17 +1043,17 @@ int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   out->n_md_bytes = 0;
   if (na) {
     if (ensure(ctx, ctx->mapq, (size_t)na * 4) || ensure(ctx, ctx->md_len, (size_t)na * 8) ||
-        ensure(ctx, ctx->md_off, (size_t)(na + 1) * 8) || ensure(ctx, ctx->scan_tmp, scan_tmp_bytes(na + 1)))
+        ensure(ctx, ctx->md_off, (size_t)(na + 1) * 8) || ensure(ctx, ctx->fmt_scan_tmp, scan_tmp_bytes(na + 1)))
       return G3A_E_HIP;
     const TextArgs t = text_args(ctx, 1);
-    launch_aln_mapq_md(t, na, P<int32_t>(ctx->aln_read), P<int32_t>(ctx->mapq), P<int64_t>(ctx->md_len),
+    launch_aln_mapq_md(t, na, P<int32_t>(ctx->w->aln_read), P<int32_t>(ctx->mapq), P<int64_t>(ctx->md_len),
                        ctx->stream);
-    scan_i64(P<int64_t>(ctx->md_len), P<int64_t>(ctx->md_off), na, ctx->scan_tmp.p, ctx->scan_tmp.bytes,
+    scan_i64(P<int64_t>(ctx->md_len), P<int64_t>(ctx->md_off), na, ctx->fmt_scan_tmp.p, ctx->fmt_scan_tmp.bytes,
              ctx->stream);
     if (readback(ctx, {P<int64_t>(ctx->md_off) + na})) return G3A_E_HIP;
     const int64_t mdb = ctx->pinned[0];
     if (ensure(ctx, ctx->md, (size_t)mdb + 16)) return G3A_E_HIP;
-    launch_aln_md_write(t, na, P<int32_t>(ctx->aln_read), P<int64_t>(ctx->md_off), P<char>(ctx->md), ctx->stream);
+    launch_aln_md_write(t, na, P<int32_t>(ctx->w->aln_read), P<int64_t>(ctx->md_off), P<char>(ctx->md), ctx->stream);
     HIPCK(hipStreamSynchronize(ctx->stream));
     HIPCK(hipGetLastError());
     out->md = (char *)std::malloc((size_t)mdb + 1);
@@ -942,9 +1067,9 @@ int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   out->ms_seed = ctx->ms[0]; out->ms_sa = ctx->ms[1]; out->ms_chain = ctx->ms[2];
   out->ms_extend = ctx->ms[3]; out->ms_traceback = ctx->ms[4]; out->ms_total = ctx->ms[5];
   int64_t c[4] = {0, 0, 0, 0};
-  HIPCK(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(c, ctx->w->counters.p, sizeof c, hipMemcpyDeviceToHost));
   int64_t cells = 0;
-  HIPCK(hipMemcpy(&cells, P<int64_t>(ctx->counters) + 24, 8, hipMemcpyDeviceToHost));
+  HIPCK(hipMemcpy(&cells, P<int64_t>(ctx->w->counters) + 24, 8, hipMemcpyDeviceToHost));
   out->fm_steps = c[0];
   out->ext_cells = cells;
   out->n_seeds = ctx->total_seeds;
@@ -970,11 +1095,12 @@ int g3a_kernel_ms(const g3a_ctx *ctx, float ms[4]) {
 
 int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *best_ms) {
   if (!ctx) return G3A_E_ARG;
-  if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
+  if (!ctx->has_output || ctx->w != &ctx->own)
+    return fail(ctx, G3A_E_STATE, "autotune needs a batch aligned on the device by g3a_bwa_align");
   HIPCK(hipSetDevice(ctx->device));
   const int form = ctx->form, sr0 = ctx->strip_rows, lb0 = ctx->lane_blocks_per_cu, pk0 = ctx->ext_packed;
   ctx->form = G3A_FORM_BATCHED;
-  int64_t *cells = P<int64_t>(ctx->counters) + 24;
+  int64_t *cells = P<int64_t>(ctx->w->counters) + 24;
   int64_t saved = 0;
   HIPCK(hipMemcpy(&saved, cells, 8, hipMemcpyDeviceToHost));
   const int srs[2] = {16, 32}, lbs[4] = {2, 4, 8, 16};
@@ -1108,7 +1234,7 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
 int g3a_learned_stats(const g3a_ctx *ctx, int64_t out[6]) {
   if (!ctx || !out) return G3A_E_ARG;
   if (hipSetDevice(ctx->device) != hipSuccess ||
-      hipMemcpy(out, (const int64_t *)ctx->counters.p + 32, 6 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      hipMemcpy(out, (const int64_t *)ctx->w->counters.p + 32, 6 * 8, hipMemcpyDeviceToHost) != hipSuccess)
     return G3A_E_HIP;
   return G3A_OK;
 }
@@ -1123,8 +1249,8 @@ int g3a_last_intervals(const g3a_ctx *ctx, int32_t *counts, int64_t *kmsn, int64
   }
   std::vector<int64_t> base((size_t)n);
   if (hipSetDevice(ctx->device) != hipSuccess ||
-      hipMemcpy(counts, ctx->n_intv.p, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(base.data(), ctx->intv_base.p, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      hipMemcpy(counts, ctx->w->n_intv.p, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(base.data(), ctx->w->intv_base.p, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess)
     return G3A_E_HIP;
   int64_t t = 0, extent = 0;
   for (int r = 0; r < n; ++r) {
@@ -1135,7 +1261,7 @@ int g3a_last_intervals(const g3a_ctx *ctx, int32_t *counts, int64_t *kmsn, int64
   if (!kmsn) return G3A_OK;
   if (cap < t) return G3A_E_ARG;
   std::vector<Intv> iv((size_t)extent);
-  if (extent && hipMemcpy(iv.data(), ctx->intv.p, (size_t)extent * sizeof(Intv), hipMemcpyDeviceToHost) != hipSuccess)
+  if (extent && hipMemcpy(iv.data(), ctx->w->intv.p, (size_t)extent * sizeof(Intv), hipMemcpyDeviceToHost) != hipSuccess)
     return G3A_E_HIP;
   int64_t w = 0;
   for (int r = 0; r < n; ++r)
@@ -1152,7 +1278,8 @@ int g3a_last_intervals(const g3a_ctx *ctx, int32_t *counts, int64_t *kmsn, int64
 
 int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms) {
   if (!ctx) return G3A_E_ARG;
-  if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "autotune needs an aligned batch on the device");
+  if (!ctx->has_output || ctx->w != &ctx->own)
+    return fail(ctx, G3A_E_STATE, "autotune needs a batch aligned on the device by g3a_bwa_align");
   const int opts[4] = {6, 8, 10, 0};
   float best = -1;
   int bb = ctx->seed_blocks_per_cu, bo = ctx->seed_ordered;
@@ -1205,7 +1332,7 @@ int g3a_last_stats(const g3a_ctx *ctx, float ms[6], int64_t work[6]) {
   if (!ctx->has_output) return G3A_E_STATE;
   for (int i = 0; i < 6; ++i) ms[i] = ctx->ms[i];
   int64_t c[32];
-  if (hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) return G3A_E_HIP;
+  if (hipMemcpy(c, ctx->w->counters.p, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) return G3A_E_HIP;
   work[0] = ctx->total_seeds;
   work[1] = ctx->total_tasks;
   work[2] = c[24];
@@ -1229,21 +1356,21 @@ void g3a_destroy(g3a_ctx *ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&ctx->occ, &ctx->kmer, &ctx->occn, &ctx->sbc, &ctx->pac, &ctx->sa, &ctx->ctg_off, &ctx->ctg_len, &ctx->ctg_alt, &ctx->seq, &ctx->off,
-                    &ctx->intv, &ctx->n_intv, &ctx->n_seed, &ctx->slow, &ctx->intv_base, &ctx->seed_off,
-                    &ctx->counters, &ctx->slow_list, &ctx->slow_scratch, &ctx->slow_n, &ctx->slow_off, &ctx->seeds,
-                    &ctx->seeds2, &ctx->ch, &ctx->nxt, &ctx->order, &ctx->tmpi, &ctx->w16, &ctx->kept, &ctx->nodes,
-                    &ctx->tasks_tmp, &ctx->n_task, &ctx->task_off, &ctx->tasks, &ctx->regs, &ctx->flag, &ctx->sec,
-                    &ctx->n_aln, &ctx->aln_off, &ctx->aln_reg, &ctx->aln_read, &ctx->aln_rank, &ctx->z_size,
-                    &ctx->cig_size, &ctx->z_off, &ctx->cig_off, &ctx->z, &ctx->cigar, &ctx->alns, &ctx->scan_tmp,
-                    &ctx->cig_words, &ctx->cig_coff, &ctx->cig_out, &ctx->seed_key, &ctx->seed_key2, &ctx->seed_idx,
-                    &ctx->seed_order, &ctx->seed_sort_tmp,
-                    &ctx->ext_res[0], &ctx->ext_res[1], &ctx->ext_key, &ctx->ext_key2, &ctx->ext_idx,
-                    &ctx->ext_sorted, &ctx->ext_slow, &ctx->ext_ctr, &ctx->ext_sort_tmp, &ctx->ext_rowbuf,
-                    &ctx->lep_pool, &ctx->eh_pool, &ctx->tb_order, &ctx->tb_ctr, &ctx->tb_nw, &ctx->li_saf, &ctx->li_keys,
+                    &ctx->own.intv, &ctx->own.n_intv, &ctx->own.n_seed, &ctx->own.slow, &ctx->own.intv_base, &ctx->own.seed_off,
+                    &ctx->own.counters, &ctx->own.slow_list, &ctx->own.slow_scratch, &ctx->own.slow_n, &ctx->own.slow_off, &ctx->own.seeds,
+                    &ctx->own.seeds2, &ctx->own.ch, &ctx->own.nxt, &ctx->own.order, &ctx->own.tmpi, &ctx->own.w16, &ctx->own.kept, &ctx->own.nodes,
+                    &ctx->own.tasks_tmp, &ctx->own.n_task, &ctx->own.task_off, &ctx->own.tasks, &ctx->own.regs, &ctx->own.flag, &ctx->own.sec,
+                    &ctx->own.n_aln, &ctx->own.aln_off, &ctx->own.aln_reg, &ctx->own.aln_read, &ctx->own.aln_rank, &ctx->own.z_size,
+                    &ctx->own.cig_size, &ctx->own.z_off, &ctx->own.cig_off, &ctx->own.z, &ctx->own.cigar, &ctx->own.alns, &ctx->own.scan_tmp,
+                    &ctx->own.cig_words, &ctx->own.cig_coff, &ctx->own.cig_out, &ctx->own.seed_key, &ctx->own.seed_key2, &ctx->own.seed_idx,
+                    &ctx->own.seed_order, &ctx->own.seed_sort_tmp,
+                    &ctx->own.ext_res[0], &ctx->own.ext_res[1], &ctx->own.ext_key, &ctx->own.ext_key2, &ctx->own.ext_idx,
+                    &ctx->own.ext_sorted, &ctx->own.ext_slow, &ctx->own.ext_ctr, &ctx->own.ext_sort_tmp, &ctx->own.ext_rowbuf,
+                    &ctx->own.lep_pool, &ctx->own.eh_pool, &ctx->own.tb_order, &ctx->own.tb_ctr, &ctx->own.tb_nw, &ctx->li_saf, &ctx->li_keys,
                     &ctx->li_root, &ctx->li_leaf, &ctx->li_leaf_a, &ctx->li_mom, &ctx->li_short, &ctx->li_ctr,
                     &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,
                     &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->mapq, &ctx->md_len,
-                    &ctx->md_off, &ctx->md};
+                    &ctx->md_off, &ctx->md, &ctx->fmt_scan_tmp};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

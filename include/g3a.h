@@ -103,6 +103,23 @@ int g3a_memcpy_index(g3a_ctx *ctx);
 int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t *offsets);
 /* bwa_align: run the pipeline on the batch resident on the device. */
 int g3a_bwa_align(g3a_ctx *ctx);
+/* bwa_align without device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No reference counterpart.  The same pipeline and output as
+ * g3a_bwa_align, with its per-batch workspace carved out of a static device
+ * array of G3A_NOALLOC_SCRATCH_BYTES per device instead of allocated: it calls
+ * no hipMalloc, hipFree or allocating library routine.  The batch (resident
+ * from g3a_memcpy_input, on a context set up by g3a_device_alloc and
+ * g3a_memcpy_index) may hold at most G3A_NOALLOC_MAX_READS reads; a batch
+ * whose intermediate data do not fit the array (many seeds or gapped
+ * alignments per read) is refused with G3A_E_ARG and the message names the
+ * limit.  Results stay in the array for g3a_memcpy_output and
+ * g3a_format_records until the next call on that device.  Every context on
+ * one device shares the array: calls that overlap on one device race, and a
+ * call overwrites the previous call's results. */
+#define G3A_NOALLOC_MAX_READS 4096
+#define G3A_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int g3a_bwa_align_noalloc(g3a_ctx *ctx);
 /* memcpy_output: copy the results of the last g3a_bwa_align to the host. */
 int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out);
 /* Names and base qualities of the batch of the last g3a_memcpy_input (same
