@@ -274,13 +274,22 @@ def pcie_inclusive(al, batch, steps: int) -> dict:
 
 
 def gather_ceiling() -> dict | None:
-    """Random-access ceiling for seeding: torch gathers of random 16-byte and
-    64-byte rows of a 1 GiB table (the Occ lines of a 1 Gbp index are 16-byte
-    rows of a 1 GB table), useful bytes per second."""
-    try:
-        import torch
-    except Exception:
+    """Random-access ceiling for seeding, measured in a child process (torch
+    carries its own HIP runtime, which must not share a process with the
+    library's)."""
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gather-ceiling"], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode != 0:
+        log(f"gather ceiling failed: {r.stderr[-300:]}")
         return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _gather_ceiling_torch() -> dict:
+    """torch gathers of random 16-byte and 64-byte rows of a 1 GiB table (the
+    Occ lines of a 1 Gbp index are 16-byte rows of a 1 GB table), useful
+    bytes per second."""
+    import torch
     out = {}
     g = torch.Generator(device="cuda")
     g.manual_seed(5)
@@ -365,7 +374,12 @@ def main() -> int:
                     help="learned form: 1 = full SA and model lookups, 2 = full SA only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (torch) to calibrate HBM byte counters")
+    ap.add_argument("--gather-ceiling", action="store_true",
+                    help="only measure the random-gather ceiling (torch) and print it as JSON")
     args = ap.parse_args()
+    if args.gather_ceiling:
+        print(json.dumps(_gather_ceiling_torch()))
+        return 0
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4
@@ -381,6 +395,8 @@ def main() -> int:
               "(reads are independent: see DESIGN.md for how the path shards)", file=sys.stderr)
         return 2
     cfg = CONFIGS[args.config]
+    log("random-gather ceiling")
+    ceiling = gather_ceiling()
     prefix, fq, prep = prepare(args.config, Path(args.data))
     log("loading reads")
     batch = load_reads(fq)
@@ -445,8 +461,6 @@ def main() -> int:
             "algorithmic_bytes": seed_bytes, "bytes_per_fm_step": step_bytes, "launch_ms": round(kms["k_seed"], 3)}
     if dominant != "k_seed":
         roof["dominant"] = dominant
-    log("random-gather ceiling")
-    ceiling = gather_ceiling()
     if ceiling:
         roof["random_gather_ceiling_GBps"] = ceiling
     # cells of the reference's extension cell set; the batched form counts
