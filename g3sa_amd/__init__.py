@@ -70,7 +70,7 @@ class LearnedInfo(C.Structure):
         ("leaves", C.c_int32), ("hidden_units", C.c_int32), ("window_cap", C.c_int32), ("pad", C.c_int32),
         ("model_bytes", C.c_int64), ("sa_bytes", C.c_int64), ("key_bytes", C.c_int64),
         ("max_err", C.c_int64), ("keys_over_window", C.c_int64), ("leaves_over_window", C.c_int64),
-        ("mean_err", C.c_double), ("build_ms", C.c_float), ("pad2", C.c_float),
+        ("mean_err", C.c_double), ("build_ms", C.c_float), ("pad2", C.c_float), ("knots", C.c_int64),
     ]
 
 

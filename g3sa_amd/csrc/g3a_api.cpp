@@ -124,7 +124,7 @@ struct g3a_ctx {
   float kms[4] = {0, 0, 0, 0};
   // last batch results
   // learned-index form: full SA, keys, model, build scratch
-  DevBuf li_saf, li_keys, li_root, li_leaf, li_leaf_a, li_mom, li_short, li_ctr;
+  DevBuf li_saf, li_keys, li_root, li_leaf, li_knot, li_leaf_a, li_short, li_ctr;
   bool li_built = false;
   uint32_t li_leaf_w = 0;
   g3a_learned_info li_info{};
@@ -1163,6 +1163,7 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
     x.keys = nullptr;
     x.li_root = nullptr;
     x.li_leaf = nullptr;
+    x.li_knot = nullptr;
     x.li_short = nullptr;
     if (info) *info = ctx->li_info;
     return G3A_OK;
@@ -1174,7 +1175,6 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
     if (ensure(ctx, ctx->li_saf, (size_t)N * 4) || ensure(ctx, ctx->li_keys, (size_t)(N + 16) * 8) ||
         ensure(ctx, ctx->li_root, ((size_t)1 << kLiRootBits) * 4 + 4) ||
         ensure(ctx, ctx->li_leaf, sizeof(LiLeaf) * kLiLeaves) || ensure(ctx, ctx->li_leaf_a, (kLiLeaves + 1) * 4) ||
-        ensure(ctx, ctx->li_mom, sizeof(double) * 5 * kLiLeaves * (kLiHinge + 1)) ||
         ensure(ctx, ctx->li_short, 2 * kLiShort * 4) || ensure(ctx, ctx->li_ctr, 64))
       return G3A_E_HIP;
     LearnedBufs b{};
@@ -1183,13 +1183,19 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
     b.root = P<uint32_t>(ctx->li_root);
     b.leaf = P<LiLeaf>(ctx->li_leaf);
     b.leaf_a = P<uint32_t>(ctx->li_leaf_a);
-    b.mom = P<double>(ctx->li_mom);
     b.shortt = P<uint32_t>(ctx->li_short);
     b.ctr = P<unsigned long long>(ctx->li_ctr);
     b.n_samples = (int64_t)(ctx->h.sa.size() / ctx->h.sa_width);
     LearnedBuildStats st{};
     HIPCK(hipEventRecord(ctx->ev[6], ctx->stream));
-    const int rc = learned_build(x, b, ctx->stream, st);
+    int64_t n_knots = 0;
+    int rc = learned_build_begin(x, b, ctx->stream, &n_knots);
+    if (rc == 0) {
+      if (ensure(ctx, ctx->li_knot, (size_t)(n_knots + 1) * sizeof(LiKnot))) return G3A_E_HIP;
+      b.knot = P<LiKnot>(ctx->li_knot);
+      rc = learned_build_end(x, b, ctx->stream, st);
+    }
+    if (rc == 1) return fail(ctx, G3A_E_ARG, "the learned index needs a text of fewer than 2^32 rows");
     if (rc == 2) return fail(ctx, G3A_E_HIP, "learned index build: HIP error");
     if (rc) {
       char m[256];
@@ -1208,9 +1214,11 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
     li.rows = N;
     li.distinct_keys = st.distinct_keys;
     li.leaves = kLiLeaves;
-    li.hidden_units = kLiHinge;
+    li.hidden_units = st.max_knots > 0 ? st.max_knots - 1 : 0;
     li.window_cap = kLiWindowCap;
-    li.model_bytes = (int64_t)sizeof(LiLeaf) * kLiLeaves + (((int64_t)1 << kLiRootBits) + 1) * 4 + 2 * kLiShort * 4;
+    li.knots = st.knots;
+    li.model_bytes = (int64_t)sizeof(LiLeaf) * kLiLeaves + (int64_t)sizeof(LiKnot) * st.knots +
+                     (((int64_t)1 << kLiRootBits) + 1) * 4 + 2 * kLiShort * 4;
     li.sa_bytes = N * 4;
     li.key_bytes = N * 8;
     li.max_err = st.max_err;
@@ -1225,6 +1233,7 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
   x.keys = P<uint64_t>(ctx->li_keys);
   x.li_root = P<uint32_t>(ctx->li_root);
   x.li_leaf = on == 1 ? P<LiLeaf>(ctx->li_leaf) : nullptr;  // 2: no model lookups
+  x.li_knot = P<LiKnot>(ctx->li_knot);
   x.li_short = P<uint32_t>(ctx->li_short);
   x.li_leaf_w = ctx->li_leaf_w;
   if (info) *info = ctx->li_info;
@@ -1367,7 +1376,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->own.ext_res[0], &ctx->own.ext_res[1], &ctx->own.ext_key, &ctx->own.ext_key2, &ctx->own.ext_idx,
                     &ctx->own.ext_sorted, &ctx->own.ext_slow, &ctx->own.ext_ctr, &ctx->own.ext_sort_tmp, &ctx->own.ext_rowbuf,
                     &ctx->own.lep_pool, &ctx->own.eh_pool, &ctx->own.tb_order, &ctx->own.tb_ctr, &ctx->own.tb_nw, &ctx->li_saf, &ctx->li_keys,
-                    &ctx->li_root, &ctx->li_leaf, &ctx->li_leaf_a, &ctx->li_mom, &ctx->li_short, &ctx->li_ctr,
+                    &ctx->li_root, &ctx->li_leaf, &ctx->li_knot, &ctx->li_leaf_a, &ctx->li_short, &ctx->li_ctr,
                     &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,
                     &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->mapq, &ctx->md_len,
                     &ctx->md_off, &ctx->md, &ctx->fmt_scan_tmp};

@@ -37,22 +37,32 @@ struct Opt {
 // suffix (2 bits per symbol, first symbol highest; symbols past the text end
 // are 0) to its row.  The root is a piecewise-linear interpolation of the
 // exact row of every 6-symbol prefix; it routes a key to one of kLiLeaves
-// leaves.  A leaf is a one-hidden-layer network in FP32: input x = the key's
-// offset from the leaf's first key, scaled to [0, 1]; kLiHinge ReLU units
-// relu(x - t_h) with their knots t_h at the leaf's key quantiles; output
-// c0 + c1 x + sum w_h relu(x - t_h) = row offset from the leaf's first row.
+// leaves.  A leaf is a one-hidden-layer network of the key with FP32
+// weights: ReLU units relu(key - K_h) with their knots K_h at keys of the
+// leaf, output y_0 + s_0 (key - K_0) + sum_h (s_h - s_{h-1}) relu(key - K_h)
+// = the predicted row offset from the leaf's first row, a continuous
+// piecewise-linear function of the key.  It is stored as its knots (K_k, the
+// output y_k there and the slope s_k after it) and evaluated on the segment
+// that holds the key as y_k + s_k (key - K_k), the difference taken in
+// 64-bit integers before it is converted to FP32: the same function, with
+// the resolution of each segment instead of the leaf's (a cluster of
+// near-equal keys gets a segment of its own) and without adding up terms of
+// opposite sign.  The knots of all leaves share one pool.
 constexpr int kLiRootBits = 12;
 constexpr int kLiLeaves = 4096;
-constexpr int kLiHinge = 30;
 constexpr int kLiWindowCap = 256;  // largest half-width of a search window
 constexpr int kLiShort = 32;       // rows whose suffix has fewer than 32 symbols ('$' row included)
 
 struct alignas(16) LiLeaf {
-  uint32_t a;     // first row routed to the leaf
-  uint32_t err;   // largest |predicted - true row| over the leaf's distinct keys
-  uint64_t kmin;  // key of row a
-  float inv_span, c0, c1, pad;
-  float t[kLiHinge], w[kLiHinge];
+  uint32_t a;    // first row routed to the leaf
+  uint32_t err;  // largest |predicted - true row| over the leaf's distinct keys
+  uint32_t k0;   // first knot in the pool
+  uint32_t nk;   // knots (hidden units + 1); 0 for a leaf without rows
+};
+
+struct alignas(16) LiKnot {
+  uint64_t key;  // K_k
+  float y, s;    // output at K_k, slope (rows per key unit) after it
 };
 
 // Device view of the index.  `occ` holds one 32-byte line per 64 BWT rows:
@@ -85,12 +95,13 @@ struct DevIndex {
   const uint32_t *sbc;
   // learned-index form (all null otherwise; texts of fewer than 2^32 rows):
   // the full suffix array, the 32-symbol key of every row, the root's
-  // 2^kLiRootBits + 1 segment rows, the leaves, and the (row, suffix length)
-  // pairs of the kLiShort short suffixes in row order
+  // 2^kLiRootBits + 1 segment rows, the leaves and their knot pool, and the
+  // (row, suffix length) pairs of the kLiShort short suffixes in row order
   const uint32_t *saf;
   const uint64_t *keys;
   const uint32_t *li_root;
   const LiLeaf *li_leaf;
+  const LiKnot *li_knot;
   const uint32_t *li_short;
   uint32_t li_leaf_w;  // routing: leaf = predicted row / li_leaf_w
 };
@@ -522,25 +533,25 @@ G3A_HD uint32_t li_route(const DevIndex &x, uint64_t key) {
   return j < (uint64_t)kLiLeaves ? (uint32_t)j : (uint32_t)(kLiLeaves - 1);
 }
 
-G3A_HD float li_leaf_x(const LiLeaf &L, uint64_t key) {
-  const uint64_t d = key > L.kmin ? key - L.kmin : 0;
-  const float v = (float)d * L.inv_span;
-  return v < 1.f ? v : 1.f;
-}
-
 // The leaf network's output: predicted row offset from the leaf's first row
-G3A_HD float li_leaf_eval(const LiLeaf &L, float v) {
-  float y = L.c0 + L.c1 * v;
-#pragma unroll 5
-  for (int h = 0; h < kLiHinge; ++h) {
-    const float u = v - L.t[h];
-    y += L.w[h] * (u > 0.f ? u : 0.f);
+// (the segment of the last knot <= key, found by bisection; keys below the
+// first knot extend its segment)
+G3A_HD float li_leaf_eval(const LiKnot *pool, const LiLeaf &L, uint64_t key) {
+  if (L.nk == 0) return 0.f;
+  const LiKnot *kn = pool + L.k0;
+  int lo = 0, hi = (int)L.nk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (kn[mid].key <= key) lo = mid;
+    else hi = mid - 1;
   }
-  return y;
+  const LiKnot k = kn[lo];
+  const float d = key >= k.key ? (float)(key - k.key) : -(float)(k.key - key);
+  return k.y + k.s * d;
 }
 
-G3A_HD int64_t li_predict(const LiLeaf &L, uint64_t key) {
-  float y = li_leaf_eval(L, li_leaf_x(L, key));
+G3A_HD int64_t li_predict(const LiKnot *pool, const LiLeaf &L, uint64_t key) {
+  float y = li_leaf_eval(pool, L, key);
   y = y > -1e9f ? (y < 5e9f ? y : 5e9f) : -1e9f;
   return (int64_t)L.a + (int64_t)(y >= 0.f ? y + 0.5f : y - 0.5f);
 }
@@ -551,7 +562,7 @@ G3A_HD int64_t li_predict(const LiLeaf &L, uint64_t key) {
 G3A_HD int64_t li_lower_bound(const DevIndex &x, uint64_t K) {
   const LiLeaf &L = x.li_leaf[li_route(x, K)];
   const int64_t N = x.n + 1;
-  const int64_t p = li_predict(L, K);
+  const int64_t p = li_predict(x.li_knot, L, K);
   int64_t E = (int64_t)L.err + 2;
   if (E > kLiWindowCap) E = kLiWindowCap;
   int64_t lo = p - E, hi = p + E + 1;
@@ -628,7 +639,7 @@ G3A_HD bool li_range(const DevIndex &x, uint64_t W, int j, uint32_t &k, uint32_t
   }
   if (!bounded || hi - lo > kLiModelMin) {
     const LiLeaf &L = x.li_leaf[li_route(x, W)];
-    const int64_t p = li_predict(L, W);
+    const int64_t p = li_predict(x.li_knot, L, W);
     int64_t E = (int64_t)L.err + 2;
     if (E > kLiWindowCap) E = kLiWindowCap;
     int64_t wl = p - E, wh = p + E + 1;

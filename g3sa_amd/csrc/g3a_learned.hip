@@ -7,10 +7,12 @@
 //   2. the key of every row: the first 32 symbols of its suffix, 2 bits
 //      each, zero past the text end (keys are non-decreasing in row order);
 //   3. the root: the first row of every 6-symbol prefix;
-//   4. the leaves: first rows (from the root's routing), knots at each
-//      leaf's key quantiles, per-segment moments of (x, row) over the
-//      distinct keys, a least-squares solve of the output layer in FP64,
-//      and the largest prediction error over the distinct keys.
+//   4. the leaves: first rows (from the root's routing), then each leaf's
+//      network trained by an error-bounded fit over its distinct keys
+//      (li_fit_pass: greedy, a knot where a line can no longer stay within
+//      250 rows of every key; a counting pass sizes the knot pool, a second
+//      pass writes it), and the largest prediction error over the distinct
+//      keys, measured with the lookup's own evaluation.
 // The model is read by li_range / li_lower_bound (g3a_dev.h), whose window
 // search is exact; the error bounds only size the windows.
 #include "g3a_dev.h"
@@ -104,56 +106,35 @@ __global__ void __launch_bounds__(256) k_li_bounds(DevIndex x, int64_t N, uint32
   for (int64_t j = j0 + 1; j <= j1; ++j) leaf_a[j] = (uint32_t)r;
 }
 
-// leaf set-up: first row, key range and knots at the key quantiles
-__global__ void __launch_bounds__(256) k_li_leaf_init(const uint64_t *keys, const uint32_t *leaf_a, LiLeaf *leaf) {
+// leaf training, pass 1: knots per leaf (thread per leaf over its rows)
+__global__ void __launch_bounds__(64) k_li_count(const uint64_t *keys, const uint32_t *leaf_a, LiLeaf *leaf) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= kLiLeaves) return;
-  li_leaf_init(keys, leaf_a[j], leaf_a[j + 1], leaf[j]);
+  LiLeaf L;
+  L.a = leaf_a[j];
+  L.err = 0;
+  L.k0 = 0;
+  L.nk = li_fit_pass(keys, leaf_a[j], leaf_a[j + 1], kLiFitEps, nullptr);
+  leaf[j] = L;
 }
 
-__device__ __forceinline__ void li_flush(double *mom, int bin, const double acc[5]) {
-  if (bin < 0) return;
-#pragma unroll
-  for (int k = 0; k < 5; ++k)
-    if (acc[k] != 0.0) atomicAdd(mom + 5 * (int64_t)bin + k, acc[k]);
-}
-
-// moments (count, sum x, sum x^2, sum y, sum x y) of the distinct keys per
-// (leaf, segment), y = row - first row of the leaf
-__global__ void __launch_bounds__(256) k_li_moments(DevIndex x, const uint32_t *leaf_a, double *mom, int64_t N) {
-  const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kChunk;
-  if (r0 >= N) return;
-  const int64_t r1 = r0 + kChunk < N ? r0 + kChunk : N;
-  int cur = -1;
-  double acc[5] = {0, 0, 0, 0, 0};
-  for (int64_t r = r0; r < r1; ++r) {
-    const uint64_t K = x.keys[r];
-    if (r > 0 && x.keys[r - 1] == K) continue;
-    const uint32_t j = li_route(x, K);
-    const LiLeaf &L = x.li_leaf[j];
-    const int64_t a = L.a, c = (int64_t)leaf_a[j + 1] - a;
-    const int bin = (int)j * (kLiHinge + 1) + li_segment(a, c, r);
-    if (bin != cur) {
-      li_flush(mom, cur, acc);
-      cur = bin;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) acc[k] = 0.0;
-    }
-    const double v = (double)li_leaf_x(L, K), y = (double)(r - a);
-    acc[0] += 1.0;
-    acc[1] += v;
-    acc[2] += v * v;
-    acc[3] += y;
-    acc[4] += v * y;
+// pool offsets of the leaves' knots (one thread: 4096 entries) and the total
+__global__ void k_li_offsets(LiLeaf *leaf, unsigned long long *total) {
+  if (blockIdx.x || threadIdx.x) return;
+  unsigned long long t = 0;
+  for (int j = 0; j < kLiLeaves; ++j) {
+    leaf[j].k0 = (uint32_t)t;
+    t += leaf[j].nk;
   }
-  li_flush(mom, cur, acc);
+  *total = t;
 }
 
-// output layer of each leaf (li_solve_leaf)
-__global__ void __launch_bounds__(64) k_li_solve(LiLeaf *leaf, const double *mom) {
+// pass 2: the knots, written at the leaf's pool offset
+__global__ void __launch_bounds__(64) k_li_fit(const uint64_t *keys, const uint32_t *leaf_a, const LiLeaf *leaf,
+                                               LiKnot *pool) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= kLiLeaves) return;
-  li_solve_leaf(leaf[j], mom + 5 * (int64_t)j * (kLiHinge + 1));
+  li_fit_pass(keys, leaf_a[j], leaf_a[j + 1], kLiFitEps, pool + leaf[j].k0);
 }
 
 // largest prediction error per leaf over its distinct keys (the same
@@ -169,7 +150,7 @@ __global__ void __launch_bounds__(256) k_li_err(DevIndex x, LiLeaf *leaf, int64_
     const uint64_t K = x.keys[r];
     if (r > 0 && x.keys[r - 1] == K) continue;
     const int j = (int)li_route(x, K);
-    const int64_t p = li_predict(x.li_leaf[j], K);
+    const int64_t p = li_predict(x.li_knot, x.li_leaf[j], K);
     const int64_t d = p > r ? p - r : r - p;
     const uint32_t e = d < 0x7fffffff ? (uint32_t)d : 0x7fffffffu;
     if (j != cur) {
@@ -190,12 +171,11 @@ inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
 
-int learned_build(DevIndex x, const LearnedBufs &b, hipStream_t s, LearnedBuildStats &st) {
+int learned_build_begin(DevIndex x, const LearnedBufs &b, hipStream_t s, int64_t *total_knots) {
   const int64_t N = x.n + 1;
   if (N >= ((int64_t)1 << 32) || !x.occn) return 1;
   unsigned long long *ctr = b.ctr;
   if (hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), s) != hipSuccess) return 2;
-  if (hipMemsetAsync(b.mom, 0, sizeof(double) * 5 * kLiLeaves * (kLiHinge + 1), s) != hipSuccess) return 2;
   hipLaunchKernelGGL(k_li_fill, dim3(blocks(N, 256)), dim3(256), 0, s, b.saf, N, 0xffffffffu);
   hipLaunchKernelGGL(k_li_fill, dim3(1), dim3(256), 0, s, b.shortt, (int64_t)(2 * kLiShort), 0xffffffffu);
   const int64_t n_samples = b.n_samples;
@@ -207,15 +187,31 @@ int learned_build(DevIndex x, const LearnedBufs &b, hipStream_t s, LearnedBuildS
   x.saf = b.saf;
   x.keys = b.keys;
   x.li_root = b.root;
-  x.li_leaf = b.leaf;
-  x.li_short = b.shortt;
   x.li_leaf_w = (uint32_t)((N + kLiLeaves - 1) / kLiLeaves);
   hipLaunchKernelGGL(k_li_bounds, dim3(blocks(N + 1, 256)), dim3(256), 0, s, x, N, b.leaf_a);
-  hipLaunchKernelGGL(k_li_leaf_init, dim3(blocks(kLiLeaves, 256)), dim3(256), 0, s, (const uint64_t *)b.keys,
+  hipLaunchKernelGGL(k_li_count, dim3(blocks(kLiLeaves, 64)), dim3(64), 0, s, (const uint64_t *)b.keys,
                      (const uint32_t *)b.leaf_a, b.leaf);
-  hipLaunchKernelGGL(k_li_moments, dim3(blocks(blocks(N, kChunk), 256)), dim3(256), 0, s, x,
-                     (const uint32_t *)b.leaf_a, b.mom, N);
-  hipLaunchKernelGGL(k_li_solve, dim3(blocks(kLiLeaves, 64)), dim3(64), 0, s, b.leaf, (const double *)b.mom);
+  hipLaunchKernelGGL(k_li_offsets, dim3(1), dim3(64), 0, s, b.leaf, ctr + 3);
+  if (hipGetLastError() != hipSuccess) return 2;
+  unsigned long long t = 0;
+  if (hipMemcpyAsync(&t, ctr + 3, sizeof t, hipMemcpyDeviceToHost, s) != hipSuccess) return 2;
+  if (hipStreamSynchronize(s) != hipSuccess) return 2;
+  *total_knots = (int64_t)t;
+  return 0;
+}
+
+int learned_build_end(DevIndex x, const LearnedBufs &b, hipStream_t s, LearnedBuildStats &st) {
+  const int64_t N = x.n + 1;
+  unsigned long long *ctr = b.ctr;
+  x.saf = b.saf;
+  x.keys = b.keys;
+  x.li_root = b.root;
+  x.li_leaf = b.leaf;
+  x.li_knot = b.knot;
+  x.li_short = b.shortt;
+  x.li_leaf_w = (uint32_t)((N + kLiLeaves - 1) / kLiLeaves);
+  hipLaunchKernelGGL(k_li_fit, dim3(blocks(kLiLeaves, 64)), dim3(64), 0, s, (const uint64_t *)b.keys,
+                     (const uint32_t *)b.leaf_a, (const LiLeaf *)b.leaf, b.knot);
   hipLaunchKernelGGL(k_li_err, dim3(blocks(blocks(N, kChunk), 256)), dim3(256), 0, s, x, b.leaf, N, ctr);
   if (hipGetLastError() != hipSuccess) return 2;
   unsigned long long c[8];
@@ -227,15 +223,18 @@ int learned_build(DevIndex x, const LearnedBufs &b, hipStream_t s, LearnedBuildS
   st.unfilled_rows = (int64_t)c[0];
   st.order_violations = (int64_t)c[1];
   st.distinct_keys = (int64_t)c[2];
+  st.knots = (int64_t)c[3];
   st.err_sum = (int64_t)c[4];
   st.keys_over_window = (int64_t)c[5];
   st.short_rows = (int64_t)c[6];
   st.bad_walks = (int64_t)c[7];
   st.max_err = 0;
   st.leaves_over_window = 0;
+  st.max_knots = 0;
   for (const LiLeaf &L : leaves) {
     st.max_err = L.err > (uint32_t)st.max_err ? (int64_t)L.err : st.max_err;
     st.leaves_over_window += L.err + 2 > (uint32_t)kLiWindowCap ? 1 : 0;
+    st.max_knots = (int32_t)L.nk > st.max_knots ? (int32_t)L.nk : st.max_knots;
   }
   st.leaf_w = x.li_leaf_w;
   const int64_t want_short = N < kLiShort ? N : kLiShort;

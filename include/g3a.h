@@ -208,27 +208,30 @@ int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *bes
 /* Learned-index form.  g3a_set_learned(ctx, 1, info) builds, on the device
  * and once per context, the full suffix array, the 32-base key of every
  * suffix-array row and a two-stage FP32 model from a key to its row
- * (piecewise-linear root over the first 6 bases, 4096 leaves of one hidden
- * layer of ReLU units), and switches seeding to it: a read's match that
- * occurs once is extended by comparing it with the text; the last round's
- * intervals are read from the sorted keys by an exact search inside the
- * 12-mer's interval (the k-mer table), narrowed in large buckets to the
- * model's window of at most +-256 rows around its prediction (checked at its
- * edges; a miss searches the whole bucket), with the FM-index as the
- * fallback where no bucket is known.  Output is identical to the FM-index-only path.  0 switches back
- * (the built model is kept); 2 uses the full suffix array (text comparison,
- * SA lookup) without the model's lookups, to measure what they add.  Texts
- * of fewer than 2^32 rows.  No reference counterpart. */
+ * (piecewise-linear root over the first 6 bases; 4096 leaves, each a
+ * one-hidden-layer ReLU network of the key trained by an error-bounded fit
+ * to within 250 rows of every distinct key), and switches seeding to it: a
+ * read's match that occurs once is extended by comparing it with the text;
+ * the last round's intervals are read from the sorted keys by an exact
+ * search inside the 12-mer's interval (the k-mer table), narrowed in large
+ * buckets to the model's window of at most +-256 rows around its prediction
+ * (checked at its edges; a miss searches the whole bucket), with the
+ * FM-index as the fallback where no bucket is known.  Output is identical to
+ * the FM-index-only path.  0 switches back (the built model is kept); 2 uses
+ * the full suffix array (text comparison, SA lookup) without the model's
+ * lookups, to measure what they add.  Texts of fewer than 2^32 rows.  No
+ * reference counterpart. */
 typedef struct g3a_learned_info {
   int64_t rows, distinct_keys;
-  int32_t leaves, hidden_units, window_cap, pad;
-  int64_t model_bytes; /* root + leaves + short-suffix table */
+  int32_t leaves, hidden_units, window_cap, pad; /* hidden_units: the most in one leaf */
+  int64_t model_bytes; /* root + leaves + knots + short-suffix table */
   int64_t sa_bytes, key_bytes;
   int64_t max_err;            /* largest |predicted - true row| over the distinct keys */
   int64_t keys_over_window;   /* distinct keys whose error exceeds the window */
   int64_t leaves_over_window; /* leaves whose largest error exceeds it */
   double mean_err;
   float build_ms, pad2;
+  int64_t knots; /* knots of all leaves (hidden units + 1 per leaf) */
 } g3a_learned_info;
 int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info);
 /* Learned-form work of the last g3a_bwa_align: [0] interval lookups,

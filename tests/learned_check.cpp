@@ -2,8 +2,8 @@
 // (g3a_dev.h, g3a_learned_fit.h compiled for the CPU):
 //   - full SA from the sample walks against the LF-walk SA of every row;
 //   - every row's key against the text read symbol by symbol; keys sorted;
-//   - the model trained as the build kernels train it (moments summed in
-//     row order instead of by atomics);
+//   - the model trained as the build kernel trains it (li_fit_leaf per
+//     leaf), and its largest error over the distinct keys;
 //   - li_lower_bound on every distinct key and on random keys against
 //     std::lower_bound; li_range on text and random j-mers against the
 //     FM-index's interval from forward extensions;
@@ -38,6 +38,7 @@ struct Model {
   std::vector<uint32_t> saf, root, leaf_a, shortt;
   std::vector<uint64_t> keys;
   std::vector<LiLeaf> leaf;
+  std::vector<LiKnot> knot;
 };
 
 // FM interval of the j-symbol string W by forward extensions from its
@@ -155,30 +156,30 @@ int main(int argc, char **argv) {
   for (int64_t r = 1; r < N; ++r)
     if (li_route(x, M.keys[r]) < li_route(x, M.keys[r - 1])) return fail("route", r, 0, 0);
   M.leaf.resize(kLiLeaves);
-  for (int j = 0; j < kLiLeaves; ++j) li_leaf_init(M.keys.data(), M.leaf_a[j], M.leaf_a[j + 1], M.leaf[j]);
-  x.li_leaf = M.leaf.data();
-  std::vector<double> mom((size_t)5 * kLiLeaves * (kLiHinge + 1), 0.0);
-  int64_t distinct = 0;
-  for (int64_t r = 0; r < N; ++r) {
-    if (r && M.keys[r - 1] == M.keys[r]) continue;
-    ++distinct;
-    const uint32_t j = li_route(x, M.keys[r]);
-    const LiLeaf &L = M.leaf[j];
-    const int64_t a = L.a, c = (int64_t)M.leaf_a[j + 1] - a;
-    double *m = mom.data() + 5 * ((size_t)j * (kLiHinge + 1) + li_segment(a, c, r));
-    const double v = (double)li_leaf_x(L, M.keys[r]), y = (double)(r - a);
-    m[0] += 1;
-    m[1] += v;
-    m[2] += v * v;
-    m[3] += y;
-    m[4] += v * y;
+  uint32_t n_knots = 0, max_knots = 0;
+  for (int j = 0; j < kLiLeaves; ++j) {
+    LiLeaf &L = M.leaf[j];
+    L.a = M.leaf_a[j];
+    L.err = 0;
+    L.k0 = n_knots;
+    L.nk = li_fit_pass(M.keys.data(), M.leaf_a[j], M.leaf_a[j + 1], kLiFitEps, nullptr);
+    n_knots += L.nk;
+    max_knots = std::max(max_knots, L.nk);
   }
-  for (int j = 0; j < kLiLeaves; ++j) li_solve_leaf(M.leaf[j], mom.data() + 5 * (size_t)j * (kLiHinge + 1));
+  M.knot.resize(n_knots + 1);
+  for (int j = 0; j < kLiLeaves; ++j)
+    if (li_fit_pass(M.keys.data(), M.leaf_a[j], M.leaf_a[j + 1], kLiFitEps, M.knot.data() + M.leaf[j].k0) !=
+        M.leaf[j].nk)
+      return fail("knot count", j, 0, 0);
+  x.li_leaf = M.leaf.data();
+  x.li_knot = M.knot.data();
+  int64_t distinct = 0;
+  for (int64_t r = 0; r < N; ++r) distinct += !(r && M.keys[r - 1] == M.keys[r]);
   int64_t esum = 0, emax = 0, over = 0;
   for (int64_t r = 0; r < N; ++r) {
     if (r && M.keys[r - 1] == M.keys[r]) continue;
     const uint32_t j = li_route(x, M.keys[r]);
-    const int64_t e = std::llabs(li_predict(M.leaf[j], M.keys[r]) - r);
+    const int64_t e = std::llabs(li_predict(x.li_knot, M.leaf[j], M.keys[r]) - r);
     M.leaf[j].err = std::max<uint32_t>(M.leaf[j].err, (uint32_t)std::min<int64_t>(e, 0x7fffffff));
     esum += e;
     emax = std::max(emax, e);
@@ -291,22 +292,20 @@ int main(int argc, char **argv) {
 
   // a model with zeroed weights: windows of +-2 around the leaf's first
   // row mostly miss, and every answer given must still be exact
-  for (auto &L : M.leaf) {
-    L.c0 = L.c1 = 0.f;
-    for (int hh = 0; hh < kLiHinge; ++hh) L.w[hh] = 0.f;
-    L.err = 0;
-  }
+  for (auto &K : M.knot) K.y = K.s = 0.f;
+  for (auto &L : M.leaf) L.err = 0;
   if (check_lookups(false, miss_z, ranges_z, rmiss_z)) return 1;
   if (rmiss_long) return fail("zeroed bucketed miss", rmiss_long, 0, 0);
 
   std::printf("ok rows=%lld distinct=%lld max_err=%lld mean_err=%.2f over_window=%lld lb_miss=%lld "
               "range_checked=%lld range_miss=%lld zeroed_lb_miss=%lld zeroed_range_miss=%lld reads=%d "
               "intervals=%lld lookups=%lld lookup_miss=%lld text_runs=%lld run_bases=%lld steps_plain=%lld "
-              "steps_learned=%lld model_narrowed=%lld model_missed=%lld bucketed_range_miss=%lld\n",
+              "steps_learned=%lld model_narrowed=%lld model_missed=%lld bucketed_range_miss=%lld knots=%u "
+              "max_knots=%u\n",
               (long long)N, (long long)distinct, (long long)emax, distinct ? (double)esum / distinct : 0.0,
               (long long)over, (long long)miss_t, (long long)ranges_t, (long long)rmiss_t, (long long)miss_z,
               (long long)rmiss_z, n_reads, (long long)n_iv, (long long)lc[0], (long long)lc[1], (long long)lc[2],
               (long long)lc[3], (long long)steps_p, (long long)steps_l, (long long)lc[4], (long long)lc[5],
-              (long long)rmiss_b);
+              (long long)rmiss_b, n_knots, max_knots);
   return 0;
 }

@@ -129,3 +129,27 @@ def test_learned_seed_lengths(ds_rep, oracle, min_seed_len):
         compare(got, ref, f"min_seed_len {min_seed_len} learned")
         _intervals_equal(iv_fm, iv, f"min_seed_len {min_seed_len}")
         assert (st["lookups"] > 0) == (min_seed_len + 1 <= 32)
+
+
+@pytest.mark.parametrize("which", ["c1", "rep"])
+def test_seeding_intervals_against_oracle(ds_c1, ds_rep, oracle, which):
+    """Stage-level parity of seeding: the GPU's intervals (k, s, m, n) of
+    every read, in each read's order, against the oracle's seeding rounds
+    (seed.cu restated), for the FM-index-only and the learned form."""
+    ds = ds_c1 if which == "c1" else ds_rep
+    names, seq, off, _ = g3sa_amd.read_fastq(str(ds.reads["main"]), limit=3000)
+    n = len(off) - 1
+    idx = oracle.load(ds.prefix)
+    try:
+        ref = [oracle.seed_intervals(idx, seq[off[i]:off[i + 1]]) for i in range(n)]
+    finally:
+        oracle.lib.orc_index_free(idx)
+    counts_ref = np.array([len(r) for r in ref], dtype=np.int32)
+    flat_ref = np.array([(k, s, m, e) for r in ref for (k, _l, s, m, e) in r], dtype=np.int64).reshape(-1, 4)
+    with g3sa_amd.Aligner(str(ds.prefix), max_reads=n, max_bases=int(off[-1])) as al:
+        for mode in (0, 1):
+            al.set_learned(mode)
+            print(f"[seeding] {which}: {n} reads, learned {mode}", flush=True)
+            al.align(seq[:off[n]], off)
+            counts, kmsn = al.intervals()
+            _intervals_equal((counts_ref, flat_ref), (counts, kmsn), f"{which} oracle vs GPU (learned {mode})")
