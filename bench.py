@@ -274,45 +274,17 @@ def pcie_inclusive(al, batch, steps: int) -> dict:
 
 
 def gather_ceiling() -> dict | None:
-    """Random-access ceiling for seeding, measured in a child process (torch
-    carries its own HIP runtime, which must not share a process with the
-    library's)."""
-    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gather-ceiling"], capture_output=True, text=True,
-                       timeout=300)
+    """Random-access ceiling for seeding: g3a-gather reads random 16-byte and
+    64-byte rows of a 1 GiB table (the narrow Occ lines of the 1 Gbp index
+    are 16-byte rows of a 1 GB table) and reports useful bytes per second."""
+    exe = g3sa_amd.BUILD / "g3a-gather"
+    if not exe.exists():
+        return None
+    r = subprocess.run([str(exe), "1024"], capture_output=True, text=True, timeout=300)
     if r.returncode != 0:
-        log(f"gather ceiling failed: {r.stderr[-300:]}")
+        log(f"g3a-gather failed: {r.stderr[-300:]}")
         return None
     return json.loads(r.stdout.strip().splitlines()[-1])
-
-
-def _gather_ceiling_torch() -> dict:
-    """torch gathers of random 16-byte and 64-byte rows of a 1 GiB table (the
-    Occ lines of a 1 Gbp index are 16-byte rows of a 1 GB table), useful
-    bytes per second."""
-    import torch
-    out = {}
-    g = torch.Generator(device="cuda")
-    g.manual_seed(5)
-    for row in (16, 64):
-        words = row // 4
-        rows = (1 << 30) // row
-        table = torch.zeros(rows, words, dtype=torch.int32, device="cuda")
-        idx = torch.randint(0, rows, (1 << 26,), device="cuda", generator=g)
-        for _ in range(2):
-            torch.index_select(table, 0, idx)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(5):
-            torch.index_select(table, 0, idx)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / 5
-        out[f"{row}B_rows_GBps"] = round(len(idx) * row / (ms * 1e-3) / 1e9, 1)
-        del table, idx
-    torch.cuda.empty_cache()
-    out["what"] = "torch.index_select of 2^26 random rows from a 1 GiB table; useful bytes per second"
-    return out
 
 
 def seed_traffic(config: str, form: str) -> int | None:
@@ -374,12 +346,7 @@ def main() -> int:
                     help="learned form: 1 = full SA and model lookups, 2 = full SA only")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (torch) to calibrate HBM byte counters")
-    ap.add_argument("--gather-ceiling", action="store_true",
-                    help="only measure the random-gather ceiling (torch) and print it as JSON")
     args = ap.parse_args()
-    if args.gather_ceiling:
-        print(json.dumps(_gather_ceiling_torch()))
-        return 0
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4

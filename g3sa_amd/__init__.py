@@ -80,7 +80,7 @@ EXPORTS = [
     "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
     "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed", "g3a_build_index_dev", "g3a_memcpy_text",
-    "g3a_format_records", "g3a_copy_text", "g3a_bwa_align_noalloc",
+    "g3a_format_records", "g3a_copy_text", "g3a_bwa_align_noalloc", "g3a_set_wide_layout",
 ]
 
 _lib = None
@@ -120,6 +120,7 @@ def load_library() -> C.CDLL:
     lib.g3a_autotune_seed.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_seed_order.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_set_ext_packed.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_set_wide_layout.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_ext_packed.argtypes = [C.c_void_p]
     lib.g3a_set_learned.argtypes = [C.c_void_p, C.c_int, C.POINTER(LearnedInfo)]
     lib.g3a_learned_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
@@ -180,7 +181,8 @@ class Aligner:
     """A device context: index in HBM, batches aligned through the C ABI."""
 
     def __init__(self, prefix: str, device: int = 0, max_reads: int = 1 << 20, max_bases: int | None = None,
-                 opt: Opt | None = None, form: str = "batched", tuning: tuple[int, int, int] | None = None):
+                 opt: Opt | None = None, form: str = "batched", tuning: tuple[int, int, int] | None = None,
+                 wide_layout: bool = False):
         self.lib = load_library()
         self.ctx = C.c_void_p()
         if max_bases is None:
@@ -189,6 +191,8 @@ class Aligner:
                                        max_reads, max_bases, C.byref(self.ctx))
         if rc != G3A_OK:
             raise RuntimeError(f"g3a_device_alloc failed ({rc})")
+        if wide_layout:
+            self._check(self.lib.g3a_set_wide_layout(self.ctx, 1))
         self._check(self.lib.g3a_memcpy_index(self.ctx))
         self.set_form(form)
         if tuning is not None:

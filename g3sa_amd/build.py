@@ -4,6 +4,7 @@ Outputs, all under g3sa_amd/_build/:
   libg3a.so   HIP kernels + C ABI (include/g3a.h) + index builder/loader
   g3a         command-line driver (`g3a index`, `g3a mem`)
   g3a-gen     synthetic reference/read generator
+  g3a-gather  random-gather ceiling of the GPU (for bench.py's roofline)
 The CPU oracle (test infrastructure) is built by oracle/Makefile into
 oracle/_build/.
 """
@@ -57,6 +58,8 @@ def build(verbose: bool = False) -> Path:
         obj = OUT / (Path(f).stem + ".o")
         jobs.append(["g++", *CXX_FLAGS, "-I", "/opt/rocm/include", "-c", str(SRC / f), "-o", str(obj)])
     jobs.append(["g++", "-O3", "-std=c++17", "-o", str(OUT / "g3a-gen"), str(SRC / "gen.cpp")])
+    # the random-gather ceiling that bench.py reports beside the seeding roofline
+    jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-o", str(OUT / "g3a-gather"), str(SRC / "g3a_gather.cpp")])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for fut in [ex.submit(_run, j) for j in jobs]:
             fut.result()

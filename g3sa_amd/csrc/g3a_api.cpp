@@ -85,7 +85,7 @@ struct g3a_ctx {
   // batch text (names, qualities) and formatted records
   bool has_text = false;
   int text_n = -1;
-  DevBuf names, name_off, quals, qual_off, txt_len, txt_off, txt;
+  DevBuf names, name_off, quals, qual_off, txt_len, txt_off, txt, txt_nm, txt_slots;
   int64_t txt_bytes = -1;
   DevBuf mapq, md_len, md_off, md;  // per alignment, for g3a_memcpy_output
   // batch input
@@ -95,6 +95,7 @@ struct g3a_ctx {
   int seed_ordered = 0;  // seed reads in minimizer order (1) or input order (0)
   int seed_threads = 0, tb_threads = 0;  // resident maxima (the pools are sized for them)
   int seed_blocks_per_cu = 0;             // 0: the occupancy maximum
+  int force_wide = 0;  // the layout of texts of 2^32 rows or more, on any text
   // per-batch workspace: `own`, or `scratch_work` during and after an
   // allocation-free call (its views stay valid for g3a_memcpy_output)
   Work own, scratch_work;
@@ -450,7 +451,7 @@ int g3a_memcpy_index(g3a_ctx *ctx) {
   DevIndex &x = ctx->idx;
   const uint64_t rows = h.seq_len + 1;
   if (rows > (4ULL << 32)) return fail(ctx, G3A_E_INDEX, "index larger than 2^34 rows");
-  const bool narrow = rows < (1ULL << 32);
+  const bool narrow = rows < (1ULL << 32) && !ctx->force_wide;
   // Occ lines, built on the device from the BWA .bwt body (k_occ_lines)
   {
     DevBuf words;
@@ -951,15 +952,20 @@ int g3a_format_records(g3a_ctx *ctx, int mode, int64_t *bytes) {
   hipStream_t s = ctx->stream;
   const int n = ctx->n_reads;
   if (ensure(ctx, ctx->txt_len, (size_t)n * 8 + 8) || ensure(ctx, ctx->txt_off, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->fmt_scan_tmp, scan_tmp_bytes(n + 1)))
+      ensure(ctx, ctx->fmt_scan_tmp, scan_tmp_bytes(n + 1)) ||
+      ensure(ctx, ctx->txt_nm, (size_t)ctx->total_alns * 4 + 4) ||
+      ensure(ctx, ctx->txt_slots, (size_t)(ctx->total_alns + n + 1) * sizeof(TextSlot)))
     return G3A_E_HIP;
-  const TextArgs t = text_args(ctx, mode);
+  TextArgs t = text_args(ctx, mode);
+  t.nm = P<int32_t>(ctx->txt_nm);
+  t.slots = P<TextSlot>(ctx->txt_slots);
   launch_text_len(t, P<int64_t>(ctx->txt_len), s);
   scan_i64(P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_off), n, ctx->fmt_scan_tmp.p, ctx->fmt_scan_tmp.bytes, s);
   if (readback(ctx, {P<int64_t>(ctx->txt_off) + n})) return G3A_E_HIP;
   const int64_t total = ctx->pinned[0];
   if (ensure(ctx, ctx->txt, (size_t)total + 16)) return G3A_E_HIP;
   launch_text_write(t, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), s);
+  launch_text_fill(t, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), ctx->total_alns + n, s);
   HIPCK(hipStreamSynchronize(s));
   HIPCK(hipGetLastError());
   ctx->txt_bytes = total;
@@ -1328,6 +1334,13 @@ int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wav
   return G3A_OK;
 }
 
+int g3a_set_wide_layout(g3a_ctx *ctx, int on) {
+  if (!ctx || (on != 0 && on != 1)) return G3A_E_ARG;
+  if (ctx->index_on_device) return fail(ctx, G3A_E_STATE, "the layout is chosen before g3a_memcpy_index");
+  ctx->force_wide = on;
+  return G3A_OK;
+}
+
 int g3a_set_ext_packed(g3a_ctx *ctx, int packed) {
   if (!ctx || (packed != 0 && packed != 1)) return G3A_E_ARG;
   ctx->ext_packed = packed;
@@ -1378,7 +1391,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->own.lep_pool, &ctx->own.eh_pool, &ctx->own.tb_order, &ctx->own.tb_ctr, &ctx->own.tb_nw, &ctx->li_saf, &ctx->li_keys,
                     &ctx->li_root, &ctx->li_leaf, &ctx->li_knot, &ctx->li_leaf_a, &ctx->li_short, &ctx->li_ctr,
                     &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,
-                    &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->mapq, &ctx->md_len,
+                    &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->txt_nm, &ctx->txt_slots, &ctx->mapq, &ctx->md_len,
                     &ctx->md_off, &ctx->md, &ctx->fmt_scan_tmp};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)

@@ -1003,7 +1003,9 @@ __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
 static inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // 16-byte list entries when every SA row fits 32 bits
-static bool seed_narrow(const DevIndex &x) { return x.n + 1 < (1LL << 32); }
+// the narrow layout is in use (texts below 2^32 rows, unless the context
+// was set to the wide layout): decided by the index, not by the text size
+static bool seed_narrow(const DevIndex &x) { return x.occn != nullptr && x.sbc != nullptr; }
 static size_t seed_lds_bytes(const DevIndex &x) {
   return seed_narrow(x) ? (size_t)((((uint64_t)x.n + 1) >> 21) + 1) * 16 : 0;
 }

@@ -19,8 +19,44 @@ __global__ void __launch_bounds__(256) k_text_len(TextArgs t, int64_t *len) {
 __global__ void __launch_bounds__(256) k_text_write(TextArgs t, const int64_t *txt_off, char *txt) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= t.n_reads) return;
-  WriteSink w{txt + txt_off[q]};
+  WriteSink16 w(txt + txt_off[q]);
   emit_read(w, t, q);
+  w.flush();
+}
+
+// SEQ and QUAL of every record slot: one wavefront per slot, lanes on
+// consecutive bytes (reads of codes and qualities and the stores coalesce)
+__global__ void __launch_bounds__(256) k_text_fill(TextArgs t, const int64_t *txt_off, char *txt, int64_t n_slots) {
+  const int64_t slot = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (slot >= n_slots) return;
+  const TextSlot d = t.slots[slot];
+  if (d.read < 0) return;
+  const int q = d.read;
+  char *dst = txt + txt_off[q] + d.seq_at;
+  const uint8_t *sq = t.seq + t.off[q];
+  const int len = t.off[q + 1] - t.off[q];
+  const char *qual = t.quals + t.qual_off[q];
+  const int64_t ql = t.qual_off[q + 1] - t.qual_off[q];
+  char *dq = dst + len + 1;
+  if (d.rev) {
+    for (int k = lane; k < len; k += 64) {
+      const uint8_t c = sq[len - 1 - k];
+      dst[k] = "TGCAN"[c < 4 ? c : 4];
+    }
+    for (int64_t k = lane; k < ql; k += 64) dq[k] = qual[ql - 1 - k];
+  } else {
+    for (int k = lane; k < len; k += 64) {
+      const uint8_t c = sq[k];
+      dst[k] = "ACGTN"[c < 4 ? c : 4];
+    }
+    for (int64_t k = lane; k < ql; k += 64) dq[k] = qual[k];
+  }
+}
+
+void launch_text_fill(const TextArgs &t, const int64_t *txt_off, char *txt, int64_t n_slots, hipStream_t s) {
+  if (n_slots > 0)
+    hipLaunchKernelGGL(k_text_fill, dim3((unsigned)((n_slots + 3) / 4)), dim3(256), 0, s, t, txt_off, txt, n_slots);
 }
 
 void launch_text_len(const TextArgs &t, int64_t *len, hipStream_t s) {

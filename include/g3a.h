@@ -176,6 +176,12 @@ int g3a_set_tuning(g3a_ctx *ctx, int strip_rows, int lane_blocks_per_cu, int wav
  * Identical results.  g3a_ext_packed returns the current setting.  No
  * reference counterpart. */
 int g3a_set_ext_packed(g3a_ctx *ctx, int packed);
+/* Index layout of texts of 2^32 rows or more (32-byte Occ lines of 64 rows
+ * with 2^32-row superblocks, 64-bit intervals in seeding, no 12-mer table,
+ * no learned form), used on any text when on = 1; called before
+ * g3a_memcpy_index.  Output is identical.  For tests of that layout on texts
+ * small enough to check.  No reference counterpart. */
+int g3a_set_wide_layout(g3a_ctx *ctx, int on);
 int g3a_ext_packed(const g3a_ctx *ctx);
 /* Autotuner of the batched form: on the batch last aligned (its tasks are
  * still on the device), time the extension stage for each lane arithmetic

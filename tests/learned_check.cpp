@@ -56,22 +56,25 @@ struct SeedOut {
   int64_t steps = 0;
 };
 
+// E, P: the list entries and interval arithmetic of the narrow layout
+// (LepNarrow, uint32_t) or of texts of 2^32 rows or more (LepWide, int64_t)
+template <class E = LepNarrow, class P = uint32_t>
 static void seed_read(const DevIndex &x, const Opt &o, const std::vector<uint8_t> &q, SeedOut &out,
                       int64_t *lc) {
   std::vector<Intv> slot(4096);
-  std::vector<LepNarrow> la(400), lf(400), lep(400), sd(400);
+  std::vector<E> la(400), lf(400), lep(400), sd(400);
   IntvSink sink{slot.data(), 4096, 0, 0, o.max_occ};
   const int len = (int)q.size();
-  GlobalBuf<LepNarrow> buf{GlobalList<LepNarrow>{la.data()}, GlobalList<LepNarrow>{lf.data()}, 400, 400};
+  GlobalBuf<E> buf{GlobalList<E>{la.data()}, GlobalList<E>{lf.data()}, 400, 400};
   if (!smem_round1(x, o, q.data(), len, sink, buf, &out.steps, lc)) {
     sink.n = 0;
     smem_round1_per_end(x, o, q.data(), len, sink, &out.steps);
   }
   const int nsm = sink.n;
   for (int k = 0; k < nsm; ++k)
-    reseed_one(x, o, q.data(), len, slot[k], GlobalList<LepNarrow>{lep.data()}, GlobalList<LepNarrow>{sd.data()}, 400,
-               400, sink, &out.steps);
-  reseed_round3<uint32_t>(x, o, q.data(), len, sink, &out.steps, lc);
+    reseed_one(x, o, q.data(), len, slot[k], GlobalList<E>{lep.data()}, GlobalList<E>{sd.data()}, 400, 400, sink,
+               &out.steps);
+  reseed_round3<P>(x, o, q.data(), len, sink, &out.steps, lc);
   out.iv.assign(slot.begin(), slot.begin() + sink.n);
 }
 
@@ -262,6 +265,13 @@ int main(int argc, char **argv) {
   xp.li_root = nullptr;
   xp.li_leaf = nullptr;
   xp.li_short = nullptr;
+  // the layout of texts of 2^32 rows or more (32-byte lines, 64-bit
+  // intervals, no k-mer table), run on this text: the seeding code that
+  // such a text takes on the device, checked here against the narrow path
+  DevIndex xw = xp;
+  xw.occn = nullptr;
+  xw.sbc = nullptr;
+  xw.kmer = nullptr;
   int64_t lc[6] = {0, 0, 0, 0, 0, 0}, n_iv = 0, steps_p = 0, steps_l = 0;
   for (int i = 0; i < n_reads; ++i) {
     const int len = 70 + (int)(rng() % 181);
@@ -277,9 +287,15 @@ int main(int argc, char **argv) {
         if (rng() % 1000 == 0) q[t] = 4;
       }
     }
-    SeedOut a, b;
+    SeedOut a, b, w;
     seed_read(xp, o, q, a, nullptr);
     seed_read(x, o, q, b, lc);
+    seed_read<LepWide, int64_t>(xw, o, q, w, nullptr);
+    if (a.iv.size() != w.iv.size()) return fail("wide seed count", i, a.iv.size(), w.iv.size());
+    for (size_t k = 0; k < a.iv.size(); ++k) {
+      const Intv &u = a.iv[k], &v = w.iv[k];
+      if (u.k != v.k || u.s != v.s || u.m != v.m || u.n != v.n) return fail("wide seed", i, (long long)k, u.m);
+    }
     steps_p += a.steps;
     steps_l += b.steps;
     if (a.iv.size() != b.iv.size()) return fail("seed count", i, a.iv.size(), b.iv.size());

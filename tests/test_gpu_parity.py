@@ -50,7 +50,7 @@ VARIANTS = [("plain", None, 1, 0, 1), ("batched", (16, 8, 8), 1, 10, 1), ("batch
             ("batched", (16, 8, 8), 0, 0, 0)]
 
 
-def run_both(ds, oracle, seq, off, label, max_reads=None):
+def run_both(ds, oracle, seq, off, label, max_reads=None, wide=False, variants=None):
     n = len(off) - 1
     idx = oracle.load(ds.prefix)
     try:
@@ -58,8 +58,8 @@ def run_both(ds, oracle, seq, off, label, max_reads=None):
     finally:
         oracle.lib.orc_index_free(idx)
     with g3sa_amd.Aligner(str(ds.prefix), max_reads=max(max_reads or n, 1),
-                          max_bases=max(int(off[-1]), 1)) as al:
-        for form, tuning, ordered, blocks, packed in VARIANTS:
+                          max_bases=max(int(off[-1]), 1), wide_layout=wide) as al:
+        for form, tuning, ordered, blocks, packed in variants or VARIANTS:
             print(f"[parity] {label}: {n} reads, form {form} {tuning} seed order {ordered} blocks {blocks} "
                   f"packed {packed}", flush=True)
             al.set_form(form)
@@ -270,6 +270,22 @@ def test_slow_seeding_path_parity(built, oracle, tmp_path):
     seq, off = _batch(reads)
     got, _ = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats")
     assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path"
+    got, _ = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats, wide layout", wide=True,
+                      variants=VARIANTS[:2])
+    assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path (wide layout)"
+
+
+@pytest.mark.parametrize("which", ["c1", "c3", "rep"])
+def test_wide_layout_parity(ds_c1, ds_rep, oracle, which):
+    """The layout of texts of 2^32 rows or more (32-byte Occ lines, 64-bit
+    intervals, no 12-mer table) on texts small enough for the oracle: the
+    seeding, SA lookup and the rest of the path that a 3.1 Gbp index takes,
+    against the oracle in the plain and batched forms."""
+    ds = ds_rep if which == "rep" else ds_c1
+    fq = ds.reads["main" if which != "c3" else "c3"]
+    names, seq, off, _ = g3sa_amd.read_fastq(str(fq), limit=4000)
+    got, _ = run_both(ds, oracle, seq, off, f"wide layout {which}", wide=True, variants=VARIANTS[:2])
+    assert got.n_alns > 0
 
 
 def test_wide_band_global_alignment_parity(built, oracle, tmp_path):
