@@ -85,7 +85,7 @@ struct g3a_ctx {
   // batch text (names, qualities) and formatted records
   bool has_text = false;
   int text_n = -1;
-  DevBuf names, name_off, quals, qual_off, txt_len, txt_off, txt, txt_nm, txt_slots;
+  DevBuf names, name_off, quals, qual_off, txt_len, txt_off, txt;
   int64_t txt_bytes = -1;
   DevBuf mapq, md_len, md_off, md;  // per alignment, for g3a_memcpy_output
   // batch input
@@ -952,20 +952,15 @@ int g3a_format_records(g3a_ctx *ctx, int mode, int64_t *bytes) {
   hipStream_t s = ctx->stream;
   const int n = ctx->n_reads;
   if (ensure(ctx, ctx->txt_len, (size_t)n * 8 + 8) || ensure(ctx, ctx->txt_off, (size_t)(n + 1) * 8) ||
-      ensure(ctx, ctx->fmt_scan_tmp, scan_tmp_bytes(n + 1)) ||
-      ensure(ctx, ctx->txt_nm, (size_t)ctx->total_alns * 4 + 4) ||
-      ensure(ctx, ctx->txt_slots, (size_t)(ctx->total_alns + n + 1) * sizeof(TextSlot)))
+      ensure(ctx, ctx->fmt_scan_tmp, scan_tmp_bytes(n + 1)))
     return G3A_E_HIP;
-  TextArgs t = text_args(ctx, mode);
-  t.nm = P<int32_t>(ctx->txt_nm);
-  t.slots = P<TextSlot>(ctx->txt_slots);
-  launch_text_len(t, P<int64_t>(ctx->txt_len), s);
+  const TextArgs t = text_args(ctx, mode);
+  launch_text(t, P<int64_t>(ctx->txt_len), nullptr, nullptr, false, s);
   scan_i64(P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_off), n, ctx->fmt_scan_tmp.p, ctx->fmt_scan_tmp.bytes, s);
   if (readback(ctx, {P<int64_t>(ctx->txt_off) + n})) return G3A_E_HIP;
   const int64_t total = ctx->pinned[0];
   if (ensure(ctx, ctx->txt, (size_t)total + 16)) return G3A_E_HIP;
-  launch_text_write(t, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), s);
-  launch_text_fill(t, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), ctx->total_alns + n, s);
+  launch_text(t, nullptr, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), true, s);
   HIPCK(hipStreamSynchronize(s));
   HIPCK(hipGetLastError());
   ctx->txt_bytes = total;
@@ -1391,7 +1386,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->own.lep_pool, &ctx->own.eh_pool, &ctx->own.tb_order, &ctx->own.tb_ctr, &ctx->own.tb_nw, &ctx->li_saf, &ctx->li_keys,
                     &ctx->li_root, &ctx->li_leaf, &ctx->li_knot, &ctx->li_leaf_a, &ctx->li_short, &ctx->li_ctr,
                     &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,
-                    &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->txt_nm, &ctx->txt_slots, &ctx->mapq, &ctx->md_len,
+                    &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->mapq, &ctx->md_len,
                     &ctx->md_off, &ctx->md, &ctx->fmt_scan_tmp};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)

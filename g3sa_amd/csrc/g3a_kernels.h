@@ -182,9 +182,9 @@ void launch_clamp_codes(uint8_t *seq, int64_t n, hipStream_t s);
 void launch_kmer_table(const DevIndex &x, uint32_t *table, hipStream_t s);
 // record formatting (g3a_text.hip): per-read text lengths, then the text at
 // txt_off; per-alignment MAPQ and MD-string lengths, then the MD strings
-void launch_text_len(const TextArgs &t, int64_t *len, hipStream_t s);
-void launch_text_write(const TextArgs &t, const int64_t *txt_off, char *txt, hipStream_t s);
-void launch_text_fill(const TextArgs &t, const int64_t *txt_off, char *txt, int64_t n_slots, hipStream_t s);
+// the batch's records: each read's text length (write = false: len), or
+// its text at txt + txt_off[q] (write = true)
+void launch_text(const TextArgs &t, int64_t *len, const int64_t *txt_off, char *txt, bool write, hipStream_t s);
 void launch_aln_mapq_md(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, int32_t *mapq, int64_t *md_len,
                         hipStream_t s);
 void launch_aln_md_write(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, const int64_t *md_off,
