@@ -273,6 +273,35 @@ def pcie_inclusive(al, batch, steps: int) -> dict:
             "what": "host codes, names and qualities in, align, SAM records formatted on the device, SAM text out"}
 
 
+def graph_ab(data_dir: Path, batch_reads: int = 262144, steps: int = 10) -> dict:
+    """One batch of the 70-250 bp reads over the 1 Mbp reference (the c3
+    workload), aligned repeatedly with the launches issued one by one and as
+    a captured HIP graph; the two outputs must be identical."""
+    prefix, fq, _ = prepare("c3", data_dir)
+    seq, off = load_reads(fq)[:2]
+    n = min(batch_reads, len(off) - 1)
+    s, o = seq[:int(off[n])], off[:n + 1]
+    out, res = {}, {"workload": CONFIGS["c3"]["workload"], "batch_reads": n, "steps": steps}
+    with g3sa_amd.Aligner(str(prefix), max_reads=n, max_bases=int(o[-1])) as al:
+        al.upload(s, o)
+        for on in (False, True):
+            al.set_graph(on)
+            for _ in range(2):  # warm-up (the first graph run captures)
+                al.run()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                al.run()
+            res[f"{'graph' if on else 'eager'}_ms_per_batch"] = round(1e3 * (time.perf_counter() - t0) / steps, 3)
+            out[on] = al.download()
+        res["graph_info"] = al.graph_info()
+    a, b = out[False], out[True]
+    res["identical"] = bool(np.array_equal(a.aln_off, b.aln_off) and np.array_equal(a.pos, b.pos)
+                            and np.array_equal(a.score, b.score) and np.array_equal(a.flag, b.flag)
+                            and np.array_equal(a.cigar, b.cigar) and np.array_equal(a.mapq, b.mapq) and a.md == b.md)
+    res["speedup"] = round(res["eager_ms_per_batch"] / res["graph_ms_per_batch"], 3)
+    return res
+
+
 def gather_ceiling() -> dict | None:
     """Random-access ceiling for seeding: g3a-gather reads random 16-byte and
     64-byte rows of a 1 GiB table (the narrow Occ lines of the 1 Gbp index
@@ -479,6 +508,8 @@ def main() -> int:
                                                  per_form["batched"]["reads_per_s"], 3)
     al.close()
     del batch, seq, off
+    log("graph on/off: one 256k-read batch of the c3 workload")
+    result["graph_ab"] = graph_ab(Path(args.data))
     if not args.no_e2e:
         log("end-to-end: g3a mem")
         result["end_to_end"] = end_to_end(prefix, fq, main_form == "learned")

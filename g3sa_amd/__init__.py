@@ -77,7 +77,7 @@ class LearnedInfo(C.Structure):
 EXPORTS = [
     "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
-    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
+    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_graph", "g3a_graph_info", "g3a_set_batch_caps", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
     "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed", "g3a_build_index_dev", "g3a_memcpy_text",
     "g3a_format_records", "g3a_copy_text", "g3a_bwa_align_noalloc", "g3a_set_wide_layout",
@@ -113,6 +113,9 @@ def load_library() -> C.CDLL:
     lib.g3a_copy_text.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
     lib.g3a_last_stats.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int64)]
     lib.g3a_set_form.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_graph_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    lib.g3a_set_batch_caps.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     lib.g3a_kernel_ms.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
     lib.g3a_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
@@ -197,6 +200,22 @@ class Aligner:
         self.set_form(form)
         if tuning is not None:
             self.set_tuning(*tuning)
+
+    def set_graph(self, on: bool) -> None:
+        """Run each batch as one HIP graph (captured again when its shape changes)."""
+        self._check(self.lib.g3a_set_graph(self.ctx, int(bool(on))))
+
+    def graph_info(self) -> dict:
+        """Whether the last batch ran as a graph, captures so far, overflow reruns."""
+        v = (C.c_int64 * 3)()
+        self._check(self.lib.g3a_graph_info(self.ctx, v))
+        return {"graph_used": bool(v[0]), "captures": v[1], "overflow_reruns": v[2]}
+
+    def set_batch_caps(self, seeds: float, tasks: float, alns: float, cig_used: float, z_per_base: float,
+                       cig_per_base: float, slow_reads: int) -> None:
+        """Capacities the next batch starts from (later batches raise them)."""
+        v = (C.c_double * 7)(seeds, tasks, alns, cig_used, z_per_base, cig_per_base, slow_reads)
+        self._check(self.lib.g3a_set_batch_caps(self.ctx, v))
 
     def set_form(self, form: str) -> None:
         """'plain' (one wavefront per extension task) or 'batched'."""

@@ -57,9 +57,20 @@ struct Work {
   // alignments
   DevBuf aln_reg, aln_read, aln_rank, z_size, cig_size, z_off, cig_off, z, cigar, alns;
   DevBuf cig_words, cig_coff, cig_out;  // compacted CIGARs for the host copy
-  DevBuf scan_tmp;
+  DevBuf scan_tmp, totals;
   // batched-form extension
-  DevBuf ext_res[2], ext_key, ext_key2, ext_idx, ext_sorted, ext_slow, ext_ctr, ext_sort_tmp, ext_rowbuf;
+  DevBuf ext_res[2], ext_key, ext_sorted, ext_slow, ext_ctr, ext_hist, ext_rowbuf;
+};
+
+// Capacities of one attempt at a batch: the buffers the pipeline writes are
+// sized for these counts before the first kernel runs, and the kernels read
+// the true counts from device memory (clamped to these), so the batch runs
+// without host round trips.  A count that does not fit sets a bit of the
+// overflow flag; the kernels after it stop, and the batch is run again with
+// that capacity grown.
+struct Caps {
+  int64_t seeds = 0, tasks = 0, alns = 0, z = 0, cig = 0, used = 0, pool = 0;
+  int slow = 0;
 };
 
 // A bump allocator over the static scratch array, active during one
@@ -131,6 +142,21 @@ struct g3a_ctx {
   g3a_learned_info li_info{};
   int64_t total_seeds = 0, total_tasks = 0, total_alns = 0, total_cig = 0, n_slow = 0;
   float ms[6] = {0, 0, 0, 0, 0, 0};
+  // batch capacities: per read (seeds, tasks, alignments, CIGAR words used)
+  // and per base (direction bytes, CIGAR words), raised by the batches seen
+  double per_read[4] = {16, 6, 2, 32}, per_base[2] = {4, 4};
+  int slow_cap = 1024;          // slow-path reads per batch before a rerun
+  int64_t pool_per_slow = 256;  // their intervals, per slow read
+  int64_t n_bases = 0;          // bases of the resident batch
+  Caps caps{};                  // of the last batch
+  uint64_t buf_gen = 0;         // bumped by every allocation (graphs hold addresses)
+  int64_t overflow_reruns = 0, graph_captures = 0;
+  // one graph per batch shape (batched or plain form, input seeding order)
+  int use_graph = 0;
+  bool graph_used = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::string gkey;
 };
 
 namespace {
@@ -171,6 +197,7 @@ int ensure(g3a_ctx *ctx, DevBuf &b, size_t bytes) {
   size_t want = bytes + bytes / 4;  // headroom so repeated batches do not reallocate
   HIPCK(hipMalloc(&b.p, want));
   b.bytes = want;
+  ++ctx->buf_gen;
   return 0;
 }
 
@@ -560,6 +587,7 @@ int g3a_memcpy_input(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_
   launch_clamp_codes(P<uint8_t>(ctx->seq), nb, ctx->stream);
   HIPCK(hipStreamSynchronize(ctx->stream));
   ctx->n_reads = n_reads;
+  ctx->n_bases = nb;
   ctx->has_input = true;
   ctx->has_output = false;
   ctx->has_text = false;
@@ -613,264 +641,450 @@ int readback(g3a_ctx *ctx, std::initializer_list<const int64_t *> src) {
 
 }  // namespace
 
-static // The extension stage over the T_t tasks in ctx->w->tasks, into ctx->w->regs.
-int run_extension(g3a_ctx *ctx, int64_t T_t, int64_t *cells) {
+
+enum : int32_t {
+  kOvSlow = 1,    // slow reads over caps.slow
+  kOvPool = 2,    // their intervals over caps.pool
+  kOvSeeds = 4,   // seeds
+  kOvTasks = 8,   // extension tasks
+  kOvAlns = 16,   // alignments
+  kOvZ = 32,      // direction bytes
+  kOvCig = 64,    // CIGAR words (traceback slots)
+  kOvUsed = 128,  // CIGAR words used
+};
+
+// counters (int64 slots of w->counters): 0 FM steps, 8 slow-read count
+// (int32), 16 chain error (int32), 24 extension cells, 32..37 learned-form
+// counters, 40 overflow flag (int32)
+constexpr int kFlagSlot32 = 80;  // the flag, as an int32 index
+
+static int64_t scale(double per, int64_t n, int64_t slack) { return (int64_t)(per * (double)n) + slack; }
+
+static Caps plan_caps(const g3a_ctx *ctx, int n) {
+  Caps c;
+  c.seeds = scale(ctx->per_read[0], n, 4096);
+  c.tasks = scale(ctx->per_read[1], n, 1024);
+  c.alns = scale(ctx->per_read[2], n, 1024);
+  c.used = scale(ctx->per_read[3], n, 1024);
+  c.z = scale(ctx->per_base[0], ctx->n_bases, 65536);
+  c.cig = scale(ctx->per_base[1], ctx->n_bases, 4 * (int64_t)n + 1024);
+  const int slow_cap = ctx->arena ? 64 : ctx->slow_cap;  // the allocation-free path: few slow-path launches
+  c.slow = n < slow_cap ? n : slow_cap;
+  c.pool = (int64_t)c.slow * ctx->pool_per_slow + 1024;
+  return c;
+}
+
+static void free_graph(g3a_ctx *ctx) {
+  if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+  ctx->gkey.clear();
+}
+
+namespace {
+
+// Buffers of the batched form's extension stage for `cap` tasks.
+int alloc_ext(g3a_ctx *ctx, int64_t cap) {
+  Work &w = *ctx->w;
+  const int lane_blocks = ctx->arena ? ctx->run_lane_blocks : ctx->n_cu * ctx->lane_blocks_per_cu;
+  return ensure(ctx, w.ext_res[0], (size_t)cap * sizeof(ExtRes)) ||
+         ensure(ctx, w.ext_res[1], (size_t)cap * sizeof(ExtRes)) || ensure(ctx, w.ext_key, (size_t)cap * 4) ||
+         ensure(ctx, w.ext_sorted, (size_t)cap * 4) || ensure(ctx, w.ext_slow, (size_t)cap * 4) ||
+         ensure(ctx, w.ext_ctr, 2 * 4 * sizeof(int32_t)) || ensure(ctx, w.ext_hist, 1024 * 4) ||
+         ensure(ctx, w.ext_rowbuf, (size_t)lane_blocks * kExtQCap * 64 * 4);
+}
+
+// Buffers of one attempt (allocation, if any, happens here and nowhere in
+// enqueue_batch, so that the launches can be captured).
+int alloc_batch(g3a_ctx *ctx, int n, const Caps &c) {
+  Work &w = *ctx->w;
+  if (alloc_reads(ctx, n)) return alloc_fail(ctx);
+  const size_t fast_entries = (size_t)n * kSlotCap;
+  const int chunk = ctx->run_slow_chunk < c.slow ? ctx->run_slow_chunk : c.slow;
+  const int64_t scan_n = std::max<int64_t>({(int64_t)n, c.alns, (int64_t)c.slow}) + 1;
+  if (ensure(ctx, w.intv, (fast_entries + (size_t)c.pool + 1024) * sizeof(Intv)) ||
+      ensure(ctx, w.slow_scratch, (size_t)(chunk > 0 ? chunk : 1) * kSlowScratch * sizeof(Intv)) ||
+      ensure(ctx, w.slow_off, (size_t)(c.slow + 1) * 8) || ensure(ctx, w.slow_n, (size_t)c.slow * 8 + 8) ||
+      ensure(ctx, w.scan_tmp, scan_tmp_bytes(scan_n)) || ensure(ctx, w.totals, 16 * 8) ||
+      alloc_seeds(ctx, c.seeds, n) || alloc_tasks(ctx, c.tasks) || alloc_alns(ctx, c.alns) ||
+      alloc_paths(ctx, c.z, c.cig) || ensure(ctx, w.cig_words, (size_t)c.alns * 4 + 4) ||
+      ensure(ctx, w.cig_coff, (size_t)(c.alns + 1) * 8) || ensure(ctx, w.cig_out, (size_t)c.used * 4 + 64))
+    return alloc_fail(ctx);
+  if (ctx->form != G3A_FORM_PLAIN && alloc_ext(ctx, c.tasks)) return alloc_fail(ctx);
+  return 0;
+}
+
+}  // namespace
+
+// The extension stage over the tasks in ctx->w->tasks (capacity cap, count
+// at n_dev on the device), into ctx->w->regs.  Its buffers are allocated
+// (alloc_batch) before it is called.
+static int run_extension(g3a_ctx *ctx, int64_t cap, const int64_t *n_dev, int64_t *cells, bool timed) {
   hipStream_t s = ctx->stream;
+  const int32_t *stop = P<int32_t>(ctx->w->counters) + kFlagSlot32;
   if (ctx->form == G3A_FORM_PLAIN) {
     ExtArgs ea{};
+    ea.stop = stop;
     ea.idx = ctx->idx;
     ea.opt = ctx->opt;
-    ea.n_tasks = T_t;
+    ea.n_tasks = cap;
+    ea.n_dev = n_dev;
     ea.seq = P<uint8_t>(ctx->seq);
     ea.off = P<int32_t>(ctx->off);
     ea.tasks = P<Task>(ctx->w->tasks);
     ea.regs = P<Region>(ctx->w->regs);
     ea.cells = cells;
-    HIPCK(hipEventRecord(ctx->kev[3], s));
+    if (timed) HIPCK(hipEventRecord(ctx->kev[3], s));
     launch_extend(ea, s);
-    HIPCK(hipEventRecord(ctx->kev[7], s));
-    ctx->kev_used[3] = 1;
-  } else if (T_t > 0) {
+    if (timed) HIPCK(hipEventRecord(ctx->kev[7], s));
+    ctx->kev_used[3] = timed;
+  } else if (cap > 0) {
     const int lane_blocks = ctx->arena ? ctx->run_lane_blocks : ctx->n_cu * ctx->lane_blocks_per_cu,
               wave_blocks = ctx->n_cu * ctx->wave_blocks_per_cu;
-    if (ensure(ctx, ctx->w->ext_res[0], (size_t)T_t * sizeof(ExtRes)) ||
-        ensure(ctx, ctx->w->ext_res[1], (size_t)T_t * sizeof(ExtRes)) || ensure(ctx, ctx->w->ext_key, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->w->ext_key2, (size_t)T_t * 4) || ensure(ctx, ctx->w->ext_idx, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->w->ext_sorted, (size_t)T_t * 4) || ensure(ctx, ctx->w->ext_slow, (size_t)T_t * 4) ||
-        ensure(ctx, ctx->w->ext_ctr, 2 * 4 * sizeof(int32_t)) ||
-        ensure(ctx, ctx->w->ext_sort_tmp, ext_sort_tmp_bytes(T_t)) ||
-        ensure(ctx, ctx->w->ext_rowbuf, (size_t)lane_blocks * kExtQCap * 64 * 4))
-      return alloc_fail(ctx);
     launch_fill_i32(P<int32_t>(ctx->w->ext_ctr), 8, 0, s);
     for (int side = 0; side < 2; ++side) {
       ExtSideArgs ea{};
+      ea.stop = stop;
       ea.idx = ctx->idx;
       ea.opt = ctx->opt;
       ea.side = side;
       ea.lane_opt_ok = ext_lane_opt_ok(ctx->opt);
-      ea.n_tasks = T_t;
+      ea.n_tasks = cap;
+      ea.n_dev = n_dev;
       ea.seq = P<uint8_t>(ctx->seq);
       ea.off = P<int32_t>(ctx->off);
       ea.tasks = P<Task>(ctx->w->tasks);
       ea.left = P<ExtRes>(ctx->w->ext_res[0]);
       ea.res = P<ExtRes>(ctx->w->ext_res[side]);
       ea.key = P<uint32_t>(ctx->w->ext_key);
-      ea.tid = P<int32_t>(ctx->w->ext_idx);
       ea.sorted = P<int32_t>(ctx->w->ext_sorted);
       ea.slow_list = P<int32_t>(ctx->w->ext_slow);
       ea.ctr = P<int32_t>(ctx->w->ext_ctr) + 4 * side;
       ea.rowbuf = P<uint32_t>(ctx->w->ext_rowbuf);
       ea.cells = cells;
       launch_ext_classify(ea, s);
-      if (ext_sort(P<uint32_t>(ctx->w->ext_key), P<uint32_t>(ctx->w->ext_key2), P<int32_t>(ctx->w->ext_idx),
-                   P<int32_t>(ctx->w->ext_sorted), T_t, ctx->w->ext_sort_tmp.p, ctx->w->ext_sort_tmp.bytes, s))
-        return fail(ctx, G3A_E_HIP, "extension task sort failed");
-      HIPCK(hipEventRecord(ctx->kev[2 + 2 * side], s));
+      ext_sort(P<uint32_t>(ctx->w->ext_key), cap, n_dev, P<int32_t>(ctx->w->ext_hist), P<int32_t>(ctx->w->ext_sorted), s);
+      if (timed) HIPCK(hipEventRecord(ctx->kev[2 + 2 * side], s));
       launch_ext_lane(ea, ctx->strip_rows, ctx->ext_packed, lane_blocks, s);
-      HIPCK(hipEventRecord(ctx->kev[3 + 2 * side], s));
+      if (timed) HIPCK(hipEventRecord(ctx->kev[3 + 2 * side], s));
       launch_ext_wave(ea, wave_blocks, s);
-      HIPCK(hipEventRecord(ctx->kev[7 + 2 * side], s));
-      ctx->kev_used[1 + side] = 1;
+      if (timed) HIPCK(hipEventRecord(ctx->kev[7 + 2 * side], s));
+      ctx->kev_used[1 + side] = timed;
     }
-    ExtFinishArgs fa{T_t, P<Task>(ctx->w->tasks), P<ExtRes>(ctx->w->ext_res[0]), P<ExtRes>(ctx->w->ext_res[1]),
-                     P<Region>(ctx->w->regs)};
+    ExtFinishArgs fa{};
+    fa.stop = stop;
+    fa.n_tasks = cap;
+    fa.n_dev = n_dev;
+    fa.tasks = P<Task>(ctx->w->tasks);
+    fa.left = P<ExtRes>(ctx->w->ext_res[0]);
+    fa.right = P<ExtRes>(ctx->w->ext_res[1]);
+    fa.regs = P<Region>(ctx->w->regs);
     launch_ext_finish(fa, s);
   }
   return 0;
 }
 
-// The pipeline on the resident batch, with ctx->w as its workspace.
-static int align_batch(g3a_ctx *ctx) {
+// Slots of the totals read back after a batch (w->totals).
+enum { kTotSlow, kTotPool, kTotSeeds, kTotTasks, kTotAlns, kTotZ, kTotCig, kTotUsed, kTotChainErr, kTotFlags, kTotN };
+
+// Every launch of one attempt at the resident batch, on ctx->stream, with
+// no host synchronization and no allocation: the sequence a graph captures.
+// timed: record the stage and kernel events (not while capturing).
+static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
   hipStream_t s = ctx->stream;
-  const int n = ctx->n_reads;
-  ctx->has_output = false;
-  ctx->total_seeds = ctx->total_tasks = ctx->total_alns = ctx->total_cig = ctx->n_slow = 0;
-  if (alloc_reads(ctx, n)) return alloc_fail(ctx);
-  // fast-path interval slots plus a pool that grows for slow reads
+  Work &w = *ctx->w;
+  int64_t *cnt = P<int64_t>(w.counters);
+  int32_t *flag = P<int32_t>(w.counters) + kFlagSlot32;
+  int32_t *n_slow_dev = P<int32_t>(w.counters) + 2 * 8;
   const size_t fast_entries = (size_t)n * kSlotCap;
-  int64_t *cnt = P<int64_t>(ctx->w->counters);
-  launch_fill_i32(P<int32_t>(ctx->w->counters), 128, 0, s);  // a kernel, not a memset node
-  HIPCK(hipEventRecord(ctx->ev[0], s));
+  launch_fill_i32(P<int32_t>(w.counters), 128, 0, s);  // a kernel, not a memset node
+  if (timed) HIPCK(hipEventRecord(ctx->ev[0], s));
   // ---- seeding
   SeedArgs sa{};
+  sa.stop = flag;
   sa.idx = ctx->idx;
   sa.opt = ctx->opt;
   sa.n_reads = n;
   sa.seq = P<uint8_t>(ctx->seq);
   sa.off = P<int32_t>(ctx->off);
-  sa.intv = P<Intv>(ctx->w->intv);
+  sa.intv = P<Intv>(w.intv);
   sa.slot_cap = kSlotCap;
-  sa.n_intv = P<int32_t>(ctx->w->n_intv);
-  sa.n_seed = P<int64_t>(ctx->w->n_seed);
-  sa.slow = P<int32_t>(ctx->w->slow);
+  sa.n_intv = P<int32_t>(w.n_intv);
+  sa.n_seed = P<int64_t>(w.n_seed);
+  sa.slow = P<int32_t>(w.slow);
   sa.fm_steps = cnt + 0;
-  sa.lep_pool = P<Intv>(ctx->w->lep_pool);
+  sa.lep_pool = P<Intv>(w.lep_pool);
   sa.li_stats = ctx->idx.saf ? cnt + 32 : nullptr;
   sa.order = nullptr;
   if (ctx->seed_ordered && n > 1) {
-    launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(ctx->w->seed_key), P<int32_t>(ctx->w->seed_idx), s);
-    if (seed_sort(P<uint32_t>(ctx->w->seed_key), P<uint32_t>(ctx->w->seed_key2), P<int32_t>(ctx->w->seed_idx),
-                  P<int32_t>(ctx->w->seed_order), n, ctx->w->seed_sort_tmp.p, ctx->w->seed_sort_tmp.bytes, s))
+    launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(w.seed_key), P<int32_t>(w.seed_idx), s);
+    if (seed_sort(P<uint32_t>(w.seed_key), P<uint32_t>(w.seed_key2), P<int32_t>(w.seed_idx),
+                  P<int32_t>(w.seed_order), n, w.seed_sort_tmp.p, w.seed_sort_tmp.bytes, s))
       return fail(ctx, G3A_E_HIP, "seed order sort failed");
-    sa.order = P<int32_t>(ctx->w->seed_order);
+    sa.order = P<int32_t>(w.seed_order);
   }
-  for (int &u : ctx->kev_used) u = 0;
-  HIPCK(hipEventRecord(ctx->kev[0], s));
+  if (timed) HIPCK(hipEventRecord(ctx->kev[0], s));
   const int seed_run = ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * 128 * ctx->n_cu < ctx->run_seed_threads
                            ? ctx->seed_blocks_per_cu * 128 * ctx->n_cu
                            : ctx->run_seed_threads;
   launch_seed(sa, seed_run, ctx->seed_blocks_per_cu, s);
-  HIPCK(hipEventRecord(ctx->kev[1], s));
-  ctx->kev_used[0] = 1;
-  launch_collect_slow(n, P<int32_t>(ctx->w->slow), P<int32_t>(ctx->w->slow_list), P<int32_t>(ctx->w->counters) + 2 * 8,
-                      P<int64_t>(ctx->w->intv_base), kSlotCap, s);
-  if (readback(ctx, {cnt + 8})) return G3A_E_HIP;
-  const int n_slow = (int)(ctx->pinned[0] & 0xffffffff);
-  ctx->n_slow = n_slow;
-  if (n_slow > 0) {
-    // counts, then pool offsets, then the intervals themselves
-    const int chunk = ctx->run_slow_chunk;
-    if (ensure(ctx, ctx->w->slow_scratch, (size_t)chunk * kSlowScratch * sizeof(Intv)) ||
-        ensure(ctx, ctx->w->slow_off, (size_t)(n_slow + 1) * 8) || ensure(ctx, ctx->w->slow_n, (size_t)n_slow * 8 + 8))
-      return alloc_fail(ctx);
-    const int32_t *list = P<int32_t>(ctx->w->slow_list);
-    for (int c0 = 0; c0 < n_slow; c0 += chunk) {
-      int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
-      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->w->slow_scratch), nullptr, nullptr, 0, 0, s);
+  if (timed) HIPCK(hipEventRecord(ctx->kev[1], s));
+  ctx->kev_used[0] = timed;
+  launch_collect_slow(n, P<int32_t>(w.slow), P<int32_t>(w.slow_list), n_slow_dev, P<int64_t>(w.intv_base), kSlotCap, s);
+  // reads whose lists outgrew the fast path: counts (pass 0), pool offsets,
+  // then the intervals (pass 1), all for the device's count of them
+  launch_cap_check32(n_slow_dev, c.slow, flag, kOvSlow, s);
+  {
+    const int chunk = ctx->run_slow_chunk < c.slow ? ctx->run_slow_chunk : c.slow;
+    const int32_t *list = P<int32_t>(w.slow_list);
+    for (int c0 = 0; c0 < c.slow; c0 += chunk) {
+      const int m = c.slow - c0 < chunk ? c.slow - c0 : chunk;
+      launch_seed_slow(sa, list + c0, m, n_slow_dev, c0, P<Intv>(w.slow_scratch), nullptr, nullptr, 0, 0, s);
     }
-    launch_gather_slow_counts(n_slow, list, P<int32_t>(ctx->w->n_intv), P<int64_t>(ctx->w->slow_n), s);
-    scan_i64(P<int64_t>(ctx->w->slow_n), P<int64_t>(ctx->w->slow_off), n_slow, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
-    if (readback(ctx, {P<int64_t>(ctx->w->slow_off) + n_slow})) return G3A_E_HIP;
-    const int64_t pool = ctx->pinned[0];
-    // grow the interval buffer, keeping the fast slots
-    const size_t need = (fast_entries + (size_t)pool + 1024) * sizeof(Intv);
-    if (ctx->w->intv.bytes < need) {
-      DevBuf nb;
-      if (ensure(ctx, nb, need)) return alloc_fail(ctx);
-      HIPCK(hipMemcpyAsync(nb.p, ctx->w->intv.p, fast_entries * sizeof(Intv), hipMemcpyDeviceToDevice, s));
-      HIPCK(hipStreamSynchronize(s));
-      free_buf(ctx->w->intv);
-      ctx->w->intv = nb;
-      sa.intv = P<Intv>(ctx->w->intv);
+    launch_gather_slow_counts(c.slow, n_slow_dev, list, P<int32_t>(w.n_intv), P<int64_t>(w.slow_n), s);
+    scan_i64(P<int64_t>(w.slow_n), P<int64_t>(w.slow_off), c.slow, w.scan_tmp.p, w.scan_tmp.bytes, s);
+    launch_cap_check(P<int64_t>(w.slow_off) + c.slow, c.pool, flag, kOvPool, s);
+    Intv *pool_base = P<Intv>(w.intv) + fast_entries;
+    for (int c0 = 0; c0 < c.slow; c0 += chunk) {
+      const int m = c.slow - c0 < chunk ? c.slow - c0 : chunk;
+      launch_seed_slow(sa, list + c0, m, n_slow_dev, c0, P<Intv>(w.slow_scratch), P<int64_t>(w.slow_off) + c0,
+                       pool_base, 1, c.slow <= chunk, s);
     }
-    Intv *pool_base = P<Intv>(ctx->w->intv) + fast_entries;
-    for (int c0 = 0; c0 < n_slow; c0 += chunk) {
-      int m = n_slow - c0 < chunk ? n_slow - c0 : chunk;
-      launch_seed_slow(sa, list + c0, m, P<Intv>(ctx->w->slow_scratch), P<int64_t>(ctx->w->slow_off) + c0, pool_base, 1,
-                       n_slow <= chunk, s);
-    }
-    launch_set_slow_base(n_slow, list, P<int64_t>(ctx->w->slow_off), (int64_t)fast_entries, P<int64_t>(ctx->w->intv_base), s);
+    launch_set_slow_base(c.slow, n_slow_dev, list, P<int64_t>(w.slow_off), (int64_t)fast_entries,
+                         P<int64_t>(w.intv_base), s);
   }
-  HIPCK(hipEventRecord(ctx->ev[1], s));
+  if (timed) HIPCK(hipEventRecord(ctx->ev[1], s));
   // ---- SA lookup and seed sort
-  scan_i64(P<int64_t>(ctx->w->n_seed), P<int64_t>(ctx->w->seed_off), n, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->w->seed_off) + n})) return G3A_E_HIP;
-  const int64_t T_s = ctx->pinned[0];
-  ctx->total_seeds = T_s;
-  if (alloc_seeds(ctx, T_s, n)) return alloc_fail(ctx);
+  scan_i64(P<int64_t>(w.n_seed), P<int64_t>(w.seed_off), n, w.scan_tmp.p, w.scan_tmp.bytes, s);
+  const int64_t *T_s = P<int64_t>(w.seed_off) + n;
+  launch_cap_check(T_s, c.seeds, flag, kOvSeeds, s);
   SaArgs la{};
+  la.stop = flag;
   la.idx = ctx->idx;
   la.opt = ctx->opt;
   la.n_reads = n;
-  la.total = T_s;
-  la.seed_off = P<int64_t>(ctx->w->seed_off);
-  la.intv = P<Intv>(ctx->w->intv);
-  la.intv_base = P<int64_t>(ctx->w->intv_base);
-  la.n_intv = P<int32_t>(ctx->w->n_intv);
-  la.seeds = P<Seed>(ctx->w->seeds);
+  la.total = c.seeds;
+  la.total_dev = T_s;
+  la.seed_off = P<int64_t>(w.seed_off);
+  la.intv = P<Intv>(w.intv);
+  la.intv_base = P<int64_t>(w.intv_base);
+  la.n_intv = P<int32_t>(w.n_intv);
+  la.seeds = P<Seed>(w.seeds);
   launch_sa_lookup(la, s);
-  SortArgs so{n, P<int64_t>(ctx->w->seed_off), P<Seed>(ctx->w->seeds), P<Seed>(ctx->w->seeds2)};
+  SortArgs so{flag, n, P<int64_t>(w.seed_off), P<Seed>(w.seeds), P<Seed>(w.seeds2)};
   launch_sort(so, s);
-  HIPCK(hipEventRecord(ctx->ev[2], s));
+  if (timed) HIPCK(hipEventRecord(ctx->ev[2], s));
   // ---- chaining
   ChainArgs ca{};
+  ca.stop = flag;
   ca.idx = ctx->idx;
   ca.opt = ctx->opt;
   ca.n_reads = n;
   ca.off = P<int32_t>(ctx->off);
-  ca.seed_off = P<int64_t>(ctx->w->seed_off);
-  ca.seeds = P<Seed>(ctx->w->seeds);
-  ca.ch = P<ChainTmp>(ctx->w->ch);
-  ca.nxt = P<int32_t>(ctx->w->nxt);
-  ca.order = P<int32_t>(ctx->w->order);
-  ca.tmpi = P<int32_t>(ctx->w->tmpi);
-  ca.w16 = P<uint16_t>(ctx->w->w16);
-  ca.kept = P<uint8_t>(ctx->w->kept);
-  ca.nodes = P<KbNode>(ctx->w->nodes);
-  ca.tasks_tmp = P<Task>(ctx->w->tasks_tmp);
-  ca.n_task = P<int32_t>(ctx->w->n_task);
-  ca.error = P<int32_t>(ctx->w->counters) + 2 * 16;
+  ca.seed_off = P<int64_t>(w.seed_off);
+  ca.seeds = P<Seed>(w.seeds);
+  ca.ch = P<ChainTmp>(w.ch);
+  ca.nxt = P<int32_t>(w.nxt);
+  ca.order = P<int32_t>(w.order);
+  ca.tmpi = P<int32_t>(w.tmpi);
+  ca.w16 = P<uint16_t>(w.w16);
+  ca.kept = P<uint8_t>(w.kept);
+  ca.nodes = P<KbNode>(w.nodes);
+  ca.tasks_tmp = P<Task>(w.tasks_tmp);
+  ca.n_task = P<int32_t>(w.n_task);
+  ca.error = P<int32_t>(w.counters) + 2 * 16;
   launch_chain(ca, s);
-  scan_i32(P<int32_t>(ctx->w->n_task), P<int64_t>(ctx->w->task_off), n, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->w->task_off) + n, cnt + 16})) return G3A_E_HIP;
-  const int64_t T_t = ctx->pinned[0];
-  if (ctx->pinned[1] & 0xffffffff) return fail(ctx, G3A_E_INTERNAL, "chain node arena overflow");
-  ctx->total_tasks = T_t;
-  if (alloc_tasks(ctx, T_t)) return alloc_fail(ctx);
-  CompactArgs cp{n, P<int64_t>(ctx->w->seed_off), P<int64_t>(ctx->w->task_off), P<Task>(ctx->w->tasks_tmp), P<Task>(ctx->w->tasks)};
+  scan_i32(P<int32_t>(w.n_task), P<int64_t>(w.task_off), n, w.scan_tmp.p, w.scan_tmp.bytes, s);
+  const int64_t *T_t = P<int64_t>(w.task_off) + n;
+  launch_cap_check(T_t, c.tasks, flag, kOvTasks, s);
+  CompactArgs cp{flag, n, P<int64_t>(w.seed_off), P<int64_t>(w.task_off), P<Task>(w.tasks_tmp), P<Task>(w.tasks)};
   launch_task_compact(cp, s);
-  HIPCK(hipEventRecord(ctx->ev[3], s));
+  if (timed) HIPCK(hipEventRecord(ctx->ev[3], s));
   // ---- extension
-  if (int rc = run_extension(ctx, T_t, cnt + 24)) return rc;
-  HIPCK(hipEventRecord(ctx->ev[4], s));
+  if (int rc = run_extension(ctx, c.tasks, T_t, cnt + 24, timed)) return rc;
+  if (timed) HIPCK(hipEventRecord(ctx->ev[4], s));
   // ---- region filter, traceback, finalize
-  RegionArgs ra{ctx->opt, n, P<int64_t>(ctx->w->task_off), P<Region>(ctx->w->regs), P<uint8_t>(ctx->w->flag),
-                P<int32_t>(ctx->w->sec), P<int32_t>(ctx->w->n_aln)};
+  RegionArgs ra{flag, ctx->opt, n, P<int64_t>(w.task_off), P<Region>(w.regs), P<uint8_t>(w.flag),
+                P<int32_t>(w.sec), P<int32_t>(w.n_aln)};
   launch_region_filter(ra, s);
-  scan_i32(P<int32_t>(ctx->w->n_aln), P<int64_t>(ctx->w->aln_off), n, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->w->aln_off) + n})) return G3A_E_HIP;
-  const int64_t T_a = ctx->pinned[0];
-  ctx->total_alns = T_a;
-  if (alloc_alns(ctx, T_a)) return alloc_fail(ctx);
-  AlnPrepArgs ap{ctx->opt, n, P<int64_t>(ctx->w->task_off), P<int64_t>(ctx->w->aln_off), P<Region>(ctx->w->regs),
-                 P<Region>(ctx->w->aln_reg), P<int32_t>(ctx->w->aln_read), P<int32_t>(ctx->w->aln_rank),
-                 P<int64_t>(ctx->w->z_size), P<int64_t>(ctx->w->cig_size)};
+  scan_i32(P<int32_t>(w.n_aln), P<int64_t>(w.aln_off), n, w.scan_tmp.p, w.scan_tmp.bytes, s);
+  const int64_t *T_a = P<int64_t>(w.aln_off) + n;
+  launch_cap_check(T_a, c.alns, flag, kOvAlns, s);
+  AlnPrepArgs ap{flag, ctx->opt, n, P<int64_t>(w.task_off), P<int64_t>(w.aln_off), P<Region>(w.regs),
+                 P<Region>(w.aln_reg), P<int32_t>(w.aln_read), P<int32_t>(w.aln_rank),
+                 P<int64_t>(w.z_size), P<int64_t>(w.cig_size)};
   launch_aln_prep(ap, s);
-  scan_i64(P<int64_t>(ctx->w->z_size), P<int64_t>(ctx->w->z_off), T_a, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
-  scan_i64(P<int64_t>(ctx->w->cig_size), P<int64_t>(ctx->w->cig_off), T_a, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->w->z_off) + T_a, P<int64_t>(ctx->w->cig_off) + T_a})) return G3A_E_HIP;
-  const int64_t T_z = ctx->pinned[0], T_c = ctx->pinned[1];
-  ctx->total_cig = T_c;
-  if (alloc_paths(ctx, T_z, T_c)) return alloc_fail(ctx);
+  // sizes past the alignments count nothing in the scans over the capacity
+  launch_zero_tail(P<int64_t>(w.z_size), c.alns, T_a, s);
+  launch_zero_tail(P<int64_t>(w.cig_size), c.alns, T_a, s);
+  scan_i64(P<int64_t>(w.z_size), P<int64_t>(w.z_off), c.alns, w.scan_tmp.p, w.scan_tmp.bytes, s);
+  scan_i64(P<int64_t>(w.cig_size), P<int64_t>(w.cig_off), c.alns, w.scan_tmp.p, w.scan_tmp.bytes, s);
+  launch_cap_check(P<int64_t>(w.z_off) + c.alns, c.z, flag, kOvZ, s);
+  launch_cap_check(P<int64_t>(w.cig_off) + c.alns, c.cig, flag, kOvCig, s);
   TbArgs tb{};
+  tb.stop = flag;
   tb.idx = ctx->idx;
   tb.opt = ctx->opt;
-  tb.n_alns = T_a;
+  tb.n_alns = c.alns;
+  tb.n_dev = T_a;
   tb.seq = P<uint8_t>(ctx->seq);
   tb.off = P<int32_t>(ctx->off);
-  tb.aln_reg = P<Region>(ctx->w->aln_reg);
-  tb.aln_read = P<int32_t>(ctx->w->aln_read);
-  tb.aln_rank = P<int32_t>(ctx->w->aln_rank);
-  tb.z_off = P<int64_t>(ctx->w->z_off);
-  tb.cig_off = P<int64_t>(ctx->w->cig_off);
-  tb.z = P<uint8_t>(ctx->w->z);
-  tb.cigar = P<uint32_t>(ctx->w->cigar);
-  tb.alns = P<Aln>(ctx->w->alns);
-  tb.eh_pool = P<int32_t>(ctx->w->eh_pool);
-  tb.tb_order = P<int32_t>(ctx->w->tb_order);
-  tb.tb_ctr = P<int32_t>(ctx->w->tb_ctr);
-  tb.tb_nw = P<int32_t>(ctx->w->tb_nw);
-  HIPCK(hipEventRecord(ctx->kev[10], s));
+  tb.aln_reg = P<Region>(w.aln_reg);
+  tb.aln_read = P<int32_t>(w.aln_read);
+  tb.aln_rank = P<int32_t>(w.aln_rank);
+  tb.z_off = P<int64_t>(w.z_off);
+  tb.cig_off = P<int64_t>(w.cig_off);
+  tb.z = P<uint8_t>(w.z);
+  tb.cigar = P<uint32_t>(w.cigar);
+  tb.alns = P<Aln>(w.alns);
+  tb.eh_pool = P<int32_t>(w.eh_pool);
+  tb.tb_order = P<int32_t>(w.tb_order);
+  tb.tb_ctr = P<int32_t>(w.tb_ctr);
+  tb.tb_nw = P<int32_t>(w.tb_nw);
+  if (timed) HIPCK(hipEventRecord(ctx->kev[10], s));
   launch_traceback(tb, ctx->run_tb_threads, ctx->n_cu, s);
-  HIPCK(hipEventRecord(ctx->kev[11], s));
-  ctx->kev_used[5] = 1;
+  if (timed) HIPCK(hipEventRecord(ctx->kev[11], s));
+  ctx->kev_used[5] = timed;
   // CIGARs packed to the words used, for the copy to the host
-  if (ensure(ctx, ctx->w->cig_words, (size_t)T_a * 4 + 4) || ensure(ctx, ctx->w->cig_coff, (size_t)(T_a + 1) * 8))
-    return alloc_fail(ctx);
-  launch_cig_words(P<Aln>(ctx->w->alns), T_a, P<int32_t>(ctx->w->cig_words), s);
-  scan_i32(P<int32_t>(ctx->w->cig_words), P<int64_t>(ctx->w->cig_coff), T_a, ctx->w->scan_tmp.p, ctx->w->scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->w->cig_coff) + T_a})) return G3A_E_HIP;
-  const int64_t T_u = ctx->pinned[0];
-  if (ensure(ctx, ctx->w->cig_out, (size_t)T_u * 4 + 64)) return alloc_fail(ctx);
-  launch_cig_gather(P<Aln>(ctx->w->alns), T_a, P<int64_t>(ctx->w->cig_coff), P<uint32_t>(ctx->w->cigar),
-                    P<uint32_t>(ctx->w->cig_out), s);
-  ctx->total_cig = T_u;
-  HIPCK(hipEventRecord(ctx->ev[5], s));
-  HIPCK(hipStreamSynchronize(s));
-  HIPCK(hipGetLastError());
-  float t[5];
-  for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], ctx->ev[i], ctx->ev[i + 1]);
-  for (int i = 0; i < 5; ++i) ctx->ms[i] = t[i];
+  launch_cig_words(P<Aln>(w.alns), c.alns, T_a, P<int32_t>(w.cig_words), s);
+  scan_i32(P<int32_t>(w.cig_words), P<int64_t>(w.cig_coff), c.alns, w.scan_tmp.p, w.scan_tmp.bytes, s);
+  launch_cap_check(P<int64_t>(w.cig_coff) + c.alns, c.used, flag, kOvUsed, s);
+  launch_cig_gather(P<Aln>(w.alns), c.alns, T_a, flag, P<int64_t>(w.cig_coff), P<uint32_t>(w.cigar),
+                    P<uint32_t>(w.cig_out), s);
+  // the totals, for the one read-back of the batch
+  Gather16 g{};
+  g.src[kTotSlow] = cnt + 8;
+  g.src[kTotPool] = P<int64_t>(w.slow_off) + c.slow;
+  g.src[kTotSeeds] = T_s;
+  g.src[kTotTasks] = T_t;
+  g.src[kTotAlns] = T_a;
+  g.src[kTotZ] = P<int64_t>(w.z_off) + c.alns;
+  g.src[kTotCig] = P<int64_t>(w.cig_off) + c.alns;
+  g.src[kTotUsed] = P<int64_t>(w.cig_coff) + c.alns;
+  g.src[kTotChainErr] = cnt + 16;
+  g.src[kTotFlags] = cnt + kFlagSlot32 / 2;
+  launch_gather_totals(g, kTotN, P<int64_t>(w.totals), s);
+  if (timed) HIPCK(hipEventRecord(ctx->ev[5], s));
+  return 0;
+}
+
+// A byte string naming everything a captured batch bakes into its launches:
+// a graph is replayed only while it is unchanged.
+static std::string graph_key(const g3a_ctx *ctx, int n, const Caps &c) {
+  std::string k;
+  auto put = [&](const void *p, size_t b) { k.append((const char *)p, b); };
+  put(&n, sizeof n);
+  put(&c, sizeof c);
+  put(&ctx->idx, sizeof ctx->idx);
+  put(&ctx->opt, sizeof ctx->opt);
+  const int v[] = {ctx->form, ctx->strip_rows, ctx->lane_blocks_per_cu, ctx->wave_blocks_per_cu, ctx->ext_packed,
+                   ctx->seed_blocks_per_cu, ctx->run_seed_threads, ctx->run_tb_threads, ctx->run_lane_blocks,
+                   ctx->run_slow_chunk};
+  put(v, sizeof v);
+  put(&ctx->buf_gen, sizeof ctx->buf_gen);
+  return k;
+}
+
+// The pipeline on the resident batch, with ctx->w as its workspace: one
+// attempt per capacity plan, redone with the overflowing capacity grown
+// until every count fits (the first attempt fits once the context has seen
+// batches like this one).
+static int align_batch(g3a_ctx *ctx) {
+  hipStream_t s = ctx->stream;
+  const int n = ctx->n_reads;
+  ctx->has_output = false;
+  ctx->total_seeds = ctx->total_tasks = ctx->total_alns = ctx->total_cig = ctx->n_slow = 0;
+  Caps c = plan_caps(ctx, n);
+  for (int attempt = 0;; ++attempt) {
+    if (attempt == 8) return fail(ctx, G3A_E_INTERNAL, "batch capacities did not settle");
+    if (ctx->arena) {  // an allocation-free attempt carves the scratch afresh
+      ctx->arena->used = 0;
+      ctx->scratch_work = Work{};
+    }
+    for (int &u : ctx->kev_used) u = 0;
+    if (alloc_batch(ctx, n, c)) return alloc_fail(ctx);
+    const bool graph = ctx->use_graph && !ctx->arena && !ctx->seed_ordered;
+    ctx->graph_used = false;
+    if (graph) {
+      const std::string key = graph_key(ctx, n, c);
+      if (!ctx->gexec || key != ctx->gkey) {
+        free_graph(ctx);
+        HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_batch(ctx, n, c, false);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &gr);
+        if (rc) {
+          if (gr) hipGraphDestroy(gr);
+          return rc;
+        }
+        HIPCK(e);
+        ctx->graph = gr;
+        HIPCK(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+        ctx->gkey = key;
+        ++ctx->graph_captures;
+      }
+      HIPCK(hipEventRecord(ctx->ev[0], s));
+      HIPCK(hipGraphLaunch(ctx->gexec, s));
+      HIPCK(hipEventRecord(ctx->ev[5], s));
+      ctx->graph_used = true;
+    } else {
+      if (int rc = enqueue_batch(ctx, n, c, true)) return rc;
+    }
+    HIPCK(hipMemcpyAsync(ctx->pinned, P<int64_t>(ctx->w->totals), kTotN * 8, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    HIPCK(hipGetLastError());
+    int64_t t[kTotN];
+    std::memcpy(t, ctx->pinned, sizeof t);
+    const int32_t flags = (int32_t)(t[kTotFlags] & 0xffffffff);
+    const int64_t n_slow = t[kTotSlow] & 0xffffffff;
+    if (flags) {
+      // the first count that did not fit (the kernels after it stopped, and
+      // the checks after it read nothing): grow it past the count
+      auto grow = [](int64_t have, int64_t need) { return std::max(2 * have, need + need / 4 + 1024); };
+      if (flags & kOvSlow) c.slow = (int)std::min<int64_t>(n, grow(c.slow, n_slow));
+      else if (flags & kOvPool) c.pool = grow(c.pool, t[kTotPool]);
+      else if (flags & kOvSeeds) c.seeds = grow(c.seeds, t[kTotSeeds]);
+      else if (flags & kOvTasks) c.tasks = grow(c.tasks, t[kTotTasks]);
+      else if (flags & kOvAlns) c.alns = grow(c.alns, t[kTotAlns]);
+      else {
+        if (flags & kOvZ) c.z = grow(c.z, t[kTotZ]);
+        if (flags & kOvCig) c.cig = grow(c.cig, t[kTotCig]);
+        if (flags & kOvUsed) c.used = grow(c.used, t[kTotUsed]);
+      }
+      ++ctx->overflow_reruns;
+      continue;
+    }
+    if (t[kTotChainErr] & 0xffffffff) return fail(ctx, G3A_E_INTERNAL, "chain node arena overflow");
+    ctx->caps = c;
+    ctx->n_slow = n_slow;
+    ctx->total_seeds = t[kTotSeeds];
+    ctx->total_tasks = t[kTotTasks];
+    ctx->total_alns = t[kTotAlns];
+    ctx->total_cig = t[kTotUsed];
+    // next batches start from what this one needed, with headroom (buffers
+    // only grow, so a smaller plan costs nothing but a new graph capture)
+    if (n >= 1024 && !ctx->arena) {
+      auto per = [&](double &r, int64_t total, int64_t units) {
+        r = 1.25 * (double)total / (double)std::max<int64_t>(units, 1);
+      };
+      per(ctx->per_read[0], t[kTotSeeds], n);
+      per(ctx->per_read[1], t[kTotTasks], n);
+      per(ctx->per_read[2], t[kTotAlns], n);
+      per(ctx->per_read[3], t[kTotUsed], n);
+      per(ctx->per_base[0], t[kTotZ], ctx->n_bases);
+      per(ctx->per_base[1], t[kTotCig], ctx->n_bases);
+      if (n_slow > 0) ctx->pool_per_slow = std::max<int64_t>(ctx->pool_per_slow, 2 * t[kTotPool] / n_slow + 1);
+      ctx->slow_cap = std::max<int>(ctx->slow_cap, (int)(n_slow + n_slow / 4) + 16);
+    }
+    break;
+  }
+  float tm[5];
+  for (float &m : ctx->ms) m = 0;
+  if (!ctx->graph_used) {
+    for (int i = 0; i < 5; ++i) hipEventElapsedTime(&tm[i], ctx->ev[i], ctx->ev[i + 1]);
+    for (int i = 0; i < 5; ++i) ctx->ms[i] = tm[i];
+  }
   hipEventElapsedTime(&ctx->ms[5], ctx->ev[0], ctx->ev[5]);
   float e = 0;
   for (float &k : ctx->kms) k = 0;
@@ -1102,6 +1316,8 @@ int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *
   const int form = ctx->form, sr0 = ctx->strip_rows, lb0 = ctx->lane_blocks_per_cu, pk0 = ctx->ext_packed;
   ctx->form = G3A_FORM_BATCHED;
   int64_t *cells = P<int64_t>(ctx->w->counters) + 24;
+  // the last batch's tasks: their capacity and their count on the device
+  const int64_t cap = ctx->caps.tasks, *n_dev = P<int64_t>(ctx->w->task_off) + ctx->n_reads;
   int64_t saved = 0;
   HIPCK(hipMemcpy(&saved, cells, 8, hipMemcpyDeviceToHost));
   const int srs[2] = {16, 32}, lbs[4] = {2, 4, 8, 16};
@@ -1117,7 +1333,8 @@ int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *
       // first run warms the buffers of this shape, the second is timed
       for (int rep = 0; rep < 2; ++rep) {
         HIPCK(hipEventRecord(ctx->ev[6], ctx->stream));
-        if (int rc = run_extension(ctx, ctx->total_tasks, cells)) return rc;
+        if (alloc_ext(ctx, cap)) return G3A_E_HIP;
+        if (int rc = run_extension(ctx, cap, n_dev, cells, false)) return rc;
         HIPCK(hipEventRecord(ctx->ev[7], ctx->stream));
         HIPCK(hipEventSynchronize(ctx->ev[7]));
         HIPCK(hipEventElapsedTime(&t, ctx->ev[6], ctx->ev[7]));
@@ -1133,7 +1350,8 @@ int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *
   ctx->lane_blocks_per_cu = blb;
   ctx->ext_packed = bpk;
   // leave the regions and counters as the aligned batch had them
-  if (int rc = run_extension(ctx, ctx->total_tasks, cells)) return rc;
+  if (alloc_ext(ctx, cap)) return G3A_E_HIP;
+  if (int rc = run_extension(ctx, cap, n_dev, cells, false)) return rc;
   HIPCK(hipMemcpyAsync(cells, &saved, 8, hipMemcpyHostToDevice, ctx->stream));
   HIPCK(hipStreamSynchronize(ctx->stream));
   ctx->form = form;
@@ -1293,11 +1511,16 @@ int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *bes
   const int opts[4] = {6, 8, 10, 0};
   float best = -1;
   int bb = ctx->seed_blocks_per_cu, bo = ctx->seed_ordered;
+  const int graph = ctx->use_graph;
+  ctx->use_graph = 0;  // the stage times come from the eager launches' events
   for (int o = 0; o < 2; ++o)
     for (int b : opts) {
       ctx->seed_blocks_per_cu = b;
       ctx->seed_ordered = o;
-      if (int rc = g3a_bwa_align(ctx)) return rc;
+      if (int rc = g3a_bwa_align(ctx)) {
+        ctx->use_graph = graph;
+        return rc;
+      }
       // the seeding stage: the order's key and sort kernels and k_seed
       if (best < 0 || ctx->ms[0] < best) {
         best = ctx->ms[0];
@@ -1307,9 +1530,39 @@ int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *bes
     }
   ctx->seed_blocks_per_cu = bb;
   ctx->seed_ordered = bo;
+  ctx->use_graph = graph;
   if (blocks_per_cu) *blocks_per_cu = bb;
   if (ordered) *ordered = bo;
   if (best_ms) *best_ms = best;
+  return G3A_OK;
+}
+
+int g3a_set_graph(g3a_ctx *ctx, int on) {
+  if (!ctx || (on != 0 && on != 1)) return G3A_E_ARG;
+  ctx->use_graph = on;
+  if (!on) {
+    HIPCK(hipSetDevice(ctx->device));
+    free_graph(ctx);
+  }
+  return G3A_OK;
+}
+
+int g3a_set_batch_caps(g3a_ctx *ctx, const double v[7]) {
+  if (!ctx || !v) return G3A_E_ARG;
+  for (int i = 0; i < 7; ++i)
+    if (!(v[i] >= 0) || v[i] > 1e9) return G3A_E_ARG;
+  for (int i = 0; i < 4; ++i) ctx->per_read[i] = v[i];
+  ctx->per_base[0] = v[4];
+  ctx->per_base[1] = v[5];
+  ctx->slow_cap = (int)v[6];
+  return G3A_OK;
+}
+
+int g3a_graph_info(const g3a_ctx *ctx, int64_t info[3]) {
+  if (!ctx || !info) return G3A_E_ARG;
+  info[0] = ctx->graph_used ? 1 : 0;
+  info[1] = ctx->graph_captures;
+  info[2] = ctx->overflow_reruns;
   return G3A_OK;
 }
 
@@ -1372,6 +1625,7 @@ void g3a_destroy(g3a_ctx *ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_graph(ctx);
   DevBuf *bufs[] = {&ctx->occ, &ctx->kmer, &ctx->occn, &ctx->sbc, &ctx->pac, &ctx->sa, &ctx->ctg_off, &ctx->ctg_len, &ctx->ctg_alt, &ctx->seq, &ctx->off,
                     &ctx->own.intv, &ctx->own.n_intv, &ctx->own.n_seed, &ctx->own.slow, &ctx->own.intv_base, &ctx->own.seed_off,
                     &ctx->own.counters, &ctx->own.slow_list, &ctx->own.slow_scratch, &ctx->own.slow_n, &ctx->own.slow_off, &ctx->own.seeds,
@@ -1381,8 +1635,8 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->own.cig_size, &ctx->own.z_off, &ctx->own.cig_off, &ctx->own.z, &ctx->own.cigar, &ctx->own.alns, &ctx->own.scan_tmp,
                     &ctx->own.cig_words, &ctx->own.cig_coff, &ctx->own.cig_out, &ctx->own.seed_key, &ctx->own.seed_key2, &ctx->own.seed_idx,
                     &ctx->own.seed_order, &ctx->own.seed_sort_tmp,
-                    &ctx->own.ext_res[0], &ctx->own.ext_res[1], &ctx->own.ext_key, &ctx->own.ext_key2, &ctx->own.ext_idx,
-                    &ctx->own.ext_sorted, &ctx->own.ext_slow, &ctx->own.ext_ctr, &ctx->own.ext_sort_tmp, &ctx->own.ext_rowbuf,
+                    &ctx->own.ext_res[0], &ctx->own.ext_res[1], &ctx->own.ext_key, &ctx->own.ext_hist, &ctx->own.totals,
+                    &ctx->own.ext_sorted, &ctx->own.ext_slow, &ctx->own.ext_ctr, &ctx->own.ext_rowbuf,
                     &ctx->own.lep_pool, &ctx->own.eh_pool, &ctx->own.tb_order, &ctx->own.tb_ctr, &ctx->own.tb_nw, &ctx->li_saf, &ctx->li_keys,
                     &ctx->li_root, &ctx->li_leaf, &ctx->li_knot, &ctx->li_leaf_a, &ctx->li_short, &ctx->li_ctr,
                     &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,

@@ -15,6 +15,7 @@ constexpr int kLepCap = 64;                    // fast-path LEP list capacity
 constexpr int kFoundCap = 16;                  // fast-path found-match (and reseed seed) list capacity
 
 struct SeedArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   DevIndex idx;
   Opt opt;
   int n_reads;
@@ -32,11 +33,17 @@ struct SeedArgs {
   int64_t *li_stats;     // learned-index form counters (6), or null
 };
 
+// Counts known only on the device (totals of a batch's seeds, tasks and
+// alignments) reach the kernels as a pointer (`*_dev`) beside the capacity
+// their buffers and grids were set up for; a kernel works on
+// min(*dev, capacity) items (all `capacity` when the pointer is null).
 struct SaArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   DevIndex idx;
   Opt opt;
   int n_reads;
-  int64_t total;
+  int64_t total;  // capacity
+  const int64_t *total_dev;
   const int64_t *seed_off;  // n_reads + 1
   const Intv *intv;
   const int64_t *intv_base;  // per read
@@ -45,12 +52,14 @@ struct SaArgs {
 };
 
 struct SortArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   int n_reads;
   const int64_t *seed_off;
   Seed *seeds, *tmp;
 };
 
 struct ChainArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   DevIndex idx;
   Opt opt;
   int n_reads;
@@ -68,6 +77,7 @@ struct ChainArgs {
 };
 
 struct CompactArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   int n_reads;
   const int64_t *seed_off, *task_off;
   const Task *tasks_tmp;
@@ -75,9 +85,11 @@ struct CompactArgs {
 };
 
 struct ExtArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   DevIndex idx;
   Opt opt;
-  int64_t n_tasks;
+  int64_t n_tasks;  // capacity
+  const int64_t *n_dev;
   const uint8_t *seq;
   const int32_t *off;
   const Task *tasks;
@@ -95,19 +107,20 @@ struct ExtRes {
 // (persistent, chunks of 64 from a device cursor) -> one wavefront per
 // remaining task (persistent) -> finish.
 struct ExtSideArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   DevIndex idx;
   Opt opt;
   int side;
   int lane_opt_ok;        // scoring admitted by the lane engine
-  int64_t n_tasks;
+  int64_t n_tasks;        // capacity
+  const int64_t *n_dev;
   const uint8_t *seq;
   const int32_t *off;
   const Task *tasks;
   const ExtRes *left;     // left results: h0 of the right side
   ExtRes *res;            // this side's results
   uint32_t *key;          // sort key: query length of lane tasks, 0 otherwise
-  int32_t *tid;           // sort values: task index
-  const int32_t *sorted;  // task indices by descending key
+  const int32_t *sorted;  // lane task indices by descending key
   int32_t *slow_list;     // tasks for the one-wavefront kernel
   int32_t *ctr;           // [0] lane tasks [1] slow tasks [2] lane chunk cursor
   uint32_t *rowbuf;       // kExtQCap * 64 words per lane-kernel block
@@ -115,13 +128,16 @@ struct ExtSideArgs {
 };
 
 struct ExtFinishArgs {
-  int64_t n_tasks;
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
+  int64_t n_tasks;  // capacity
+  const int64_t *n_dev;
   const Task *tasks;
   const ExtRes *left, *right;
   Region *regs;
 };
 
 struct RegionArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   Opt opt;
   int n_reads;
   const int64_t *task_off;
@@ -132,6 +148,7 @@ struct RegionArgs {
 };
 
 struct AlnPrepArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   Opt opt;
   int n_reads;
   const int64_t *task_off, *aln_off;
@@ -142,9 +159,11 @@ struct AlnPrepArgs {
 };
 
 struct TbArgs {
+  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
   DevIndex idx;
   Opt opt;
-  int64_t n_alns;
+  int64_t n_alns;  // capacity
+  const int64_t *n_dev;
   const uint8_t *seq;
   const int32_t *off;
   const Region *aln_reg;
@@ -164,8 +183,9 @@ int seed_grid_threads(int n_cu, const DevIndex &x);
 // blocks_per_cu: the grid the caller runs (0: the occupancy maximum), which
 // sets the LDS share of each block for the LEP lists
 void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_t s);
-void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
-                      Intv *pool, int pass, int from_scratch, hipStream_t s);
+// slow reads list[0, n_list) of the whole list's *n_dev (from entry c0 of it)
+void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, const int32_t *n_dev, int c0, Intv *scratch,
+                      const int64_t *pool_off, Intv *pool, int pass, int from_scratch, hipStream_t s);
 void launch_sa_lookup(const SaArgs &a, hipStream_t s);
 void launch_sort(const SortArgs &a, hipStream_t s);
 void launch_chain(const ChainArgs &a, hipStream_t s);
@@ -193,10 +213,11 @@ void launch_aln_md_write(const TextArgs &t, int64_t n_alns, const int32_t *aln_r
 // null) from the BWA .bwt body, rows 0..n
 void launch_occ_lines(const uint32_t *W, uint64_t n, uint64_t primary, uint32_t *wide, uint32_t *narrow,
                       uint32_t *sbc, hipStream_t s);
-// descending radix sort of (key < 512, value) pairs
-size_t ext_sort_tmp_bytes(int64_t n);
-int ext_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,
-             size_t tmp_bytes, hipStream_t s);
+// The lane tasks (key > 0, key < 512) of key[0, min(*n_dev, cap)) by
+// descending key into out[0, number of lane tasks): a counting sort
+// (histogram, bin starts, scatter with per-block reservations); equal keys
+// in no fixed order.  hist: 1024 words of scratch.
+void ext_sort(const uint32_t *key, int64_t cap, const int64_t *n_dev, int32_t *hist, int32_t *out, hipStream_t s);
 void launch_region_filter(const RegionArgs &a, hipStream_t s);
 void launch_aln_prep(const AlnPrepArgs &a, hipStream_t s);
 int traceback_grid_threads(int n_cu);
@@ -204,9 +225,21 @@ int64_t traceback_pool_words(int n_threads);
 void launch_traceback(const TbArgs &a, int n_threads, int n_cu, hipStream_t s);
 void launch_collect_slow(int n, const int32_t *slow, int32_t *list, int32_t *count, int64_t *intv_base, int slot_cap,
                          hipStream_t s);
-void launch_gather_slow_counts(int n, const int32_t *list, const int32_t *n_intv, int64_t *out, hipStream_t s);
-void launch_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_t base, int64_t *intv_base,
-                          hipStream_t s);
+// over list entries [0, cap) of which *n_dev are set (entries past it: 0)
+void launch_gather_slow_counts(int cap, const int32_t *n_dev, const int32_t *list, const int32_t *n_intv, int64_t *out,
+                               hipStream_t s);
+void launch_set_slow_base(int cap, const int32_t *n_dev, const int32_t *list, const int64_t *off, int64_t base,
+                          int64_t *intv_base, hipStream_t s);
+// p[i] = 0 for *n_dev <= i < cap
+void launch_zero_tail(int64_t *p, int64_t cap, const int64_t *n_dev, hipStream_t s);
+// flag |= bit when *total > cap (*total read as int32 when total32)
+void launch_cap_check(const int64_t *total, int64_t cap, int32_t *flag, int32_t bit, hipStream_t s);
+void launch_cap_check32(const int32_t *total, int64_t cap, int32_t *flag, int32_t bit, hipStream_t s);
+// dst[i] = *src[i] for the k pointers in src (k <= 16), one kernel
+struct Gather16 {
+  const int64_t *src[16];
+};
+void launch_gather_totals(const Gather16 &g, int k, int64_t *dst, hipStream_t s);
 
 // exclusive scans (rocPRIM) over int32 or int64 counts into int64 offsets;
 // out has n+1 entries, out[n] = total
@@ -217,7 +250,9 @@ void launch_seed_keys(const uint8_t *seq, const int32_t *off, int n, uint32_t *k
 size_t seed_sort_tmp_bytes(int64_t n);
 int seed_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,
               size_t tmp_bytes, hipStream_t s);
-void launch_cig_words(const Aln *alns, int64_t n, int32_t *words, hipStream_t s);
-void launch_cig_gather(Aln *alns, int64_t n, const int64_t *coff, const uint32_t *cig, uint32_t *out, hipStream_t s);
+// over alignments [0, cap) of which *n_dev are set (words past it: 0)
+void launch_cig_words(const Aln *alns, int64_t cap, const int64_t *n_dev, int32_t *words, hipStream_t s);
+void launch_cig_gather(Aln *alns, int64_t cap, const int64_t *n_dev, const int32_t *stop, const int64_t *coff,
+                       const uint32_t *cig, uint32_t *out, hipStream_t s);
 
 }  // namespace g3a

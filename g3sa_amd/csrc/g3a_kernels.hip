@@ -17,12 +17,21 @@
 #include "g3a_kernels.h"
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 
 namespace g3a {
+
+// min(*dev, cap): a count known on the device, clamped to the capacity the
+// launch was set up for (cap itself when dev is null)
+__device__ __forceinline__ int64_t dcount(const int64_t *dev, int64_t cap) {
+  if (!dev) return cap;
+  const int64_t v = *dev;
+  return v < cap ? v : cap;
+}
 
 // ---------------------------------------------------------------------------
 // seeding
@@ -156,10 +165,17 @@ __global__ void __launch_bounds__(256) k_seed_keys(const uint8_t *seq, const int
 // puts all intervals at the pool offsets computed from the counts, copying
 // them from the scratch when it still holds them all (`from_scratch`: every
 // slow read had its own scratch in pass 0) and searching again otherwise.
-__global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *list, int n_list, Intv *scratch,
-                                                  const int64_t *pool_off, Intv *pool, int pass, int from_scratch) {
+__global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *list, int n_list, const int32_t *n_dev,
+                                                  int c0, Intv *scratch, const int64_t *pool_off, Intv *pool, int pass,
+                                                  int from_scratch) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_list) return;
+  int nv = n_list;
+  if (n_dev) {
+    const int m = *n_dev - c0;
+    nv = m < 0 ? 0 : m < n_list ? m : n_list;
+  }
+  if (t >= nv) return;
   const int r = list[t];
   const int off = a.off[r], len = a.off[r + 1] - off;
   const uint8_t *q = a.seq + off;
@@ -194,8 +210,9 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
 // SA lookup (saLookup, bwa/src/aux.cu:1365-1425): thread per seed
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.total) return;
+  if (s >= dcount(a.total_dev, a.total)) return;
   // read owning seed s: last r with seed_off[r] <= s
   int lo = 0, hi = a.n_reads;
   while (hi - lo > 1) {
@@ -232,6 +249,7 @@ __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
 }
 
 __global__ void __launch_bounds__(128) k_sort_seeds(SortArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
   const int64_t b = a.seed_off[r];
@@ -243,6 +261,7 @@ __global__ void __launch_bounds__(128) k_sort_seeds(SortArgs a) {
 // chaining + chain filter + per-read task list
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) k_chain(ChainArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
   const int64_t b = a.seed_off[r];
@@ -283,6 +302,7 @@ __global__ void __launch_bounds__(128) k_chain(ChainArgs a) {
 }
 
 __global__ void __launch_bounds__(128) k_task_compact(CompactArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
   const int64_t src = a.seed_off[r], dst = a.task_off[r];
@@ -442,10 +462,11 @@ __device__ ExtOut extend_wave(const Opt &o, int qlen, const uint8_t *qbuf, int t
 }
 
 __global__ void __launch_bounds__(64) k_extend(ExtArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   __shared__ uint8_t qbuf[kQMax];
   __shared__ int Hbuf[2 * (kQMax + 2)], Ebuf[2 * (kQMax + 2)];
   const int64_t t = blockIdx.x;
-  if (t >= a.n_tasks) return;
+  if (t >= dcount(a.n_dev, a.n_tasks)) return;
   const Task tk = a.tasks[t];
   const int off = a.off[tk.read], l_seq = a.off[tk.read + 1] - off;
   const uint8_t *read = a.seq + off;
@@ -592,9 +613,11 @@ __device__ __forceinline__ SideTask side_task(const ExtSideArgs &a, int64_t t) {
 }
 
 __global__ void __launch_bounds__(256) k_ext_classify(ExtSideArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = t < a.n_tasks;
+  const bool in = t < dcount(a.n_dev, a.n_tasks);
   bool lane = false, slow = false;
+  uint32_t key = 0;  // 0 for tasks not on the lane engine and past the tasks
   if (in) {
     const SideTask st = side_task(a, t);
     if (st.qlen <= 0) {
@@ -605,9 +628,9 @@ __global__ void __launch_bounds__(256) k_ext_classify(ExtSideArgs a) {
       lane = a.lane_opt_ok && ext_lane_ok(a.opt, st.qlen, w, st.h0);
       slow = !lane;
     }
-    a.key[t] = lane ? (uint32_t)st.qlen : 0u;
-    a.tid[t] = (int32_t)t;
+    if (lane) key = (uint32_t)st.qlen;
   }
+  if (t < a.n_tasks) a.key[t] = key;
   __shared__ int32_t sh[5];
   block_reserve(lane, a.ctr, sh);
   const int slot = block_reserve(slow, a.ctr + 1, sh);
@@ -616,6 +639,7 @@ __global__ void __launch_bounds__(256) k_ext_classify(ExtSideArgs a) {
 
 template <int R, bool SYM, bool PK>
 __global__ void __launch_bounds__(64) k_ext_lane(ExtSideArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   constexpr int NW = R / 8;
   __shared__ uint32_t wt[6 * 64 * NW];
   const int lane = threadIdx.x;
@@ -672,6 +696,7 @@ __global__ void __launch_bounds__(64) k_ext_lane(ExtSideArgs a) {
 }
 
 __global__ void __launch_bounds__(64) k_ext_wave(ExtSideArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   __shared__ uint8_t qbuf[kQMax];
   __shared__ int Hbuf[2 * (kQMax + 2)], Ebuf[2 * (kQMax + 2)];
   const int lane = threadIdx.x;
@@ -712,8 +737,9 @@ __global__ void __launch_bounds__(64) k_ext_wave(ExtSideArgs a) {
 
 // Region of each task from its two sides (localExtending, aux.cu:2326-2370).
 __global__ void __launch_bounds__(256) k_ext_finish(ExtFinishArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.n_tasks) return;
+  if (t >= dcount(a.n_dev, a.n_tasks)) return;
   const Task tk = a.tasks[t];
   const ExtRes l = a.left[t], r = a.right[t];
   Region g;
@@ -778,6 +804,7 @@ __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
 // region filter, alignment list, traceback + finalize
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) k_region_filter(RegionArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
   const int64_t b = a.task_off[r];
@@ -786,6 +813,7 @@ __global__ void __launch_bounds__(128) k_region_filter(RegionArgs a) {
 }
 
 __global__ void __launch_bounds__(128) k_aln_prep(AlnPrepArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n_reads) return;
   const int64_t src = a.task_off[r], dst = a.aln_off[r];
@@ -816,16 +844,19 @@ __device__ __forceinline__ int tb_class(const TbArgs &a, int64_t t) {
 }
 // tb_ctr[0..2]: alignments of classes 0..2
 __global__ void __launch_bounds__(256) k_tb_count(TbArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = t < a.n_alns ? tb_class(a, t) : 3;
+  const int c = t < dcount(a.n_dev, a.n_alns) ? tb_class(a, t) : 3;
   __shared__ int32_t sh[5];
   for (int k = 0; k < 3; ++k) block_reserve(c == k, a.tb_ctr + k, sh);
 }
 // The list holds classes 0, 1, 2 in turn from the front and class 3 from the
 // back; tb_ctr[4..7] are the placement cursors.
 __global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = t < a.n_alns;
+  const int64_t n_alns = dcount(a.n_dev, a.n_alns);
+  const bool in = t < n_alns;
   const int c = in ? tb_class(a, t) : -1;
   const int64_t base1 = a.tb_ctr[0], base2 = base1 + a.tb_ctr[1];
   int at = -1;
@@ -835,7 +866,7 @@ __global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
     if (c == k) at = r;
   }
   if (in) {
-    const int64_t pos = c == 0 ? at : c == 1 ? base1 + at : c == 2 ? base2 + at : a.n_alns - 1 - at;
+    const int64_t pos = c == 0 ? at : c == 1 ? base1 + at : c == 2 ? base2 + at : n_alns - 1 - at;
     a.tb_order[pos] = (int32_t)t;
   }
 }
@@ -847,6 +878,7 @@ __global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
 // the ungapped class.
 template <int RB>
 __global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a, int cls) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   __shared__ int32_t ring_h[RB * 64], ring_e[RB * 64];
   const int lane = threadIdx.x;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + lane;
@@ -896,6 +928,7 @@ __device__ __forceinline__ int col_get(const int (&v)[NH], int c) {
 }
 template <int NH>
 __global__ void __launch_bounds__(256) k_tb_wave_dp(TbArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int L = threadIdx.x & 63;
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -982,10 +1015,12 @@ __global__ void __launch_bounds__(256) k_tb_wave_dp(TbArgs a) {
 }
 
 __global__ void __launch_bounds__(64) k_traceback(TbArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
   int32_t *eh_h = a.eh_pool + gtid * 2 * kQMax, *eh_e = eh_h + kQMax;
-  for (int64_t e = a.tb_ctr[0] + a.tb_ctr[1] + a.tb_ctr[2] + gtid; e < a.n_alns; e += nthreads) {
+  const int64_t n_alns = dcount(a.n_dev, a.n_alns);
+  for (int64_t e = a.tb_ctr[0] + a.tb_ctr[1] + a.tb_ctr[2] + gtid; e < n_alns; e += nthreads) {
     const int64_t t = a.tb_order[e];
     const int r = a.aln_read[t];
     const int off = a.off[r], l_seq = a.off[r + 1] - off;
@@ -1056,11 +1091,11 @@ void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_
     hipLaunchKernelGGL((k_seed<LepWide, false>), dim3(nb), dim3(128), 0, s, a);
   }
 }
-void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, Intv *scratch, const int64_t *pool_off,
-                      Intv *pool, int pass, int from_scratch, hipStream_t s) {
+void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, const int32_t *n_dev, int c0, Intv *scratch,
+                      const int64_t *pool_off, Intv *pool, int pass, int from_scratch, hipStream_t s) {
   if (n_list)
-    hipLaunchKernelGGL(k_seed_slow, dim3(blocks(n_list, 64)), dim3(64), 0, s, a, list, n_list, scratch, pool_off, pool,
-                       pass, from_scratch);
+    hipLaunchKernelGGL(k_seed_slow, dim3(blocks(n_list, 64)), dim3(64), 0, s, a, list, n_list, n_dev, c0, scratch,
+                       pool_off, pool, pass, from_scratch);
 }
 void launch_sa_lookup(const SaArgs &a, hipStream_t s) {
   if (a.total) hipLaunchKernelGGL(k_sa_lookup, dim3(blocks(a.total, 256)), dim3(256), 0, s, a);
@@ -1152,13 +1187,32 @@ __global__ void k_collect_slow(int n, const int32_t *slow, int32_t *list, int32_
   intv_base[r] = (int64_t)r * slot_cap;
   if (slow[r]) list[atomicAdd(count, 1)] = r;
 }
-__global__ void k_gather_slow_counts(int n, const int32_t *list, const int32_t *n_intv, int64_t *out) {
+__global__ void k_gather_slow_counts(int cap, const int32_t *n_dev, const int32_t *list, const int32_t *n_intv,
+                                     int64_t *out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) out[t] = n_intv[list[t]];
+  const int n = *n_dev < cap ? *n_dev : cap;
+  if (t < cap) out[t] = t < n ? n_intv[list[t]] : 0;
 }
-__global__ void k_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_t base, int64_t *intv_base) {
+__global__ void k_set_slow_base(int cap, const int32_t *n_dev, const int32_t *list, const int64_t *off, int64_t base,
+                                int64_t *intv_base) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = *n_dev < cap ? *n_dev : cap;
   if (t < n) intv_base[list[t]] = base + off[t];
+}
+__global__ void k_zero_tail(int64_t *p, int64_t cap, const int64_t *n_dev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap && i >= dcount(n_dev, cap)) p[i] = 0;
+}
+// only the first count that does not fit is flagged: the kernels after it
+// stop, so the counts checked later are not meaningful
+__global__ void k_cap_check(const int64_t *total, const int32_t *total32, int64_t cap, int32_t *flag, int32_t bit) {
+  if (*flag) return;
+  const int64_t v = total ? *total : (int64_t)*total32;
+  if (v > cap) *flag = bit;
+}
+__global__ void k_gather_totals(Gather16 g, int k, int64_t *dst) {
+  const int i = threadIdx.x;
+  if (i < k) dst[i] = *g.src[i];
 }
 __global__ void k_zero_first(int64_t *p) { *p = 0; }
 
@@ -1166,14 +1220,15 @@ __global__ void k_zero_first(int64_t *p) { *p = 0; }
 // two CIGARs into a slot sized for the worst case; the copy holds only the
 // words used, alignment by alignment (reference-format CIGAR, then the
 // global one), and the records point into it.
-__global__ void __launch_bounds__(256) k_cig_words(const Aln *alns, int64_t n, int32_t *words) {
+__global__ void __launch_bounds__(256) k_cig_words(const Aln *alns, int64_t cap, const int64_t *n_dev, int32_t *words) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) words[i] = alns[i].n_cigar + alns[i].n_core;
+  if (i < cap) words[i] = i < dcount(n_dev, cap) ? alns[i].n_cigar + alns[i].n_core : 0;
 }
-__global__ void __launch_bounds__(256) k_cig_gather(Aln *alns, int64_t n, const int64_t *coff, const uint32_t *cig,
-                                                    uint32_t *out) {
+__global__ void __launch_bounds__(256) k_cig_gather(Aln *alns, int64_t cap, const int64_t *n_dev, const int32_t *stop,
+                                                    const int64_t *coff, const uint32_t *cig, uint32_t *out) {
+  if (stop && *stop) return;  // a capacity check before this kernel failed: the batch is redone
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= dcount(n_dev, cap)) return;
   Aln a = alns[i];
   const int64_t o = coff[i];
   for (int k = 0; k < a.n_cigar; ++k) out[o + k] = cig[a.cigar_off + k];
@@ -1188,12 +1243,26 @@ void launch_collect_slow(int n, const int32_t *slow, int32_t *list, int32_t *cou
   if (n) hipLaunchKernelGGL(k_collect_slow, dim3(blocks(n, 256)), dim3(256), 0, s, n, slow, list, count, intv_base,
                             slot_cap);
 }
-void launch_gather_slow_counts(int n, const int32_t *list, const int32_t *n_intv, int64_t *out, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_gather_slow_counts, dim3(blocks(n, 256)), dim3(256), 0, s, n, list, n_intv, out);
+void launch_gather_slow_counts(int cap, const int32_t *n_dev, const int32_t *list, const int32_t *n_intv, int64_t *out,
+                               hipStream_t s) {
+  if (cap) hipLaunchKernelGGL(k_gather_slow_counts, dim3(blocks(cap, 256)), dim3(256), 0, s, cap, n_dev, list, n_intv, out);
 }
-void launch_set_slow_base(int n, const int32_t *list, const int64_t *off, int64_t base, int64_t *intv_base,
-                          hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_set_slow_base, dim3(blocks(n, 256)), dim3(256), 0, s, n, list, off, base, intv_base);
+void launch_set_slow_base(int cap, const int32_t *n_dev, const int32_t *list, const int64_t *off, int64_t base,
+                          int64_t *intv_base, hipStream_t s) {
+  if (cap)
+    hipLaunchKernelGGL(k_set_slow_base, dim3(blocks(cap, 256)), dim3(256), 0, s, cap, n_dev, list, off, base, intv_base);
+}
+void launch_zero_tail(int64_t *p, int64_t cap, const int64_t *n_dev, hipStream_t s) {
+  if (cap > 0) hipLaunchKernelGGL(k_zero_tail, dim3(blocks(cap, 256)), dim3(256), 0, s, p, cap, n_dev);
+}
+void launch_cap_check(const int64_t *total, int64_t cap, int32_t *flag, int32_t bit, hipStream_t s) {
+  hipLaunchKernelGGL(k_cap_check, dim3(1), dim3(1), 0, s, total, (const int32_t *)nullptr, cap, flag, bit);
+}
+void launch_cap_check32(const int32_t *total, int64_t cap, int32_t *flag, int32_t bit, hipStream_t s) {
+  hipLaunchKernelGGL(k_cap_check, dim3(1), dim3(1), 0, s, (const int64_t *)nullptr, total, cap, flag, bit);
+}
+void launch_gather_totals(const Gather16 &g, int k, int64_t *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_totals, dim3(1), dim3(64), 0, s, g, k, dst);
 }
 
 struct WidenI32 {
@@ -1222,12 +1291,15 @@ int seed_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, 
              : -1;
 }
 
-void launch_cig_words(const Aln *alns, int64_t n, int32_t *words, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_cig_words, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, alns, n, words);
+void launch_cig_words(const Aln *alns, int64_t cap, const int64_t *n_dev, int32_t *words, hipStream_t s) {
+  if (cap > 0)
+    hipLaunchKernelGGL(k_cig_words, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, alns, cap, n_dev, words);
 }
-void launch_cig_gather(Aln *alns, int64_t n, const int64_t *coff, const uint32_t *cig, uint32_t *out, hipStream_t s) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_cig_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, alns, n, coff, cig, out);
+void launch_cig_gather(Aln *alns, int64_t cap, const int64_t *n_dev, const int32_t *stop, const int64_t *coff,
+                       const uint32_t *cig, uint32_t *out, hipStream_t s) {
+  if (cap > 0)
+    hipLaunchKernelGGL(k_cig_gather, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, alns, cap, n_dev, stop,
+                       coff, cig, out);
 }
 
 size_t scan_tmp_bytes(int64_t n) {
@@ -1250,19 +1322,74 @@ void scan_i32(const int32_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_
                             rocprim::plus<int64_t>(), s);
 }
 
-size_t ext_sort_tmp_bytes(int64_t n) {
-  size_t b = 0;
-  rocprim::radix_sort_pairs_desc(nullptr, b, (const uint32_t *)nullptr, (uint32_t *)nullptr, (const int32_t *)nullptr,
-                                 (int32_t *)nullptr, (size_t)(n > 0 ? n : 1), 0, 9);
-  return b + 256;
+// Counting sort of the lane tasks by descending key (kernels of its own:
+// no memset nodes when the batch is captured in a graph).  hist[0, 512):
+// tasks per key; hist[512, 1024): each key's next free place in out.
+constexpr int kExtKeys = 512;
+__global__ void __launch_bounds__(256) k_ext_hist(const uint32_t *key, int64_t cap, const int64_t *n_dev,
+                                                  int32_t *hist) {
+  __shared__ int32_t h[kExtKeys];
+  const int64_t n = dcount(n_dev, cap);
+  for (int k = threadIdx.x; k < kExtKeys; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t k = key[t];
+    if (k > 0 && k < kExtKeys) atomicAdd(&h[k], 1);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kExtKeys; k += blockDim.x)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
 }
-int ext_sort(const uint32_t *key_in, uint32_t *key_out, const int32_t *val_in, int32_t *val_out, int64_t n, void *tmp,
-             size_t tmp_bytes, hipStream_t s) {
-  if (n <= 0) return 0;
-  return rocprim::radix_sort_pairs_desc(tmp, tmp_bytes, key_in, key_out, val_in, val_out, (size_t)n, 0, 9, s) ==
-                 hipSuccess
-             ? 0
-             : -1;
+// one block of kExtKeys threads: start of key k = tasks with keys above k
+__global__ void __launch_bounds__(kExtKeys) k_ext_bins(int32_t *hist) {
+  __shared__ int32_t c[kExtKeys];
+  const int k = threadIdx.x;
+  c[k] = hist[kExtKeys - 1 - k];  // descending keys in ascending slots
+  __syncthreads();
+  for (int o = 1; o < kExtKeys; o <<= 1) {
+    const int v = k >= o ? c[k - o] : 0;
+    __syncthreads();
+    c[k] += v;
+    __syncthreads();
+  }
+  const int key = kExtKeys - 1 - k;
+  hist[kExtKeys + key] = c[k] - hist[key];  // exclusive
+}
+// A block places a tile of 4096 tasks: ranks within the tile from LDS
+// counters, then one global reservation per key present in the tile (few
+// blocks, so the reservations hardly contend).
+constexpr int kScatterPer = 16;  // tasks per thread
+__global__ void __launch_bounds__(256) k_ext_scatter(const uint32_t *key, int64_t cap, const int64_t *n_dev,
+                                                     int32_t *hist, int32_t *out) {
+  __shared__ int32_t h[kExtKeys], base[kExtKeys];
+  const int64_t tile = (int64_t)blockIdx.x * (256 * kScatterPer);
+  const int64_t n = dcount(n_dev, cap);
+  if (tile >= n) return;  // the whole block: past the tasks
+  for (int k = threadIdx.x; k < kExtKeys; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  uint32_t k[kScatterPer];
+  int r[kScatterPer];
+#pragma unroll
+  for (int i = 0; i < kScatterPer; ++i) {
+    const int64_t t = tile + i * 256 + threadIdx.x;
+    k[i] = t < n ? key[t] : 0;
+    r[i] = k[i] > 0 && k[i] < kExtKeys ? atomicAdd(&h[k[i]], 1) : 0;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kExtKeys; j += blockDim.x)
+    if (h[j]) base[j] = atomicAdd(&hist[kExtKeys + j], h[j]);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScatterPer; ++i)
+    if (k[i] > 0 && k[i] < kExtKeys) out[base[k[i]] + r[i]] = (int32_t)(tile + i * 256 + threadIdx.x);
+}
+void ext_sort(const uint32_t *key, int64_t cap, const int64_t *n_dev, int32_t *hist, int32_t *out, hipStream_t s) {
+  launch_fill_i32(hist, 2 * kExtKeys, 0, s);
+  if (cap <= 0) return;
+  hipLaunchKernelGGL(k_ext_hist, dim3((unsigned)std::min<int64_t>(blocks(cap, 256), 2048)), dim3(256), 0, s, key, cap,
+                     n_dev, hist);
+  hipLaunchKernelGGL(k_ext_bins, dim3(1), dim3(kExtKeys), 0, s, hist);
+  hipLaunchKernelGGL(k_ext_scatter, dim3(blocks(cap, 256 * kScatterPer)), dim3(256), 0, s, key, cap, n_dev, hist, out);
 }
 
 }  // namespace g3a

@@ -401,7 +401,7 @@ int g3a_mem_main(int argc, char **argv) {
   mallopt(M_TRIM_THRESHOLD, 1 << 30);
   const char *out_fn = nullptr;
   int batch = 250000, gpus = 1, threads = 8;
-  bool sam = false, learned = false;
+  bool sam = false, learned = false, graph = true;
   int i = 1;
   for (; i < argc && argv[i][0] == '-' && argv[i][1]; ++i) {
     const char *a = argv[i];
@@ -412,6 +412,7 @@ int g3a_mem_main(int argc, char **argv) {
     else if (!std::strcmp(a, "-t")) { const char *v = val(); threads = v ? std::atoi(v) : threads; }
     else if (!std::strcmp(a, "-S")) sam = true;
     else if (!std::strcmp(a, "--learned")) learned = true;
+    else if (!std::strcmp(a, "--no-graph")) graph = false;
     else if (!std::strcmp(a, "-b")) {}
     else if (!std::strcmp(a, "-v") || !std::strcmp(a, "-i") || !std::strcmp(a, "-z") || !std::strcmp(a, "-l") ||
              !std::strcmp(a, "-p") || !std::strcmp(a, "-F") || !std::strcmp(a, "-L") || !std::strcmp(a, "-m"))
@@ -423,7 +424,8 @@ int g3a_mem_main(int argc, char **argv) {
   }
   if (i + 2 > argc) {
     std::fprintf(stderr,
-                 "usage: g3a mem [-o out] [-Z reads_per_batch] [-S] [-t threads] [--learned] <prefix> <reads.fq>\n");
+                 "usage: g3a mem [-o out] [-Z reads_per_batch] [-S] [-t threads] [--learned] [--no-graph] <prefix> <reads.fq>\n"
+                 "  each batch runs as one HIP graph unless --no-graph\n");
     return 1;
   }
   if (gpus != 1) {
@@ -465,7 +467,7 @@ int g3a_mem_main(int argc, char **argv) {
       return 1;
     }
     ctxs.push_back(c);
-    if ((rc = g3a_memcpy_index(c))) {
+    if ((rc = g3a_memcpy_index(c)) || (rc = g3a_set_graph(c, graph ? 1 : 0))) {
       std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(c));
       destroy_all();
       return 1;

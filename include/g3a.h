@@ -166,6 +166,31 @@ void g3a_destroy(g3a_ctx *ctx);
 #define G3A_FORM_PLAIN 0
 #define G3A_FORM_BATCHED 1
 int g3a_set_form(g3a_ctx *ctx, int form);
+/* Run each g3a_bwa_align as one HIP graph (on = 1; default 0).  A batch
+ * runs with no host round trip in either mode: its buffers are sized before
+ * the first kernel from the seeds, tasks and alignments per read of the
+ * batches seen so far, the kernels read the true counts from device memory,
+ * and a count that outgrows its buffer stops the kernels after it and makes
+ * the call run the batch again with that buffer grown.  With on = 1 the
+ * launches are captured into a graph, replayed while the batch's read count,
+ * buffers, index, options and tuning stay the same, and captured again when
+ * they change (a batch seeded in minimizer order, g3a_set_seed_order(ctx, 1),
+ * and the allocation-free entry point run without a graph).  Output is the
+ * same in both modes.  Stage times (g3a_last_stats) then give only the
+ * total.  No reference counterpart: the reference launches its stages one
+ * by one with host synchronizations (bwa/src/bwamem.cu:252, 340). */
+int g3a_set_graph(g3a_ctx *ctx, int on);
+/* [0] 1 when the last g3a_bwa_align ran as a graph, [1] graph captures so
+ * far, [2] batch attempts redone because a count outgrew its buffer.  No
+ * reference counterpart. */
+int g3a_graph_info(const g3a_ctx *ctx, int64_t info[3]);
+/* The capacities the next batch starts from: v = {seeds, extension tasks,
+ * alignments, CIGAR words used, per read; direction bytes, CIGAR slot words,
+ * per base; slow-path reads per batch}.  Later batches raise them again
+ * from the counts they see.  Small values make batches run again with grown
+ * buffers (the path tests exercise); the defaults suit 150 bp reads.  No
+ * reference counterpart. */
+int g3a_set_batch_caps(g3a_ctx *ctx, const double v[7]);
 /* Tuning of the batched form: rows per register strip (16 or 32) and
  * persistent blocks per compute unit for the lane and wavefront kernels.
  * No reference counterpart. */

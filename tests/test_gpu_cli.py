@@ -53,6 +53,9 @@ def test_cli_byte_identical(ds_c1, tmp_path, mode):
     print(f"[cli] {mode}: batches of 700 reads", flush=True)
     got = gpu_out(ds_c1, fq, flags + ["-Z", "700"], tmp_path, f"{mode}_many")
     assert got == ref, first_diff(got, ref)
+    print(f"[cli] {mode}: batches of 700 reads, launches without graphs", flush=True)
+    got = gpu_out(ds_c1, fq, flags + ["-Z", "700", "--no-graph"], tmp_path, f"{mode}_many_eager")
+    assert got == ref, first_diff(got, ref)
     # FASTQ read in 997-byte chunks: records cut at every possible offset
     print(f"[cli] {mode}: 997-byte input chunks", flush=True)
     got = gpu_out(ds_c1, fq, flags + ["-Z", "1500"], tmp_path, f"{mode}_chunks", chunk=997)
