@@ -34,6 +34,7 @@ steps.
 from __future__ import annotations
 
 import argparse
+import threading
 import json
 import os
 import re
@@ -256,6 +257,52 @@ def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = Tr
     }
 
 
+def _part(batch, a: int, b: int):
+    """Reads [a, b) of a loaded batch, with offsets starting at 0."""
+    seq, off, names, name_off, quals, qual_off = batch
+    return (seq[int(off[a]):int(off[b])], (off[a:b + 1] - off[a]).astype(np.int32),
+            names[int(name_off[a]):int(name_off[b])], name_off[a:b + 1] - name_off[a],
+            quals[int(qual_off[a]):int(qual_off[b])], qual_off[a:b + 1] - qual_off[a])
+
+
+def two_context_steps(al, batch, steps: int, warmup: int) -> dict:
+    """The step with the batch split in halves over `al` and a sibling context
+    that shares its device index (g3a_device_alloc_sibling), each half
+    aligned and formatted from its own host thread at the same time; the
+    resident batch of `al` is restored afterwards."""
+    off = batch[1]
+    n = len(off) - 1
+    h = n // 2
+    parts = [_part(batch, 0, h), _part(batch, h, n)]
+    sib = al.sibling(n - h + 16, int(off[n] - off[h]) + 1024)
+    ctxs = [al, sib]
+    for c, p in zip(ctxs, parts):
+        c.upload(p[0], p[1])
+        c.upload_text(*p[2:])
+
+    def both():
+        def one(c):
+            c.run()
+            c.format(sam=True)
+        th = [threading.Thread(target=one, args=(c,)) for c in ctxs]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+
+    for _ in range(max(warmup, 1)):
+        both()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        both()
+        log(f"two contexts: step {i + 1}/{steps} done")
+    el = time.perf_counter() - t0
+    sib.close()
+    al.upload(batch[0], batch[1])
+    al.upload_text(*batch[2:])
+    return {"reads_per_s": round(n * steps / el, 1), "ms_per_step": round(1e3 * el / steps, 3)}
+
+
 def pcie_inclusive(al, batch, steps: int) -> dict:
     """The same step with the batch coming from host memory and the SAM
     records going back to it (reads in over PCIe, text out)."""
@@ -430,6 +477,12 @@ def main() -> int:
                 log(f"seeding autotuned (learned form): {learned_tuning}")
         per_form[form] = run_form(al, n, args.steps, args.warmup, form,
                                   autotune=not args.tuning and (form == "batched" or "batched" not in per_form))
+        # the headline step: halves on two contexts at the same time
+        one = {k: per_form[form][k] for k in ("reads_per_s", "ms_per_step")}
+        two = two_context_steps(al, batch, args.steps, args.warmup)
+        log(f"[{form}] one context {one['ms_per_step']:.1f} ms, two contexts {two['ms_per_step']:.1f} ms per step")
+        per_form[form].update(two)
+        per_form[form]["one_context"] = one
         if form == "learned":
             per_form[form]["learned_work"] = al.learned_stats()
     main_form = "learned" if "learned" in per_form else ("batched" if "batched" in per_form else forms[0])
@@ -479,7 +532,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "reads": n, "bases": int(off[-1]),
                    "form": "batched+learned" if main_form == "learned" else main_form,
-                   "step": "resident reads -> alignments -> SAM records with MAPQ/NM/MD/AS in HBM"},
+                   "contexts": 2,
+                   "step": "resident reads -> alignments -> SAM records with MAPQ/NM/MD/AS in HBM; the batch's "
+                           "halves on two contexts sharing one device index, at the same time"},
         "device_pipeline_reads_per_s": m["device_pipeline_reads_per_s"],
         "sam_format_ms": m["sam_format_ms"],
         "sam_bytes": m["sam_bytes"],
@@ -492,6 +547,7 @@ def main() -> int:
         "tuning": m.get("tuning"),
         "seed_tuning": seed_tuning,
         "forms": {f: {"reads_per_s": v["reads_per_s"], "ms_per_step": v["ms_per_step"],
+                      "one_context": v["one_context"],
                       "device_pipeline_reads_per_s": v["device_pipeline_reads_per_s"],
                       "sam_format_ms": v["sam_format_ms"], "kernel_ms": v["kernel_ms"],
                       "stages_ms": v["stages_ms"], "fm_steps": v["work"]["fm_steps"]} for f, v in per_form.items()},

@@ -77,7 +77,7 @@ class LearnedInfo(C.Structure):
 EXPORTS = [
     "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
-    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_graph", "g3a_graph_info", "g3a_set_batch_caps", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
+    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_graph", "g3a_graph_info", "g3a_set_batch_caps", "g3a_device_alloc_sibling", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
     "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed", "g3a_build_index_dev", "g3a_memcpy_text",
     "g3a_format_records", "g3a_copy_text", "g3a_bwa_align_noalloc", "g3a_set_wide_layout",
@@ -116,6 +116,7 @@ def load_library() -> C.CDLL:
     lib.g3a_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_graph_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     lib.g3a_set_batch_caps.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    lib.g3a_device_alloc_sibling.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_void_p)]
     lib.g3a_kernel_ms.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
     lib.g3a_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
@@ -200,6 +201,19 @@ class Aligner:
         self.set_form(form)
         if tuning is not None:
             self.set_tuning(*tuning)
+
+    def sibling(self, max_reads: int, max_bases: int | None = None) -> "Aligner":
+        """A second context sharing this one's device index (its own stream and
+        workspace, this one's form and tuning); batches on the two can run at
+        the same time from two threads."""
+        if max_bases is None:
+            max_bases = max_reads * 320
+        ctx = C.c_void_p()
+        self._check(self.lib.g3a_device_alloc_sibling(self.ctx, max_reads, max_bases, C.byref(ctx)))
+        al = Aligner.__new__(Aligner)
+        al.lib = self.lib
+        al.ctx = ctx
+        return al
 
     def set_graph(self, on: bool) -> None:
         """Run each batch as one HIP graph (captured again when its shape changes)."""

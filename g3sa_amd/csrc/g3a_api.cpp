@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -71,6 +72,19 @@ struct Work {
 struct Caps {
   int64_t seeds = 0, tasks = 0, alns = 0, z = 0, cig = 0, used = 0, pool = 0;
   int slow = 0;
+};
+
+// Device index buffers shared by a context and its siblings: the store owns
+// them and frees them when the last context holding it is destroyed (each
+// context's own fields are then non-owning views).
+struct IndexStore {
+  int device = 0;
+  std::vector<DevBuf> bufs;
+  ~IndexStore() {
+    hipSetDevice(device);
+    for (DevBuf &b : bufs)
+      if (b.p && !b.view) hipFree(b.p);
+  }
 };
 
 // A bump allocator over the static scratch array, active during one
@@ -154,6 +168,10 @@ struct g3a_ctx {
   // one graph per batch shape (batched or plain form, input seeding order)
   int use_graph = 0;
   bool graph_used = false;
+  // the device index is shared (g3a_device_alloc_sibling): set on the
+  // context that made the index and on each sibling
+  std::shared_ptr<IndexStore> store;
+  bool sibling = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::string gkey;
@@ -267,7 +285,36 @@ hipError_t scratch_address(int device, uint8_t **out) {
   return hipSuccess;
 }
 
+// Stream-side set-up of a context (its stream already created): events,
+// page-locked staging, the device's scratch array.
+int init_runtime(g3a_ctx *ctx) {
+  for (auto &ev : ctx->ev) hipEventCreate(&ev);
+  for (auto &ev : ctx->kev) hipEventCreate(&ev);
+  if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+      ctx->n_cu < 1)
+    ctx->n_cu = 256;
+  if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->stage[0], g3a_ctx::kStage, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->stage[1], g3a_ctx::kStage, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->stage_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->stage_ev[1], hipEventDisableTiming) != hipSuccess)
+    return G3A_E_HIP;
+  // the allocation-free entry point's scratch array on this device (set up
+  // here, with the device current)
+  if (scratch_address(ctx->device, &ctx->scratch) != hipSuccess) return G3A_E_HIP;
+  return 0;
+}
+
+// The device index's buffers of a context (shared with its siblings).
+std::vector<DevBuf *> index_bufs(g3a_ctx *c) {
+  return {&c->occ,     &c->kmer,       &c->occn,    &c->sbc,     &c->pac,     &c->sa,       &c->ctg_off,
+          &c->ctg_len, &c->ctg_alt,    &c->ctg_names, &c->ctg_name_off, &c->log_tab, &c->li_saf, &c->li_keys,
+          &c->li_root, &c->li_leaf,    &c->li_knot, &c->li_leaf_a, &c->li_short, &c->li_ctr};
+}
+
 }  // namespace
+
+static int reserve_workspace(g3a_ctx *ctx);
 
 extern "C" {
 
@@ -357,24 +404,74 @@ int g3a_device_alloc(int device, const char *prefix, const g3a_opt *o, int max_r
     delete ctx;
     return G3A_E_HIP;
   }
-  for (auto &ev : ctx->ev) hipEventCreate(&ev);
-  for (auto &ev : ctx->kev) hipEventCreate(&ev);
-  if (hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      ctx->n_cu < 1)
-    ctx->n_cu = 256;
-  if (hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&ctx->stage[0], g3a_ctx::kStage, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&ctx->stage[1], g3a_ctx::kStage, hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->stage_ev[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->stage_ev[1], hipEventDisableTiming) != hipSuccess) {
+  if (init_runtime(ctx)) {
     g3a_destroy(ctx);
     return G3A_E_HIP;
   }
-  // the allocation-free entry point's scratch array on this device (set
-  // up here, with the device current)
-  if (scratch_address(device, &ctx->scratch) != hipSuccess) {
+  *out = ctx;
+  return G3A_OK;
+}
+
+int g3a_device_alloc_sibling(g3a_ctx *src, int max_reads, int64_t max_bases, g3a_ctx **out) {
+  if (!src || !out || max_reads < 1 || max_bases < 1) return G3A_E_ARG;
+  *out = nullptr;
+  if (!src->index_on_device) return fail(src, G3A_E_STATE, "the source context has no index on the device");
+  if (hipSetDevice(src->device) != hipSuccess) return G3A_E_HIP;
+  g3a_ctx *ctx = new g3a_ctx;
+  ctx->device = src->device;
+  ctx->max_reads = max_reads;
+  ctx->max_bases = max_bases;
+  ctx->opt = src->opt;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return G3A_E_HIP;
+  }
+  if (init_runtime(ctx)) {
     g3a_destroy(ctx);
     return G3A_E_HIP;
+  }
+  // the index buffers the source still owns move into the shared store
+  if (!src->store) {
+    src->store = std::make_shared<IndexStore>();
+    src->store->device = src->device;
+  }
+  const std::vector<DevBuf *> from = index_bufs(src), to = index_bufs(ctx);
+  for (DevBuf *b : from)
+    if (b->p && !b->view) {
+      src->store->bufs.push_back(*b);
+      b->view = true;
+    }
+  for (size_t i = 0; i < from.size(); ++i) {
+    *to[i] = *from[i];
+    to[i]->view = true;
+  }
+  ctx->store = src->store;
+  ctx->sibling = true;
+  ctx->index_on_device = true;
+  ctx->idx = src->idx;
+  ctx->force_wide = src->force_wide;
+  ctx->li_built = src->li_built;
+  ctx->li_leaf_w = src->li_leaf_w;
+  ctx->li_info = src->li_info;
+  // the source's form and tuning
+  ctx->form = src->form;
+  ctx->strip_rows = src->strip_rows;
+  ctx->lane_blocks_per_cu = src->lane_blocks_per_cu;
+  ctx->wave_blocks_per_cu = src->wave_blocks_per_cu;
+  ctx->ext_packed = src->ext_packed;
+  ctx->seed_blocks_per_cu = src->seed_blocks_per_cu;
+  ctx->seed_ordered = src->seed_ordered;
+  ctx->seed_threads = src->seed_threads;
+  ctx->tb_threads = src->tb_threads;
+  ctx->use_graph = src->use_graph;
+  std::memcpy(ctx->per_read, src->per_read, sizeof ctx->per_read);
+  std::memcpy(ctx->per_base, src->per_base, sizeof ctx->per_base);
+  ctx->slow_cap = src->slow_cap;
+  ctx->pool_per_slow = src->pool_per_slow;
+  if (reserve_workspace(ctx)) {
+    const std::string e = ctx->err;
+    g3a_destroy(ctx);
+    return fail(src, G3A_E_HIP, "sibling workspace: " + e);
   }
   *out = ctx;
   return G3A_OK;
@@ -473,6 +570,7 @@ static int reserve_workspace(g3a_ctx *ctx) {
 
 int g3a_memcpy_index(g3a_ctx *ctx) {
   if (!ctx) return G3A_E_ARG;
+  if (ctx->store) return fail(ctx, G3A_E_STATE, "the device index is shared with sibling contexts");
   HIPCK(hipSetDevice(ctx->device));
   const HostIndex &h = ctx->h;
   DevIndex &x = ctx->idx;
@@ -1388,6 +1486,8 @@ int g3a_set_learned(g3a_ctx *ctx, int on, g3a_learned_info *info) {
     return G3A_OK;
   }
   if (!x.occn) return fail(ctx, G3A_E_ARG, "the learned index needs a text of fewer than 2^32 rows");
+  if (!ctx->li_built && ctx->sibling)
+    return fail(ctx, G3A_E_STATE, "a sibling context shares its learned index: build it on the source first");
   if (!ctx->li_built) {
     HIPCK(hipSetDevice(ctx->device));
     const int64_t N = x.n + 1;

@@ -184,6 +184,17 @@ int g3a_set_graph(g3a_ctx *ctx, int on);
  * far, [2] batch attempts redone because a count outgrew its buffer.  No
  * reference counterpart. */
 int g3a_graph_info(const g3a_ctx *ctx, int64_t info[3]);
+/* A second context on src's device that shares src's device index (and its
+ * learned index, when src built it before this call): its own stream,
+ * staging and workspace for batches of up to max_reads reads and max_bases
+ * bases, and src's options, form and tuning.  Batches on src and its
+ * siblings can run at the same time, each from its own host thread; while
+ * seeding on one waits on memory, extension on the other keeps the vector
+ * units busy.  The index buffers are freed with the last context that holds
+ * them.  A context whose index is shared refuses g3a_memcpy_index, and a
+ * sibling refuses to build a learned index.  No reference counterpart: the
+ * reference allocates one process_data_t per GPU (bwa/src/pipeline.cpp:54-67). */
+int g3a_device_alloc_sibling(g3a_ctx *src, int max_reads, int64_t max_bases, g3a_ctx **out);
 /* The capacities the next batch starts from: v = {seeds, extension tasks,
  * alignments, CIGAR words used, per read; direction bytes, CIGAR slot words,
  * per base; slow-path reads per batch}.  Later batches raise them again
