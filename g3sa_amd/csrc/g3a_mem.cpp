@@ -458,15 +458,20 @@ int g3a_mem_main(int argc, char **argv) {
   auto destroy_all = [&] {
     for (g3a_ctx *c : ctxs) g3a_destroy(c);
   };
+  // the first context loads the index (and builds the learned index); the
+  // second shares both (g3a_device_alloc_sibling)
   for (int k = 0; k < kCtx; ++k) {
     g3a_ctx *c = nullptr;
-    int rc = g3a_device_alloc(0, prefix, &opt, batch, ctx_bases, &c);
+    int rc = k == 0 ? g3a_device_alloc(0, prefix, &opt, batch, ctx_bases, &c)
+                    : g3a_device_alloc_sibling(ctxs[0], batch, ctx_bases, &c);
     if (rc) {
-      std::fprintf(stderr, "g3a mem: cannot create context (%d)\n", rc);
+      if (k == 0) std::fprintf(stderr, "g3a mem: cannot create context (%d)\n", rc);
+      else std::fprintf(stderr, "g3a mem: cannot create context: %s\n", g3a_last_error(ctxs[0]));
       destroy_all();
       return 1;
     }
     ctxs.push_back(c);
+    if (k > 0) continue;
     if ((rc = g3a_memcpy_index(c)) || (rc = g3a_set_graph(c, graph ? 1 : 0))) {
       std::fprintf(stderr, "g3a mem: %s\n", g3a_last_error(c));
       destroy_all();
@@ -479,12 +484,11 @@ int g3a_mem_main(int argc, char **argv) {
         destroy_all();
         return 1;
       }
-      if (k == 0)
-        std::fprintf(stderr,
-                     "g3a mem: learned index: %d leaves x %d hidden units, model %lld bytes, max error %lld rows "
-                     "(mean %.1f), %lld keys beyond the %d-row window, built in %.0f ms\n",
-                     li.leaves, li.hidden_units, (long long)li.model_bytes, (long long)li.max_err, li.mean_err,
-                     (long long)li.keys_over_window, li.window_cap, li.build_ms);
+      std::fprintf(stderr,
+                   "g3a mem: learned index: %d leaves x %d hidden units, model %lld bytes, max error %lld rows "
+                   "(mean %.1f), %lld keys beyond the %d-row window, built in %.0f ms\n",
+                   li.leaves, li.hidden_units, (long long)li.model_bytes, (long long)li.max_err, li.mean_err,
+                   (long long)li.keys_over_window, li.window_cap, li.build_ms);
     }
   }
   FILE *out = out_fn ? std::fopen(out_fn, "wb") : stdout;
