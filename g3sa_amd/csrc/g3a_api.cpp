@@ -753,8 +753,9 @@ enum : int32_t {
 
 // counters (int64 slots of w->counters): 0 FM steps, 8 slow-read count
 // (int32), 16 chain error (int32), 24 extension cells, 32..37 learned-form
-// counters, 40 overflow flag (int32)
-constexpr int kFlagSlot32 = 80;  // the flag, as an int32 index
+// counters, 40 overflow flag (int32), 41 k_seed's read counter (int32)
+constexpr int kFlagSlot32 = 80;      // the flag, as an int32 index
+constexpr int kNextReadSlot32 = 82;  // k_seed's read counter (int32)
 
 static int64_t scale(double per, int64_t n, int64_t slack) { return (int64_t)(per * (double)n) + slack; }
 
@@ -914,6 +915,7 @@ static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
   sa.fm_steps = cnt + 0;
   sa.lep_pool = P<Intv>(w.lep_pool);
   sa.li_stats = ctx->idx.saf ? cnt + 32 : nullptr;
+  sa.next_read = P<int32_t>(w.counters) + kNextReadSlot32;
   sa.order = nullptr;
   if (ctx->seed_ordered && n > 1) {
     launch_seed_keys(sa.seq, sa.off, n, P<uint32_t>(w.seed_key), P<int32_t>(w.seed_idx), s);

@@ -31,6 +31,7 @@ struct SeedArgs {
   const int32_t *order;  // reads in processing order, or null for input order
   int lep_lds;           // LEP entries per thread kept in LDS (narrow form; set by launch_seed)
   int64_t *li_stats;     // learned-index form counters (6), or null
+  int32_t *next_read;    // zeroed before k_seed: reads are handed out by an atomic counter (null: fixed stride)
 };
 
 // Counts known only on the device (totals of a batch's seeds, tasks and

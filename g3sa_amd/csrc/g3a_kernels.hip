@@ -36,7 +36,9 @@ __device__ __forceinline__ int64_t dcount(const int64_t *dev, int64_t cap) {
 // ---------------------------------------------------------------------------
 // seeding
 // ---------------------------------------------------------------------------
-// Persistent grid: each thread takes reads r = tid, tid + T, ...  Its LEP
+// Persistent grid: each thread takes the next read from a device counter
+// (one atomic per read), so that a thread held up by a long read takes fewer
+// (r = tid, tid + T, ... when no counter is given).  Its LEP
 // lists live in a global pool slice of its own, contiguous per thread (a
 // divergent list access then touches one cache line; private memory would
 // scatter every entry over eight lines of the per-lane swizzle).
@@ -53,7 +55,8 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
   // learned-index form: range lookups, lookups answered by the FM-index,
   // text runs, run bases, lookups the model narrowed, model windows missed
   int64_t lc[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = gtid; i < a.n_reads; i += nthreads) {
+  for (int i = a.next_read ? atomicAdd(a.next_read, 1) : gtid; i < a.n_reads;
+       i = a.next_read ? atomicAdd(a.next_read, 1) : i + nthreads) {
     const int r = a.order ? a.order[i] : i;
     const int off = a.off[r], len = a.off[r + 1] - off;
     const uint8_t *q = a.seq + off;
