@@ -5,7 +5,7 @@
 //   k_seed_slow      thread per read, for reads whose lists outgrew the fast
 //                    path's private buffers (global-memory scratch)
 //   k_sa_lookup      thread per seed: SA interval row -> reference position
-//   k_sort_seeds     thread per read
+//   k_sort_seeds     wavefront per read (seeds ranked across its lanes)
 //   k_chain          thread per read: B-tree chaining, chain filter, tasks
 //   k_task_compact   thread per read: per-read task lists -> dense task array
 //   k_extend         one wavefront per task: banded local extension, left
@@ -251,13 +251,35 @@ __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
   a.seeds[s] = out;
 }
 
-__global__ void __launch_bounds__(128) k_sort_seeds(SortArgs a) {
+// One wavefront per read: with at most 64 seeds each lane holds one, counts
+// the seeds that sort before it (seed_less, ties by input position: the
+// stable order sort_seeds gives) and stores it at that rank; a read with
+// more seeds is sorted by lane 0 with sort_seeds.
+__global__ void __launch_bounds__(256) k_sort_seeds(SortArgs a) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.n_reads) return;
+  const int lane = threadIdx.x & 63;
+  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (r >= a.n_reads) return;  // the whole wavefront
   const int64_t b = a.seed_off[r];
   const int n = (int)(a.seed_off[r + 1] - b);
-  if (n > 1) sort_seeds(a.seeds + b, n, a.tmp + b);
+  if (n <= 1) return;
+  Seed *sd = a.seeds + b;
+  if (n > 64) {
+    if (lane == 0) sort_seeds(sd, n, a.tmp + b);
+    return;
+  }
+  Seed v{};
+  uint32_t key = 0xffffffffu;  // past the seeds: sorts last
+  if (lane < n) {
+    v = sd[lane];
+    key = (uint32_t)v.qbeg << 16 | (uint32_t)v.len;  // qbeg, len < 2^16 (reads of at most 320 bases)
+  }
+  int rank = 0;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t ki = (uint32_t)__shfl((int)key, i);
+    rank += ki < key || (ki == key && i < lane);
+  }
+  if (lane < n) sd[rank] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1104,7 +1126,7 @@ void launch_sa_lookup(const SaArgs &a, hipStream_t s) {
   if (a.total) hipLaunchKernelGGL(k_sa_lookup, dim3(blocks(a.total, 256)), dim3(256), 0, s, a);
 }
 void launch_sort(const SortArgs &a, hipStream_t s) {
-  if (a.n_reads) hipLaunchKernelGGL(k_sort_seeds, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+  if (a.n_reads) hipLaunchKernelGGL(k_sort_seeds, dim3(blocks(a.n_reads, 4)), dim3(256), 0, s, a);
 }
 void launch_chain(const ChainArgs &a, hipStream_t s) {
   if (a.n_reads) hipLaunchKernelGGL(k_chain, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
