@@ -414,6 +414,8 @@ def main() -> int:
                     help="fixed batched-form tuning instead of the autotuner (e.g. 16,16)")
     ap.add_argument("--seed-blocks", type=int, default=None, metavar="N",
                     help="fixed persistent seeding blocks per CU instead of the autotuner (0: occupancy maximum)")
+    ap.add_argument("--seed-block", type=int, default=1024, metavar="T",
+                    help="with --seed-blocks: seeding threads per block (a multiple of 64, at most 1024)")
     ap.add_argument("--seed-order", type=int, default=1, choices=[0, 1],
                     help="with --seed-blocks: seed in minimizer order (1) or input order (0)")
     ap.add_argument("--form", default="all", choices=["all", "both", "batched", "plain", "learned"],
@@ -453,9 +455,10 @@ def main() -> int:
     al.upload_text(*batch[2:])
     # seeding grid: fixed, or the autotuner's pick on one aligned pass
     if args.seed_blocks is not None:
-        al.set_seed_blocks(args.seed_blocks)
+        al.set_seed_grid(args.seed_block, args.seed_blocks)
         al.set_seed_order(args.seed_order)
-        seed_tuning = {"seed_blocks_per_cu": args.seed_blocks, "seed_ordered": args.seed_order}
+        seed_tuning = {"seed_threads_per_block": args.seed_block, "seed_blocks_per_cu": args.seed_blocks,
+                       "seed_ordered": args.seed_order}
     else:
         al.run()
         seed_tuning = al.autotune_seed()

@@ -119,7 +119,10 @@ struct g3a_ctx {
   DevBuf seq, off;
   int seed_ordered = 0;  // seed reads in minimizer order (1) or input order (0)
   int seed_threads = 0, tb_threads = 0;  // resident maxima (the pools are sized for them)
-  int seed_blocks_per_cu = 0;             // 0: the occupancy maximum
+  // k_seed grid: one 1024-thread block per compute unit keeps a single copy
+  // of the superblock counts in LDS and the rest for list entries
+  int seed_blocks_per_cu = 1;             // 0: the occupancy maximum
+  int seed_block = 1024;                  // k_seed threads per block
   int force_wide = 0;  // the layout of texts of 2^32 rows or more, on any text
   // per-batch workspace: `own`, or `scratch_work` during and after an
   // allocation-free call (its views stay valid for g3a_memcpy_output)
@@ -460,6 +463,7 @@ int g3a_device_alloc_sibling(g3a_ctx *src, int max_reads, int64_t max_bases, g3a
   ctx->wave_blocks_per_cu = src->wave_blocks_per_cu;
   ctx->ext_packed = src->ext_packed;
   ctx->seed_blocks_per_cu = src->seed_blocks_per_cu;
+  ctx->seed_block = src->seed_block;
   ctx->seed_ordered = src->seed_ordered;
   ctx->seed_threads = src->seed_threads;
   ctx->tb_threads = src->tb_threads;
@@ -485,7 +489,7 @@ int g3a_device_alloc_sibling(g3a_ctx *src, int max_reads, int64_t max_bases, g3a
 // traceback and extension grids, so that their per-thread pools fit the
 // scratch array.
 static void set_grids(g3a_ctx *ctx, int n, bool noalloc) {
-  if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx);
+  if (!ctx->seed_threads) ctx->seed_threads = seed_grid_threads(ctx->n_cu, ctx->idx, ctx->seed_block);
   if (!ctx->tb_threads) ctx->tb_threads = traceback_grid_threads(ctx->n_cu);
   ctx->run_seed_threads = ctx->seed_threads;
   ctx->run_tb_threads = ctx->tb_threads;
@@ -493,7 +497,7 @@ static void set_grids(g3a_ctx *ctx, int n, bool noalloc) {
   ctx->run_slow_chunk = 16384;
   if (noalloc) {
     const int need = ((n + 127) / 128) * 128;
-    ctx->run_seed_threads = std::max(128, std::min(ctx->seed_threads, need));
+    ctx->run_seed_threads = std::max(ctx->seed_block, std::min(ctx->seed_threads, need));
     ctx->run_tb_threads = std::min(ctx->tb_threads, 4096);
     ctx->run_lane_blocks = std::min(ctx->run_lane_blocks, 64);
     ctx->run_slow_chunk = 16;
@@ -925,10 +929,11 @@ static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
     sa.order = P<int32_t>(w.seed_order);
   }
   if (timed) HIPCK(hipEventRecord(ctx->kev[0], s));
-  const int seed_run = ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * 128 * ctx->n_cu < ctx->run_seed_threads
-                           ? ctx->seed_blocks_per_cu * 128 * ctx->n_cu
-                           : ctx->run_seed_threads;
-  launch_seed(sa, seed_run, ctx->seed_blocks_per_cu, s);
+  const int seed_run =
+      ctx->seed_blocks_per_cu > 0 && ctx->seed_blocks_per_cu * ctx->seed_block * ctx->n_cu < ctx->run_seed_threads
+          ? ctx->seed_blocks_per_cu * ctx->seed_block * ctx->n_cu
+          : ctx->run_seed_threads;
+  launch_seed(sa, seed_run, ctx->seed_blocks_per_cu, ctx->seed_block, s);
   if (timed) HIPCK(hipEventRecord(ctx->kev[1], s));
   ctx->kev_used[0] = timed;
   launch_collect_slow(n, P<int32_t>(w.slow), P<int32_t>(w.slow_list), n_slow_dev, P<int64_t>(w.intv_base), kSlotCap, s);
@@ -1079,7 +1084,7 @@ static std::string graph_key(const g3a_ctx *ctx, int n, const Caps &c) {
   put(&ctx->idx, sizeof ctx->idx);
   put(&ctx->opt, sizeof ctx->opt);
   const int v[] = {ctx->form, ctx->strip_rows, ctx->lane_blocks_per_cu, ctx->wave_blocks_per_cu, ctx->ext_packed,
-                   ctx->seed_blocks_per_cu, ctx->run_seed_threads, ctx->run_tb_threads, ctx->run_lane_blocks,
+                   ctx->seed_blocks_per_cu, ctx->seed_block, ctx->run_seed_threads, ctx->run_tb_threads, ctx->run_lane_blocks,
                    ctx->run_slow_chunk};
   put(v, sizeof v);
   put(&ctx->buf_gen, sizeof ctx->buf_gen);
@@ -1467,6 +1472,18 @@ int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu) {
   return G3A_OK;
 }
 
+int g3a_set_seed_grid(g3a_ctx *ctx, int threads_per_block, int blocks_per_cu) {
+  if (!ctx || blocks_per_cu < 0 || blocks_per_cu > 64 || threads_per_block < 64 || threads_per_block > 1024 ||
+      threads_per_block % 64)
+    return G3A_E_ARG;
+  if (threads_per_block != ctx->seed_block) {
+    ctx->seed_block = threads_per_block;
+    ctx->seed_threads = 0;  // the resident maximum (and the list pool) follow the block size
+  }
+  ctx->seed_blocks_per_cu = blocks_per_cu;
+  return G3A_OK;
+}
+
 int g3a_set_seed_order(g3a_ctx *ctx, int ordered) {
   if (!ctx || (ordered != 0 && ordered != 1)) return G3A_E_ARG;
   ctx->seed_ordered = ordered;
@@ -1606,18 +1623,20 @@ int g3a_last_intervals(const g3a_ctx *ctx, int32_t *counts, int64_t *kmsn, int64
   return G3A_OK;
 }
 
-int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms) {
+int g3a_autotune_seed(g3a_ctx *ctx, int *threads_per_block, int *blocks_per_cu, int *ordered, float *best_ms) {
   if (!ctx) return G3A_E_ARG;
   if (!ctx->has_output || ctx->w != &ctx->own)
     return fail(ctx, G3A_E_STATE, "autotune needs a batch aligned on the device by g3a_bwa_align");
-  const int opts[4] = {6, 8, 10, 0};
+  // grids of 16 waves per compute unit, from one block (one LDS copy of the
+  // superblock counts) to six smaller ones
+  const int grids[4][2] = {{1024, 1}, {512, 2}, {256, 4}, {128, 6}};
   float best = -1;
-  int bb = ctx->seed_blocks_per_cu, bo = ctx->seed_ordered;
+  int bt = ctx->seed_block, bb = ctx->seed_blocks_per_cu, bo = ctx->seed_ordered;
   const int graph = ctx->use_graph;
   ctx->use_graph = 0;  // the stage times come from the eager launches' events
   for (int o = 0; o < 2; ++o)
-    for (int b : opts) {
-      ctx->seed_blocks_per_cu = b;
+    for (const auto &g : grids) {
+      g3a_set_seed_grid(ctx, g[0], g[1]);
       ctx->seed_ordered = o;
       if (int rc = g3a_bwa_align(ctx)) {
         ctx->use_graph = graph;
@@ -1626,13 +1645,15 @@ int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *bes
       // the seeding stage: the order's key and sort kernels and k_seed
       if (best < 0 || ctx->ms[0] < best) {
         best = ctx->ms[0];
-        bb = b;
+        bt = g[0];
+        bb = g[1];
         bo = o;
       }
     }
-  ctx->seed_blocks_per_cu = bb;
+  g3a_set_seed_grid(ctx, bt, bb);
   ctx->seed_ordered = bo;
   ctx->use_graph = graph;
+  if (threads_per_block) *threads_per_block = bt;
   if (blocks_per_cu) *blocks_per_cu = bb;
   if (ordered) *ordered = bo;
   if (best_ms) *best_ms = best;

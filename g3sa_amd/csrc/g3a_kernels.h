@@ -179,11 +179,12 @@ struct TbArgs {
   int32_t *tb_nw;     // n_alns: global-alignment scores from k_tb_wave_dp
 };
 
-// threads of k_seed's persistent grid (occupancy-limited, n_cu compute units)
-int seed_grid_threads(int n_cu, const DevIndex &x);
+// threads of k_seed's persistent grid in blocks of `block` threads
+// (occupancy-limited, n_cu compute units)
+int seed_grid_threads(int n_cu, const DevIndex &x, int block);
 // blocks_per_cu: the grid the caller runs (0: the occupancy maximum), which
-// sets the LDS share of each block for the LEP lists
-void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_t s);
+// with the block size sets the LDS share of each block for the LEP lists
+void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, int block, hipStream_t s);
 // slow reads list[0, n_list) of the whole list's *n_dev (from entry c0 of it)
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, const int32_t *n_dev, int c0, Intv *scratch,
                       const int64_t *pool_off, Intv *pool, int pass, int from_scratch, hipStream_t s);

@@ -103,7 +103,7 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
 // model's intervals in the last round); a separate kernel, so that the
 // FM-only one keeps its register budget
 template <class E, bool L>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(L ? 5 : 4))) k_seed(SeedArgs a) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(L ? 5 : 4))) k_seed(SeedArgs a) {
   extern __shared__ uint4 sb_lds[];
   const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
   const int nthreads = gridDim.x * blockDim.x;
@@ -1069,51 +1069,57 @@ static bool seed_narrow(const DevIndex &x) { return x.occn != nullptr && x.sbc !
 static size_t seed_lds_bytes(const DevIndex &x) {
   return seed_narrow(x) ? (size_t)((((uint64_t)x.n + 1) >> 21) + 1) * 16 : 0;
 }
-int seed_grid_threads(int n_cu, const DevIndex &x) {
+int seed_grid_threads(int n_cu, const DevIndex &x, int block) {
   int per_cu = 0;
   hipError_t e = seed_narrow(x)
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepNarrow, false>, 128,
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepNarrow, false>, block,
                                                                     seed_lds_bytes(x))
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepWide, false>, 128, 0);
-  if (e != hipSuccess || per_cu < 1) per_cu = 8;
-  return n_cu * per_cu * 128;
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seed<LepWide, false>, block, 0);
+  if (e != hipSuccess || per_cu < 1) per_cu = 1024 / block;
+  return n_cu * per_cu * block;
 }
-void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, hipStream_t s) {
+void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, int block, hipStream_t s) {
   if (!a.n_reads) return;
-  const int need = blocks(a.n_reads, 128), have = n_threads / 128;
+  const int need = blocks(a.n_reads, block), have = n_threads / block;
   int nb = need < have ? need : have;
+  if (nb < 1) nb = 1;
   if (seed_narrow(a.idx)) {
     void (*kern)(SeedArgs) = a.idx.saf ? k_seed<LepNarrow, true> : k_seed<LepNarrow, false>;
-    // The compute unit's LDS (160 KB, at most 64 KB per block) is shared by
-    // the blocks it is to hold; the LEP entries per thread are the most that
-    // still lets that many blocks be resident (a block that is not resident
-    // would only start when another finishes, doubling the kernel's time).
+    // The compute unit's LDS (160 KB) is shared by the blocks it is to hold,
+    // and each block keeps one copy of the superblock counts, so larger
+    // blocks leave more room for list entries; the LEP entries per thread are
+    // the most that still lets that many blocks be resident (a block that is
+    // not resident would only start when another finishes, doubling the
+    // kernel's time).
     const size_t sb = seed_lds_bytes(a.idx);
+    int dev = 0, n_cu = 0, max_lds = 65536;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+          max_lds < 16384)
+        max_lds = 65536;
+    }
     int want = blocks_per_cu;
-    if (want <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&want, kern, 128, sb) != hipSuccess || want < 1))
-      want = 8;
+    if (want <= 0 && (hipOccupancyMaxActiveBlocksPerMultiprocessor(&want, kern, block, sb) != hipSuccess || want < 1))
+      want = 1024 / block;
     size_t per_block = (size_t)160 * 1024 / (size_t)want - 1024;  // slack for allocation granules
-    per_block = per_block < (size_t)64 * 1024 ? per_block : (size_t)64 * 1024;
-    int capl = per_block > sb ? (int)((per_block - sb) / (128 * 12)) : 0;
+    per_block = per_block < (size_t)max_lds ? per_block : (size_t)max_lds;
+    int capl = per_block > sb ? (int)((per_block - sb) / ((size_t)block * 12)) : 0;
     capl = capl < kLepCap ? capl : kLepCap;
     int resident = 0;
     for (; capl > 0; --capl) {
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, 128, sb + (size_t)capl * 128 * 12) !=
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, kern, block, sb + (size_t)capl * block * 12) !=
           hipSuccess)
         resident = 0;
       if (resident >= want) break;
     }
     // no more blocks than the compute units hold at once
-    int dev = 0, n_cu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n_cu > 0 &&
-        resident > 0 && nb > resident * n_cu)
-      nb = resident * n_cu;
+    if (n_cu > 0 && resident > 0 && nb > resident * n_cu) nb = resident * n_cu;
     SeedArgs b = a;
     b.lep_lds = capl;
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(128), sb + (size_t)capl * 128 * 12, s, b);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(block), sb + (size_t)capl * block * 12, s, b);
   } else {
-    hipLaunchKernelGGL((k_seed<LepWide, false>), dim3(nb), dim3(128), 0, s, a);
+    hipLaunchKernelGGL((k_seed<LepWide, false>), dim3(nb), dim3(block), 0, s, a);
   }
 }
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, const int32_t *n_dev, int c0, Intv *scratch,

@@ -227,13 +227,18 @@ int g3a_ext_packed(const g3a_ctx *ctx);
  * for every setting; the batch's output is left as it was.  No reference
  * counterpart. */
 int g3a_autotune(g3a_ctx *ctx, int *strip_rows, int *lane_blocks_per_cu, float *best_ms);
-/* Persistent seeding blocks (128 threads) per compute unit; 0 restores the
- * default, the occupancy maximum.  Each
- * block's share of the compute unit's LDS holds the first entries of its
- * threads' LEP lists (the rest spill to a global slice per thread), so the
- * grid trades latency hiding against list traffic.  No reference
- * counterpart. */
+/* Persistent seeding blocks per compute unit at the current block size
+ * (g3a_set_seed_grid); 0: the occupancy maximum.  Each block's share of the
+ * compute unit's LDS holds the first entries of its threads' LEP lists (the
+ * rest spill to a global slice per thread), so the grid trades latency
+ * hiding against list traffic.  No reference counterpart. */
 int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu);
+/* The seeding grid in full: threads per block (a multiple of 64, at most
+ * 1024; default 1024) and blocks per compute unit (0: the occupancy maximum;
+ * default 1).  Each block keeps one
+ * copy of the Occ superblock counts in LDS, so larger blocks leave more of
+ * the compute unit's LDS to the threads' lists.  No reference counterpart. */
+int g3a_set_seed_grid(g3a_ctx *ctx, int threads_per_block, int blocks_per_cu);
 /* Seeding order: 1 seeds the reads in the order of a minimizer
  * key (the smallest hash over each read's canonical 16-mers), so that
  * overlapping reads run in neighbouring lanes and share cached Occ lines;
@@ -241,12 +246,13 @@ int g3a_set_seed_blocks(g3a_ctx *ctx, int blocks_per_cu);
  * counterpart. */
 int g3a_set_seed_order(g3a_ctx *ctx, int ordered);
 /* Seeding autotuner: re-aligns the batch last aligned once per seeding grid
- * (blocks per compute unit 6, 8, 10 and the occupancy maximum; each block's
- * LDS share holds the first entries of its threads' LEP lists, so fewer
- * blocks keep more in LDS) and order (input, minimizer), keeps the pair with
- * the fastest seeding stage and reports it with its time.  The batch's
- * output is identical for every setting.  No reference counterpart. */
-int g3a_autotune_seed(g3a_ctx *ctx, int *blocks_per_cu, int *ordered, float *best_ms);
+ * (threads per block x blocks per compute unit: 1024x1, 512x2, 256x4, 128x6;
+ * each block keeps one copy of the superblock counts in LDS and the rest of
+ * its share holds the first entries of its threads' LEP lists, so fewer,
+ * larger blocks keep more in LDS) and order (input, minimizer), keeps the
+ * setting with the fastest seeding stage and reports it with its time.  The
+ * batch's output is identical for every setting.  No reference counterpart. */
+int g3a_autotune_seed(g3a_ctx *ctx, int *threads_per_block, int *blocks_per_cu, int *ordered, float *best_ms);
 /* Learned-index form.  g3a_set_learned(ctx, 1, info) builds, on the device
  * and once per context, the full suffix array, the 32-base key of every
  * suffix-array row and a two-stage FP32 model from a key to its row

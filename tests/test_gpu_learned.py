@@ -24,7 +24,7 @@ def _intervals_equal(a, b, label):
         raise AssertionError(f"{label}: intervals differ at {bad.tolist()}: fm {ia[bad].tolist()} learned {ib[bad].tolist()}")
 
 
-def _run(ds, oracle, seq, off, label, forms=(("batched", 1, 0), ("batched", 0, 10), ("plain", 1, 0)), modes=(1,)):
+def _run(ds, oracle, seq, off, label, forms=(("batched", 1, (1024, 1)), ("batched", 0, (128, 10)), ("plain", 1, (256, 0))), modes=(1,)):
     idx = oracle.load(ds.prefix)
     try:
         ref = oracle.align(idx, seq, off, threads=oracle_threads())
@@ -35,7 +35,7 @@ def _run(ds, oracle, seq, off, label, forms=(("batched", 1, 0), ("batched", 0, 1
         for form, ordered, blocks in forms:
             al.set_form(form)
             al.set_seed_order(ordered)
-            al.set_seed_blocks(blocks)
+            al.set_seed_grid(*blocks)
             al.set_learned(False)
             print(f"[learned] {label}: {len(off) - 1} reads, FM-only, form {form} order {ordered} blocks {blocks}",
                   flush=True)
@@ -69,12 +69,12 @@ def test_learned_config1(ds_c1, oracle):
 
 def test_learned_config3_style(ds_c1, oracle):
     names, seq, off, _ = g3sa_amd.read_fastq(str(ds_c1.reads["c3"]))
-    _run(ds_c1, oracle, seq, off, "config3-style", forms=(("batched", 1, 0),))
+    _run(ds_c1, oracle, seq, off, "config3-style", forms=(("batched", 1, (1024, 1)),))
 
 
 def test_learned_repeats(ds_rep, oracle):
     names, seq, off, _ = g3sa_amd.read_fastq(str(ds_rep.reads["main"]))
-    info, st = _run(ds_rep, oracle, seq, off, "repeats", forms=(("batched", 1, 0),))
+    info, st = _run(ds_rep, oracle, seq, off, "repeats", forms=(("batched", 1, (1024, 1)),))
     assert info["distinct_keys"] < info["rows"]  # repeated 32-mers share keys
 
 
@@ -94,7 +94,7 @@ def test_learned_edge_cases(ds_c1, oracle):
         np.full(150, 0, np.uint8),                            # poly-A: long runs of equal keys
     ] + reads[9:40]
     seq2, off2 = _batch(cases)
-    _run(ds_c1, oracle, seq2, off2, "edge-cases", forms=(("batched", 1, 0),))
+    _run(ds_c1, oracle, seq2, off2, "edge-cases", forms=(("batched", 1, (1024, 1)),))
 
 
 @pytest.mark.parametrize("min_seed_len", [9, 11, 15, 26, 31])

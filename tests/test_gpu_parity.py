@@ -46,8 +46,9 @@ def compare(gpu: g3sa_amd.Result, ref: dict, label: str) -> None:
 # reads seeded in minimizer order and in input order; seeding grids whose LDS
 # share holds 9, 25 and the default number of LEP entries per thread (the
 # rest of a list spills to its global slice)
-VARIANTS = [("plain", None, 1, 0, 1), ("batched", (16, 8, 8), 1, 10, 1), ("batched", (32, 4, 8), 0, 4, 1),
-            ("batched", (16, 8, 8), 0, 0, 0)]
+# (form, tuning, seed order, seeding grid (threads per block, blocks per CU), packed)
+VARIANTS = [("plain", None, 1, (1024, 1), 1), ("batched", (16, 8, 8), 1, (128, 10), 1),
+            ("batched", (32, 4, 8), 0, (256, 4), 1), ("batched", (16, 8, 8), 0, (512, 0), 0)]
 
 
 def run_both(ds, oracle, seq, off, label, max_reads=None, wide=False, variants=None):
@@ -66,7 +67,7 @@ def run_both(ds, oracle, seq, off, label, max_reads=None, wide=False, variants=N
             if tuning:
                 al.set_tuning(*tuning)
             al.set_seed_order(ordered)
-            al.set_seed_blocks(blocks)
+            al.set_seed_grid(*blocks)
             al.set_ext_packed(packed)
             got = al.align(seq, off)
             compare(got, ref, f"{label} [{form} {tuning} order {ordered}]")
@@ -169,7 +170,7 @@ def test_scoring_variants_parity(ds_c1, oracle, w, b):
             if tuning:
                 al.set_tuning(*tuning)
             al.set_seed_order(ordered)
-            al.set_seed_blocks(blocks)
+            al.set_seed_grid(*blocks)
             al.set_ext_packed(packed)
             compare(al.align(seq, off), ref, f"scoring w={w} b={b} [{form} {tuning} order {ordered}]")
 
@@ -200,7 +201,8 @@ def test_autotune_keeps_output(ds_c1, oracle):
         compare(al.align(seq, off), ref, "next batch after autotune")
         print("[parity] seeding autotune", flush=True)
         st = al.autotune_seed()
-        assert st["seed_blocks_per_cu"] in (6, 8, 10, 0) and st["seed_ordered"] in (0, 1)
+        assert (st["seed_threads_per_block"], st["seed_blocks_per_cu"]) in ((1024, 1), (512, 2), (256, 4), (128, 6))
+        assert st["seed_ordered"] in (0, 1)
         compare(al.download(), ref, "after seeding autotune")
         compare(al.align(seq, off), ref, "next batch after seeding autotune")
 
