@@ -52,7 +52,7 @@ struct Work {
   // seeding
   DevBuf intv, n_intv, n_seed, slow, intv_base, seed_off, counters, slow_list, slow_scratch, slow_n, slow_off;
   // seeds / chains
-  DevBuf seeds, seeds2, ch, nxt, order, tmpi, w16, kept, nodes, tasks_tmp, n_task, task_off;
+  DevBuf seeds, seeds2, ch, nxt, order, tmpi, w16, kept, nodes, tasks_tmp, n_task, task_off, chain_big;
   // tasks / regions
   DevBuf tasks, regs, flag, sec, n_aln, aln_off;
   // alignments
@@ -511,6 +511,7 @@ static int alloc_reads(g3a_ctx *ctx, int n) {
       ensure(ctx, ctx->w->slow, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->intv_base, (size_t)n * 8 + 8) ||
       ensure(ctx, ctx->w->seed_off, (size_t)(n + 1) * 8) || ensure(ctx, ctx->w->counters, 64 * 8) ||
       ensure(ctx, ctx->w->n_task, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->task_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->w->chain_big, (size_t)n * 4 + 4) ||
       ensure(ctx, ctx->w->n_aln, (size_t)n * 4 + 4) || ensure(ctx, ctx->w->aln_off, (size_t)(n + 1) * 8) ||
       ensure(ctx, ctx->w->slow_list, (size_t)n * 4 + 4) ||
       ensure(ctx, ctx->w->scan_tmp, scan_tmp_bytes((ctx->arena ? n : ctx->max_reads) + 1)) ||
@@ -760,6 +761,7 @@ enum : int32_t {
 // counters, 40 overflow flag (int32), 41 k_seed's read counter (int32)
 constexpr int kFlagSlot32 = 80;      // the flag, as an int32 index
 constexpr int kNextReadSlot32 = 82;  // k_seed's read counter (int32)
+constexpr int kChainBigSlot32 = 84;  // count of k_chain's large reads (int32)
 
 static int64_t scale(double per, int64_t n, int64_t slack) { return (int64_t)(per * (double)n) + slack; }
 
@@ -960,7 +962,7 @@ static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
                          P<int64_t>(w.intv_base), s);
   }
   if (timed) HIPCK(hipEventRecord(ctx->ev[1], s));
-  // ---- SA lookup and seed sort
+  // ---- SA lookup (the seeds are sorted by the chaining kernel)
   scan_i64(P<int64_t>(w.n_seed), P<int64_t>(w.seed_off), n, w.scan_tmp.p, w.scan_tmp.bytes, s);
   const int64_t *T_s = P<int64_t>(w.seed_off) + n;
   launch_cap_check(T_s, c.seeds, flag, kOvSeeds, s);
@@ -977,8 +979,6 @@ static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
   la.n_intv = P<int32_t>(w.n_intv);
   la.seeds = P<Seed>(w.seeds);
   launch_sa_lookup(la, s);
-  SortArgs so{flag, n, P<int64_t>(w.seed_off), P<Seed>(w.seeds), P<Seed>(w.seeds2)};
-  launch_sort(so, s);
   if (timed) HIPCK(hipEventRecord(ctx->ev[2], s));
   // ---- chaining
   ChainArgs ca{};
@@ -989,6 +989,7 @@ static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
   ca.off = P<int32_t>(ctx->off);
   ca.seed_off = P<int64_t>(w.seed_off);
   ca.seeds = P<Seed>(w.seeds);
+  ca.sort_tmp = P<Seed>(w.seeds2);
   ca.ch = P<ChainTmp>(w.ch);
   ca.nxt = P<int32_t>(w.nxt);
   ca.order = P<int32_t>(w.order);
@@ -999,6 +1000,8 @@ static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
   ca.tasks_tmp = P<Task>(w.tasks_tmp);
   ca.n_task = P<int32_t>(w.n_task);
   ca.error = P<int32_t>(w.counters) + 2 * 16;
+  ca.big = P<int32_t>(w.chain_big);
+  ca.n_big = P<int32_t>(w.counters) + kChainBigSlot32;
   launch_chain(ca, s);
   scan_i32(P<int32_t>(w.n_task), P<int64_t>(w.task_off), n, w.scan_tmp.p, w.scan_tmp.bytes, s);
   const int64_t *T_t = P<int64_t>(w.task_off) + n;
@@ -1753,7 +1756,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->own.intv, &ctx->own.n_intv, &ctx->own.n_seed, &ctx->own.slow, &ctx->own.intv_base, &ctx->own.seed_off,
                     &ctx->own.counters, &ctx->own.slow_list, &ctx->own.slow_scratch, &ctx->own.slow_n, &ctx->own.slow_off, &ctx->own.seeds,
                     &ctx->own.seeds2, &ctx->own.ch, &ctx->own.nxt, &ctx->own.order, &ctx->own.tmpi, &ctx->own.w16, &ctx->own.kept, &ctx->own.nodes,
-                    &ctx->own.tasks_tmp, &ctx->own.n_task, &ctx->own.task_off, &ctx->own.tasks, &ctx->own.regs, &ctx->own.flag, &ctx->own.sec,
+                    &ctx->own.tasks_tmp, &ctx->own.n_task, &ctx->own.task_off, &ctx->own.chain_big, &ctx->own.tasks, &ctx->own.regs, &ctx->own.flag, &ctx->own.sec,
                     &ctx->own.n_aln, &ctx->own.aln_off, &ctx->own.aln_reg, &ctx->own.aln_read, &ctx->own.aln_rank, &ctx->own.z_size,
                     &ctx->own.cig_size, &ctx->own.z_off, &ctx->own.cig_off, &ctx->own.z, &ctx->own.cigar, &ctx->own.alns, &ctx->own.scan_tmp,
                     &ctx->own.cig_words, &ctx->own.cig_coff, &ctx->own.cig_out, &ctx->own.seed_key, &ctx->own.seed_key2, &ctx->own.seed_idx,

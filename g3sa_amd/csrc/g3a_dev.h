@@ -955,6 +955,81 @@ struct LepBuf {
 template <class E>
 using GlobalBuf = LepBuf<GlobalList<E>, GlobalList<E>>;
 
+// The backward steps of rounds 1 and 2, one list entry after another.  A
+// step rebuilds its list in place (entry nc is written only after every
+// entry j >= nc has been read) and returns the new length, or -1 when a list
+// outgrows its capacity.  Other policies (a wavefront's lanes over the
+// entries) give the same lists.
+struct SerialBack {
+  // Round 1 at read position i (base c, -1 past the read or at an N): an
+  // entry that extends goes on, deduplicated by size; the first that does
+  // not, when none before it did, is a new start, stored in `fl` when the
+  // match is long enough to be emitted.
+  template <class L, class LF>
+  static G3A_HD int round1(const DevIndex &x, const Opt &o, const L &list, int np, int i, int c, const LF &fl,
+                           int &nf, int cap, int cap_f, int &last_start, int64_t *steps) {
+    using I = typename L::I;
+    using P = decltype(I::s);
+    int nc = 0;
+    P last_s = 0;  // size of the last entry written (read only when nc > 0)
+    for (int j = 0; j < np; ++j) {
+      const I pj = list.get(j);
+      I ok;
+      bool ext_ok = false;
+      if (c >= 0) {
+        backward_ext_ks(x, pj, c, ok);
+        if (steps) ++*steps;
+        ext_ok = ok.s >= 1;
+      }
+      if (!ext_ok) {
+        if (nc == 0 && i + 1 < last_start) {
+          last_start = i + 1;
+          if (pj.n - i >= o.min_seed_len) {
+            I p = pj;
+            p.m = i + 1;
+            if (nf == cap_f) return -1;
+            fl.put(nf++, p);
+          }
+        }
+      } else if (nc == 0 || ok.s != last_s) {
+        ok.m = pj.m;
+        ok.n = pj.n;
+        if (nc == cap) return -1;
+        list.put(nc++, ok);
+        last_s = ok.s;
+      }
+    }
+    return nc;
+  }
+  // Round 2 at pivot-side position m (base = read[m - 1]): an extension at
+  // least as large as the last kept one goes on; the first entry that falls
+  // short, when long enough, becomes a seed.
+  template <class L, class LF, class P>
+  static G3A_HD int reseed(const DevIndex &x, const Opt &o, const L &leps, int num_old, int m, int base,
+                           P min_seed_intv, const LF &seeds, int &ns, int cap_s, int64_t *steps) {
+    using I = typename L::I;
+    P curr = min_seed_intv;
+    bool collected = false;
+    int nl = 0;
+    for (int j = 0; j < num_old; ++j) {
+      I lep = leps.get(j), ext;
+      backward_ext_ks(x, lep, base, ext);
+      if (steps) ++*steps;
+      if (ext.s >= curr) {
+        ext.m = m - 1;
+        ext.n = lep.n;
+        leps.put(nl++, ext);
+        curr = ext.s;
+      } else if (!collected && lep.n - lep.m + 1 >= o.min_seed_len) {
+        if (ns == cap_s) return -1;
+        seeds.put(ns++, lep);
+        collected = true;
+      }
+    }
+    return nl;
+  }
+};
+
 // Round 1 (preseedAndFilterV2, bwa/src/seed.cu:25-104 with
 // preseedOnePos2Backward, seed.cu:454-531).  The reference keeps, for each
 // end n, the longest match read[m(n)..n] and outputs it when it is right
@@ -966,7 +1041,7 @@ using GlobalBuf = LepBuf<GlobalList<E>, GlobalList<E>>;
 // reference's.  Returns false when a LEP list outgrows `buf`.
 // kLearn = false compiles the learned-index form out (the FM-only kernel
 // keeps its registers); with kLearn it is used when x.saf / x.li_leaf are set.
-template <bool kLearn = true, class B>
+template <bool kLearn = true, class B, class BW = SerialBack>
 G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int len, IntvSink &out, B &buf,
                         int64_t *steps, int64_t *lc = nullptr) {
   using I = typename B::I;
@@ -976,7 +1051,7 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     if (q[x0] > 3) { ++x0; continue; }
     // the backward pass rebuilds the list in place: entry nc is written only
     // after entry j >= nc has been read
-    const auto &prev = buf.a, &curr = buf.a;
+    const auto &prev = buf.a;
     int np = 0;
     I ik;
     intv_init(x, ik, q[x0]);
@@ -1051,38 +1126,10 @@ G3A_HD bool smem_round1(const DevIndex &x, const Opt &o, const uint8_t *q, int l
     // backward search; SMEMs are found in decreasing start order
     int nf = 0, last_start = len + 1;  // start of the last match found (none yet)
     const auto &fl = buf.f;
-    P last_s = 0;  // size of the last entry written to curr (read only when nc > 0)
     for (i = x0 - 1; i >= -1; --i) {
-      int c = i < 0 ? -1 : (q[i] < 4 ? q[i] : -1);
-      int nc = 0;
-      for (int j = 0; j < np; ++j) {
-        const I pj = prev.get(j);
-        I ok;
-        bool ext_ok = false;
-        if (c >= 0) {
-          backward_ext_ks(x, pj, c, ok);
-          if (steps) ++*steps;
-          ext_ok = ok.s >= 1;
-        }
-        if (!ext_ok) {
-          // a new start; only matches long enough to be emitted are stored
-          if (nc == 0 && i + 1 < last_start) {
-            last_start = i + 1;
-            if (pj.n - i >= o.min_seed_len) {
-              I p = pj;
-              p.m = i + 1;
-              if (nf == buf.cap_f) return false;
-              fl.put(nf++, p);
-            }
-          }
-        } else if (nc == 0 || ok.s != last_s) {
-          ok.m = pj.m;
-          ok.n = pj.n;
-          if (nc == buf.cap) return false;
-          curr.put(nc++, ok);
-          last_s = ok.s;
-        }
-      }
+      const int c = i < 0 ? -1 : (q[i] < 4 ? q[i] : -1);
+      const int nc = BW::round1(x, o, prev, np, i, c, fl, nf, buf.cap, buf.cap_f, last_start, steps);
+      if (nc < 0) return false;
       if (nc == 0) break;
       np = nc;
     }
@@ -1134,7 +1181,7 @@ G3A_HD void smem_round1_per_end(const DevIndex &x, const Opt &o, const uint8_t *
 // Round 2, reseedV2 (bwa/src/seed.cu:639-913), for one SMEM.  `leps` and
 // `seeds` are scratch lists of capacities cap and cap_s.  Returns false on
 // overflow.
-template <class LA, class LF>
+template <class LA, class LF, class BW = SerialBack>
 G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int len, const Intv &smem,
                        const LA &leps, const LF &seeds, int cap, int cap_s, IntvSink &out, int64_t *steps) {
   using I = typename LA::I;
@@ -1175,27 +1222,10 @@ G3A_HD bool reseed_one(const DevIndex &x, const Opt &o, const uint8_t *q, int le
     leps.put(b, ta);
   }
   for (int m = pivot; m > 0; --m) {
-    int base = q[m - 1];
+    const int base = q[m - 1];
     if (base >= 4) break;
-    int num_old = nl;
-    P curr = min_seed_intv;
-    bool collected = false;
-    nl = 0;
-    for (int j = 0; j < num_old; ++j) {
-      I lep = leps.get(j), ext;
-      backward_ext_ks(x, lep, base, ext);
-      if (steps) ++*steps;
-      if (ext.s >= curr) {
-        ext.m = m - 1;
-        ext.n = lep.n;
-        leps.put(nl++, ext);
-        curr = ext.s;
-      } else if (!collected && lep.n - lep.m + 1 >= o.min_seed_len) {
-        if (ns == cap_s) return false;
-        seeds.put(ns++, lep);
-        collected = true;
-      }
-    }
+    nl = BW::reseed(x, o, leps, nl, m, base, min_seed_intv, seeds, ns, cap_s, steps);
+    if (nl < 0) return false;
     if (nl == 0) break;
   }
   if (nl > 0) {

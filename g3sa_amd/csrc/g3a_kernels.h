@@ -52,12 +52,6 @@ struct SaArgs {
   Seed *seeds;
 };
 
-struct SortArgs {
-  const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
-  int n_reads;
-  const int64_t *seed_off;
-  Seed *seeds, *tmp;
-};
 
 struct ChainArgs {
   const int32_t *stop;  // when *stop is set the kernel returns at once (null: never)
@@ -66,7 +60,8 @@ struct ChainArgs {
   int n_reads;
   const int32_t *off;
   const int64_t *seed_off;
-  const Seed *seeds;
+  Seed *seeds;     // as the SA lookup wrote them; sorted in place only by a read that runs on one thread
+  Seed *sort_tmp;  // sort_seeds scratch, one entry per seed
   ChainTmp *ch;
   int32_t *nxt, *order, *tmpi;
   uint16_t *w16;
@@ -75,6 +70,7 @@ struct ChainArgs {
   Task *tasks_tmp;
   int32_t *n_task;
   int32_t *error;
+  int32_t *big, *n_big;  // reads of more than 128 seeds and their count (zero before the launch)
 };
 
 struct CompactArgs {
@@ -189,7 +185,6 @@ void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, int block,
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, const int32_t *n_dev, int c0, Intv *scratch,
                       const int64_t *pool_off, Intv *pool, int pass, int from_scratch, hipStream_t s);
 void launch_sa_lookup(const SaArgs &a, hipStream_t s);
-void launch_sort(const SortArgs &a, hipStream_t s);
 void launch_chain(const ChainArgs &a, hipStream_t s);
 void launch_task_compact(const CompactArgs &a, hipStream_t s);
 void launch_extend(const ExtArgs &a, hipStream_t s);

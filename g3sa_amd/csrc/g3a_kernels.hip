@@ -2,11 +2,11 @@
 //
 // Stage kernels (one launch each, all on the context's stream):
 //   k_seed           thread per read: SMEM + reseed + third-round intervals
-//   k_seed_slow      thread per read, for reads whose lists outgrew the fast
-//                    path's private buffers (global-memory scratch)
+//   k_seed_slow      wavefront per read, for reads whose lists outgrew the
+//                    fast path's buffers (LDS lists, list entries on lanes)
 //   k_sa_lookup      thread per seed: SA interval row -> reference position
-//   k_sort_seeds     wavefront per read (seeds ranked across its lanes)
-//   k_chain          thread per read: B-tree chaining, chain filter, tasks
+//   k_chain          wavefront per read: seed sort, chaining, chain filter,
+//                    tasks (lane 0 in global memory for the rare large read)
 //   k_task_compact   thread per read: per-read task lists -> dense task array
 //   k_extend         one wavefront per task: banded local extension, left
 //                    then right, anti-diagonal sweep over 64-row tiles
@@ -32,6 +32,50 @@ __device__ __forceinline__ int64_t dcount(const int64_t *dev, int64_t cap) {
   const int64_t v = *dev;
   return v < cap ? v : cap;
 }
+
+// Register arrays of a wavefront: element e of an array of NC chunks lives
+// in chunk e / 64, lane e % 64.
+template <class T>
+__device__ __forceinline__ T lane_read(T v, int l) {  // v of lane l (l wave-uniform)
+  if constexpr (sizeof(T) == 8) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return (T)((uint64_t)hi << 32 | lo);
+  } else {
+    return (T)__builtin_amdgcn_readlane((int)v, l);
+  }
+}
+template <int NC, class T>
+__device__ __forceinline__ T elem(const T (&v)[NC], int e) {  // e wave-uniform
+  T x = lane_read(v[0], e & 63);
+#pragma unroll
+  for (int k = 1; k < NC; ++k)
+    if ((e >> 6) == k) x = lane_read(v[k], e & 63);
+  return x;
+}
+template <int NC, class T>
+__device__ __forceinline__ void set_elem(T (&v)[NC], int e, T x, int lane) {  // e wave-uniform
+#pragma unroll
+  for (int k = 0; k < NC; ++k)
+    if (e == 64 * k + lane) v[k] = x;
+}
+template <int NC, class T>
+__device__ __forceinline__ T gather(const T (&v)[NC], int e) {  // e per lane, every lane active
+  T x = __shfl(v[0], e & 63);
+#pragma unroll
+  for (int k = 1; k < NC; ++k) {
+    const T y = __shfl(v[k], e & 63);
+    if ((e >> 6) == k) x = y;
+  }
+  return x;
+}
+__device__ __forceinline__ void wave_lds_sync() {  // this wavefront's LDS stores are seen by its lanes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 
 // ---------------------------------------------------------------------------
 // seeding
@@ -162,23 +206,166 @@ __global__ void __launch_bounds__(256) k_seed_keys(const uint8_t *seq, const int
   idx[r] = r;
 }
 
-// Slow path for the reads listed in `list`: global-memory LEP lists of
-// kSlowLep entries, which no read of <= 320 bases can outgrow.  Pass 0 counts
-// and keeps the first 2 * kSlowLep intervals in the read's scratch; pass 1
-// puts all intervals at the pool offsets computed from the counts, copying
-// them from the scratch when it still holds them all (`from_scratch`: every
-// slow read had its own scratch in pass 0) and searching again otherwise.
+// The backward steps of rounds 1 and 2 on a wavefront: entry j on lane
+// j % 64 of chunk j / 64 extends in parallel with the others, then one pass
+// in list order over the lanes' results takes the decisions that depend on
+// the entries before (which entries go on, the new start, the seed), and the
+// entries that go on are written back in order.  The lists are in LDS and
+// every lane sees the same list; the results equal SerialBack's.
+struct WaveBack {
+  static constexpr int kChunks = (kMaxReadLen + 63) / 64;  // no list of a read holds more entries
+  template <class L, class LF>
+  static __device__ int round1(const DevIndex &x, const Opt &o, const L &list, int np, int i, int c, const LF &fl,
+                               int &nf, int cap, int cap_f, int &last_start, int64_t *) {
+    using I = typename L::I;
+    const int lane = threadIdx.x & 63;
+    I ok[kChunks];
+    int pm[kChunks], pn[kChunks];
+    uint64_t ext[kChunks], fail[kChunks];
+    if (np > 64 * kChunks) return -1;  // (no read of <= kMaxReadLen bases gets here)
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+      const int j = 64 * k + lane;
+      const bool live = j < np;
+      bool e = false;
+      pm[k] = pn[k] = 0;
+      if (live) {
+        const I pj = list.get(j);
+        pm[k] = pj.m;
+        pn[k] = pj.n;
+        if (c >= 0) {
+          backward_ext_ks(x, pj, c, ok[k]);
+          e = ok[k].s >= 1;
+        }
+      }
+      ext[k] = __ballot(live && e);
+      fail[k] = __ballot(live && !e);
+    }
+    int first_ext = np, first_fail = np;
+#pragma unroll
+    for (int k = kChunks - 1; k >= 0; --k) {
+      if (ext[k]) first_ext = 64 * k + __ffsll((unsigned long long)ext[k]) - 1;
+      if (fail[k]) first_fail = 64 * k + __ffsll((unsigned long long)fail[k]) - 1;
+    }
+    // a new start: the first entry that does not extend, none before it did
+    if (first_fail < first_ext && i + 1 < last_start) {
+      last_start = i + 1;
+      I p = list.get(first_fail);
+      if (p.n - i >= o.min_seed_len) {
+        p.m = i + 1;
+        if (nf == cap_f) return -1;
+        fl.put(nf++, p);  // every lane stores the same entry
+      }
+    }
+    // the entries that go on: each extension whose size differs from the
+    // last one kept
+    uint64_t keep[kChunks];
+    int nc = 0;
+    decltype(I::s) last_s = 0;
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+      keep[k] = 0;
+      for (uint64_t m = ext[k]; m; m &= m - 1) {
+        const int b = __ffsll((unsigned long long)m) - 1;
+        const auto sb = lane_read(ok[k].s, b);
+        if (nc == 0 || sb != last_s) {
+          keep[k] |= 1ull << b;
+          ++nc;
+          last_s = sb;
+        }
+      }
+    }
+    if (nc > cap) return -1;
+    wave_lds_sync();  // every entry was read above
+    int before = 0;
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+      if ((keep[k] >> lane) & 1) {
+        I e = ok[k];
+        e.m = pm[k];
+        e.n = pn[k];
+        list.put(before + __popcll(keep[k] & ((1ull << lane) - 1)), e);
+      }
+      before += __popcll(keep[k]);
+    }
+    wave_lds_sync();
+    return nc;
+  }
+  template <class L, class LF, class P>
+  static __device__ int reseed(const DevIndex &x, const Opt &o, const L &leps, int num_old, int m, int base,
+                               P min_seed_intv, const LF &seeds, int &ns, int cap_s, int64_t *) {
+    using I = typename L::I;
+    const int lane = threadIdx.x & 63;
+    I ex[kChunks];
+    int ln[kChunks];
+    if (num_old > 64 * kChunks) return -1;  // (no read of <= kMaxReadLen bases gets here)
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+      const int j = 64 * k + lane;
+      ln[k] = 0;
+      if (j < num_old) {
+        const I lep = leps.get(j);
+        backward_ext_ks(x, lep, base, ex[k]);
+        ex[k].m = m - 1;
+        ex[k].n = lep.n;
+        ln[k] = lep.n - lep.m + 1;
+      }
+    }
+    P curr = min_seed_intv;
+    bool collected = false;
+    int nl = 0;
+    uint64_t keep[kChunks];
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+      keep[k] = 0;
+      const int n = num_old - 64 * k < 64 ? num_old - 64 * k : 64;
+      for (int b = 0; b < n; ++b) {
+        const P sb = lane_read(ex[k].s, b);
+        if (sb >= curr) {
+          keep[k] |= 1ull << b;
+          ++nl;
+          curr = sb;
+        } else if (!collected && lane_read(ln[k], b) >= o.min_seed_len) {
+          if (ns == cap_s) return -1;
+          seeds.put(ns++, leps.get(64 * k + b));  // every lane stores the same entry
+          collected = true;
+        }
+      }
+    }
+    wave_lds_sync();  // every entry was read above
+    int before = 0;
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k) {
+      if ((keep[k] >> lane) & 1) leps.put(before + __popcll(keep[k] & ((1ull << lane) - 1)), ex[k]);
+      before += __popcll(keep[k]);
+    }
+    wave_lds_sync();
+    return nl;
+  }
+};
+
+// Slow path for the reads listed in `list`, one wavefront per read: LEP
+// lists of kSlowLep entries in LDS, which no read of <= 320 bases can
+// outgrow, the backward steps on the wavefront's lanes (WaveBack) and the
+// rest of the search on every lane alike.  Pass 0 counts and keeps the first
+// 2 * kSlowLep intervals in the read's scratch; pass 1 puts all intervals at
+// the pool offsets computed from the counts, copying them from the scratch
+// when it still holds them all (`from_scratch`: every slow read had its own
+// scratch in pass 0) and searching again otherwise.
 __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *list, int n_list, const int32_t *n_dev,
                                                   int c0, Intv *scratch, const int64_t *pool_off, Intv *pool, int pass,
                                                   int from_scratch) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ LepWide lds_a[kSlowLep], lds_f[kSlowLep];
+  const int t = blockIdx.x, lane = threadIdx.x;
   int nv = n_list;
   if (n_dev) {
     const int m = *n_dev - c0;
     nv = m < 0 ? 0 : m < n_list ? m : n_list;
   }
-  if (t >= nv) return;
+  if (t >= nv) return;  // the whole wavefront
   const int r = list[t];
   const int off = a.off[r], len = a.off[r + 1] - off;
   const uint8_t *q = a.seq + off;
@@ -186,27 +373,29 @@ __global__ void __launch_bounds__(64) k_seed_slow(SeedArgs a, const int32_t *lis
   if (pass && from_scratch && a.n_intv[r] <= 2 * kSlowLep) {
     const Intv *src = sc + 3 * kSlowLep;
     Intv *dst = pool + pool_off[t];
-    for (int k = 0, n = a.n_intv[r]; k < n; ++k) dst[k] = src[k];
+    for (int k = lane, n = a.n_intv[r]; k < n; k += 64) dst[k] = src[k];
     return;
   }
   Intv *slot = pass ? pool + pool_off[t] : sc + 3 * kSlowLep;
   const int cap = pass ? (int)(pool_off[t + 1] - pool_off[t]) : 2 * kSlowLep;
-  IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};
-  GlobalBuf<LepWide> buf{GlobalList<LepWide>{reinterpret_cast<LepWide *>(sc)},
-                         GlobalList<LepWide>{reinterpret_cast<LepWide *>(sc) + kSlowLep}, kSlowLep, kSlowLep};
-  if (!smem_round1(a.idx, a.opt, q, len, sink, buf, nullptr)) {
+  IntvSink sink{slot, cap, 0, 0, a.opt.max_occ};  // every lane pushes the same intervals
+  GlobalBuf<LepWide> buf{GlobalList<LepWide>{lds_a}, GlobalList<LepWide>{lds_f}, kSlowLep, kSlowLep};
+  if (!smem_round1<true, GlobalBuf<LepWide>, WaveBack>(a.idx, a.opt, q, len, sink, buf, nullptr)) {
     sink.n = 0;
     sink.n_seed = 0;
     smem_round1_per_end(a.idx, a.opt, q, len, sink, nullptr);
   }
   const int nsm = sink.n;
   for (int k = 0; k < nsm; ++k) {
-    Intv smem = slot[k];
-    reseed_one(a.idx, a.opt, q, len, smem, buf.a, buf.f, kSlowLep, kSlowLep, sink, nullptr);
+    const Intv smem = slot[k];
+    reseed_one<GlobalList<LepWide>, GlobalList<LepWide>, WaveBack>(a.idx, a.opt, q, len, smem, buf.a, buf.f,
+                                                                    kSlowLep, kSlowLep, sink, nullptr);
   }
   reseed_round3<int64_t>(a.idx, a.opt, q, len, sink, nullptr);
-  a.n_intv[r] = sink.n;
-  a.n_seed[r] = sink.n_seed;
+  if (lane == 0) {
+    a.n_intv[r] = sink.n;
+    a.n_seed[r] = sink.n_seed;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -251,50 +440,24 @@ __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
   a.seeds[s] = out;
 }
 
-// One wavefront per read: with at most 64 seeds each lane holds one, counts
-// the seeds that sort before it (seed_less, ties by input position: the
-// stable order sort_seeds gives) and stores it at that rank; a read with
-// more seeds is sorted by lane 0 with sort_seeds.
-__global__ void __launch_bounds__(256) k_sort_seeds(SortArgs a) {
-  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  const int lane = threadIdx.x & 63;
-  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (r >= a.n_reads) return;  // the whole wavefront
-  const int64_t b = a.seed_off[r];
-  const int n = (int)(a.seed_off[r + 1] - b);
-  if (n <= 1) return;
-  Seed *sd = a.seeds + b;
-  if (n > 64) {
-    if (lane == 0) sort_seeds(sd, n, a.tmp + b);
-    return;
-  }
-  Seed v{};
-  uint32_t key = 0xffffffffu;  // past the seeds: sorts last
-  if (lane < n) {
-    v = sd[lane];
-    key = (uint32_t)v.qbeg << 16 | (uint32_t)v.len;  // qbeg, len < 2^16 (reads of at most 320 bases)
-  }
-  int rank = 0;
-  for (int i = 0; i < n; ++i) {
-    const uint32_t ki = (uint32_t)__shfl((int)key, i);
-    rank += ki < key || (ki == key && i < lane);
-  }
-  if (lane < n) sd[rank] = v;
-}
+// ---------------------------------------------------------------------------
+// Seed sort + chaining + chain filter + per-read task list
+// (BTreeChaining and CHAINFILTERING, bwa/src/aux.cu:1755-1988; the seed
+// order of sortSeeds, aux.cu:1696-1752)
+// ---------------------------------------------------------------------------
 
-// ---------------------------------------------------------------------------
-// chaining + chain filter + per-read task list
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_chain(ChainArgs a) {
-  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.n_reads) return;
+// One read on one thread, in global memory: sort_seeds, then the B-tree
+// chaining, the filter and the read's tasks.  The wavefront kernel below
+// hands a read here when it has more seeds than its registers hold, or when
+// its B-tree would differ from a sorted array (see there).
+__device__ void chain_one(const ChainArgs &a, int r) {
   const int64_t b = a.seed_off[r];
   const int ns = (int)(a.seed_off[r + 1] - b);
   a.n_task[r] = 0;
   if (ns == 0) return;
   const int l_seq = a.off[r + 1] - a.off[r];
-  const Seed *s = a.seeds + b;
+  Seed *s = a.seeds + b;
+  if (ns > 1) sort_seeds(s, ns, a.sort_tmp + b);
   ChainTmp *ch = a.ch + b;
   int32_t *nxt = a.nxt + b, *order = a.order + b;
   const int64_t nb0 = (b + 32 * (int64_t)r) / 4, nb1 = (a.seed_off[r + 1] + 32 * (int64_t)(r + 1)) / 4;
@@ -324,6 +487,364 @@ __global__ void __launch_bounds__(128) k_chain(ChainArgs a) {
     }
   }
   a.n_task[r] = t;
+}
+
+constexpr int kChainWaveSeeds = 128;  // most seeds a read may have on the wavefront path
+struct ChainStage {                   // one wavefront's LDS: the permutations of the sorts
+  int64_t rbeg[kChainWaveSeeds];
+  int32_t qbeg[kChainWaveSeeds], len[kChainWaveSeeds], rid[kChainWaveSeeds];
+};
+
+// One read of ns <= 64 * NC seeds on one wavefront.  Returns false, having
+// written nothing, when the read must take chain_one instead.
+//
+// The reference's B-tree of chains (t = 6, up to 11 keys a node) is, while it
+// has one node, a sorted array in which a key is inserted after the first
+// equal key and the lookup returns the first equal key or else the last
+// smaller one; with more nodes and distinct keys its lookups and in-order
+// traversal are those of the sorted array.  Both hold here: chain positions
+// sit in a lane array in that order, and a read whose chains repeat a
+// position once there are more than 11 of them goes to chain_one.
+template <int NC>
+__device__ bool chain_wave(const ChainArgs &a, int r, int64_t b, int ns, int lane, ChainStage &st) {
+  const Opt &o = a.opt;
+  const int64_t l_pac = a.idx.l_pac;
+  const Seed *sd = a.seeds + b;
+  // ---- sort: seed_less, ties by input position (the stable order of sort_seeds)
+  {
+    Seed v[NC];
+    uint32_t key[NC];
+    int rank[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int e = 64 * k + lane;
+      key[k] = 0xffffffffu;
+      rank[k] = 0;
+      if (e < ns) {
+        v[k] = sd[e];
+        key[k] = (uint32_t)v[k].qbeg << 16 | (uint32_t)v[k].len;  // qbeg, len < 2^16
+      }
+    }
+    for (int i = 0; i < ns; ++i) {
+      const uint32_t ki = elem<NC>(key, i);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) rank[k] += ki < key[k] || (ki == key[k] && i < 64 * k + lane);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (64 * k + lane < ns) {
+        st.rbeg[rank[k]] = v[k].rbeg;
+        st.qbeg[rank[k]] = v[k].qbeg;
+        st.len[rank[k]] = v[k].len;
+        st.rid[rank[k]] = v[k].rid;
+      }
+  }
+  wave_lds_sync();
+  int64_t s_rb[NC];
+  int s_qb[NC], s_ln[NC], s_rid[NC], s_nxt[NC], s_ch[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int e = 64 * k + lane;
+    const bool in = e < ns;
+    s_rb[k] = in ? st.rbeg[e] : 0;
+    s_qb[k] = in ? st.qbeg[e] : 0;
+    s_ln[k] = in ? st.len[e] : 0;
+    s_rid[k] = in ? st.rid[e] : 0;
+    s_nxt[k] = -1;
+    s_ch[k] = -1;
+  }
+  wave_lds_sync();  // the staging is reused below
+  // ---- chaining (BTreeChaining, aux.cu:1755-1825): chains by creation, and
+  // their positions in B-tree order
+  int64_t c_pos[NC], t_pos[NC];
+  int c_rid[NC], c_n[NC], c_head[NC], c_tail[NC], t_cid[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    c_pos[k] = t_pos[k] = 0;
+    c_rid[k] = c_n[k] = c_head[k] = c_tail[k] = t_cid[k] = 0;
+  }
+  int nch = 0;
+  bool dup = false;
+  for (int i = 0; i < ns; ++i) {
+    const int64_t prb = elem<NC>(s_rb, i);
+    const int pqb = elem<NC>(s_qb, i), pln = elem<NC>(s_ln, i), prid = elem<NC>(s_rid, i);
+    int n_lt = 0, first_eq = -1;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const bool live = 64 * k + lane < nch;
+      const uint64_t lt = __ballot(live && t_pos[k] < prb), eq = __ballot(live && t_pos[k] == prb);
+      n_lt += __popcll(lt);
+      if (first_eq < 0 && eq) first_eq = 64 * k + __ffsll((unsigned long long)eq) - 1;
+    }
+    if (nch) {
+      const int low = first_eq >= 0 ? first_eq : n_lt - 1;
+      if (low >= 0) {
+        // test_and_merge (aux.cu:54-84) against chain c
+        const int c = elem<NC>(t_cid, low);
+        const int ct = elem<NC>(c_tail, c), chd = elem<NC>(c_head, c);
+        const int64_t last_rb = elem<NC>(s_rb, ct), first_rb = elem<NC>(s_rb, chd);
+        const int last_qb = elem<NC>(s_qb, ct), last_ln = elem<NC>(s_ln, ct), first_qb = elem<NC>(s_qb, chd);
+        const int64_t qend = last_qb + last_ln, rend = last_rb + last_ln;
+        bool merged = false;
+        if (prid == elem<NC>(c_rid, c)) {
+          if (pqb >= first_qb && pqb + pln <= qend && prb >= first_rb && prb + pln <= rend) {
+            merged = true;  // contained in the chain: dropped
+          } else if (!((last_rb < l_pac || first_rb < l_pac) && prb >= l_pac)) {
+            const int64_t xd = pqb - last_qb, yd = prb - last_rb;
+            if (yd >= 0 && xd - yd <= o.w && yd - xd <= o.w && xd - last_ln < o.max_chain_gap &&
+                yd - last_ln < o.max_chain_gap) {
+              set_elem<NC>(s_nxt, ct, i, lane);
+              set_elem<NC>(s_ch, i, c, lane);
+              set_elem<NC>(c_tail, c, i, lane);
+              set_elem<NC>(c_n, c, elem<NC>(c_n, c) + 1, lane);
+              merged = true;
+            }
+          }
+        }
+        if (merged) continue;
+      }
+    }
+    if (prid < 0 || prid >= 455) continue;  // bwa/src/aux.cu:1799
+    set_elem<NC>(c_pos, nch, prb, lane);
+    set_elem<NC>(c_rid, nch, prid, lane);
+    set_elem<NC>(c_n, nch, 1, lane);
+    set_elem<NC>(c_head, nch, i, lane);
+    set_elem<NC>(c_tail, nch, i, lane);
+    set_elem<NC>(s_ch, i, nch, lane);
+    // into the ordered positions: after the first equal one, else before
+    // the first larger one; the slots above move up by one
+    const int ins = first_eq >= 0 ? first_eq + 1 : n_lt;
+    dup |= first_eq >= 0;
+#pragma unroll
+    for (int k = NC - 1; k >= 0; --k) {
+      int64_t pp = __shfl_up(t_pos[k], 1);
+      int pc = __shfl_up(t_cid[k], 1);
+      if (k > 0 && lane == 0) {
+        pp = lane_read(t_pos[k - 1], 63);
+        pc = lane_read(t_cid[k - 1], 63);
+      }
+      const int e = 64 * k + lane;
+      if (e > ins && e <= nch) {
+        t_pos[k] = pp;
+        t_cid[k] = pc;
+      } else if (e == ins) {
+        t_pos[k] = prb;
+        t_cid[k] = nch;
+      }
+    }
+    ++nch;
+    if (dup && nch > kKbN) return false;  // a B-tree of several nodes with repeated keys
+  }
+  if (nch == 0) {
+    if (lane == 0) a.n_task[r] = 0;
+    return true;
+  }
+  // ---- chain filter (sortChainsDecreasingWeight + CHAINFILTERING_filter_kernel,
+  // aux.cu:1828-1988): weights by a walk of each chain's seeds, one chain a lane
+  int c_w[NC], c_b[NC], c_e[NC];
+  {
+    int j[NC], wq[NC], wr[NC];
+    int64_t endq[NC], endr[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const bool live = 64 * k + lane < nch;
+      j[k] = live ? c_head[k] : -1;
+      wq[k] = wr[k] = 0;
+      endq[k] = endr[k] = 0;
+      c_b[k] = (uint16_t)gather<NC>(s_qb, live ? c_head[k] : 0);
+      const int tl = live ? c_tail[k] : 0;
+      c_e[k] = (uint16_t)(gather<NC>(s_qb, tl) + gather<NC>(s_ln, tl));
+    }
+    for (;;) {
+      bool more = false;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) more |= j[k] >= 0;
+      if (!__any(more)) break;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        const int jj = j[k] >= 0 ? j[k] : 0;
+        const int q = gather<NC>(s_qb, jj), l = gather<NC>(s_ln, jj), nx = gather<NC>(s_nxt, jj);
+        const int64_t rb = gather<NC>(s_rb, jj);
+        if (j[k] >= 0) {
+          if (q >= endq[k]) wq[k] += l;
+          else if (q + l > endq[k]) wq[k] += (int)(q + l - endq[k]);
+          endq[k] = endq[k] > q + l ? endq[k] : q + l;
+          if (rb >= endr[k]) wr[k] += l;
+          else if (rb + l > endr[k]) wr[k] += (int)(rb + l - endr[k]);
+          endr[k] = endr[k] > rb + l ? endr[k] : rb + l;
+          j[k] = nx;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      int w = wr[k] < wq[k] ? wr[k] : wq[k];
+      w = w < 1 << 30 ? w : (1 << 30) - 1;
+      c_w[k] = (uint16_t)w;  // the filter's 16-bit weights
+    }
+  }
+  // B-tree order, then a stable sort by decreasing weight (all of a read's
+  // chains: at most 128 here)
+  int o_cid[NC], o_w[NC], o_be[NC];
+  {
+    int rank[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int cid = t_cid[k];
+      o_cid[k] = cid;
+      o_w[k] = 64 * k + lane < nch ? gather<NC>(c_w, cid) : -1;
+      o_be[k] = (int)((uint32_t)gather<NC>(c_b, cid) | (uint32_t)gather<NC>(c_e, cid) << 16);
+      rank[k] = 0;
+    }
+    for (int i = 0; i < nch; ++i) {
+      const int wi = elem<NC>(o_w, i);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) rank[k] += wi > o_w[k] || (wi == o_w[k] && i < 64 * k + lane);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (64 * k + lane < nch) {
+        st.qbeg[rank[k]] = o_cid[k];
+        st.len[rank[k]] = o_w[k];
+        st.rid[rank[k]] = o_be[k];
+      }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int e = 64 * k + lane;
+      if (e < nch) {
+        o_cid[k] = st.qbeg[e];
+        o_w[k] = st.len[e];
+        o_be[k] = st.rid[e];
+      }
+    }
+    wave_lds_sync();
+  }
+  // the filter: chain i is dropped iff an earlier kept chain j overlaps it by
+  // >= mask_level of the shorter and outweighs it (no ALT contigs)
+  uint64_t kept[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) kept[k] = 0;
+  for (int i = 0; i < nch; ++i) {
+    const int be = elem<NC>(o_be, i), wi = elem<NC>(o_w, i);
+    const uint16_t bi = (uint16_t)(be & 0xffff), ei = (uint16_t)((uint32_t)be >> 16);
+    bool drop = false;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      bool hit = false;
+      if (64 * k + lane < i && ((kept[k] >> lane) & 1)) {
+        const uint16_t bj = (uint16_t)(o_be[k] & 0xffff), ej = (uint16_t)((uint32_t)o_be[k] >> 16);
+        const int wj = o_w[k];
+        const int b_max = bj > bi ? bj : bi, e_min = ej < ei ? ej : ei;
+        if (e_min > b_max) {
+          const int li = ei - bi, lj = ej - bj, min_l = li < lj ? li : lj;
+          if (e_min - b_max >= min_l * o.mask_level && min_l < o.max_chain_gap)
+            hit = (uint16_t)wi < (uint16_t)wj * o.drop_ratio && (uint16_t)wj - (uint16_t)wi >= o.min_seed_len << 1;
+        }
+      }
+      drop |= __ballot(hit) != 0;
+    }
+    if (!drop) kept[i >> 6] |= 1ull << (i & 63);
+  }
+  // ---- tasks: kept chains in order, each chain's seeds in chain order
+  // (ascending seed index) with the chain's reference window
+  // (ExtendingPairGenerate, aux.cu:2101-2192)
+  int64_t s_lo[NC], s_hi[NC];
+  const int l_seq = a.off[r + 1] - a.off[r];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int q = s_qb[k], l = s_ln[k];
+    s_lo[k] = s_rb[k] - (q + cal_max_gap(o, q));
+    s_hi[k] = s_rb[k] + l + ((l_seq - q - l) + cal_max_gap(o, l_seq - q - l));
+  }
+  Task *out = a.tasks_tmp + b;
+  int t = 0;
+#pragma unroll
+  for (int kk = 0; kk < NC; ++kk) {
+    uint64_t m = kept[kk];
+    while (m) {
+      const int i = 64 * kk + __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const int c = elem<NC>(o_cid, i);
+      int64_t rmax0 = l_pac << 1, rmax1 = 0;
+      for (int j = elem<NC>(c_head, c); j >= 0; j = elem<NC>(s_nxt, j)) {
+        const int64_t lo = elem<NC>(s_lo, j), hi = elem<NC>(s_hi, j);
+        rmax0 = rmax0 < lo ? rmax0 : lo;
+        rmax1 = rmax1 > hi ? rmax1 : hi;
+      }
+      rmax0 = rmax0 > 0 ? rmax0 : 0;
+      rmax1 = rmax1 < l_pac << 1 ? rmax1 : l_pac << 1;
+      const int64_t mid = elem<NC>(s_rb, elem<NC>(c_head, c));
+      if (rmax0 < l_pac && l_pac < rmax1) {
+        if (mid < l_pac) rmax1 = l_pac;
+        else rmax0 = l_pac;
+      }
+      int is_rev;
+      const int rid = pos2rid(a.idx, depos(a.idx, mid, is_rev));
+      int64_t far_beg = a.idx.ctg_off[rid], far_end = far_beg + a.idx.ctg_len[rid];
+      if (is_rev) {
+        const int64_t tb = far_beg;
+        far_beg = (l_pac << 1) - far_end;
+        far_end = (l_pac << 1) - tb;
+      }
+      rmax0 = rmax0 > far_beg ? rmax0 : far_beg;
+      rmax1 = rmax1 < far_end ? rmax1 : far_end;
+      int before = 0;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        const uint64_t in = __ballot(s_ch[k] == c);
+        if (s_ch[k] == c) {
+          Task tk;
+          tk.rbeg = s_rb[k];
+          tk.rmax0 = rmax0;
+          tk.rmax1 = rmax1;
+          tk.qbeg = s_qb[k];
+          tk.len = s_ln[k];
+          tk.read = r;
+          tk.rid = rid;
+          out[t + before + __popcll(in & ((1ull << lane) - 1))] = tk;
+        }
+        before += __popcll(in);
+      }
+      t += before;
+    }
+  }
+  if (lane == 0) a.n_task[r] = t;
+  return true;
+}
+
+// Reads of more seeds than the wavefront path holds, listed so that k_chain
+// starts them first: each runs on one lane for far longer than a wavefront
+// read, and started in read order the last of them would be the kernel's
+// tail.
+__global__ void k_chain_big(const int32_t *stop, int n_reads, const int64_t *seed_off, int32_t *list,
+                            int32_t *n_list) {
+  if (stop && *stop) return;  // a capacity check before this kernel failed: the batch is redone
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n_reads && seed_off[r + 1] - seed_off[r] > kChainWaveSeeds) list[atomicAdd(n_list, 1)] = r;
+}
+
+// One wavefront per read (four a block).  Wavefront g first takes the g-th
+// listed large read (chain_one on lane 0), then read g unless it is a listed
+// one; reads that chain_wave hands back also run chain_one on lane 0.
+__global__ void __launch_bounds__(256) k_chain(ChainArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
+  __shared__ ChainStage stage[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+  if (r >= a.n_reads) return;  // the whole wavefront
+  if (r < *a.n_big && lane == 0) chain_one(a, a.big[r]);
+  const int64_t b = a.seed_off[r];
+  const int ns = __builtin_amdgcn_readfirstlane((int)(a.seed_off[r + 1] - b));
+  bool done = true;
+  if (ns == 0) {
+    if (lane == 0) a.n_task[r] = 0;
+  } else if (ns <= 64) {
+    done = chain_wave<1>(a, r, b, ns, lane, stage[wv]);
+  } else if (ns <= kChainWaveSeeds) {
+    done = chain_wave<2>(a, r, b, ns, lane, stage[wv]);
+  }  // else: listed
+  if (!done && lane == 0) chain_one(a, r);
 }
 
 __global__ void __launch_bounds__(128) k_task_compact(CompactArgs a) {
@@ -1125,17 +1646,17 @@ void launch_seed(const SeedArgs &a, int n_threads, int blocks_per_cu, int block,
 void launch_seed_slow(const SeedArgs &a, const int32_t *list, int n_list, const int32_t *n_dev, int c0, Intv *scratch,
                       const int64_t *pool_off, Intv *pool, int pass, int from_scratch, hipStream_t s) {
   if (n_list)
-    hipLaunchKernelGGL(k_seed_slow, dim3(blocks(n_list, 64)), dim3(64), 0, s, a, list, n_list, n_dev, c0, scratch,
-                       pool_off, pool, pass, from_scratch);
+    hipLaunchKernelGGL(k_seed_slow, dim3(n_list), dim3(64), 0, s, a, list, n_list, n_dev, c0, scratch, pool_off, pool,
+                       pass, from_scratch);
 }
 void launch_sa_lookup(const SaArgs &a, hipStream_t s) {
   if (a.total) hipLaunchKernelGGL(k_sa_lookup, dim3(blocks(a.total, 256)), dim3(256), 0, s, a);
 }
-void launch_sort(const SortArgs &a, hipStream_t s) {
-  if (a.n_reads) hipLaunchKernelGGL(k_sort_seeds, dim3(blocks(a.n_reads, 4)), dim3(256), 0, s, a);
-}
 void launch_chain(const ChainArgs &a, hipStream_t s) {
-  if (a.n_reads) hipLaunchKernelGGL(k_chain, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);
+  if (!a.n_reads) return;
+  hipLaunchKernelGGL(k_chain_big, dim3(blocks(a.n_reads, 256)), dim3(256), 0, s, a.stop, a.n_reads, a.seed_off,
+                     a.big, a.n_big);
+  hipLaunchKernelGGL(k_chain, dim3(blocks(a.n_reads, 4)), dim3(256), 0, s, a);
 }
 void launch_task_compact(const CompactArgs &a, hipStream_t s) {
   if (a.n_reads) hipLaunchKernelGGL(k_task_compact, dim3(blocks(a.n_reads, 128)), dim3(128), 0, s, a);

@@ -272,9 +272,16 @@ def test_slow_seeding_path_parity(built, oracle, tmp_path):
     seq, off = _batch(reads)
     got, _ = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats")
     assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path"
-    got, _ = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats, wide layout", wide=True,
-                      variants=VARIANTS[:2])
+    got, ref = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats, wide layout", wide=True,
+                        variants=VARIANTS[:2])
     assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path (wide layout)"
+    # the slow path of the learned-index form (text runs, k-mer and model lookups)
+    with g3sa_amd.Aligner(str(prefix), max_reads=len(off) - 1, max_bases=int(off[-1])) as al:
+        al.set_learned(1)
+        print("[parity] tandem-repeats, learned form", flush=True)
+        got = al.align(seq, off)
+        compare(got, ref, "tandem-repeats, learned form")
+        assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path (learned form)"
 
 
 @pytest.mark.parametrize("which", ["c1", "c3", "rep"])
