@@ -446,27 +446,20 @@ __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
 // order of sortSeeds, aux.cu:1696-1752)
 // ---------------------------------------------------------------------------
 
-// One read on one thread, in global memory: sort_seeds, then the B-tree
-// chaining, the filter and the read's tasks.  The wavefront kernel below
-// hands a read here when it has more seeds than its registers hold, or when
-// its B-tree would differ from a sorted array (see there).
-__device__ void chain_one(const ChainArgs &a, int r) {
-  const int64_t b = a.seed_off[r];
-  const int ns = (int)(a.seed_off[r + 1] - b);
-  a.n_task[r] = 0;
-  if (ns == 0) return;
+// One read on one thread from its sorted seeds `s`: the B-tree chaining, the
+// filter and the read's tasks, with the temporaries at the pointers given
+// (global slices, or the thread's LDS slice in k_chain_small).
+__device__ __forceinline__ void chain_read(const ChainArgs &a, int r, int64_t b, const Seed *s, int ns, ChainTmp *ch,
+                                           int32_t *nxt, int32_t *order, KbNode *nodes, int cap_nodes,
+                                           uint16_t *w16, int32_t *tmpi, uint8_t *kept) {
   const int l_seq = a.off[r + 1] - a.off[r];
-  Seed *s = a.seeds + b;
-  if (ns > 1) sort_seeds(s, ns, a.sort_tmp + b);
-  ChainTmp *ch = a.ch + b;
-  int32_t *nxt = a.nxt + b, *order = a.order + b;
-  const int64_t nb0 = (b + 32 * (int64_t)r) / 4, nb1 = (a.seed_off[r + 1] + 32 * (int64_t)(r + 1)) / 4;
-  int nch = chain_build(a.idx, a.opt, s, ns, ch, nxt, a.nodes + nb0, (int)(nb1 - nb0), order);
-  if (nch < 0) {  // cannot happen with the arena sizing above; reported as an error
+  int nch = chain_build(a.idx, a.opt, s, ns, ch, nxt, nodes, cap_nodes, order);
+  if (nch < 0) {  // cannot happen with the node counts given; reported as an error
     atomicOr(a.error, 1);
+    a.n_task[r] = 0;
     return;
   }
-  int nk = chain_filter(a.opt, s, nxt, ch, order, nch, a.w16 + b, a.tmpi + 2 * b, a.kept + b);
+  int nk = chain_filter(a.opt, s, nxt, ch, order, nch, w16, tmpi, kept);
   int t = 0;
   Task *out = a.tasks_tmp + b;
   for (int c = 0; c < nk; ++c) {
@@ -487,6 +480,53 @@ __device__ void chain_one(const ChainArgs &a, int r) {
     }
   }
   a.n_task[r] = t;
+}
+
+// One read on one thread, in global memory: sort_seeds, then chain_read.
+// The wavefront kernel below hands a read here when it has more seeds than
+// its registers hold, or when its B-tree would differ from a sorted array
+// (see there).
+__device__ void chain_one(const ChainArgs &a, int r) {
+  const int64_t b = a.seed_off[r];
+  const int ns = (int)(a.seed_off[r + 1] - b);
+  a.n_task[r] = 0;
+  if (ns == 0) return;
+  Seed *s = a.seeds + b;
+  if (ns > 1) sort_seeds(s, ns, a.sort_tmp + b);
+  const int64_t nb0 = (b + 32 * (int64_t)r) / 4, nb1 = (a.seed_off[r + 1] + 32 * (int64_t)(r + 1)) / 4;
+  chain_read(a, r, b, s, ns, a.ch + b, a.nxt + b, a.order + b, a.nodes + nb0, (int)(nb1 - nb0), a.w16 + b,
+             a.tmpi + 2 * b, a.kept + b);
+}
+
+// Reads of at most kChainSmall seeds, one a thread, every temporary in the
+// thread's LDS slice: with at most 11 chains the B-tree never grows past its
+// root, so one node holds it.  The slice is 162 words (a multiple of 8
+// bytes whose 8-byte words fall on distinct bank pairs across the lanes).
+constexpr int kChainSmall = 8;
+struct ChainSmall {
+  Seed s[kChainSmall];
+  ChainTmp ch[kChainSmall];
+  int32_t nxt[kChainSmall], order[kChainSmall], tmpi[2 * kChainSmall];
+  uint16_t w16[kChainSmall];
+  uint8_t kept[kChainSmall];
+  KbNode node;
+};
+constexpr int kChainSmallWords = 162;
+static_assert(sizeof(ChainSmall) <= 4 * kChainSmallWords, "LDS slice of k_chain_small");
+static_assert(kChainSmall <= kKbN, "one B-tree node per read");
+
+__global__ void __launch_bounds__(64) k_chain_small(ChainArgs a) {
+  if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
+  __shared__ uint64_t slices[64 * kChainSmallWords / 2];
+  const int r = blockIdx.x * 64 + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const int64_t b = a.seed_off[r];
+  const int ns = (int)(a.seed_off[r + 1] - b);
+  if (ns == 0 || ns > kChainSmall) return;  // k_chain's
+  ChainSmall &m = *reinterpret_cast<ChainSmall *>(slices + threadIdx.x * (kChainSmallWords / 2));
+  for (int i = 0; i < ns; ++i) m.s[i] = a.seeds[b + i];
+  sort_seeds(m.s, ns, nullptr);  // insertion sort: no scratch below 33 seeds
+  chain_read(a, r, b, m.s, ns, m.ch, m.nxt, m.order, &m.node, 1, m.w16, m.tmpi, m.kept);
 }
 
 constexpr int kChainWaveSeeds = 128;  // most seeds a read may have on the wavefront path
@@ -824,9 +864,10 @@ __global__ void k_chain_big(const int32_t *stop, int n_reads, const int64_t *see
   if (r < n_reads && seed_off[r + 1] - seed_off[r] > kChainWaveSeeds) list[atomicAdd(n_list, 1)] = r;
 }
 
-// One wavefront per read (four a block).  Wavefront g first takes the g-th
-// listed large read (chain_one on lane 0), then read g unless it is a listed
-// one; reads that chain_wave hands back also run chain_one on lane 0.
+// One wavefront per read (four a block) for reads of more than kChainSmall
+// seeds.  Wavefront g first takes the g-th listed large read (chain_one on
+// lane 0), then read g unless it is a listed or a small one; reads that
+// chain_wave hands back also run chain_one on lane 0.
 __global__ void __launch_bounds__(256) k_chain(ChainArgs a) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   __shared__ ChainStage stage[4];
@@ -839,6 +880,8 @@ __global__ void __launch_bounds__(256) k_chain(ChainArgs a) {
   bool done = true;
   if (ns == 0) {
     if (lane == 0) a.n_task[r] = 0;
+  } else if (ns <= kChainSmall) {
+    // k_chain_small's
   } else if (ns <= 64) {
     done = chain_wave<1>(a, r, b, ns, lane, stage[wv]);
   } else if (ns <= kChainWaveSeeds) {
@@ -1656,6 +1699,7 @@ void launch_chain(const ChainArgs &a, hipStream_t s) {
   if (!a.n_reads) return;
   hipLaunchKernelGGL(k_chain_big, dim3(blocks(a.n_reads, 256)), dim3(256), 0, s, a.stop, a.n_reads, a.seed_off,
                      a.big, a.n_big);
+  hipLaunchKernelGGL(k_chain_small, dim3(blocks(a.n_reads, 64)), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_chain, dim3(blocks(a.n_reads, 4)), dim3(256), 0, s, a);
 }
 void launch_task_compact(const CompactArgs &a, hipStream_t s) {
