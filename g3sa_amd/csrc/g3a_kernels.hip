@@ -448,7 +448,7 @@ __global__ void __launch_bounds__(256) k_sa_lookup(SaArgs a) {
 
 // One read on one thread from its sorted seeds `s`: the B-tree chaining, the
 // filter and the read's tasks, with the temporaries at the pointers given
-// (global slices, or the thread's LDS slice in k_chain_small).
+// (global slices).
 __device__ __forceinline__ void chain_read(const ChainArgs &a, int r, int64_t b, const Seed *s, int ns, ChainTmp *ch,
                                            int32_t *nxt, int32_t *order, KbNode *nodes, int cap_nodes,
                                            uint16_t *w16, int32_t *tmpi, uint8_t *kept) {
@@ -498,35 +498,238 @@ __device__ void chain_one(const ChainArgs &a, int r) {
              a.tmpi + 2 * b, a.kept + b);
 }
 
-// Reads of at most kChainSmall seeds, one a thread, every temporary in the
-// thread's LDS slice: with at most 11 chains the B-tree never grows past its
-// root, so one node holds it.  The slice is 162 words (a multiple of 8
-// bytes whose 8-byte words fall on distinct bank pairs across the lanes).
-constexpr int kChainSmall = 8;
-struct ChainSmall {
-  Seed s[kChainSmall];
-  ChainTmp ch[kChainSmall];
-  int32_t nxt[kChainSmall], order[kChainSmall], tmpi[2 * kChainSmall];
-  uint16_t w16[kChainSmall];
-  uint8_t kept[kChainSmall];
-  KbNode node;
+// Reads of at most kChainThread seeds (91% at 1 Gbp), one a thread, as
+// chain_wave does them (the B-tree as a sorted array; a read that repeats a
+// chain position past 11 chains runs chain_one), with everything in the
+// thread's LDS slice: 32-bit words, word k of the lane at lds[64 k + lane]
+// (no bank conflicts), byte arrays four entries a word.  The seeds: rbeg
+// (two words), qbeg | len << 16, rid; a chain: head | tail << 8 | n << 16
+// (its position and contig are its head seed's); the B-tree order: slot
+// positions (two words) and chain ids; the filter: chain ids, weights and
+// begin | end << 16 in weight order.
+constexpr int kChainThread = 16;
+struct ChainLds {
+  enum : int {
+    kRbLo = 0, kRbHi = 16, kQl = 32, kRid = 48,  // seeds
+    kNxt = 64,                                   // bytes: next seed of the chain
+    kCh = 68,                                    // chains
+    kTpLo = 84, kTpHi = 100, kTc = 116,          // B-tree order: positions, chain ids (bytes)
+    kOc = 120, kOw = 124, kObe = 132,            // filter order: chain ids (bytes), weights (halves), begin | end
+    kWords = 148
+  };
+  uint32_t *w;
+  __device__ uint32_t get(int k) const { return w[64 * k]; }
+  __device__ void set(int k, uint32_t v) const { w[64 * k] = v; }
+  __device__ int getb(int k, int e) const { return (int)(int8_t)reinterpret_cast<const uint8_t *>(w + 64 * (k + (e >> 2)))[e & 3]; }
+  __device__ void setb(int k, int e, int v) const { reinterpret_cast<uint8_t *>(w + 64 * (k + (e >> 2)))[e & 3] = (uint8_t)v; }
+  __device__ int geth(int k, int e) const { return reinterpret_cast<const uint16_t *>(w + 64 * (k + (e >> 1)))[e & 1]; }
+  __device__ void seth(int k, int e, int v) const { reinterpret_cast<uint16_t *>(w + 64 * (k + (e >> 1)))[e & 1] = (uint16_t)v; }
+  __device__ int64_t rb(int i) const { return (int64_t)((uint64_t)get(kRbHi + i) << 32 | get(kRbLo + i)); }
+  __device__ int qb(int i) const { return (int)(get(kQl + i) & 0xffff); }
+  __device__ int ln(int i) const { return (int)(get(kQl + i) >> 16); }
+  __device__ int64_t tp(int s) const { return (int64_t)((uint64_t)get(kTpHi + s) << 32 | get(kTpLo + s)); }
 };
-constexpr int kChainSmallWords = 162;
-static_assert(sizeof(ChainSmall) <= 4 * kChainSmallWords, "LDS slice of k_chain_small");
-static_assert(kChainSmall <= kKbN, "one B-tree node per read");
+static_assert(kChainThread <= 16, "the slice's arrays hold 16 entries");
 
-__global__ void __launch_bounds__(64) k_chain_small(ChainArgs a) {
+__device__ bool chain_thread(const ChainArgs &a, int r, int64_t b, int ns, const ChainLds &m) {
+  const Opt &o = a.opt;
+  const int64_t l_pac = a.idx.l_pac;
+  // the seeds, sorted by insertion as they arrive (seed_less, stable)
+  for (int i = 0; i < ns; ++i) {
+    const Seed v = a.seeds[b + i];
+    const uint32_t key = (uint32_t)v.qbeg | (uint32_t)v.len << 16;  // qbeg, len < 2^16
+    int j = i;
+    for (; j > 0; --j) {
+      const uint32_t kj = m.get(ChainLds::kQl + j - 1);
+      const bool less = (int)(key & 0xffff) < (int)(kj & 0xffff) ||
+                        ((key & 0xffff) == (kj & 0xffff) && (int)(key >> 16) < (int)(kj >> 16));
+      if (!less) break;
+      m.set(ChainLds::kRbLo + j, m.get(ChainLds::kRbLo + j - 1));
+      m.set(ChainLds::kRbHi + j, m.get(ChainLds::kRbHi + j - 1));
+      m.set(ChainLds::kQl + j, kj);
+      m.set(ChainLds::kRid + j, m.get(ChainLds::kRid + j - 1));
+    }
+    m.set(ChainLds::kRbLo + j, (uint32_t)(uint64_t)v.rbeg);
+    m.set(ChainLds::kRbHi + j, (uint32_t)((uint64_t)v.rbeg >> 32));
+    m.set(ChainLds::kQl + j, key);
+    m.set(ChainLds::kRid + j, (uint32_t)v.rid);
+  }
+  for (int i = 0; i < ns; ++i) m.setb(ChainLds::kNxt, i, -1);
+  // chaining (BTreeChaining, aux.cu:1755-1825)
+  int nch = 0;
+  bool dup = false;
+  for (int i = 0; i < ns; ++i) {
+    const int64_t prb = m.rb(i);
+    const int pqb = m.qb(i), pln = m.ln(i), prid = (int)m.get(ChainLds::kRid + i);
+    // the first slot whose position is >= prb
+    int lo = 0, hi = nch;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (m.tp(mid) < prb) lo = mid + 1;
+      else hi = mid;
+    }
+    const int n_lt = lo, first_eq = lo < nch && m.tp(lo) == prb ? lo : -1;
+    if (nch) {
+      const int low = first_eq >= 0 ? first_eq : n_lt - 1;
+      if (low >= 0) {
+        // test_and_merge (aux.cu:54-84) against chain c
+        const int c = m.getb(ChainLds::kTc, low);
+        const uint32_t cw = m.get(ChainLds::kCh + c);
+        const int chd = (int)(cw & 0xff), ct = (int)((cw >> 8) & 0xff);
+        const int64_t last_rb = m.rb(ct), first_rb = m.rb(chd);
+        const int last_qb = m.qb(ct), last_ln = m.ln(ct), first_qb = m.qb(chd);
+        const int64_t qend = last_qb + last_ln, rend = last_rb + last_ln;
+        bool merged = false;
+        if (prid == (int)m.get(ChainLds::kRid + chd)) {
+          if (pqb >= first_qb && pqb + pln <= qend && prb >= first_rb && prb + pln <= rend) {
+            merged = true;  // contained in the chain: dropped
+          } else if (!((last_rb < l_pac || first_rb < l_pac) && prb >= l_pac)) {
+            const int64_t xd = pqb - last_qb, yd = prb - last_rb;
+            if (yd >= 0 && xd - yd <= o.w && yd - xd <= o.w && xd - last_ln < o.max_chain_gap &&
+                yd - last_ln < o.max_chain_gap) {
+              m.setb(ChainLds::kNxt, ct, i);
+              m.set(ChainLds::kCh + c, (uint32_t)chd | (uint32_t)i << 8 | ((cw >> 16) + 1) << 16);
+              merged = true;
+            }
+          }
+        }
+        if (merged) continue;
+      }
+    }
+    if (prid < 0 || prid >= 455) continue;  // bwa/src/aux.cu:1799
+    m.set(ChainLds::kCh + nch, (uint32_t)i | (uint32_t)i << 8 | 1u << 16);
+    const int ins = first_eq >= 0 ? first_eq + 1 : n_lt;
+    dup |= first_eq >= 0;
+    for (int sl = nch; sl > ins; --sl) {
+      m.set(ChainLds::kTpLo + sl, m.get(ChainLds::kTpLo + sl - 1));
+      m.set(ChainLds::kTpHi + sl, m.get(ChainLds::kTpHi + sl - 1));
+      m.setb(ChainLds::kTc, sl, m.getb(ChainLds::kTc, sl - 1));
+    }
+    m.set(ChainLds::kTpLo + ins, (uint32_t)(uint64_t)prb);
+    m.set(ChainLds::kTpHi + ins, (uint32_t)((uint64_t)prb >> 32));
+    m.setb(ChainLds::kTc, ins, nch);
+    ++nch;
+    if (dup && nch > kKbN) return false;  // a B-tree of several nodes with repeated keys
+  }
+  Task *out = a.tasks_tmp + b;
+  if (nch == 0) {
+    a.n_task[r] = 0;
+    return true;
+  }
+  // chain filter (aux.cu:1828-1988): weights in B-tree order, then a stable
+  // sort by decreasing weight
+  for (int sl = 0; sl < nch; ++sl) {
+    const int c = m.getb(ChainLds::kTc, sl);
+    const uint32_t cw = m.get(ChainLds::kCh + c);
+    const int chd = (int)(cw & 0xff), ct = (int)((cw >> 8) & 0xff);
+    int64_t endq = 0, endr = 0;
+    int wq = 0, wr = 0;
+    for (int j = chd; j >= 0; j = m.getb(ChainLds::kNxt, j)) {
+      const int q = m.qb(j), l = m.ln(j);
+      const int64_t rbj = m.rb(j);
+      if (q >= endq) wq += l;
+      else if (q + l > endq) wq += (int)(q + l - endq);
+      endq = endq > q + l ? endq : q + l;
+      if (rbj >= endr) wr += l;
+      else if (rbj + l > endr) wr += (int)(rbj + l - endr);
+      endr = endr > rbj + l ? endr : rbj + l;
+    }
+    int wt = wr < wq ? wr : wq;
+    wt = wt < 1 << 30 ? wt : (1 << 30) - 1;
+    const int wv = (uint16_t)wt;
+    const uint32_t be = (uint32_t)(uint16_t)m.qb(chd) | (uint32_t)(uint16_t)(m.qb(ct) + m.ln(ct)) << 16;
+    int k = sl;
+    for (; k > 0 && m.geth(ChainLds::kOw, k - 1) < wv; --k) {
+      m.setb(ChainLds::kOc, k, m.getb(ChainLds::kOc, k - 1));
+      m.seth(ChainLds::kOw, k, m.geth(ChainLds::kOw, k - 1));
+      m.set(ChainLds::kObe + k, m.get(ChainLds::kObe + k - 1));
+    }
+    m.setb(ChainLds::kOc, k, c);
+    m.seth(ChainLds::kOw, k, wv);
+    m.set(ChainLds::kObe + k, be);
+  }
+  // the filter: chain i is dropped iff an earlier kept chain j overlaps it by
+  // >= mask_level of the shorter and outweighs it (no ALT contigs)
+  uint32_t kept = 0;
+  for (int i = 0; i < nch; ++i) {
+    const uint32_t bei = m.get(ChainLds::kObe + i);
+    const uint16_t bi = (uint16_t)(bei & 0xffff), ei = (uint16_t)(bei >> 16);
+    const uint16_t wi = (uint16_t)m.geth(ChainLds::kOw, i);
+    bool drop = false;
+    for (int j = 0; j < i && !drop; ++j) {
+      if (!((kept >> j) & 1)) continue;
+      const uint32_t bej = m.get(ChainLds::kObe + j);
+      const uint16_t bj = (uint16_t)(bej & 0xffff), ej = (uint16_t)(bej >> 16);
+      const int b_max = bj > bi ? bj : bi, e_min = ej < ei ? ej : ei;
+      if (e_min > b_max) {
+        const int li = ei - bi, lj = ej - bj, min_l = li < lj ? li : lj;
+        if (e_min - b_max >= min_l * o.mask_level && min_l < o.max_chain_gap) {
+          const uint16_t wj = (uint16_t)m.geth(ChainLds::kOw, j);
+          drop = wi < wj * o.drop_ratio && wj - wi >= o.min_seed_len << 1;
+        }
+      }
+    }
+    if (!drop) kept |= 1u << i;
+  }
+  // tasks: kept chains in order, each chain's seeds in chain order with the
+  // chain's reference window (ExtendingPairGenerate, aux.cu:2101-2192)
+  const int l_seq = a.off[r + 1] - a.off[r];
+  int t = 0;
+  for (uint32_t km = kept; km; km &= km - 1) {
+    const int c = m.getb(ChainLds::kOc, __ffs(km) - 1);
+    const int chd = (int)(m.get(ChainLds::kCh + c) & 0xff);
+    int64_t rmax0 = l_pac << 1, rmax1 = 0;
+    for (int j = chd; j >= 0; j = m.getb(ChainLds::kNxt, j)) {
+      const int q = m.qb(j), l = m.ln(j);
+      const int64_t rbj = m.rb(j);
+      const int64_t lo = rbj - (q + cal_max_gap(o, q));
+      const int64_t hi = rbj + l + ((l_seq - q - l) + cal_max_gap(o, l_seq - q - l));
+      rmax0 = rmax0 < lo ? rmax0 : lo;
+      rmax1 = rmax1 > hi ? rmax1 : hi;
+    }
+    rmax0 = rmax0 > 0 ? rmax0 : 0;
+    rmax1 = rmax1 < l_pac << 1 ? rmax1 : l_pac << 1;
+    const int64_t mid = m.rb(chd);
+    if (rmax0 < l_pac && l_pac < rmax1) {
+      if (mid < l_pac) rmax1 = l_pac;
+      else rmax0 = l_pac;
+    }
+    int is_rev;
+    const int rid = pos2rid(a.idx, depos(a.idx, mid, is_rev));
+    int64_t far_beg = a.idx.ctg_off[rid], far_end = far_beg + a.idx.ctg_len[rid];
+    if (is_rev) {
+      const int64_t tb = far_beg;
+      far_beg = (l_pac << 1) - far_end;
+      far_end = (l_pac << 1) - tb;
+    }
+    rmax0 = rmax0 > far_beg ? rmax0 : far_beg;
+    rmax1 = rmax1 < far_end ? rmax1 : far_end;
+    for (int j = chd; j >= 0; j = m.getb(ChainLds::kNxt, j)) {
+      Task tk;
+      tk.rbeg = m.rb(j);
+      tk.rmax0 = rmax0;
+      tk.rmax1 = rmax1;
+      tk.qbeg = m.qb(j);
+      tk.len = m.ln(j);
+      tk.read = r;
+      tk.rid = rid;
+      out[t++] = tk;
+    }
+  }
+  a.n_task[r] = t;
+  return true;
+}
+
+__global__ void __launch_bounds__(64) k_chain_thread(ChainArgs a) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  __shared__ uint64_t slices[64 * kChainSmallWords / 2];
+  __shared__ uint32_t lds[64 * ChainLds::kWords];
   const int r = blockIdx.x * 64 + threadIdx.x;
   if (r >= a.n_reads) return;
   const int64_t b = a.seed_off[r];
   const int ns = (int)(a.seed_off[r + 1] - b);
-  if (ns == 0 || ns > kChainSmall) return;  // k_chain's
-  ChainSmall &m = *reinterpret_cast<ChainSmall *>(slices + threadIdx.x * (kChainSmallWords / 2));
-  for (int i = 0; i < ns; ++i) m.s[i] = a.seeds[b + i];
-  sort_seeds(m.s, ns, nullptr);  // insertion sort: no scratch below 33 seeds
-  chain_read(a, r, b, m.s, ns, m.ch, m.nxt, m.order, &m.node, 1, m.w16, m.tmpi, m.kept);
+  if (ns == 0 || ns > kChainThread) return;  // k_chain's
+  const ChainLds m{lds + threadIdx.x};
+  if (!chain_thread(a, r, b, ns, m)) chain_one(a, r);
 }
 
 constexpr int kChainWaveSeeds = 128;  // most seeds a read may have on the wavefront path
@@ -864,7 +1067,7 @@ __global__ void k_chain_big(const int32_t *stop, int n_reads, const int64_t *see
   if (r < n_reads && seed_off[r + 1] - seed_off[r] > kChainWaveSeeds) list[atomicAdd(n_list, 1)] = r;
 }
 
-// One wavefront per read (four a block) for reads of more than kChainSmall
+// One wavefront per read (four a block) for reads of more than kChainThread
 // seeds.  Wavefront g first takes the g-th listed large read (chain_one on
 // lane 0), then read g unless it is a listed or a small one; reads that
 // chain_wave hands back also run chain_one on lane 0.
@@ -880,8 +1083,8 @@ __global__ void __launch_bounds__(256) k_chain(ChainArgs a) {
   bool done = true;
   if (ns == 0) {
     if (lane == 0) a.n_task[r] = 0;
-  } else if (ns <= kChainSmall) {
-    // k_chain_small's
+  } else if (ns <= kChainThread) {
+    // k_chain_thread's
   } else if (ns <= 64) {
     done = chain_wave<1>(a, r, b, ns, lane, stage[wv]);
   } else if (ns <= kChainWaveSeeds) {
@@ -1699,7 +1902,7 @@ void launch_chain(const ChainArgs &a, hipStream_t s) {
   if (!a.n_reads) return;
   hipLaunchKernelGGL(k_chain_big, dim3(blocks(a.n_reads, 256)), dim3(256), 0, s, a.stop, a.n_reads, a.seed_off,
                      a.big, a.n_big);
-  hipLaunchKernelGGL(k_chain_small, dim3(blocks(a.n_reads, 64)), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_chain_thread, dim3(blocks(a.n_reads, 64)), dim3(64), 0, s, a);
   hipLaunchKernelGGL(k_chain, dim3(blocks(a.n_reads, 4)), dim3(256), 0, s, a);
 }
 void launch_task_compact(const CompactArgs &a, hipStream_t s) {
