@@ -77,7 +77,7 @@ class LearnedInfo(C.Structure):
 EXPORTS = [
     "g3a_opt_init", "g3a_device_alloc", "g3a_memcpy_index", "g3a_memcpy_input", "g3a_bwa_align",
     "g3a_memcpy_output", "g3a_align_batch", "g3a_batch_out_free", "g3a_destroy", "g3a_last_error",
-    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_graph", "g3a_graph_info", "g3a_set_batch_caps", "g3a_device_alloc_sibling", "g3a_set_seed_grid", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
+    "g3a_build_index", "g3a_last_stats", "g3a_set_form", "g3a_set_graph", "g3a_graph_info", "g3a_set_batch_caps", "g3a_device_alloc_sibling", "g3a_set_seed_grid", "g3a_set_chain_serial", "g3a_set_tuning", "g3a_kernel_ms", "g3a_autotune",
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
     "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed", "g3a_build_index_dev", "g3a_memcpy_text",
     "g3a_format_records", "g3a_copy_text", "g3a_bwa_align_noalloc", "g3a_set_wide_layout",
@@ -118,6 +118,7 @@ def load_library() -> C.CDLL:
     lib.g3a_set_batch_caps.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     lib.g3a_device_alloc_sibling.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_void_p)]
     lib.g3a_set_seed_grid.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.g3a_set_chain_serial.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_kernel_ms.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
     lib.g3a_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]
     lib.g3a_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
@@ -254,6 +255,10 @@ class Aligner:
     def set_seed_blocks(self, blocks_per_cu: int) -> None:
         """Persistent seeding blocks per compute unit (0: occupancy maximum)."""
         self._check(self.lib.g3a_set_seed_blocks(self.ctx, blocks_per_cu))
+
+    def set_chain_serial(self, on: bool) -> None:
+        """Chain every read by the B-tree code on one thread (True) or by the tiers (False)."""
+        self._check(self.lib.g3a_set_chain_serial(self.ctx, int(bool(on))))
 
     def set_seed_grid(self, threads_per_block: int, blocks_per_cu: int) -> None:
         """Seeding threads per block (128-1024) and blocks per compute unit."""

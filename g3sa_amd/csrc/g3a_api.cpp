@@ -111,6 +111,7 @@ struct g3a_ctx {
   bool has_text = false;
   int text_n = -1;
   DevBuf names, name_off, quals, qual_off, txt_len, txt_off, txt;
+  DevBuf txt_bound, stage_off, txt_stage, fmt_flag;  // the one-pass formatter's bounds, staging and overflow flag
   int64_t txt_bytes = -1;
   DevBuf mapq, md_len, md_off, md;  // per alignment, for g3a_memcpy_output
   // batch input
@@ -123,6 +124,7 @@ struct g3a_ctx {
   // of the superblock counts in LDS and the rest for list entries
   int seed_blocks_per_cu = 1;             // 0: the occupancy maximum
   int seed_block = 1024;                  // k_seed threads per block
+  int chain_serial = 0;                   // chaining: 0 thread / wavefront tiers, 1 the B-tree code per read
   int force_wide = 0;  // the layout of texts of 2^32 rows or more, on any text
   // per-batch workspace: `own`, or `scratch_work` during and after an
   // allocation-free call (its views stay valid for g3a_memcpy_output)
@@ -464,6 +466,7 @@ int g3a_device_alloc_sibling(g3a_ctx *src, int max_reads, int64_t max_bases, g3a
   ctx->ext_packed = src->ext_packed;
   ctx->seed_blocks_per_cu = src->seed_blocks_per_cu;
   ctx->seed_block = src->seed_block;
+  ctx->chain_serial = src->chain_serial;
   ctx->seed_ordered = src->seed_ordered;
   ctx->seed_threads = src->seed_threads;
   ctx->tb_threads = src->tb_threads;
@@ -1000,6 +1003,7 @@ static int enqueue_batch(g3a_ctx *ctx, int n, const Caps &c, bool timed) {
   ca.tasks_tmp = P<Task>(w.tasks_tmp);
   ca.n_task = P<int32_t>(w.n_task);
   ca.error = P<int32_t>(w.counters) + 2 * 16;
+  ca.serial = ctx->chain_serial;
   ca.big = P<int32_t>(w.chain_big);
   ca.n_big = P<int32_t>(w.counters) + kChainBigSlot32;
   launch_chain(ca, s);
@@ -1088,7 +1092,7 @@ static std::string graph_key(const g3a_ctx *ctx, int n, const Caps &c) {
   put(&ctx->opt, sizeof ctx->opt);
   const int v[] = {ctx->form, ctx->strip_rows, ctx->lane_blocks_per_cu, ctx->wave_blocks_per_cu, ctx->ext_packed,
                    ctx->seed_blocks_per_cu, ctx->seed_block, ctx->run_seed_threads, ctx->run_tb_threads, ctx->run_lane_blocks,
-                   ctx->run_slow_chunk};
+                   ctx->run_slow_chunk, ctx->chain_serial};
   put(v, sizeof v);
   put(&ctx->buf_gen, sizeof ctx->buf_gen);
   return k;
@@ -1277,12 +1281,35 @@ int g3a_format_records(g3a_ctx *ctx, int mode, int64_t *bytes) {
       ensure(ctx, ctx->fmt_scan_tmp, scan_tmp_bytes(n + 1)))
     return G3A_E_HIP;
   const TextArgs t = text_args(ctx, mode);
-  launch_text(t, P<int64_t>(ctx->txt_len), nullptr, nullptr, false, s);
+  // one formatting pass into a staging buffer laid out by per-read bounds,
+  // then the texts moved to their places; should a read outgrow its bound,
+  // the texts are made again by a counting pass and a writing pass
+  if (ensure(ctx, ctx->txt_bound, (size_t)n * 8 + 8) || ensure(ctx, ctx->stage_off, (size_t)(n + 1) * 8) ||
+      ensure(ctx, ctx->fmt_flag, 8))
+    return G3A_E_HIP;
+  launch_text_bound(t, P<int64_t>(ctx->txt_bound), s);
+  scan_i64(P<int64_t>(ctx->txt_bound), P<int64_t>(ctx->stage_off), n, ctx->fmt_scan_tmp.p, ctx->fmt_scan_tmp.bytes,
+           s);
+  if (readback(ctx, {P<int64_t>(ctx->stage_off) + n})) return G3A_E_HIP;
+  if (ensure(ctx, ctx->txt_stage, (size_t)ctx->pinned[0] + 16)) return G3A_E_HIP;
+  launch_fill_i32(P<int32_t>(ctx->fmt_flag), 2, 0, s);
+  launch_text_staged(t, P<int64_t>(ctx->txt_len), P<int64_t>(ctx->stage_off), P<char>(ctx->txt_stage), s);
   scan_i64(P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_off), n, ctx->fmt_scan_tmp.p, ctx->fmt_scan_tmp.bytes, s);
-  if (readback(ctx, {P<int64_t>(ctx->txt_off) + n})) return G3A_E_HIP;
-  const int64_t total = ctx->pinned[0];
-  if (ensure(ctx, ctx->txt, (size_t)total + 16)) return G3A_E_HIP;
-  launch_text(t, nullptr, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), true, s);
+  launch_text_check(n, P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_bound), P<int64_t>(ctx->fmt_flag), s);
+  if (readback(ctx, {P<int64_t>(ctx->txt_off) + n, P<int64_t>(ctx->fmt_flag)})) return G3A_E_HIP;
+  int64_t total = ctx->pinned[0];
+  if (ctx->pinned[1]) {
+    launch_text(t, P<int64_t>(ctx->txt_len), nullptr, nullptr, false, s);
+    scan_i64(P<int64_t>(ctx->txt_len), P<int64_t>(ctx->txt_off), n, ctx->fmt_scan_tmp.p, ctx->fmt_scan_tmp.bytes, s);
+    if (readback(ctx, {P<int64_t>(ctx->txt_off) + n})) return G3A_E_HIP;
+    total = ctx->pinned[0];
+    if (ensure(ctx, ctx->txt, (size_t)total + 16)) return G3A_E_HIP;
+    launch_text(t, nullptr, P<int64_t>(ctx->txt_off), P<char>(ctx->txt), true, s);
+  } else {
+    if (ensure(ctx, ctx->txt, (size_t)total + 16)) return G3A_E_HIP;
+    launch_text_pack(n, P<char>(ctx->txt_stage), P<int64_t>(ctx->stage_off), P<int64_t>(ctx->txt_len),
+                     P<int64_t>(ctx->txt_off), P<char>(ctx->txt), s);
+  }
   HIPCK(hipStreamSynchronize(s));
   HIPCK(hipGetLastError());
   ctx->txt_bytes = total;
@@ -1484,6 +1511,12 @@ int g3a_set_seed_grid(g3a_ctx *ctx, int threads_per_block, int blocks_per_cu) {
     ctx->seed_threads = 0;  // the resident maximum (and the list pool) follow the block size
   }
   ctx->seed_blocks_per_cu = blocks_per_cu;
+  return G3A_OK;
+}
+
+int g3a_set_chain_serial(g3a_ctx *ctx, int on) {
+  if (!ctx || (on != 0 && on != 1)) return G3A_E_ARG;
+  ctx->chain_serial = on;
   return G3A_OK;
 }
 
@@ -1766,7 +1799,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->own.lep_pool, &ctx->own.eh_pool, &ctx->own.tb_order, &ctx->own.tb_ctr, &ctx->own.tb_nw, &ctx->li_saf, &ctx->li_keys,
                     &ctx->li_root, &ctx->li_leaf, &ctx->li_knot, &ctx->li_leaf_a, &ctx->li_short, &ctx->li_ctr,
                     &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,
-                    &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->mapq, &ctx->md_len,
+                    &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->txt_bound, &ctx->stage_off, &ctx->txt_stage, &ctx->fmt_flag, &ctx->mapq, &ctx->md_len,
                     &ctx->md_off, &ctx->md, &ctx->fmt_scan_tmp};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)

@@ -70,7 +70,8 @@ struct ChainArgs {
   Task *tasks_tmp;
   int32_t *n_task;
   int32_t *error;
-  int32_t *big, *n_big;  // reads of more than 128 seeds and their count (zero before the launch)
+  int32_t *big, *n_big;  // reads of more than 64 seeds and their count (zero before the launch)
+  int serial;            // 1: every read by the B-tree code on one thread (chain_one)
 };
 
 struct CompactArgs {
@@ -202,6 +203,15 @@ void launch_kmer_table(const DevIndex &x, uint32_t *table, hipStream_t s);
 // the batch's records: each read's text length (write = false: len), or
 // its text at txt + txt_off[q] (write = true)
 void launch_text(const TextArgs &t, int64_t *len, const int64_t *txt_off, char *txt, bool write, hipStream_t s);
+// One formatting pass: every read's text into a staging buffer at stage_off
+// (offsets from k_text_bound's bounds), its length into len; then
+// launch_text_check flags a read longer than its bound and launch_text_pack
+// moves the texts to their places.
+void launch_text_bound(const TextArgs &t, int64_t *bound, hipStream_t s);
+void launch_text_staged(const TextArgs &t, int64_t *len, const int64_t *stage_off, char *stage, hipStream_t s);
+void launch_text_check(int n, const int64_t *len, const int64_t *bound, int64_t *flag, hipStream_t s);
+void launch_text_pack(int n, const char *stage, const int64_t *stage_off, const int64_t *len, const int64_t *txt_off,
+                      char *txt, hipStream_t s);
 void launch_aln_mapq_md(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, int32_t *mapq, int64_t *md_len,
                         hipStream_t s);
 void launch_aln_md_write(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, const int64_t *md_off,

@@ -732,7 +732,7 @@ __global__ void __launch_bounds__(64) k_chain_thread(ChainArgs a) {
   if (!chain_thread(a, r, b, ns, m)) chain_one(a, r);
 }
 
-constexpr int kChainWaveSeeds = 128;  // most seeds a read may have on the wavefront path
+constexpr int kChainWaveSeeds = 64;  // most seeds a read may have on the wavefront path
 struct ChainStage {                   // one wavefront's LDS: the permutations of the sorts
   int64_t rbeg[kChainWaveSeeds];
   int32_t qbeg[kChainWaveSeeds], len[kChainWaveSeeds], rid[kChainWaveSeeds];
@@ -1068,15 +1068,20 @@ __global__ void k_chain_big(const int32_t *stop, int n_reads, const int64_t *see
 }
 
 // One wavefront per read (four a block) for reads of more than kChainThread
-// seeds.  Wavefront g first takes the g-th listed large read (chain_one on
-// lane 0), then read g unless it is a listed or a small one; reads that
-// chain_wave hands back also run chain_one on lane 0.
+// and at most kChainWaveSeeds seeds.  Wavefront g first takes the g-th
+// listed large read (chain_one on lane 0), then read g unless it is a listed
+// or a small one; reads that chain_wave hands back also run chain_one on
+// lane 0.
 __global__ void __launch_bounds__(256) k_chain(ChainArgs a) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   __shared__ ChainStage stage[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
   if (r >= a.n_reads) return;  // the whole wavefront
+  if (a.serial) {  // every read by the B-tree code on one thread
+    if (lane == 0) chain_one(a, r);
+    return;
+  }
   if (r < *a.n_big && lane == 0) chain_one(a, a.big[r]);
   const int64_t b = a.seed_off[r];
   const int ns = __builtin_amdgcn_readfirstlane((int)(a.seed_off[r + 1] - b));
@@ -1085,10 +1090,8 @@ __global__ void __launch_bounds__(256) k_chain(ChainArgs a) {
     if (lane == 0) a.n_task[r] = 0;
   } else if (ns <= kChainThread) {
     // k_chain_thread's
-  } else if (ns <= 64) {
-    done = chain_wave<1>(a, r, b, ns, lane, stage[wv]);
   } else if (ns <= kChainWaveSeeds) {
-    done = chain_wave<2>(a, r, b, ns, lane, stage[wv]);
+    done = chain_wave<1>(a, r, b, ns, lane, stage[wv]);
   }  // else: listed
   if (!done && lane == 0) chain_one(a, r);
 }
@@ -1900,6 +1903,10 @@ void launch_sa_lookup(const SaArgs &a, hipStream_t s) {
 }
 void launch_chain(const ChainArgs &a, hipStream_t s) {
   if (!a.n_reads) return;
+  if (a.serial) {
+    hipLaunchKernelGGL(k_chain, dim3(blocks(a.n_reads, 4)), dim3(256), 0, s, a);
+    return;
+  }
   hipLaunchKernelGGL(k_chain_big, dim3(blocks(a.n_reads, 256)), dim3(256), 0, s, a.stop, a.n_reads, a.seed_off,
                      a.big, a.n_big);
   hipLaunchKernelGGL(k_chain_thread, dim3(blocks(a.n_reads, 64)), dim3(64), 0, s, a);

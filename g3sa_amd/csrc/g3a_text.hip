@@ -465,7 +465,48 @@ k_fmt(TextArgs t, int64_t *len_out, const int64_t *txt_off, char *txt) {
   const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * kFmtWaves + wv);
   if (q >= t.n_reads) return;  // the whole wavefront
   const int64_t n = format_read<kWrite>(t, q, lane, ev_all[wv], kWrite ? txt + txt_off[q] : nullptr);
-  if (!kWrite && lane == 0) len_out[q] = n;
+  if (len_out && lane == 0) len_out[q] = n;
+}
+
+// An upper bound on the text of each read, from what is known before the
+// records are formatted.  A record is its NAME, SEQ and QUAL, the contig's
+// name, at most 12 bytes a CIGAR operation (a count of at most 28 bits and
+// the letter; the SAM core operations are a subset of n_core), the MD string
+// and 256 bytes for every other field (numbers of at most 20 characters,
+// tags and separators).  The MD string is at most 2 span + 1 bytes: an event
+// writes the number of matches before it (at most one digit more than
+// there are matches) and one base or '^' and the deleted bases, so never
+// more than twice the reference bases it covers, plus one for the last run.
+__global__ void __launch_bounds__(256) k_text_bound(TextArgs t, int64_t *bound) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= t.n_reads) return;
+  const int64_t base = (t.name_off[q + 1] - t.name_off[q]) + (t.qual_off[q + 1] - t.qual_off[q]) +
+                       (t.off[q + 1] - t.off[q]) + 256;
+  const int64_t a0 = t.aln_off[q], a1 = t.aln_off[q + 1];
+  int64_t u = a1 > a0 ? 0 : base;
+  for (int64_t j = a0; j < a1; ++j) {
+    const Aln &a = t.alns[j];
+    u += base + (t.ctg_name_off[a.rid + 1] - t.ctg_name_off[a.rid]) + 12 * ((int64_t)a.n_cigar + a.n_core + 4) +
+         2 * (a.re - a.rb) + 1;
+  }
+  bound[q] = u;
+}
+
+// flag |= some read's text is longer than its bound
+__global__ void __launch_bounds__(256) k_text_check(int n, const int64_t *len, const int64_t *bound, int64_t *flag) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n && len[q] > bound[q]) atomicOr((unsigned long long *)flag, 1ull);
+}
+
+// read q's text from the staging buffer to its place, a wavefront a read
+__global__ void __launch_bounds__(256) k_text_pack(int n, const char *stage, const int64_t *stage_off,
+                                                   const int64_t *len, const int64_t *txt_off, char *txt) {
+  const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (q >= n) return;  // the whole wavefront
+  const char *src = stage + stage_off[q];
+  char *dst = txt + txt_off[q];
+  const int64_t l = len[q];
+  for (int64_t k = threadIdx.x & 63; k < l; k += 64) dst[k] = src[k];
 }
 
 // MAPQ and MD-string length of every alignment (thread per alignment;
@@ -510,6 +551,25 @@ void launch_text(const TextArgs &t, int64_t *len, const int64_t *txt_off, char *
     hipLaunchKernelGGL(k_fmt<true>, dim3(blocks), dim3(64 * kFmtWaves), 0, s, t, len, txt_off, txt);
   else
     hipLaunchKernelGGL(k_fmt<false>, dim3(blocks), dim3(64 * kFmtWaves), 0, s, t, len, txt_off, txt);
+}
+
+void launch_text_staged(const TextArgs &t, int64_t *len, const int64_t *stage_off, char *stage, hipStream_t s) {
+  if (t.n_reads <= 0) return;
+  const unsigned blocks = (unsigned)((t.n_reads + kFmtWaves - 1) / kFmtWaves);
+  hipLaunchKernelGGL(k_fmt<true>, dim3(blocks), dim3(64 * kFmtWaves), 0, s, t, len, stage_off, stage);
+}
+void launch_text_bound(const TextArgs &t, int64_t *bound, hipStream_t s) {
+  if (t.n_reads > 0)
+    hipLaunchKernelGGL(k_text_bound, dim3((unsigned)((t.n_reads + 255) / 256)), dim3(256), 0, s, t, bound);
+}
+void launch_text_check(int n, const int64_t *len, const int64_t *bound, int64_t *flag, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_text_check, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, len, bound, flag);
+}
+void launch_text_pack(int n, const char *stage, const int64_t *stage_off, const int64_t *len, const int64_t *txt_off,
+                      char *txt, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_text_pack, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, stage, stage_off, len, txt_off,
+                       txt);
 }
 
 void launch_aln_mapq_md(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, int32_t *mapq, int64_t *md_len,

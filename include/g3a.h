@@ -245,6 +245,11 @@ int g3a_set_seed_grid(g3a_ctx *ctx, int threads_per_block, int blocks_per_cu);
  * 0 (default) seeds them in input order.  Results are identical.  No reference
  * counterpart. */
 int g3a_set_seed_order(g3a_ctx *ctx, int ordered);
+/* Chaining path: 0 (default) chains a read of at most 16 seeds on one
+ * thread and a larger one on a wavefront, both keeping the B-tree of chains
+ * as a sorted array; 1 runs the B-tree code for every read on one thread.
+ * The results are identical.  No reference counterpart. */
+int g3a_set_chain_serial(g3a_ctx *ctx, int on);
 /* Seeding autotuner: re-aligns the batch last aligned once per seeding grid
  * (threads per block x blocks per compute unit: 1024x1, 512x2, 256x4, 128x6;
  * each block keeps one copy of the superblock counts in LDS and the rest of

@@ -40,13 +40,13 @@ def compare(gpu: g3sa_amd.Result, ref: dict, label: str) -> None:
         assert np.array_equal(gk, rk), f"{label}: global CIGAR of alignment {i}"
 
 
-# (form, tuning, seeding order, seeding blocks per CU, packed lane
-# arithmetic): the plain form and the batched form at both strip heights
-# with two strip rows per register, and at 16 rows with one cell per lane;
-# reads seeded in minimizer order and in input order; seeding grids whose LDS
-# share holds 9, 25 and the default number of LEP entries per thread (the
-# rest of a list spills to its global slice)
-# (form, tuning, seed order, seeding grid (threads per block, blocks per CU), packed)
+# (form, tuning, seed order, seeding grid (threads per block, blocks per
+# CU), packed lane arithmetic): the plain form and the batched form at both
+# strip heights with two strip rows per register, and at 16 rows with one
+# cell per lane; reads seeded in minimizer order and in input order; seeding
+# grids of 1024, 512, 256 and 128 threads per block, whose LDS shares hold
+# different numbers of LEP entries per thread (the rest of a list spills to
+# its global slice)
 VARIANTS = [("plain", None, 1, (1024, 1), 1), ("batched", (16, 8, 8), 1, (128, 10), 1),
             ("batched", (32, 4, 8), 0, (256, 4), 1), ("batched", (16, 8, 8), 0, (512, 0), 0)]
 
@@ -238,11 +238,9 @@ def test_api_error_paths(ds_c1):
         lib.g3a_destroy(ctx)
 
 
-def test_slow_seeding_path_parity(built, oracle, tmp_path):
-    """Reads inside long tandem repeats grow their LEP lists past the fast
-    seeding path's capacity and are seeded again on the slow path (global
-    lists of 400 entries); the output still equals the oracle's."""
-    from types import SimpleNamespace
+def _tandem_dataset(tmp_path):
+    """A 200 kbp reference with three 4 kbp tandem repeats (1% point
+    divergence) and 400 reads, three in four inside the repeats."""
     rng = np.random.default_rng(17)
     L = 200_000
     g = rng.integers(0, 4, L).astype(np.uint8)
@@ -270,6 +268,15 @@ def test_slow_seeding_path_parity(built, oracle, tmp_path):
         r[sub] = rng.integers(0, 4, int(sub.sum()))
         reads.append(r)
     seq, off = _batch(reads)
+    return prefix, seq, off
+
+
+def test_slow_seeding_path_parity(built, oracle, tmp_path):
+    """Reads inside long tandem repeats grow their LEP lists past the fast
+    seeding path's capacity and are seeded again on the slow path (LDS lists
+    of 400 entries); the output still equals the oracle's."""
+    from types import SimpleNamespace
+    prefix, seq, off = _tandem_dataset(tmp_path)
     got, _ = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats")
     assert got.work["n_slow_reads"] > 0, "no read took the slow seeding path"
     got, ref = run_both(SimpleNamespace(prefix=prefix), oracle, seq, off, "tandem-repeats, wide layout", wide=True,
@@ -395,3 +402,39 @@ def _oracle_compat(ds, fq, tmp_path):
     subprocess.run([str(ROOT / "oracle" / "_build" / "g3a_oracle"), "-t", str(oracle_threads()), "-m", "compat",
                     "-o", str(out), str(ds.prefix), str(fq)], check=True, capture_output=True)
     return out.read_bytes()
+
+
+@pytest.mark.parametrize("which", ["rep", "tandem"])
+def test_chain_tiers_equal_serial(ds_rep, tmp_path, which):
+    """Chaining by the tiers (a thread per read of at most 16 seeds, a
+    wavefront per read of at most 64, the B-tree code on one lane above)
+    equals the B-tree code for every read, field by field, in the learned and
+    the FM-index forms, on repeat-rich references whose reads reach every
+    tier."""
+    if which == "rep":
+        prefix = ds_rep.prefix
+        names, seq, off, _ = g3sa_amd.read_fastq(str(ds_rep.reads["main"]), limit=20000)
+    else:
+        prefix, seq, off = _tandem_dataset(tmp_path)
+    n = len(off) - 1
+    with g3sa_amd.Aligner(str(prefix), max_reads=n, max_bases=int(off[-1])) as al:
+        for learned in (0, 1):
+            if learned:
+                al.set_learned(1)
+            al.set_chain_serial(False)
+            tiers = al.align(seq, off)
+            counts, kmsn = al.intervals()
+            seeds = np.bincount(np.repeat(np.arange(n), counts), weights=np.minimum(kmsn[:, 1], 500),
+                                minlength=n)
+            print(f"[chain] {which} learned {learned}: {n} reads, seeds per read max {int(seeds.max())}, "
+                  f"over 16: {int((seeds > 16).sum())}, over 64: {int((seeds > 64).sum())}", flush=True)
+            al.set_chain_serial(True)
+            serial = al.align(seq, off)
+            al.set_chain_serial(False)
+            np.testing.assert_array_equal(tiers.aln_off, serial.aln_off)
+            for f in g3sa_amd.Result.FIELDS:
+                np.testing.assert_array_equal(getattr(tiers, f), getattr(serial, f), err_msg=f"{which}: {f}")
+            np.testing.assert_array_equal(tiers.cigar, serial.cigar)
+            assert (seeds > 16).sum() > 0, "no read reached the wavefront tier"
+            if which == "tandem":
+                assert (seeds > 64).sum() > 0, "no read reached the one-lane tier"
