@@ -70,7 +70,7 @@ struct ChainArgs {
   Task *tasks_tmp;
   int32_t *n_task;
   int32_t *error;
-  int32_t *big, *n_big;  // reads of more than 64 seeds and their count (zero before the launch)
+  int32_t *big, *n_big;  // reads of more than 128 seeds and their count (zero before the launch)
   int serial;            // 1: every read by the B-tree code on one thread (chain_one)
 };
 

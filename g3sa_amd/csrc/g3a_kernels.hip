@@ -61,7 +61,8 @@ __device__ __forceinline__ void set_elem(T (&v)[NC], int e, T x, int lane) {  //
     if (e == 64 * k + lane) v[k] = x;
 }
 template <int NC, class T>
-__device__ __forceinline__ T gather(const T (&v)[NC], int e) {  // e per lane, every lane active
+__device__ __forceinline__ T gather(const T (&v)[NC], int e) {  // e per lane; call it with every lane active
+                                                                 // (a shuffle from an inactive lane gives 0)
   T x = __shfl(v[0], e & 63);
 #pragma unroll
   for (int k = 1; k < NC; ++k) {
@@ -732,7 +733,7 @@ __global__ void __launch_bounds__(64) k_chain_thread(ChainArgs a) {
   if (!chain_thread(a, r, b, ns, m)) chain_one(a, r);
 }
 
-constexpr int kChainWaveSeeds = 64;  // most seeds a read may have on the wavefront path
+constexpr int kChainWaveSeeds = 128;  // most seeds a read may have on the wavefront path
 struct ChainStage {                   // one wavefront's LDS: the permutations of the sorts
   int64_t rbeg[kChainWaveSeeds];
   int32_t qbeg[kChainWaveSeeds], len[kChainWaveSeeds], rid[kChainWaveSeeds];
@@ -935,7 +936,10 @@ __device__ bool chain_wave(const ChainArgs &a, int r, int64_t b, int ns, int lan
     for (int k = 0; k < NC; ++k) {
       const int cid = t_cid[k];
       o_cid[k] = cid;
-      o_w[k] = 64 * k + lane < nch ? gather<NC>(c_w, cid) : -1;
+      // every lane takes part in the shuffle: a lane that reads from one
+      // outside a divergent branch gets 0
+      const int wc = gather<NC>(c_w, cid);
+      o_w[k] = 64 * k + lane < nch ? wc : -1;
       o_be[k] = (int)((uint32_t)gather<NC>(c_b, cid) | (uint32_t)gather<NC>(c_e, cid) << 16);
       rank[k] = 0;
     }
@@ -1090,8 +1094,10 @@ __global__ void __launch_bounds__(256) k_chain(ChainArgs a) {
     if (lane == 0) a.n_task[r] = 0;
   } else if (ns <= kChainThread) {
     // k_chain_thread's
-  } else if (ns <= kChainWaveSeeds) {
+  } else if (ns <= 64) {
     done = chain_wave<1>(a, r, b, ns, lane, stage[wv]);
+  } else if (ns <= kChainWaveSeeds) {
+    done = chain_wave<2>(a, r, b, ns, lane, stage[wv]);
   }  // else: listed
   if (!done && lane == 0) chain_one(a, r);
 }
