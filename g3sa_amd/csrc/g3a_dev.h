@@ -1796,27 +1796,36 @@ G3A_HD int push_cigar(uint32_t *cigar, int n, int op, int len) {
 // columns (enough when 2w+2 <= RB) at a stride, e.g. one lane's slice of a
 // wavefront's LDS block.
 struct EhArray {
+  static constexpr bool kStage = false;
   int32_t *h, *e;
   G3A_HD int32_t &H(int j) { return h[j]; }
   G3A_HD int32_t &E(int j) { return e[j]; }
 };
+// The ring also carries `zst`, RB bytes (16-byte aligned) where a row's
+// direction bytes are gathered before they are stored 16 bytes at a time:
+// a thread's row is then whole 16-byte pieces of its alignment's matrix
+// instead of 4-byte pieces, which the memory side merges far better when
+// every lane of a wavefront writes its own alignment.
 template <int RB>
 struct EhRing {
+  static constexpr bool kStage = true;
   int32_t *h, *e;
   int stride;
+  uint32_t *zst;
   G3A_HD int32_t &H(int j) { return h[(j & (RB - 1)) * stride]; }
   G3A_HD int32_t &E(int j) { return e[(j & (RB - 1)) * stride]; }
 };
 
-// z-matrix row stride: the band's column count rounded up to 4 bytes
+// z-matrix row stride: the band's column count rounded up to 16 bytes
 G3A_HD int z_stride(int qlen, int w) {
   const int n_col = qlen < 2 * w + 1 ? qlen : 2 * w + 1;
-  return (n_col + 3) & ~3;
+  return (n_col + 15) & ~15;
 }
 
 // ksw_global2 (bwa/src/ksw.cu:706-812, DR-9): banded global alignment.
-// global_dp fills the direction bytes (a row's written four at a time) and
-// returns the score; global_walk turns them into the CIGAR.
+// global_dp fills the direction bytes (a row's written four at a time, or
+// gathered in eh.zst and written sixteen at a time) and returns the score;
+// global_walk turns them into the CIGAR.
 template <class Q, class T, class EH>
 G3A_HD int global_dp(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH eh, uint8_t *z) {
   const int MINUS_INF = -0x40000000;
@@ -1857,11 +1866,24 @@ G3A_HD int global_dp(const Opt &o, int qlen, Q tq, int tlen, T tt, int w, EH eh,
       const int c = j - beg;
       zw |= d << (8 * (c & 3));
       if ((c & 3) == 3) {
-        zi[c >> 2] = zw;
+        if constexpr (EH::kStage) eh.zst[c >> 2] = zw;
+        else zi[c >> 2] = zw;
         zw = 0;
       }
     }
-    if ((end - beg) & 3) zi[(end - beg) >> 2] = zw;
+    if constexpr (EH::kStage) {
+      if ((end - beg) & 3) eh.zst[(end - beg) >> 2] = zw;
+      // the row's words, whole 16-byte pieces (bytes past the band are
+      // never read)
+#if defined(__HIPCC__)
+      for (int q = 0; q < ((end - beg + 15) >> 4); ++q)
+        reinterpret_cast<uint4 *>(zi)[q] = reinterpret_cast<const uint4 *>(eh.zst)[q];
+#else
+      for (int q = 0; q < ((end - beg + 15) >> 4) * 4; ++q) zi[q] = eh.zst[q];
+#endif
+    } else {
+      if ((end - beg) & 3) zi[(end - beg) >> 2] = zw;
+    }
     eh.H(end) = h1;
     eh.E(end) = MINUS_INF;
   }

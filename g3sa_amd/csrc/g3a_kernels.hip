@@ -1681,6 +1681,7 @@ template <int RB>
 __global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a, int cls) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
   __shared__ int32_t ring_h[RB * 64], ring_e[RB * 64];
+  __shared__ uint4 zstage[64 * RB / 16];  // each lane's row of direction bytes, RB bytes
   const int lane = threadIdx.x;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + lane;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -1700,8 +1701,9 @@ __global__ void __launch_bounds__(64) k_traceback_dp(TbArgs a, int cls) {
                          a.z + a.z_off[t], a.cigar + a.cig_off[t], a.cig_off[t], out, a.tb_nw + t);
     else if (2 * w + 2 <= RB)
       trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t],
-                         EhRing<RB>{ring_h + lane, ring_e + lane, 64}, a.z + a.z_off[t], a.cigar + a.cig_off[t],
-                         a.cig_off[t], out);
+                         EhRing<RB>{ring_h + lane, ring_e + lane, 64,
+                                    reinterpret_cast<uint32_t *>(zstage + lane * (RB / 16))},
+                         a.z + a.z_off[t], a.cigar + a.cig_off[t], a.cig_off[t], out);
     else
       trace_and_finalize(a.idx, a.opt, a.seq + off, l_seq, g, a.aln_rank[t], EhArray{eh_h, eh_e},
                          a.z + a.z_off[t], a.cigar + a.cig_off[t], a.cig_off[t], out);
