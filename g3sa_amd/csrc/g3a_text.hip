@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(256) k_text_bound(TextArgs t, int64_t *bound) 
     u += base + (t.ctg_name_off[a.rid + 1] - t.ctg_name_off[a.rid]) + 12 * ((int64_t)a.n_cigar + a.n_core + 4) +
          2 * (a.re - a.rb) + 1;
   }
-  bound[q] = u;
+  bound[q] = (u + 3) & ~(int64_t)3;  // staging offsets stay 4-byte aligned
 }
 
 // flag |= some read's text is longer than its bound
@@ -498,15 +498,33 @@ __global__ void __launch_bounds__(256) k_text_check(int n, const int64_t *len, c
   if (q < n && len[q] > bound[q]) atomicOr((unsigned long long *)flag, 1ull);
 }
 
-// read q's text from the staging buffer to its place, a wavefront a read
+// read q's text from the staging buffer (4-byte aligned) to its place, a
+// wavefront a read: each lane writes aligned words of the destination, each
+// made of two source words shifted into place; the words at either end that
+// the read shares with its neighbours are written byte by byte.  A source
+// word read past the text stays inside the staging buffer (bounds are
+// rounded up to 4 bytes and the buffer has 16 to spare).
 __global__ void __launch_bounds__(256) k_text_pack(int n, const char *stage, const int64_t *stage_off,
                                                    const int64_t *len, const int64_t *txt_off, char *txt) {
   const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (q >= n) return;  // the whole wavefront
+  const int lane = threadIdx.x & 63;
   const char *src = stage + stage_off[q];
-  char *dst = txt + txt_off[q];
-  const int64_t l = len[q];
-  for (int64_t k = threadIdx.x & 63; k < l; k += 64) dst[k] = src[k];
+  const int64_t d0 = txt_off[q], l = len[q], d1 = d0 + l;
+  if (l <= 0) return;
+  const uint32_t *sw = reinterpret_cast<const uint32_t *>(src);
+  uint32_t *dw = reinterpret_cast<uint32_t *>(txt);
+  const int64_t w0 = d0 >> 2, w1 = (d1 + 3) >> 2;
+  for (int64_t k = w0 + lane; k < w1; k += 64) {
+    const int64_t a = 4 * k;
+    if (a >= d0 && a + 4 <= d1) {
+      const int64_t o = a - d0;  // source offset of the word's first byte, >= 0
+      const uint32_t lo = sw[o >> 2], hi = sw[(o >> 2) + 1];
+      dw[k] = __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(o & 3));
+    } else {
+      for (int64_t x = a > d0 ? a : d0, e = a + 4 < d1 ? a + 4 : d1; x < e; ++x) txt[x] = src[x - d0];
+    }
+  }
 }
 
 // MAPQ and MD-string length of every alignment (thread per alignment;
