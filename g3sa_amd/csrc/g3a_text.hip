@@ -498,24 +498,27 @@ __global__ void __launch_bounds__(256) k_text_check(int n, const int64_t *len, c
   if (q < n && len[q] > bound[q]) atomicOr((unsigned long long *)flag, 1ull);
 }
 
-// read q's text from the staging buffer (4-byte aligned) to its place, a
-// wavefront a read: each lane writes aligned words of the destination, each
+// read q's text from the staging buffer (4-byte aligned) to its place, 16
+// lanes a read (a wavefront moves four reads at once: the copy is a few
+// words a lane, so its time is the loads' latency): each lane writes aligned
+// words of the destination, each
 // made of two source words shifted into place; the words at either end that
 // the read shares with its neighbours are written byte by byte.  A source
 // word read past the text stays inside the staging buffer (bounds are
 // rounded up to 4 bytes and the buffer has 16 to spare).
 __global__ void __launch_bounds__(256) k_text_pack(int n, const char *stage, const int64_t *stage_off,
                                                    const int64_t *len, const int64_t *txt_off, char *txt) {
-  const int q = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (q >= n) return;  // the whole wavefront
-  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 16 + (threadIdx.x >> 4);
+  if (q >= n) return;
+  const int lane = threadIdx.x & 15;
   const char *src = stage + stage_off[q];
   const int64_t d0 = txt_off[q], l = len[q], d1 = d0 + l;
   if (l <= 0) return;
   const uint32_t *sw = reinterpret_cast<const uint32_t *>(src);
   uint32_t *dw = reinterpret_cast<uint32_t *>(txt);
   const int64_t w0 = d0 >> 2, w1 = (d1 + 3) >> 2;
-  for (int64_t k = w0 + lane; k < w1; k += 64) {
+#pragma unroll 4
+  for (int64_t k = w0 + lane; k < w1; k += 16) {
     const int64_t a = 4 * k;
     if (a >= d0 && a + 4 <= d1) {
       const int64_t o = a - d0;  // source offset of the word's first byte, >= 0
@@ -586,8 +589,8 @@ void launch_text_check(int n, const int64_t *len, const int64_t *bound, int64_t 
 void launch_text_pack(int n, const char *stage, const int64_t *stage_off, const int64_t *len, const int64_t *txt_off,
                       char *txt, hipStream_t s) {
   if (n > 0)
-    hipLaunchKernelGGL(k_text_pack, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, stage, stage_off, len, txt_off,
-                       txt);
+    hipLaunchKernelGGL(k_text_pack, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, s, n, stage, stage_off, len,
+                       txt_off, txt);
 }
 
 void launch_aln_mapq_md(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, int32_t *mapq, int64_t *md_len,
