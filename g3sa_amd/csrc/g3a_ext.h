@@ -14,9 +14,13 @@
 // edge, the column limit 32*(i/32+1)+w+1, the corner rule and the quirk rows
 // (only in the few columns of a strip where one of them can apply to some
 // lane; the other columns run without any test).  It admits every task
-// whose scores fit its 12-bit cells; tasks whose best score equals h0 after
-// at least 32 rows (where the reference's 32-lane residue rule decides the
-// position) go to the one-wavefront-per-task kernel.
+// whose scores fit its 12-bit cells.  When no cell exceeds h0, the
+// reference's position is decided by its 32-lane residue rule: no cell
+// unless every residue of the rows mod 32 holds a cell equal to h0, and then
+// the first such row and its last such column.  While the best is h0 the
+// engine records, in the rare columns whose maximum equals h0, which rows
+// hold h0 (cells outside the target or query, or outside the band, never
+// do: they are below every real cell or 0).
 //
 // Exact pruning: after each strip the lane stops once no later cell can
 // reach the current best.  A later cell's value comes from a positive cell of
@@ -139,16 +143,16 @@ G3A_HD bool ext_lane_opt_ok(const Opt &o) {
 }
 
 // Can the engine compute this task exactly?  (Scores must fit its 12-bit
-// cells; every rule of the cell set is applied in the general columns.)
+// cells; every rule of the cell set is applied in the general columns; h0 is
+// at least 1, so that no cell outside the cell set, all 0, can equal it.)
 G3A_HD bool ext_lane_ok(const Opt &o, int qlen, int w, int h0) {
   (void)w;
-  if (qlen <= 0 || qlen > kExtQCap || h0 < 0) return false;
+  if (qlen <= 0 || qlen > kExtQCap || h0 < 1) return false;
   return h0 + mat_max(o) * qlen <= 4095;
 }
 
 struct ExtLaneOut {
   int score, qle, tle;
-  int redo;  // best == h0 after >= 32 rows: the position needs the full rule
   int64_t cells;
 };
 
@@ -187,6 +191,20 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
     rowbuf[(int64_t)j * rs] = (uint32_t)hv | (qc << 24);
   }
   int best = h0, bi = -1, bj = -1;
+  // while best == h0: the row residues (mod 32) holding a cell equal to h0,
+  // the first such row and its last such column
+  uint32_t h0_rows = 0;
+  int h0_i = -1, h0_j = -1;
+  auto h0_note = [&](int row, int col) {
+    if (row >= tlen || col >= qlen) return;
+    h0_rows |= 1u << (row & 31);
+    if (h0_i < 0 || row < h0_i) {
+      h0_i = row;
+      h0_j = col;
+    } else if (row == h0_i && col > h0_j) {
+      h0_j = col;
+    }
+  };
   int64_t cells = 0;
   int i0 = 0;
   for (;;) {
@@ -297,6 +315,11 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
         if (r & 1) cmax = max3i(cmax, hc[r - 1], h);
       }
       rowbuf[(int64_t)j * rs] = (uint32_t)hc[R - 1] | ((uint32_t)e << 12) | (qc << 24);
+      if (best == h0 && cmax == h0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (hc[r] == h0) h0_note(i0 + r, j);
+      }
       // the column's best cell: only when it can change the result, find the
       // smallest row holding the column maximum
       if (cmax > best || (cmax == best && bi >= i0)) {
@@ -390,6 +413,11 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
 #pragma unroll
         for (int r = 0; r < H; ++r) plain_cell(r, e, cmax_u);
         e_mid = e;
+        if (best == h0 && cmax_u == h0) {
+#pragma unroll
+          for (int r = 0; r < H; ++r)
+            if (hc[r] == h0) h0_note(i0 + r, c0);
+        }
         if (cmax_u > best || (cmax_u == best && bi >= i0)) {
 #pragma unroll
           for (int r = H - 1; r >= 0; --r)
@@ -439,6 +467,13 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
           cm = p2_max(cm, h);
         }
         e_mid = p2_lo(eP);
+        if (best == h0 && (p2_lo(cm) == h0 || p2_hi(cm) == h0)) {
+#pragma unroll
+          for (int r = 0; r < H; ++r) {
+            if (p2_lo(hP[r]) == h0) h0_note(i0 + r, c);
+            if (p2_hi(hP[r]) == h0) h0_note(i0 + H + r, c - 1);
+          }
+        }
         // column c-1 is complete
         const int hl = p2_hi(hP[H - 1]);
         rowbuf[(int64_t)(c - 1) * rs] = (uint32_t)hl | ((uint32_t)p2_hi(eP) << 12) | (qcp << 24);
@@ -472,6 +507,11 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
       int e = e_mid, cmax_l = 0;
 #pragma unroll
       for (int r = H; r < R; ++r) plain_cell(r, e, cmax_l);
+      if (best == h0 && cmax_l == h0) {
+#pragma unroll
+        for (int r = H; r < R; ++r)
+          if (hc[r] == h0) h0_note(i0 + r, c1 - 1);
+      }
       rowbuf[(int64_t)(c1 - 1) * rs] = (uint32_t)hc[R - 1] | ((uint32_t)e << 12) | (qcp << 24);
       finish(c1 - 1, cmax_u, rmin_u, cmax_l, [&](int r) { return hc[H + r]; }, hc[R - 1]);
     };
@@ -517,11 +557,16 @@ G3A_HD ExtLaneOut ext_lane(const Opt &o, int qlen, QB qb, int tlen, TB tb, int w
     }
     if ((u > ub ? u : ub) < best) break;
   }
+  if (best == h0) {  // the residue rule (bi and bj are still -1)
+    if (h0_rows == 0xffffffffu) {
+      bi = h0_i;
+      bj = h0_j;
+    }
+  }
   ExtLaneOut out;
   out.score = best;
   out.qle = bj + 1;
   out.tle = bi + 1;
-  out.redo = best == h0 && (i0 < tlen ? i0 : tlen) >= 32;
   out.cells = cells;
   return out;
 }

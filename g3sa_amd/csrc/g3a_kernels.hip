@@ -1486,7 +1486,6 @@ __global__ void __launch_bounds__(64) k_ext_lane(ExtSideArgs a) {
       ExtRes o{r.score, r.qle, r.tle, 0};
       a.res[t] = o;
       cells += r.cells;
-      if (r.redo) a.slow_list[atomicAdd(a.ctr + 1, 1)] = (int32_t)t;
     }
   }
   if (a.cells) {
