@@ -1915,7 +1915,12 @@ struct Aln {
 
 // Cigar words an alignment may need: the global-alignment CIGAR (at most
 // lq + rlen operations) followed by the clipped reference-format copy.
-G3A_HD int64_t aln_cigar_cap(int lq, int64_t rlen) { return 2 * ((int64_t)lq + rlen) + 4; }
+// An ungapped alignment (w = 0) needs at most three operations (clip, match,
+// clip) and one core operation: eight words, so that neighbouring alignments
+// share cache lines instead of each writing a few bytes into its own.
+G3A_HD int64_t aln_cigar_cap(int lq, int64_t rlen, int w) { return w == 0 ? 8 : 2 * ((int64_t)lq + rlen) + 4; }
+// Offset of the core CIGAR inside an alignment's words.
+G3A_HD int64_t aln_core_at(int lq, int64_t rlen, int w) { return w == 0 ? 4 : (int64_t)lq + rlen + 4; }
 // Direction bytes of the global alignment.
 G3A_HD int64_t aln_z_size(int lq, int64_t rlen, int w) {
   if (w == 0) return 0;
@@ -1924,7 +1929,7 @@ G3A_HD int64_t aln_z_size(int lq, int64_t rlen, int w) {
 
 // traceback (aux.cu:2874-2949) and finalize (aux.cu:2952-3059) of the ai-th
 // kept region of a read of length l_seq.  `read` is the read, `cig` has
-// aln_cigar_cap() words (core at cig + lq + rlen + 4 - ... see below), z has
+// aln_cigar_cap() words (the core CIGAR at aln_core_at()), z has
 // aln_z_size() bytes, eh_h/eh_e lq+1 entries.
 // pre_nw: when not null, the direction bytes and the score (*pre_nw) of the
 // global alignment are already computed (k_tb_wave_dp) and only the walk runs.
@@ -1938,7 +1943,7 @@ G3A_HD void trace_and_finalize(const DevIndex &x, const Opt &o, const uint8_t *r
   const uint8_t *query = read + g.qb;
   const int64_t rb = g.rb;
   const int w = global_bw(o, lq, rlen, g.score);
-  const int64_t core_at = (int64_t)lq + rlen + 4;  // core CIGAR offset inside this record
+  const int64_t core_at = aln_core_at(lq, rlen, w);  // core CIGAR offset inside this record
   uint32_t *core = cig + core_at;
   int n_core = 0, nw = 0;
   TextReader tr(x.pac, l_pac);

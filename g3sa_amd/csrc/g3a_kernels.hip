@@ -1627,7 +1627,7 @@ __global__ void __launch_bounds__(128) k_aln_prep(AlnPrepArgs a) {
     const int64_t rlen = g.re - g.rb;
     const int w = global_bw(a.opt, lq, rlen, g.score);
     a.z_size[dst + k] = aln_z_size(lq, rlen, w);
-    a.cig_size[dst + k] = aln_cigar_cap(lq, rlen);
+    a.cig_size[dst + k] = aln_cigar_cap(lq, rlen, w);
   }
 }
 
