@@ -437,7 +437,8 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(reads are independent: see DESIGN.md for how the path shards)", file=sys.stderr)
+              "(reads are independent: a multi-GPU run would give each GPU its share of a batch's reads, "
+              "each with its own copy of the index)", file=sys.stderr)
         return 2
     cfg = CONFIGS[args.config]
     log("random-gather ceiling")
