@@ -1345,29 +1345,6 @@ __device__ __forceinline__ int wave_min_i32(int v) {
 
 // Slot of this lane in a wave-aggregated append (one atomic per wavefront).
 // Every lane of the wavefront must call it.
-// The same for a 256-thread block: one atomic per block (atomics on one
-// counter from every wavefront of a large grid serialise at the L2).  Every
-// thread of the block must call it; `sh` is 5 ints of LDS.
-__device__ __forceinline__ int block_reserve(bool pred, int32_t *ctr, int32_t *sh) {
-  const uint64_t m = __ballot(pred);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) sh[wave] = __popcll(m);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int tot = 0;
-    for (int w = 0; w < 4; ++w) {
-      const int c = sh[w];
-      sh[w] = tot;
-      tot += c;
-    }
-    sh[4] = tot ? atomicAdd(ctr, tot) : 0;
-  }
-  __syncthreads();
-  const int r = pred ? sh[4] + sh[wave] + __popcll(m & ((1ull << lane) - 1)) : -1;
-  __syncthreads();  // sh is free for the next call
-  return r;
-}
-
 __device__ __forceinline__ int wave_reserve(bool pred, int32_t *ctr) {
   const uint64_t m = __ballot(pred);
   if (!m) return -1;
@@ -1413,29 +1390,52 @@ __device__ __forceinline__ SideTask side_task(const ExtSideArgs &a, int64_t t) {
   return st;
 }
 
+// Tiles of kTilePer * 256 elements for the classifying kernels: a tile
+// counts its members in LDS and reserves its places with one global atomic a
+// counter (a block of 256 doing so contended on the few counters).  The
+// order of a list's entries is that of the reservations; its consumers
+// compute each entry on its own.
+constexpr int kTilePer = 16;
+
 __global__ void __launch_bounds__(256) k_ext_classify(ExtSideArgs a) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = t < dcount(a.n_dev, a.n_tasks);
-  bool lane = false, slow = false;
-  uint32_t key = 0;  // 0 for tasks not on the lane engine and past the tasks
-  if (in) {
-    const SideTask st = side_task(a, t);
-    if (st.qlen <= 0) {
-      ExtRes r{st.h0, 0, 0, 0};
-      a.res[t] = r;
-    } else {
-      const int w = ext_band_w(a.opt, st.qlen, st.end_bonus);
-      lane = a.lane_opt_ok && ext_lane_ok(a.opt, st.qlen, w, st.h0);
-      slow = !lane;
+  __shared__ int32_t cnt[2], base[2];
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t tile = (int64_t)blockIdx.x * (256 * kTilePer);
+  const int64_t n = dcount(a.n_dev, a.n_tasks);
+  int n_lane = 0, slow_at[kTilePer];
+#pragma unroll
+  for (int i = 0; i < kTilePer; ++i) {
+    const int64_t t = tile + i * 256 + threadIdx.x;
+    bool lane = false, slow = false;
+    uint32_t key = 0;  // 0 for tasks not on the lane engine and past the tasks
+    if (t < n) {
+      const SideTask st = side_task(a, t);
+      if (st.qlen <= 0) {
+        ExtRes r{st.h0, 0, 0, 0};
+        a.res[t] = r;
+      } else {
+        const int w = ext_band_w(a.opt, st.qlen, st.end_bonus);
+        lane = a.lane_opt_ok && ext_lane_ok(a.opt, st.qlen, w, st.h0);
+        slow = !lane;
+      }
+      if (lane) key = (uint32_t)st.qlen;
     }
-    if (lane) key = (uint32_t)st.qlen;
+    if (t < a.n_tasks) a.key[t] = key;
+    n_lane += lane;
+    slow_at[i] = slow ? atomicAdd(&cnt[1], 1) : -1;
   }
-  if (t < a.n_tasks) a.key[t] = key;
-  __shared__ int32_t sh[5];
-  block_reserve(lane, a.ctr, sh);
-  const int slot = block_reserve(slow, a.ctr + 1, sh);
-  if (slow) a.slow_list[slot] = (int32_t)t;
+  if (n_lane) atomicAdd(&cnt[0], n_lane);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (cnt[0]) atomicAdd(a.ctr, cnt[0]);
+    base[1] = cnt[1] ? atomicAdd(a.ctr + 1, cnt[1]) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kTilePer; ++i)
+    if (slow_at[i] >= 0) a.slow_list[base[1] + slow_at[i]] = (int32_t)(tile + i * 256 + threadIdx.x);
 }
 
 template <int R, bool SYM, bool PK>
@@ -1642,32 +1642,55 @@ __device__ __forceinline__ int tb_class(const TbArgs &a, int64_t t) {
   const int w = global_bw(a.opt, g.qe - g.qb, g.re - g.rb, g.score);
   return w == 0 ? 3 : 2 * w + 2 <= 16 ? 0 : 2 * w + 2 <= 32 ? 1 : 2;
 }
-// tb_ctr[0..2]: alignments of classes 0..2
+// tb_ctr[0..2]: alignments of classes 0..2 (tiles of kTilePer * 256)
 __global__ void __launch_bounds__(256) k_tb_count(TbArgs a) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = t < dcount(a.n_dev, a.n_alns) ? tb_class(a, t) : 3;
-  __shared__ int32_t sh[5];
-  for (int k = 0; k < 3; ++k) block_reserve(c == k, a.tb_ctr + k, sh);
+  __shared__ int32_t cnt[3];
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t tile = (int64_t)blockIdx.x * (256 * kTilePer);
+  const int64_t n = dcount(a.n_dev, a.n_alns);
+  int c3[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < kTilePer; ++i) {
+    const int64_t t = tile + i * 256 + threadIdx.x;
+    const int c = t < n ? tb_class(a, t) : 3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c3[k] += c == k;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (c3[k]) atomicAdd(&cnt[k], c3[k]);
+  __syncthreads();
+  if (threadIdx.x < 3 && cnt[threadIdx.x]) atomicAdd(a.tb_ctr + threadIdx.x, cnt[threadIdx.x]);
 }
 // The list holds classes 0, 1, 2 in turn from the front and class 3 from the
-// back; tb_ctr[4..7] are the placement cursors.
+// back; tb_ctr[4..7] are the placement cursors (tiles of kTilePer * 256).
 __global__ void __launch_bounds__(256) k_tb_order(TbArgs a) {
   if (a.stop && *a.stop) return;  // a capacity check before this kernel failed: the batch is redone
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int32_t cnt[4], base[4];
+  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t tile = (int64_t)blockIdx.x * (256 * kTilePer);
   const int64_t n_alns = dcount(a.n_dev, a.n_alns);
-  const bool in = t < n_alns;
-  const int c = in ? tb_class(a, t) : -1;
-  const int64_t base1 = a.tb_ctr[0], base2 = base1 + a.tb_ctr[1];
-  int at = -1;
-  __shared__ int32_t sh[5];
-  for (int k = 0; k < 4; ++k) {
-    const int r = block_reserve(c == k, a.tb_ctr + 4 + k, sh);
-    if (c == k) at = r;
+  int cls[kTilePer], at[kTilePer];
+#pragma unroll
+  for (int i = 0; i < kTilePer; ++i) {
+    const int64_t t = tile + i * 256 + threadIdx.x;
+    cls[i] = t < n_alns ? tb_class(a, t) : -1;
+    at[i] = cls[i] >= 0 ? atomicAdd(&cnt[cls[i]], 1) : -1;
   }
-  if (in) {
-    const int64_t pos = c == 0 ? at : c == 1 ? base1 + at : c == 2 ? base2 + at : n_alns - 1 - at;
-    a.tb_order[pos] = (int32_t)t;
+  __syncthreads();
+  if (threadIdx.x < 4) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(a.tb_ctr + 4 + threadIdx.x, cnt[threadIdx.x]) : 0;
+  __syncthreads();
+  const int64_t base1 = a.tb_ctr[0], base2 = base1 + a.tb_ctr[1];
+#pragma unroll
+  for (int i = 0; i < kTilePer; ++i) {
+    const int c = cls[i];
+    if (c < 0) continue;
+    const int r = base[c] + at[i];
+    const int64_t pos = c == 0 ? r : c == 1 ? base1 + r : c == 2 ? base2 + r : n_alns - 1 - r;
+    a.tb_order[pos] = (int32_t)(tile + i * 256 + threadIdx.x);
   }
 }
 
@@ -1926,7 +1949,7 @@ void launch_extend(const ExtArgs &a, hipStream_t s) {
   if (a.n_tasks) hipLaunchKernelGGL(k_extend, dim3((unsigned)a.n_tasks), dim3(64), 0, s, a);
 }
 void launch_ext_classify(const ExtSideArgs &a, hipStream_t s) {
-  if (a.n_tasks) hipLaunchKernelGGL(k_ext_classify, dim3(blocks(a.n_tasks, 256)), dim3(256), 0, s, a);
+  if (a.n_tasks) hipLaunchKernelGGL(k_ext_classify, dim3(blocks(a.n_tasks, 256 * kTilePer)), dim3(256), 0, s, a);
 }
 void launch_ext_lane(const ExtSideArgs &a, int strip_rows, int packed, int n_blocks, hipStream_t s) {
   if (!a.n_tasks) return;
@@ -1965,9 +1988,9 @@ int64_t traceback_pool_words(int n_threads) { return (int64_t)n_threads * 2 * kQ
 void launch_traceback(const TbArgs &a, int n_threads, int n_cu, hipStream_t s) {
   if (!a.n_alns) return;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, a.tb_ctr, 8, 0);
-  const dim3 g256((unsigned)((a.n_alns + 255) / 256));
-  hipLaunchKernelGGL(k_tb_count, g256, dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_tb_order, g256, dim3(256), 0, s, a);
+  const dim3 gt((unsigned)((a.n_alns + 256 * kTilePer - 1) / (256 * kTilePer)));
+  hipLaunchKernelGGL(k_tb_count, gt, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tb_order, gt, dim3(256), 0, s, a);
   // the pool has a slice for each of the n_threads threads
   auto dp_blocks = [&](const void *k) {
     int per_cu = 0;

@@ -228,6 +228,11 @@ def test_api_error_paths(ds_c1):
         assert lib.g3a_memcpy_input(ctx, 51, seq.ctypes.data, big.ctypes.data) == -1   # > max_reads
         assert lib.g3a_set_form(ctx, 7) == -1
         assert lib.g3a_set_tuning(ctx, 24, 8, 8) == -1
+        assert lib.g3a_set_seed_grid(ctx, 100, 1) == -1          # not a multiple of 64
+        assert lib.g3a_set_seed_grid(ctx, 2048, 1) == -1         # over 1024 threads
+        assert lib.g3a_set_seed_grid(ctx, 256, -1) == -1
+        assert lib.g3a_set_chain_serial(ctx, 2) == -1
+        assert lib.g3a_set_seed_grid(ctx, 256, 4) == 0
         # still usable
         assert lib.g3a_memcpy_input(ctx, 50, seq.ctypes.data, off.ctypes.data) == 0
         assert lib.g3a_bwa_align(ctx) == 0
