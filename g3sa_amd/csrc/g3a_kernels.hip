@@ -71,6 +71,16 @@ __device__ __forceinline__ T gather(const T (&v)[NC], int e) {  // e per lane; c
   }
   return x;
 }
+// The next item of a work counter for each active lane: one atomic per
+// wavefront, its active lanes taking consecutive items.
+__device__ __forceinline__ int wave_next(int32_t *ctr) {
+  const uint64_t m = __ballot(true);
+  const int leader = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)m) - 1);
+  int base = 0;
+  if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(ctr, __popcll(m));
+  base = __builtin_amdgcn_readlane(base, leader);
+  return base + __popcll(m & ((1ull << (threadIdx.x & 63)) - 1));
+}
 __device__ __forceinline__ void wave_lds_sync() {  // this wavefront's LDS stores are seen by its lanes
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -82,7 +92,8 @@ __device__ __forceinline__ void wave_lds_sync() {  // this wavefront's LDS store
 // seeding
 // ---------------------------------------------------------------------------
 // Persistent grid: each thread takes the next read from a device counter
-// (one atomic per read), so that a thread held up by a long read takes fewer
+// (one atomic per wavefront and turn, its active lanes taking consecutive
+// reads), so that a thread held up by a long read takes fewer
 // (r = tid, tid + T, ... when no counter is given).  Its LEP
 // lists live in a global pool slice of its own, contiguous per thread (a
 // divergent list access then touches one cache line; private memory would
@@ -100,8 +111,8 @@ __device__ __forceinline__ void seed_reads(SeedArgs &a, B &buf, int gtid, int nt
   // learned-index form: range lookups, lookups answered by the FM-index,
   // text runs, run bases, lookups the model narrowed, model windows missed
   int64_t lc[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = a.next_read ? atomicAdd(a.next_read, 1) : gtid; i < a.n_reads;
-       i = a.next_read ? atomicAdd(a.next_read, 1) : i + nthreads) {
+  for (int i = a.next_read ? wave_next(a.next_read) : gtid; i < a.n_reads;
+       i = a.next_read ? wave_next(a.next_read) : i + nthreads) {
     const int r = a.order ? a.order[i] : i;
     const int off = a.off[r], len = a.off[r + 1] - off;
     const uint8_t *q = a.seq + off;
@@ -1341,19 +1352,6 @@ __device__ __forceinline__ int wave_min_i32(int v) {
     v = v < t ? v : t;
   }
   return v;
-}
-
-// Slot of this lane in a wave-aggregated append (one atomic per wavefront).
-// Every lane of the wavefront must call it.
-__device__ __forceinline__ int wave_reserve(bool pred, int32_t *ctr) {
-  const uint64_t m = __ballot(pred);
-  if (!m) return -1;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(ctr, __popcll(m));
-  base = __shfl(base, leader);
-  return pred ? base + __popcll(m & ((1ull << lane) - 1)) : -1;
 }
 
 // Inputs of one side of task t (ExtendingPairGenerate, bwa/src/aux.cu:2101-2192;
