@@ -62,6 +62,12 @@ class BatchOut(C.Structure):
     ]
 
 
+class PeStat(C.Structure):
+    """g3a_pestat: insert-size statistics of one pair orientation."""
+    _fields_ = [("low", C.c_int32), ("high", C.c_int32), ("failed", C.c_int32), ("avg", C.c_double),
+                ("std", C.c_double)]
+
+
 class LearnedInfo(C.Structure):
     """g3a_learned_info (include/g3a.h)."""
 
@@ -81,6 +87,7 @@ EXPORTS = [
     "g3a_set_seed_blocks", "g3a_autotune_seed", "g3a_set_seed_order", "g3a_set_learned", "g3a_learned_stats",
     "g3a_last_intervals", "g3a_set_ext_packed", "g3a_ext_packed", "g3a_build_index_dev", "g3a_memcpy_text",
     "g3a_format_records", "g3a_copy_text", "g3a_bwa_align_noalloc", "g3a_set_wide_layout",
+    "g3a_mem_pestat", "g3a_mem_pair",
 ]
 
 _lib = None
@@ -128,6 +135,8 @@ def load_library() -> C.CDLL:
     lib.g3a_set_seed_order.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_set_ext_packed.argtypes = [C.c_void_p, C.c_int]
     lib.g3a_set_wide_layout.argtypes = [C.c_void_p, C.c_int]
+    lib.g3a_mem_pestat.argtypes = [C.c_void_p, C.c_int, C.POINTER(PeStat)]
+    lib.g3a_mem_pair.argtypes = [C.c_void_p, C.POINTER(PeStat), C.c_int64, C.c_void_p]
     lib.g3a_ext_packed.argtypes = [C.c_void_p]
     lib.g3a_set_learned.argtypes = [C.c_void_p, C.c_int, C.POINTER(LearnedInfo)]
     lib.g3a_learned_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
@@ -255,6 +264,22 @@ class Aligner:
     def set_seed_blocks(self, blocks_per_cu: int) -> None:
         """Persistent seeding blocks per compute unit (0: occupancy maximum)."""
         self._check(self.lib.g3a_set_seed_blocks(self.ctx, blocks_per_cu))
+
+    def mem_pestat(self, max_ins: int = 10000) -> list[dict]:
+        """Insert-size statistics of the last batch, read as pairs (2p, 2p+1),
+        per orientation FF, FR, RF, RR (bwa/src/aux.cu:435-500)."""
+        pes = (PeStat * 4)()
+        self._check(self.lib.g3a_mem_pestat(self.ctx, int(max_ins), pes))
+        return [{"low": p.low, "high": p.high, "failed": p.failed, "avg": p.avg, "std": p.std} for p in pes]
+
+    def mem_pair(self, pes: list[dict], id0: int = 0) -> np.ndarray:
+        """Best pair of hits per pair of the last batch (bwa/src/aux.cu:1000-1070):
+        rows of (score, sub, n_sub, hit of read 1, hit of read 2, proper-pair bit)."""
+        arr = (PeStat * 4)(*[PeStat(p["low"], p["high"], p["failed"], p["avg"], p["std"]) for p in pes])
+        n_pairs = self.n_reads // 2
+        out = np.zeros((max(n_pairs, 1), 6), dtype=np.int32)
+        self._check(self.lib.g3a_mem_pair(self.ctx, arr, int(id0), out.ctypes.data))
+        return out[:n_pairs]
 
     def set_chain_serial(self, on: bool) -> None:
         """Chain every read by the B-tree code on one thread (True) or by the tiers (False)."""

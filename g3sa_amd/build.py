@@ -32,7 +32,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result"]
 
-LIB_HIP = ["g3a_kernels.hip", "g3a_learned.hip", "g3a_index_gpu.hip", "g3a_text.hip", "g3a_api.cpp"]
+LIB_HIP = ["g3a_kernels.hip", "g3a_learned.hip", "g3a_index_gpu.hip", "g3a_text.hip", "g3a_pair.hip", "g3a_api.cpp"]
 LIB_CXX = ["index_build.cpp", "index_io.cpp"]
 
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -133,6 +134,7 @@ struct g3a_ctx {
   ScratchArena *arena = nullptr;  // set while an allocation-free call runs
   uint8_t *scratch = nullptr;     // this device's static scratch array
   DevBuf fmt_scan_tmp;            // scan storage of the formatter and output copy
+  DevBuf pe_hist, pe_stat, pe_v, pe_out;  // paired-end statistics and pair choice
   // grids of the running call (sizes of the per-thread pools)
   int run_seed_threads = 0, run_tb_threads = 0, run_lane_blocks = 0, run_slow_chunk = 16384;
   // batched-form extension
@@ -1327,6 +1329,61 @@ int g3a_copy_text(g3a_ctx *ctx, char *dst, int64_t cap) {
   return G3A_OK;
 }
 
+static_assert(sizeof(g3a_pestat) == sizeof(PeStat) && offsetof(g3a_pestat, avg) == offsetof(PeStat, avg),
+              "g3a_pestat and PeStat share one layout");
+
+// Paired-end arguments over the last batch's alignments (reads 2p, 2p+1 are
+// pair p).
+static int pair_args(g3a_ctx *ctx, PairArgs &a) {
+  if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "no alignment results");
+  if (ctx->n_reads & 1) return fail(ctx, G3A_E_ARG, "a paired batch holds an even number of reads");
+  a = PairArgs{};
+  a.n_pairs = ctx->n_reads / 2;
+  a.aln_off = P<int64_t>(ctx->w->aln_off);
+  a.alns = P<Aln>(ctx->w->alns);
+  a.ctg_off = P<int64_t>(ctx->ctg_off);
+  a.l_pac = ctx->idx.l_pac;
+  const Opt &o = ctx->opt;
+  a.a = o.a, a.b = o.b, a.o_del = o.o_del, a.e_del = o.e_del, a.o_ins = o.o_ins, a.e_ins = o.e_ins;
+  a.min_seed_len = o.min_seed_len;
+  a.mask_level = o.mask_level;
+  return 0;
+}
+
+int g3a_mem_pestat(g3a_ctx *ctx, int max_ins, g3a_pestat pes[4]) {
+  if (!ctx || !pes) return G3A_E_ARG;
+  if (max_ins < 0 || max_ins > G3A_PE_MAX_INS)
+    return fail(ctx, G3A_E_ARG, "max_ins must lie in [0, " + std::to_string(G3A_PE_MAX_INS) + "]");
+  HIPCK(hipSetDevice(ctx->device));
+  PairArgs a;
+  if (int rc = pair_args(ctx, a)) return rc;
+  a.max_ins = max_ins;
+  if (int rc = ensure(ctx, ctx->pe_hist, 16 * ((size_t)max_ins + 1))) return rc;
+  if (int rc = ensure(ctx, ctx->pe_stat, 4 * sizeof(PeStat))) return rc;
+  a.hist = P<uint32_t>(ctx->pe_hist);
+  a.stat = P<PeStat>(ctx->pe_stat);
+  launch_pe_stat(a, ctx->stream);
+  HIPCK(hipGetLastError());
+  return stage_d2h(ctx, pes, ctx->pe_stat.p, 4 * sizeof(PeStat));
+}
+
+int g3a_mem_pair(g3a_ctx *ctx, const g3a_pestat pes[4], int64_t id0, int32_t *out) {
+  if (!ctx || !pes || !out) return G3A_E_ARG;
+  HIPCK(hipSetDevice(ctx->device));
+  PairArgs a;
+  if (int rc = pair_args(ctx, a)) return rc;
+  if (a.n_pairs == 0) return 0;
+  if (int rc = ensure(ctx, ctx->pe_v, 16 * (size_t)std::max<int64_t>(ctx->total_alns, 1))) return rc;
+  if (int rc = ensure(ctx, ctx->pe_out, 24 * (size_t)a.n_pairs)) return rc;
+  std::memcpy(a.pes, pes, sizeof a.pes);
+  a.v = P<uint64_t>(ctx->pe_v);
+  a.out = P<int32_t>(ctx->pe_out);
+  a.id0 = id0;
+  launch_pe_pair(a, ctx->stream);
+  HIPCK(hipGetLastError());
+  return stage_d2h(ctx, out, ctx->pe_out.p, 24 * (size_t)a.n_pairs);
+}
+
 int g3a_memcpy_output(g3a_ctx *ctx, g3a_batch_out *out) {
   if (!ctx || !out) return G3A_E_ARG;
   if (!ctx->has_output) return fail(ctx, G3A_E_STATE, "no alignment results");
@@ -1800,7 +1857,7 @@ void g3a_destroy(g3a_ctx *ctx) {
                     &ctx->li_root, &ctx->li_leaf, &ctx->li_knot, &ctx->li_leaf_a, &ctx->li_short, &ctx->li_ctr,
                     &ctx->ctg_names, &ctx->ctg_name_off, &ctx->log_tab, &ctx->names, &ctx->name_off, &ctx->quals,
                     &ctx->qual_off, &ctx->txt_len, &ctx->txt_off, &ctx->txt, &ctx->txt_bound, &ctx->stage_off, &ctx->txt_stage, &ctx->fmt_flag, &ctx->mapq, &ctx->md_len,
-                    &ctx->md_off, &ctx->md, &ctx->fmt_scan_tmp};
+                    &ctx->md_off, &ctx->md, &ctx->fmt_scan_tmp, &ctx->pe_hist, &ctx->pe_stat, &ctx->pe_v, &ctx->pe_out};
   for (DevBuf *b : bufs) free_buf(*b);
   for (auto &ev : ctx->ev)
     if (ev) hipEventDestroy(ev);

@@ -216,6 +216,30 @@ void launch_aln_mapq_md(const TextArgs &t, int64_t n_alns, const int32_t *aln_re
                         hipStream_t s);
 void launch_aln_md_write(const TextArgs &t, int64_t n_alns, const int32_t *aln_read, const int64_t *md_off,
                          char *md, hipStream_t s);
+
+// Paired-end statistics and pair choice (g3a_pair.hip).  Reads 2p and 2p+1
+// are the two ends of pair p.
+struct PeStat {  // layout of g3a_pestat (bwa/src/bwa.h:250-254)
+  int32_t low, high, failed;
+  double avg, std;
+};
+struct PairArgs {
+  int64_t n_pairs;
+  const int64_t *aln_off;  // 2 * n_pairs + 1
+  const Aln *alns;
+  const int64_t *ctg_off;
+  int64_t l_pac;
+  int a, b, o_del, e_del, o_ins, e_ins, min_seed_len, max_ins;
+  float mask_level;
+  uint32_t *hist;  // 4 * (max_ins + 1)
+  PeStat *stat;    // 4, written by launch_pe_stat
+  PeStat pes[4];   // read by launch_pe_pair
+  uint64_t *v;     // two words per alignment
+  int32_t *out;    // six words per pair
+  int64_t id0;
+};
+void launch_pe_stat(const PairArgs &a, hipStream_t s);
+void launch_pe_pair(const PairArgs &a, hipStream_t s);
 // Occ lines (wide, and narrow with superblock counts when `narrow` is not
 // null) from the BWA .bwt body, rows 0..n
 void launch_occ_lines(const uint32_t *W, uint64_t n, uint64_t primary, uint32_t *wide, uint32_t *narrow,

@@ -100,6 +100,16 @@ extern "C" void memcpy_output(aligned_chunk *ac, process_data_t *p) {
   g3a_batch_out_free(&b);
 }
 
+// Paired-end statistics of the last batch (reads 2i, 2i+1 as pairs) in the
+// reference's own struct: mem_pestat (bwa/src/aux.cu:435-500), whose host
+// caller passes opt->max_ins (bwa/src/bwa.h:185) and the process's pes
+// (bwa/src/bwa.h:327).  The reference declares this port but never calls it.
+static_assert(sizeof(mem_pestat_t) == sizeof(g3a_pestat), "mem_pestat_t and g3a_pestat share one layout");
+extern "C" void g3a_shim_pestat(process_data_t *p, const mem_opt_t *opt, mem_pestat_t pes[4]) {
+  if (g3a_mem_pestat(ctx_of(p), opt->max_ins, reinterpret_cast<g3a_pestat *>(pes)) != G3A_OK)
+    die(ctx_of(p), "mem_pestat");
+}
+
 extern "C" void destruct_proc(process_data_t *p) {
   g3a_destroy(ctx_of(p));
   delete p;

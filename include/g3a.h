@@ -138,6 +138,33 @@ int g3a_memcpy_text(g3a_ctx *ctx, int n_reads, const char *names, const int64_t 
 int g3a_format_records(g3a_ctx *ctx, int mode, int64_t *bytes);
 /* Copies the formatted records to dst (cap >= their length). */
 int g3a_copy_text(g3a_ctx *ctx, char *dst, int64_t cap);
+/* Paired-end primitives over the last g3a_bwa_align, whose batch holds read
+ * pairs as reads 2p and 2p+1 (an odd n_reads is G3A_E_ARG).  The reference
+ * carries them as device ports that its pipeline never launches (it aligns
+ * pairs as single-end reads); its host caller would bind them where BWA calls
+ * mem_pestat and mem_pair.
+ *
+ * Insert-size distribution per orientation (0 FF, 1 FR, 2 RF, 3 RR) from the
+ * pairs whose ends have a unique top hit on one contig, with sizes in
+ * (0, max_ins]: mem_pestat, bwa/src/aux.cu:435-500 (reference default max_ins
+ * 10000, bwa/src/fastmap.cpp:97).  Layout of mem_pestat_t
+ * (bwa/src/bwa.h:250-254). */
+typedef struct {
+  int32_t low, high; /* proper-pair bounds of the insert size */
+  int32_t failed;    /* too few pairs in this orientation */
+  double avg, std;
+} g3a_pestat;
+#define G3A_PE_MAX_INS (1 << 20)
+int g3a_mem_pestat(g3a_ctx *ctx, int max_ins, g3a_pestat pes[4]);
+/* Best pair of hits per pair (mem_pair, bwa/src/aux.cu:1000-1070, with every
+ * hit of an end taken as primary) and the proper-pair bit of the two top hits
+ * (bwa/src/aux.cu:1288-1295).  out[6 * p + k], k = 0..5: pair score (0 when
+ * no consistent pair exists), second-best pair score, number of pairs within
+ * the gap-open distance of the second best, index of the chosen hit of read 1
+ * and of read 2 in their alignment lists (-1 when none), and 2 when the top
+ * hits form a proper pair under pes (else 0).  id0 is the id of the batch's
+ * first pair (the reference hashes pair ids into its tie-break). */
+int g3a_mem_pair(g3a_ctx *ctx, const g3a_pestat pes[4], int64_t id0, int32_t *out);
 /* memcpy_input + bwa_align + memcpy_output */
 int g3a_align_batch(g3a_ctx *ctx, int n_reads, const uint8_t *seq, const int32_t *offsets, g3a_batch_out *out);
 void g3a_batch_out_free(g3a_batch_out *out);

@@ -1273,6 +1273,195 @@ int orc_mapq(const orc_opt *opt, int score, int qb, int qe, int64_t rb, int64_t 
 }
 
 /* ------------------------------------------------------------------------ */
+/* paired-end primitives (device ports the reference never launches)          */
+/* ------------------------------------------------------------------------ */
+/* bwa/src/aux.cuh:404-409 */
+#define ORC_MIN_RATIO 0.8
+#define ORC_MIN_DIR_CNT 10
+#define ORC_MIN_DIR_RATIO 0.05
+#define ORC_OUTLIER_BOUND 2.0
+#define ORC_MAPPING_BOUND 3.0
+#define ORC_MAX_STDDEV 4.0
+
+/* mem_infer_dir, bwa/src/aux.cu:411-418 */
+int orc_infer_dir(int64_t l_pac, int64_t b1, int64_t b2, int64_t *dist) {
+  int64_t p2;
+  int r1 = (b1 >= l_pac), r2 = (b2 >= l_pac);
+  p2 = r1 == r2 ? b2 : (l_pac << 1) - 1 - b2;
+  *dist = p2 > b1 ? p2 - b1 : b1 - p2;
+  return (r1 == r2 ? 0 : 1) ^ (p2 > b1 ? 0 : 3);
+}
+
+/* cal_sub, bwa/src/aux.cu:420-431, over one read's hits (best first) */
+int orc_cal_sub(const orc_opt *opt, int n, const int32_t *qb, const int32_t *qe, const int32_t *score) {
+  int j;
+  for (j = 1; j < n; ++j) {
+    int b_max = qb[j] > qb[0] ? qb[j] : qb[0];
+    int e_min = qe[j] < qe[0] ? qe[j] : qe[0];
+    if (e_min > b_max) {
+      int min_l = qe[j] - qb[j] < qe[0] - qb[0] ? qe[j] - qb[j] : qe[0] - qb[0];
+      if (e_min - b_max >= min_l * opt->mask_level) break;
+    }
+  }
+  return j < n ? score[j] : opt->min_seed_len * opt->a;
+}
+
+static int cmp_u64(const void *x, const void *y) {
+  uint64_t a = *(const uint64_t *)x, b = *(const uint64_t *)y;
+  return a < b ? -1 : a > b;
+}
+
+/* mem_pestat, bwa/src/aux.cu:435-500: reads 2i and 2i+1 are the ends of
+ * pair i; each read's hits are aln_off[r]..aln_off[r+1], best first.  The
+ * sizes of each orientation are collected, sorted and scanned as there. */
+void orc_mem_pestat(const orc_opt *opt, int64_t l_pac, int max_ins, int n, const int64_t *aln_off,
+                    const int32_t *rid, const int64_t *rb, const int32_t *qb, const int32_t *qe,
+                    const int32_t *score, orc_pestat pes[4]) {
+  uint64_t *isize[4];
+  size_t cnt[4] = {0, 0, 0, 0}, max = 0;
+  int i, d;
+  memset(pes, 0, 4 * sizeof(orc_pestat));
+  for (d = 0; d < 4; ++d) isize[d] = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(n / 2 + 1));
+  for (i = 0; i < n >> 1; ++i) {
+    int64_t a0 = aln_off[2 * i], a1 = aln_off[2 * i + 1], e1 = aln_off[2 * i + 2], is;
+    int dir;
+    if (a1 == a0 || e1 == a1) continue;
+    if (orc_cal_sub(opt, (int)(a1 - a0), qb + a0, qe + a0, score + a0) > ORC_MIN_RATIO * score[a0]) continue;
+    if (orc_cal_sub(opt, (int)(e1 - a1), qb + a1, qe + a1, score + a1) > ORC_MIN_RATIO * score[a1]) continue;
+    if (rid[a0] != rid[a1]) continue;
+    dir = orc_infer_dir(l_pac, rb[a0], rb[a1], &is);
+    if (is && is <= max_ins) isize[dir][cnt[dir]++] = (uint64_t)is;
+  }
+  for (d = 0; d < 4; ++d) {
+    orc_pestat *r = &pes[d];
+    uint64_t *q = isize[d];
+    size_t qn = cnt[d], k;
+    int p25, p50, p75, x;
+    if (qn < ORC_MIN_DIR_CNT) {
+      r->failed = 1;
+      continue;
+    }
+    qsort(q, qn, sizeof(uint64_t), cmp_u64);
+    p25 = (int)q[(int)(.25 * qn + .499)];
+    p50 = (int)q[(int)(.50 * qn + .499)];
+    p75 = (int)q[(int)(.75 * qn + .499)];
+    (void)p50;
+    r->low = (int)(p25 - ORC_OUTLIER_BOUND * (p75 - p25) + .499);
+    if (r->low < 1) r->low = 1;
+    r->high = (int)(p75 + ORC_OUTLIER_BOUND * (p75 - p25) + .499);
+    for (k = 0, x = 0, r->avg = 0; k < qn; ++k)
+      if (q[k] >= (uint64_t)r->low && q[k] <= (uint64_t)r->high) r->avg += q[k], ++x;
+    r->avg /= x;
+    for (k = 0, r->std = 0; k < qn; ++k)
+      if (q[k] >= (uint64_t)r->low && q[k] <= (uint64_t)r->high) r->std += (q[k] - r->avg) * (q[k] - r->avg);
+    r->std = sqrt(r->std / x);
+    r->low = (int)(p25 - ORC_MAPPING_BOUND * (p75 - p25) + .499);
+    r->high = (int)(p75 + ORC_MAPPING_BOUND * (p75 - p25) + .499);
+    if (r->low > r->avg - ORC_MAX_STDDEV * r->std) r->low = (int)(r->avg - ORC_MAX_STDDEV * r->std + .499);
+    if (r->high < r->avg + ORC_MAX_STDDEV * r->std) r->high = (int)(r->avg + ORC_MAX_STDDEV * r->std + .499);
+    if (r->low < 1) r->low = 1;
+  }
+  for (d = 0; d < 4; ++d) max = max > cnt[d] ? max : cnt[d];
+  for (d = 0; d < 4; ++d)
+    if (pes[d].failed == 0 && cnt[d] < (int)max * ORC_MIN_DIR_RATIO) pes[d].failed = 1;
+  for (d = 0; d < 4; ++d) free(isize[d]);
+}
+
+/* hash_64, bwa/src/aux.cu:703-714 (Thomas Wang's 64-bit integer mix) */
+uint64_t orc_hash_64(uint64_t key) {
+  key += ~(key << 32);
+  key ^= (key >> 22);
+  key += ~(key << 13);
+  key ^= (key >> 8);
+  key += (key << 3);
+  key ^= (key >> 15);
+  key += ~(key << 27);
+  key ^= (key >> 31);
+  return key;
+}
+
+typedef struct { uint64_t x, y; } orc_pair64;
+static int cmp_pair64(const void *p, const void *q) {
+  const orc_pair64 *a = (const orc_pair64 *)p, *b = (const orc_pair64 *)q;
+  if (a->x != b->x) return a->x < b->x ? -1 : 1;
+  return a->y < b->y ? -1 : a->y > b->y;
+}
+
+/* mem_pair, bwa/src/aux.cu:1000-1070, for one pair whose ends have n0 and n1
+ * hits (all primary), given as arrays of n0 + n1 entries (read 1's first).
+ * Returns the best pair score (0 if none) and sets *sub, *n_sub and z[]. */
+int orc_mem_pair(const orc_opt *opt, int64_t l_pac, const int64_t *ctg_off, const orc_pestat pes[4], int id,
+                 int n0, int n1, const int32_t *rid, const int64_t *rb, const int32_t *score, int *sub,
+                 int *n_sub, int z[2]) {
+  orc_pair64 *v, *u;
+  size_t nv = 0, nu = 0, mu = 16;
+  int r, i, k, y[4], ret;
+  int nn[2] = {n0, n1};
+  v = (orc_pair64 *)malloc(sizeof(orc_pair64) * (size_t)(n0 + n1 + 1));
+  u = (orc_pair64 *)malloc(sizeof(orc_pair64) * mu);
+  for (r = 0; r < 2; ++r) {
+    for (i = 0; i < nn[r]; ++i) {
+      int e = r ? n0 + i : i;
+      orc_pair64 key;
+      key.x = rb[e] < l_pac ? (uint64_t)rb[e] : (uint64_t)((l_pac << 1) - 1 - rb[e]);
+      key.x = (uint64_t)rid[e] << 32 | (key.x - (uint64_t)ctg_off[rid[e]]);
+      key.y = (uint64_t)score[e] << 32 | (uint64_t)(i << 2 | (rb[e] >= l_pac) << 1 | r);
+      v[nv++] = key;
+    }
+  }
+  qsort(v, nv, sizeof(orc_pair64), cmp_pair64);
+  y[0] = y[1] = y[2] = y[3] = -1;
+  for (i = 0; i < (int)nv; ++i) {
+    for (r = 0; r < 2; ++r) {
+      int dir = r << 1 | (int)(v[i].y >> 1 & 1), which;
+      if (pes[dir].failed) continue;
+      which = r << 1 | (int)((v[i].y & 1) ^ 1);
+      if (y[which] < 0) continue;
+      for (k = y[which]; k >= 0; --k) {
+        int64_t dist;
+        int q;
+        double ns;
+        if ((int)(v[k].y & 3) != which) continue;
+        dist = (int64_t)v[i].x - (int64_t)v[k].x;
+        if (dist > pes[dir].high) break;
+        if (dist < pes[dir].low) continue;
+        ns = (dist - pes[dir].avg) / pes[dir].std;
+        {
+          double qd = (v[i].y >> 32) + (v[k].y >> 32) + .721 * log(2. * erfc(fabs(ns) * M_SQRT1_2)) * opt->a + .499;
+          q = qd > 0 ? (int)qd : 0; /* the reference's (int) of -inf or NaN, then clamped at 0 */
+        }
+        if (nu == mu) {
+          mu <<= 1;
+          u = (orc_pair64 *)realloc(u, sizeof(orc_pair64) * mu);
+        }
+        u[nu].y = (uint64_t)k << 32 | (uint64_t)(uint32_t)i;
+        u[nu].x = (uint64_t)q << 32 |
+                  (orc_hash_64(u[nu].y ^ (uint64_t)(int64_t)(int32_t)((uint32_t)id << 8)) & 0xffffffffU);
+        ++nu;
+      }
+    }
+    y[v[i].y & 3] = i;
+  }
+  if (nu) {
+    int tmp = opt->a + opt->b;
+    tmp = tmp > opt->o_del + opt->e_del ? tmp : opt->o_del + opt->e_del;
+    tmp = tmp > opt->o_ins + opt->e_ins ? tmp : opt->o_ins + opt->e_ins;
+    qsort(u, nu, sizeof(orc_pair64), cmp_pair64);
+    i = (int)(u[nu - 1].y >> 32);
+    k = (int)(u[nu - 1].y << 32 >> 32);
+    z[v[i].y & 1] = (int)(v[i].y << 32 >> 34);
+    z[v[k].y & 1] = (int)(v[k].y << 32 >> 34);
+    ret = (int)(u[nu - 1].x >> 32);
+    *sub = nu > 1 ? (int)(u[nu - 2].x >> 32) : 0;
+    for (i = (int)nu - 2, *n_sub = 0; i >= 0; --i)
+      if (*sub - (int)(u[i].x >> 32) <= tmp) ++*n_sub;
+  } else ret = 0, *sub = 0, *n_sub = 0;
+  free(u);
+  free(v);
+  return ret;
+}
+
+/* ------------------------------------------------------------------------ */
 /* per-read pipeline                                                         */
 /* ------------------------------------------------------------------------ */
 typedef struct {

@@ -123,6 +123,21 @@ int orc_ksw_global2(int qlen, const uint8_t *query, int tlen, const uint8_t *tar
                     const int8_t *mat, int o_del, int e_del, int o_ins, int e_ins, int w,
                     int *n_cigar, uint32_t *cigar, int cigar_cap);
 
+/* Paired-end primitives (bwa/src/aux.cu:411-500, 703-714, 1000-1070). */
+typedef struct {
+  int low, high, failed;
+  double avg, std;
+} orc_pestat;
+int orc_infer_dir(int64_t l_pac, int64_t b1, int64_t b2, int64_t *dist);
+int orc_cal_sub(const orc_opt *opt, int n, const int32_t *qb, const int32_t *qe, const int32_t *score);
+void orc_mem_pestat(const orc_opt *opt, int64_t l_pac, int max_ins, int n, const int64_t *aln_off,
+                    const int32_t *rid, const int64_t *rb, const int32_t *qb, const int32_t *qe,
+                    const int32_t *score, orc_pestat pes[4]);
+uint64_t orc_hash_64(uint64_t key);
+int orc_mem_pair(const orc_opt *opt, int64_t l_pac, const int64_t *ctg_off, const orc_pestat pes[4], int id,
+                 int n0, int n1, const int32_t *rid, const int64_t *rb, const int32_t *score, int *sub,
+                 int *n_sub, int z[2]);
+
 /* MAPQ (bwa/src/aux.cu:676-700) with sub = csub = sub_n = 0 and frac_rep = 0. */
 int orc_mapq(const orc_opt *opt, int score, int qb, int qe, int64_t rb, int64_t re);
 

@@ -48,6 +48,18 @@ class OrcBatch(C.Structure):
     ]
 
 
+class OrcPeStat(C.Structure):
+    _fields_ = [("low", C.c_int), ("high", C.c_int), ("failed", C.c_int), ("avg", C.c_double), ("std", C.c_double)]
+
+
+def read_ann(prefix) -> tuple[int, np.ndarray]:
+    """l_pac and contig offsets from a BWA .ann file."""
+    lines = Path(str(prefix) + ".ann").read_text().split("\n")
+    l_pac, n_seqs = int(lines[0].split()[0]), int(lines[0].split()[1])
+    offs = [int(lines[2 + 2 * i].split()[0]) for i in range(n_seqs)]
+    return l_pac, np.array(offs, dtype=np.int64)
+
+
 class Oracle:
     def __init__(self, lib_path: Path | None = None):
         path = Path(lib_path) if lib_path is not None else LIB
@@ -83,9 +95,54 @@ class Oracle:
         lib.orc_index_contig_name.restype = C.c_char_p
         lib.orc_index_n_contigs.argtypes = [C.c_void_p]
         lib.orc_mapq.argtypes = [C.POINTER(OrcOpt), C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64]
+        lib.orc_infer_dir.argtypes = [C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_int64)]
+        lib.orc_cal_sub.argtypes = [C.POINTER(OrcOpt), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.orc_mem_pestat.argtypes = [C.POINTER(OrcOpt), C.c_int64, C.c_int, C.c_int] + [C.c_void_p] * 6 + \
+            [C.POINTER(OrcPeStat)]
+        lib.orc_hash_64.argtypes = [C.c_uint64]
+        lib.orc_hash_64.restype = C.c_uint64
+        lib.orc_mem_pair.argtypes = [C.POINTER(OrcOpt), C.c_int64, C.c_void_p, C.POINTER(OrcPeStat), C.c_int,
+                                     C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
         self.lib = lib
         self.opt = OrcOpt()
         lib.orc_opt_init(C.byref(self.opt))
+
+    # paired-end primitives ------------------------------------------------
+    def infer_dir(self, l_pac, b1, b2):
+        d = C.c_int64()
+        r = self.lib.orc_infer_dir(l_pac, b1, b2, C.byref(d))
+        return r, d.value
+
+    def cal_sub(self, qb, qe, score):
+        qb, qe, score = (np.ascontiguousarray(v, dtype=np.int32) for v in (qb, qe, score))
+        return self.lib.orc_cal_sub(C.byref(self.opt), len(qb), qb.ctypes.data, qe.ctypes.data, score.ctypes.data)
+
+    def pestat(self, l_pac, aln_off, rid, rb, qb, qe, score, max_ins=10000) -> list[dict]:
+        """mem_pestat over reads 2i, 2i+1 as pairs; per-read hits aln_off[r]..aln_off[r+1]."""
+        aln_off = np.ascontiguousarray(aln_off, dtype=np.int64)
+        rid, qb, qe, score = (np.ascontiguousarray(v, dtype=np.int32) for v in (rid, qb, qe, score))
+        rb = np.ascontiguousarray(rb, dtype=np.int64)
+        pes = (OrcPeStat * 4)()
+        self.lib.orc_mem_pestat(C.byref(self.opt), l_pac, max_ins, len(aln_off) - 1, aln_off.ctypes.data,
+                                rid.ctypes.data, rb.ctypes.data, qb.ctypes.data, qe.ctypes.data, score.ctypes.data,
+                                pes)
+        return [{"low": p.low, "high": p.high, "failed": p.failed, "avg": p.avg, "std": p.std} for p in pes]
+
+    def hash_64(self, key: int) -> int:
+        return self.lib.orc_hash_64(key)
+
+    def mem_pair(self, l_pac, ctg_off, pes, pair_id, rid, rb, score, n0):
+        """mem_pair of one pair: rid/rb/score hold read 1's n0 hits, then read 2's.
+        Returns (score, sub, n_sub, z0, z1)."""
+        arr = (OrcPeStat * 4)(*[OrcPeStat(p["low"], p["high"], p["failed"], p["avg"], p["std"]) for p in pes])
+        ctg_off = np.ascontiguousarray(ctg_off, dtype=np.int64)
+        rid, score = (np.ascontiguousarray(v, dtype=np.int32) for v in (rid, score))
+        rb = np.ascontiguousarray(rb, dtype=np.int64)
+        sub, n_sub, z = C.c_int(), C.c_int(), (C.c_int * 2)(-1, -1)
+        o = self.lib.orc_mem_pair(C.byref(self.opt), l_pac, ctg_off.ctypes.data, arr, pair_id, n0, len(rid) - n0,
+                                  rid.ctypes.data, rb.ctypes.data, score.ctypes.data, C.byref(sub), C.byref(n_sub), z)
+        return o, sub.value, n_sub.value, z[0], z[1]
 
     def mat(self, a=1, b=4) -> np.ndarray:
         m = np.full(25, -1, dtype=np.int8)
