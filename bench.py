@@ -257,6 +257,31 @@ def run_form(al, n: int, steps: int, warmup: int, form: str, autotune: bool = Tr
     }
 
 
+def paired_end_times(al, n: int, reps: int = 3) -> dict:
+    """Insert-size statistics and best-pair choice over the batch the context
+    aligned last (its reads are pairs 2p, 2p+1), after the timed steps: wall
+    time of each call, which ends with its results on the host (4 orientation
+    records; six int32 per pair)."""
+    if n % 2:
+        return None
+    ts, tp = [], []
+    pes, rows = None, None
+    for _ in range(reps):
+        t = time.perf_counter()
+        pes = al.mem_pestat(10000)
+        ts.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        rows = al.mem_pair(pes)
+        tp.append(time.perf_counter() - t)
+    fr = pes[1]
+    return {"pairs": n // 2, "pestat_ms": round(1e3 * min(ts), 3), "pair_ms": round(1e3 * min(tp), 3),
+            "pair_rows_bytes": 24 * (n // 2),
+            "fr": {"avg": round(fr["avg"], 3), "std": round(fr["std"], 3), "low": fr["low"], "high": fr["high"]},
+            "failed": [p["failed"] for p in pes],
+            "paired_frac": round(float((rows[:, 0] > 0).mean()), 4),
+            "proper_frac": round(float((rows[:, 5] > 0).mean()), 4)}
+
+
 def _part(batch, a: int, b: int):
     """Reads [a, b) of a loaded batch, with offsets starting at 0."""
     seq, off, names, name_off, quals, qual_off = batch
@@ -481,6 +506,8 @@ def main() -> int:
                 log(f"seeding autotuned (learned form): {learned_tuning}")
         per_form[form] = run_form(al, n, args.steps, args.warmup, form,
                                   autotune=not args.tuning and (form == "batched" or "batched" not in per_form))
+        per_form[form]["paired_end"] = paired_end_times(al, n)
+        log(f"[{form}] paired-end primitives: {per_form[form]['paired_end']}")
         # the headline step: halves on two contexts at the same time
         one = {k: per_form[form][k] for k in ("reads_per_s", "ms_per_step")}
         two = two_context_steps(al, batch, args.steps, args.warmup)
@@ -543,6 +570,7 @@ def main() -> int:
         "sam_format_ms": m["sam_format_ms"],
         "sam_bytes": m["sam_bytes"],
         "pcie_inclusive": pcie,
+        "paired_end": m["paired_end"],
         "stages_ms": mean,
         "kernel_ms": kms,
         "work": work,

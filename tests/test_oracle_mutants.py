@@ -15,6 +15,7 @@ from pathlib import Path
 import pytest
 
 import oracle_lib
+from test_oracle_pairs import check_pairs
 from test_oracle_pins import (chain_idx, check_chain, check_emitters, check_global, check_quirk,  # noqa: F401
                               check_reseed, emitter_case, planted_reseed)
 
@@ -62,6 +63,22 @@ MUTANTS = [
      "int clip_l = rev ? len - al->qe : al->qb, clip_r = rev ? al->qb : len - al->qe;",
      "int clip_l = al->qb, clip_r = len - al->qe;", "emit"),
     ("compat: unmapped position -1", "oracle_main.c", 'sputs(&s, " * 0 * ");', 'sputs(&s, " * -1 * ");', "emit"),
+    # paired-end primitives (bwa/src/aux.cu:411-500, 1000-1070)
+    ("pairs: orientation code", "oracle.c", "return (r1 == r2 ? 0 : 1) ^ (p2 > b1 ? 0 : 3);",
+     "return (r1 == r2 ? 0 : 1) ^ (p2 > b1 ? 0 : 2);", "pairs"),
+    ("pairs: strict overlap in cal_sub", "oracle.c", "if (e_min - b_max >= min_l * opt->mask_level) break;",
+     "if (e_min - b_max > min_l * opt->mask_level) break;", "pairs"),
+    ("pairs: outlier bound 1.5", "oracle.c", "#define ORC_OUTLIER_BOUND 2.0", "#define ORC_OUTLIER_BOUND 1.5",
+     "pairs"),
+    ("pairs: quartile index not rounded", "oracle.c", "p75 = (int)q[(int)(.75 * qn + .499)];",
+     "p75 = (int)q[(int)(.75 * qn)];", "pairs"),
+    ("pairs: sample deviation", "oracle.c", "r->std = sqrt(r->std / x);", "r->std = sqrt(r->std / (x - 1));",
+     "pairs"),
+    ("pairs: direction ratio 0.1", "oracle.c", "#define ORC_MIN_DIR_RATIO 0.05", "#define ORC_MIN_DIR_RATIO 0.1",
+     "pairs"),
+    ("pairs: pair score penalty sign", "oracle.c", "+ .721 * log(2. * erfc(", "- .721 * log(2. * erfc(", "pairs"),
+    ("pairs: best pair counted as near-best", "oracle.c", "for (i = (int)nu - 2, *n_sub = 0; i >= 0; --i)",
+     "for (i = (int)nu - 1, *n_sub = 0; i >= 0; --i)", "pairs"),
 ]
 
 
@@ -97,5 +114,7 @@ def test_pin_catches_mutant(name, fname, old, new, pin, tmp_path, chain_idx, pla
             check_reseed(orc, planted_reseed)
         elif pin == "quirk":
             check_quirk(orc)
+        elif pin == "pairs":
+            check_pairs(orc)
         else:
             check_emitters(cli, emitter_case, tmp_path)

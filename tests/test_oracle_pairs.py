@@ -23,6 +23,14 @@ def orc():
     return oracle_lib.Oracle()
 
 
+def check_pairs(orc):
+    """Every pin of this file against one oracle build (for the mutation checks)."""
+    for t in (test_infer_dir_orientations, test_cal_sub_cases, test_pestat_uniform_sizes_closed_form,
+              test_pestat_outliers_and_stddev_bound, test_pestat_filters_and_direction_counts,
+              test_mem_pair_single_candidate, test_mem_pair_picks_the_best_and_counts_near_ones):
+        t(orc)
+
+
 def rev(fwd_end):
     """Text coordinate of a reverse-strand hit whose forward-strand end base is fwd_end."""
     return 2 * L - 1 - fwd_end
@@ -54,6 +62,8 @@ def test_cal_sub_cases(orc):
     assert orc.cal_sub([0, 40], [60, 90], [60, 50]) == floor
     # overlap of 30 against 50: 30 >= 25 -> sub-hit
     assert orc.cal_sub([0, 30], [60, 80], [60, 50]) == 50
+    # overlap of exactly half the shorter hit counts (>=)
+    assert orc.cal_sub([0, 35], [60, 85], [60, 50]) == 50
     # the first qualifying hit is taken, not the best
     assert orc.cal_sub([0, 0, 0], [150, 150, 150], [150, 90, 140]) == 90
 
@@ -107,6 +117,15 @@ def test_pestat_outliers_and_stddev_bound(orc):
     assert fr["std"] == pytest.approx(math.sqrt(500.0), rel=1e-14)
     # mapping band [300 - 120, 340 + 120] = [180, 460]; avg +- 4 sd = [220.6, 399.4] lies inside
     assert (fr["low"], fr["high"]) == (180, 460)
+    # n = 41, sorted 300 x10, 310 x10, 320 x11, 330 x9, 365: quartile indices
+    # (int)(10.25 + .499) = 10 and (int)(30.75 + .499) = 31 -> 310, 330; the
+    # outlier band [270, 370] keeps 365 (a band of 1.5 IQR, or p75 read at
+    # index 30, would drop it)
+    sizes = [300] * 10 + [310] * 10 + [320] * 11 + [330] * 9 + [365]
+    fr = orc.pestat(L, *_pairs_to_arrays(_fr(sizes)))[1]
+    assert fr["avg"] == 12955 / 41
+    assert fr["std"] == pytest.approx(float(np.std(np.array(sizes, dtype=np.float64))), rel=1e-12)
+    assert (fr["low"], fr["high"]) == (250, 390)
 
 
 def test_pestat_filters_and_direction_counts(orc):
