@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "g3a_kernels.h"
+#include "g3a_pair.h"
 
 #pragma clang fp contract(off)
 
@@ -110,9 +111,10 @@ __device__ __forceinline__ int64_t wave_sum(int64_t v) {
 // sizes, the mean and standard deviation of those within the outlier bounds,
 // and the mapping bounds.  The mean's running sum is exact (integers below
 // 2^53), so it is summed here as an integer; the variance's running sum
-// rounds at every term and is added up in the reference's order (ascending
-// size) by one lane.  Then the orientations with too few pairs against the
-// most frequent one fail (bwa/src/aux.cu:492-498).
+// rounds at every term, in the reference's order (ascending size): the c
+// equal terms of one size are added by repeat_add, which gives the result of
+// c single rounded adds.  Then the orientations with too few pairs against
+// the most frequent one fail (bwa/src/aux.cu:492-498).
 __global__ void __launch_bounds__(256) k_pe_stat(PairArgs a) {
   __shared__ int64_t cnt[4];
   const int d = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -155,15 +157,20 @@ __global__ void __launch_bounds__(256) k_pe_stat(PairArgs a) {
     x = wave_sum(x);
     double avg = (double)sum;
     avg /= (double)x;
+    // every lane runs the same sum; the wavefront loads 64 counts at a time
+    const int64_t hi = high < M ? high : M;
     double sd = 0;
-    if (lane == 0) {
-      for (int64_t v = low; v <= high && v <= M; ++v) {
-        const double dv = (double)v - avg;
-        const double sq = dv * dv;
-        for (uint32_t c = h[v]; c; --c) sd += sq;
+    for (int64_t base = low; base <= hi; base += 64) {
+      const uint32_t c = base + lane <= hi ? h[base + lane] : 0;
+      const int nj = (int)(hi - base + 1 < 64 ? hi - base + 1 : 64);
+      for (int j = 0; j < nj; ++j) {
+        const uint32_t cj = __shfl(c, j);
+        if (cj) {
+          const double dv = (double)(base + j) - avg;
+          sd = repeat_add(sd, dv * dv, cj);
+        }
       }
     }
-    sd = __shfl(sd, 0);
     sd = sqrt(sd / (double)x);
     st.avg = avg;
     st.std = sd;
