@@ -1358,7 +1358,9 @@ int g3a_mem_pestat(g3a_ctx *ctx, int max_ins, g3a_pestat pes[4]) {
   PairArgs a;
   if (int rc = pair_args(ctx, a)) return rc;
   a.max_ins = max_ins;
-  if (int rc = ensure(ctx, ctx->pe_hist, 16 * ((size_t)max_ins + 1))) return rc;
+  // up to 16 histogram copies, within 64 MiB
+  a.copies = (int)std::max<size_t>(1, std::min<size_t>(16, ((size_t)64 << 20) / (16 * ((size_t)max_ins + 1))));
+  if (int rc = ensure(ctx, ctx->pe_hist, 16 * ((size_t)max_ins + 1) * (size_t)a.copies)) return rc;
   if (int rc = ensure(ctx, ctx->pe_stat, 4 * sizeof(PeStat))) return rc;
   a.hist = P<uint32_t>(ctx->pe_hist);
   a.stat = P<PeStat>(ctx->pe_stat);

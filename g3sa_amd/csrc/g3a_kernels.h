@@ -231,7 +231,8 @@ struct PairArgs {
   int64_t l_pac;
   int a, b, o_del, e_del, o_ins, e_ins, min_seed_len, max_ins;
   float mask_level;
-  uint32_t *hist;  // 4 * (max_ins + 1)
+  uint32_t *hist;  // copies * 4 * (max_ins + 1)
+  int copies;      // histogram copies that spread the counting atomics
   PeStat *stat;    // 4, written by launch_pe_stat
   PeStat pes[4];   // read by launch_pe_pair
   uint64_t *v;     // two words per alignment

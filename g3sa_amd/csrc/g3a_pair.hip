@@ -76,6 +76,9 @@ __global__ void __launch_bounds__(256) k_pe_zero(uint32_t *h, int64_t n) {
 // unique top hit on one contig, counted in its orientation's histogram
 // (bwa/src/aux.cu:441-459).  The reference collects the sizes in per-
 // orientation lists and sorts them; the histogram holds the same multiset.
+// Sizes cluster on a few hundred words, so each block counts into one of
+// `copies` histograms (atomics on one word serialise in L2), which k_pe_fold
+// adds into the first.
 __global__ void __launch_bounds__(256) k_pe_isize(PairArgs a) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= a.n_pairs) return;
@@ -87,7 +90,17 @@ __global__ void __launch_bounds__(256) k_pe_isize(PairArgs a) {
   if (r0->rid != r1->rid) return;
   int64_t is;
   const int dir = infer_dir(a.l_pac, r0->rb, r1->rb, &is);
-  if (is && is <= a.max_ins) atomicAdd(a.hist + (int64_t)dir * (a.max_ins + 1) + is, 1u);
+  if (is && is <= a.max_ins)
+    atomicAdd(a.hist + ((int64_t)(blockIdx.x % a.copies) * 4 + dir) * (a.max_ins + 1) + is, 1u);
+}
+
+__global__ void __launch_bounds__(256) k_pe_fold(PairArgs a) {
+  const int64_t nb = 4 * ((int64_t)a.max_ins + 1);
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < nb; i += (int64_t)gridDim.x * 256) {
+    uint32_t s = a.hist[i];
+    for (int c = 1; c < a.copies; ++c) s += a.hist[c * nb + i];
+    a.hist[i] = s;
+  }
 }
 
 // Inclusive prefix sum over a wavefront.
@@ -304,10 +317,12 @@ __global__ void __launch_bounds__(256) k_pe_pair(PairArgs a) {
 }  // namespace
 
 void launch_pe_stat(const PairArgs &a, hipStream_t s) {
-  const int64_t nh = 4 * ((int64_t)a.max_ins + 1);
+  const int64_t nb = 4 * ((int64_t)a.max_ins + 1), nh = nb * a.copies;
   hipLaunchKernelGGL(k_pe_zero, dim3((unsigned)std::min<int64_t>((nh + 255) / 256, 4096)), dim3(256), 0, s, a.hist, nh);
   if (a.n_pairs)
     hipLaunchKernelGGL(k_pe_isize, dim3((unsigned)((a.n_pairs + 255) / 256)), dim3(256), 0, s, a);
+  if (a.copies > 1)
+    hipLaunchKernelGGL(k_pe_fold, dim3((unsigned)std::min<int64_t>((nb + 255) / 256, 1024)), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_pe_stat, dim3(1), dim3(256), 0, s, a);
 }
 
