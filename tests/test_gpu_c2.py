@@ -3,13 +3,16 @@ interspersed repeats (GC per 10 kb window ~ N(0.41, 0.05), 200 families of
 300 bp-6 kbp at 5-15% divergence), its index built on the GPU and compared
 byte for byte with the host build, and 100,000 of its paired 2x150 bp reads
 (aligned as single-end reads) through the plain, batched and batched +
-learned-index forms, every field against the oracle."""
+learned-index forms, every field against the oracle; then the paired-end
+statistics and pair choice of its 50,000 pairs against the oracle."""
 from __future__ import annotations
 
 import pytest
 
 import g3sa_amd
+import oracle_lib
 from conftest import gen, oracle_threads
+from test_gpu_pairs import _check
 from test_gpu_parity import compare
 
 pytestmark = pytest.mark.gpu
@@ -51,4 +54,8 @@ def test_c2_full_index_parity(built, oracle, tmp_path):
             got = al.align(seq, off)
             compare(got, ref, f"c2 {form}")
             assert got.work["n_seeds"] == ref["work"]["sa_reads"]
+        print(f"[c2] {n // 2} pairs: insert-size statistics and pair choice", flush=True)
+        l_pac, ctg = oracle_lib.read_ann(prefix)
+        pes, st = _check(al, got, oracle, l_pac, ctg, 10000, 1000, "c2 pairs")
+        assert pes[1]["failed"] == 0 and st["proper"] > 0.9 * (n // 2)
     assert got.n_alns > 0.95 * n
