@@ -23,6 +23,10 @@
  *   global alignment ........ textbook Gotoh score and CIGAR consistency
  *   finalize / MAPQ ......... reads sampled at known places; MAPQ known
  *                             answers
+ *   paired-end statistics ... closed-form size sets (quartiles, mean,
+ *   and pair choice           population deviation, bounds, failure rules);
+ *                             hand-placed hits for the pair scores
+ *   hash_64 ................. PARITY UNPINNED (bijection on samples only)
  */
 #define _GNU_SOURCE
 #include "oracle.h"

@@ -10,6 +10,9 @@
  * Where the reference's behaviour is undefined (data races, uninitialised
  * reads), the oracle implements a documented resolution; each is marked
  * "DR" at the point of use.
+ *
+ * Parity unpinned: orc_hash_64 (no known-answer vectors; see its
+ * declaration).
  */
 #ifndef G3A_ORACLE_H
 #define G3A_ORACLE_H
@@ -133,6 +136,10 @@ int orc_cal_sub(const orc_opt *opt, int n, const int32_t *qb, const int32_t *qe,
 void orc_mem_pestat(const orc_opt *opt, int64_t l_pac, int max_ins, int n, const int64_t *aln_off,
                     const int32_t *rid, const int64_t *rb, const int32_t *qb, const int32_t *qe,
                     const int32_t *score, orc_pestat pes[4]);
+/* Parity unpinned: checked only as a bijection on samples (no known-answer
+ * vectors here); constants read against bwa/src/aux.cu:703-714 and
+ * bwa/src/utils.h:102-112.  Used only as a tie-break between equal pair
+ * scores. */
 uint64_t orc_hash_64(uint64_t key);
 int orc_mem_pair(const orc_opt *opt, int64_t l_pac, const int64_t *ctg_off, const orc_pestat pes[4], int id,
                  int n0, int n1, const int32_t *rid, const int64_t *rb, const int32_t *score, int *sub,
