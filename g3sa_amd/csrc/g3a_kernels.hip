@@ -520,14 +520,21 @@ __device__ void chain_one(const ChainArgs &a, int r) {
 // positions (two words) and chain ids; the filter: chain ids, weights and
 // begin | end << 16 in weight order.
 constexpr int kChainThread = 16;
+// A thread's LDS slice, 104 words (6 blocks of 64 threads per compute unit).
+// A seed's word at kRbHi holds bits 32..47 of its position (texts stay below
+// 2^48 rows) over its contig id, or 0xffff for an id outside [0, 455): such
+// seeds never start a chain (bwa/src/aux.cu:1799) and no chain's id equals
+// the mark, so the comparisons of ids are unchanged.  The filter's arrays
+// (kOc, kOw, kObe) take the place of the B-tree's positions (kTpLo, kTpHi),
+// which the chaining no longer needs once they are written.
 struct ChainLds {
   enum : int {
-    kRbLo = 0, kRbHi = 16, kQl = 32, kRid = 48,  // seeds
-    kNxt = 64,                                   // bytes: next seed of the chain
-    kCh = 68,                                    // chains
-    kTpLo = 84, kTpHi = 100, kTc = 116,          // B-tree order: positions, chain ids (bytes)
-    kOc = 120, kOw = 124, kObe = 132,            // filter order: chain ids (bytes), weights (halves), begin | end
-    kWords = 148
+    kRbLo = 0, kRbHi = 16, kQl = 32,    // seeds: position bits 0..31; bits 32..47 | contig id; qbeg | len
+    kNxt = 48,                          // bytes: next seed of the chain
+    kCh = 52,                           // chains
+    kTpLo = 68, kTpHi = 84, kTc = 100,  // B-tree order: positions, chain ids (bytes)
+    kOc = 68, kOw = 72, kObe = 80,      // filter order: chain ids (bytes), weights (halves), begin | end
+    kWords = 104
   };
   uint32_t *w;
   __device__ uint32_t get(int k) const { return w[64 * k]; }
@@ -536,12 +543,14 @@ struct ChainLds {
   __device__ void setb(int k, int e, int v) const { reinterpret_cast<uint8_t *>(w + 64 * (k + (e >> 2)))[e & 3] = (uint8_t)v; }
   __device__ int geth(int k, int e) const { return reinterpret_cast<const uint16_t *>(w + 64 * (k + (e >> 1)))[e & 1]; }
   __device__ void seth(int k, int e, int v) const { reinterpret_cast<uint16_t *>(w + 64 * (k + (e >> 1)))[e & 1] = (uint16_t)v; }
-  __device__ int64_t rb(int i) const { return (int64_t)((uint64_t)get(kRbHi + i) << 32 | get(kRbLo + i)); }
+  __device__ int64_t rb(int i) const { return (int64_t)((uint64_t)(get(kRbHi + i) >> 16) << 32 | get(kRbLo + i)); }
+  __device__ int rid(int i) const { return (int)(get(kRbHi + i) & 0xffff); }
   __device__ int qb(int i) const { return (int)(get(kQl + i) & 0xffff); }
   __device__ int ln(int i) const { return (int)(get(kQl + i) >> 16); }
   __device__ int64_t tp(int s) const { return (int64_t)((uint64_t)get(kTpHi + s) << 32 | get(kTpLo + s)); }
 };
 static_assert(kChainThread <= 16, "the slice's arrays hold 16 entries");
+static_assert(ChainLds::kObe + 16 <= ChainLds::kTc && ChainLds::kTpHi + 16 <= ChainLds::kTc, "slice layout");
 
 __device__ bool chain_thread(const ChainArgs &a, int r, int64_t b, int ns, const ChainLds &m) {
   const Opt &o = a.opt;
@@ -559,12 +568,11 @@ __device__ bool chain_thread(const ChainArgs &a, int r, int64_t b, int ns, const
       m.set(ChainLds::kRbLo + j, m.get(ChainLds::kRbLo + j - 1));
       m.set(ChainLds::kRbHi + j, m.get(ChainLds::kRbHi + j - 1));
       m.set(ChainLds::kQl + j, kj);
-      m.set(ChainLds::kRid + j, m.get(ChainLds::kRid + j - 1));
     }
+    const uint32_t rid16 = v.rid >= 0 && v.rid < 455 ? (uint32_t)v.rid : 0xffffu;
     m.set(ChainLds::kRbLo + j, (uint32_t)(uint64_t)v.rbeg);
-    m.set(ChainLds::kRbHi + j, (uint32_t)((uint64_t)v.rbeg >> 32));
+    m.set(ChainLds::kRbHi + j, (uint32_t)((uint64_t)v.rbeg >> 32) << 16 | rid16);
     m.set(ChainLds::kQl + j, key);
-    m.set(ChainLds::kRid + j, (uint32_t)v.rid);
   }
   for (int i = 0; i < ns; ++i) m.setb(ChainLds::kNxt, i, -1);
   // chaining (BTreeChaining, aux.cu:1755-1825)
@@ -572,7 +580,7 @@ __device__ bool chain_thread(const ChainArgs &a, int r, int64_t b, int ns, const
   bool dup = false;
   for (int i = 0; i < ns; ++i) {
     const int64_t prb = m.rb(i);
-    const int pqb = m.qb(i), pln = m.ln(i), prid = (int)m.get(ChainLds::kRid + i);
+    const int pqb = m.qb(i), pln = m.ln(i), prid = m.rid(i);
     // the first slot whose position is >= prb
     int lo = 0, hi = nch;
     while (lo < hi) {
@@ -592,7 +600,7 @@ __device__ bool chain_thread(const ChainArgs &a, int r, int64_t b, int ns, const
         const int last_qb = m.qb(ct), last_ln = m.ln(ct), first_qb = m.qb(chd);
         const int64_t qend = last_qb + last_ln, rend = last_rb + last_ln;
         bool merged = false;
-        if (prid == (int)m.get(ChainLds::kRid + chd)) {
+        if (prid == m.rid(chd)) {
           if (pqb >= first_qb && pqb + pln <= qend && prb >= first_rb && prb + pln <= rend) {
             merged = true;  // contained in the chain: dropped
           } else if (!((last_rb < l_pac || first_rb < l_pac) && prb >= l_pac)) {
