@@ -15,8 +15,8 @@ alignment + finalize) and formatted on the device into SAM records with
 MAPQ, NM, MD and AS (the last SAM byte in HBM).  `value` is reads per second
 of such steps.  The same steps through PCIe (reads in from the host, SAM out
 to the host) are reported as `pcie_inclusive`; the `g3a mem` command line over
-the FASTQ file as `end_to_end`.  The CPU baseline is the oracle (C, the box's
-16 host threads) on a bounded sample of the same reads, alignment plus record
+the FASTQ file as `end_to_end`.  The CPU baseline is the oracle (C, up to 16
+host threads) on a bounded sample of the same reads, alignment plus record
 formatting; its SAM for that sample is compared byte for byte with `g3a mem
 -S` on the same reads at the workload's full index size (`parity_sample`).
 
@@ -163,8 +163,8 @@ def cpu_model() -> str:
 
 
 def cpu_baseline(prefix: Path, fq: Path, sample: int, sam_out: Path | None = None) -> dict | None:
-    """The oracle on the first `sample` reads: alignment on the host threads
-    the box gives one GPU (16 at most), then SAM record formatting; writing
+    """The oracle on the first `sample` reads: alignment on up to 16 host
+    threads (those this process may use), then SAM record formatting; writing
     its SAM to `sam_out` (for the parity sample) is not timed."""
     if not ORACLE_CLI.exists():
         return None

@@ -2,7 +2,7 @@
 
 Datasets are generated with the project's deterministic generator (g3a-gen)
 into a per-session temporary directory; nothing here reads the reference
-repository, so the same fixtures work on the GPU box.
+repository, so the same fixtures work on any machine with the built code.
 """
 from __future__ import annotations
 
@@ -26,7 +26,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def built():
-    """The native build must exist (tests never build it themselves on the box)."""
+    """The native build must exist (tests never build it themselves)."""
     missing = [p for p in (g3sa_amd.LIB_PATH, g3sa_amd.CLI_PATH, g3sa_amd.GEN_PATH,
                            ROOT / "oracle" / "_build" / "liboracle.so",
                            ROOT / "oracle" / "_build" / "g3a_oracle") if not p.exists()]
